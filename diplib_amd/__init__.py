@@ -1,0 +1,12 @@
+"""diplib_amd -- MI355X-native Gaussian FIR filtering with DIPlib's semantics.
+
+The compute path is libdipamd.so (HIP kernels for gfx950 behind the C ABI in
+include/dipamd.h); this package is its PyDIP-style front end on PyTorch-ROCm
+device tensors.  Importing the package does not touch the GPU.
+"""
+from ._lib import DipError, lib  # noqa: F401
+from .gauss import (Gauss, GaussFIR, MakeHalfGaussian, OneDimensionalFilter,  # noqa: F401
+                    SeparableConvolution, best_gauss_method, gauss_radius, image_struct)
+
+__all__ = ["Gauss", "GaussFIR", "SeparableConvolution", "OneDimensionalFilter", "MakeHalfGaussian", "DipError",
+           "gauss_radius", "best_gauss_method", "lib"]

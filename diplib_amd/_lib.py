@@ -1,0 +1,97 @@
+"""ctypes binding of libdipamd.so, the C ABI declared in include/dipamd.h.
+
+The library is the only compute path: when it is missing or fails to load, every
+call raises.  There is no CPU fallback.
+"""
+import ctypes
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libdipamd.so")
+
+MAX_DIMS = 8
+
+DT_UINT8, DT_UINT16, DT_UINT32, DT_SINT8, DT_SINT16, DT_SINT32, DT_SFLOAT, DT_DFLOAT, DT_BIN = range(9)
+
+BC_CODES = {
+    "": 0, "default": 0, "mirror": 0, "asym mirror": 1, "antisym reflect": 2, "periodic": 3,
+    "asym periodic": 4, "add zeros": 5, "add max": 6, "add min": 7, "zero order": 8,
+    "first order": 9, "second order": 10, "third order": 11, "already expanded": 12,
+}
+SYM_CODES = {"": 0, "general": 0, "even": 1, "odd": 2, "conj": 3, "d-even": 4, "d-odd": 5, "d-conj": 6}
+
+EXACT, FAST, NO_FUSION, FORCE_FUSION = 0, 1, 2, 4
+
+E_PARAMETER, E_NOT_FORGED, E_ARRAY_LENGTH, E_SIZES, E_UNSUPPORTED, E_HIP, E_ORDER = -1, -2, -3, -4, -5, -6, -7
+
+
+class DipamdImage(ctypes.Structure):
+    _fields_ = [("origin", ctypes.c_void_p), ("dtype", ctypes.c_int32), ("ndim", ctypes.c_int32),
+                ("sizes", ctypes.c_int64 * MAX_DIMS), ("strides", ctypes.c_int64 * MAX_DIMS)]
+
+
+class DipamdFilter(ctypes.Structure):
+    _fields_ = [("taps", ctypes.POINTER(ctypes.c_double)), ("ntaps", ctypes.c_int32), ("origin", ctypes.c_int32),
+                ("symmetry", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class DipamdBatchImage(ctypes.Structure):
+    _fields_ = [("inp", ctypes.c_void_p), ("out", ctypes.c_void_p), ("width", ctypes.c_int32),
+                ("height", ctypes.c_int32), ("in_stride_y", ctypes.c_int64), ("out_stride_y", ctypes.c_int64),
+                ("sigma_x", ctypes.c_double), ("sigma_y", ctypes.c_double), ("order_x", ctypes.c_int32),
+                ("order_y", ctypes.c_int32)]
+
+
+class DipError(RuntimeError):
+    """Error raised by the library (dip::ParameterError / dip::RunTimeError counterpart)."""
+
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Loads libdipamd.so (once).  Raises when it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built: run `python -m diplib_amd.build` "
+                           "(or __graft_entry__.build()); there is no fallback path")
+    L = ctypes.CDLL(LIB_PATH)
+    P, I32, U32, D = ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32, ctypes.c_double
+    IMG = ctypes.POINTER(DipamdImage)
+    L.dipamd_gauss_fir.argtypes = [IMG, IMG, P, I32, P, I32, P, I32, D, U32, P]
+    L.dipamd_separable_convolution.argtypes = [IMG, IMG, ctypes.POINTER(DipamdFilter), I32, P, I32, P, U32, P]
+    L.dipamd_make_half_gaussian.argtypes = [D, I32, D, I32, P, I32]
+    L.dipamd_gauss_radius.argtypes = [D, I32, D, I32]
+    L.dipamd_last_error.restype = ctypes.c_char_p
+    L.dipamd_version.restype = ctypes.c_char_p
+    if hasattr(L, "dipamd_batch_create"):
+        L.dipamd_batch_create.argtypes = [ctypes.POINTER(P), ctypes.POINTER(DipamdBatchImage), I32, I32, I32, P, I32,
+                                          D, U32, I32, P]
+        L.dipamd_batch_run.argtypes = [P, P]
+        L.dipamd_batch_run_graph.argtypes = [P, P]
+        L.dipamd_batch_destroy.argtypes = [P]
+        L.dipamd_batch_info.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(I32)]
+    if hasattr(L, "dipamd_gauss_fir_backward"):
+        L.dipamd_gauss_fir_backward.argtypes = [IMG, IMG, IMG, P, P, I32, P, I32, P, I32, D, U32, P]
+    _lib = L
+    return L
+
+
+def check(status):
+    if status < 0:
+        msg = lib().dipamd_last_error().decode(errors="replace")
+        raise DipError(status, msg)
+    return status
+
+
+EXPORTED_SYMBOLS = [
+    "dipamd_gauss_fir", "dipamd_separable_convolution", "dipamd_make_half_gaussian", "dipamd_batch_create",
+    "dipamd_batch_run", "dipamd_batch_run_graph", "dipamd_batch_destroy", "dipamd_batch_num_configs",
+    "dipamd_batch_info", "dipamd_gauss_fir_backward", "dipamd_gauss_radius", "dipamd_last_error", "dipamd_version",
+]

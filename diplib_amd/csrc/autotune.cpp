@@ -1,0 +1,97 @@
+// Schedule choice for the fused 2-D kernel, keyed by (dtype, radius bucket).
+//
+// A table of measured choices is read from the file named by DIPAMD_TUNE_FILE (or
+// diplib_amd/tune_cache.txt next to the library) when present; lines are
+//     <is_double> <radius_lo> <radius_hi> <config or -1>
+// with -1 meaning "run one pass per axis".  Without a table the built-in defaults
+// below apply.  DIPAMD_FUSED_CONFIG=<n> forces a config (or -1) for every call.
+#include <dlfcn.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "dipamd_internal.hpp"
+#include "sep_kernels.hpp"
+
+namespace dipamd {
+
+namespace {
+
+struct TuneRow {
+    int is_double, rlo, rhi, cfg;
+};
+
+std::once_flag g_tune_once;
+std::vector<TuneRow> g_tune;
+int g_forced = -2;
+
+std::string library_dir() {
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void *>(&choose_fused_config), &info) && info.dli_fname) {
+        std::string p(info.dli_fname);
+        size_t s = p.find_last_of('/');
+        if (s != std::string::npos) return p.substr(0, s);
+    }
+    return ".";
+}
+
+void load_table() {
+    if (const char *f = std::getenv("DIPAMD_FUSED_CONFIG")) g_forced = std::atoi(f);
+    std::string path;
+    if (const char *f = std::getenv("DIPAMD_TUNE_FILE")) {
+        path = f;
+    } else {
+        path = library_dir() + "/tune_cache.txt";
+    }
+    FILE *fp = std::fopen(path.c_str(), "r");
+    if (!fp) return;
+    char line[256];
+    while (std::fgets(line, sizeof(line), fp)) {
+        if (line[0] == '#') continue;
+        TuneRow r;
+        if (std::sscanf(line, "%d %d %d %d", &r.is_double, &r.rlo, &r.rhi, &r.cfg) == 4) g_tune.push_back(r);
+    }
+    std::fclose(fp);
+}
+
+bool fits(int cfg, bool is_double, int rx, int ry) {
+    if (cfg < 0 || cfg >= kNumFusedConfigs) return false;
+    if (is_double && cfg == 2) return false;
+    return fused_lds_bytes(cfg, rx, ry, is_double) <= 65536;
+}
+
+}  // namespace
+
+int choose_fused_config(bool is_double, int rx, int ry, int64_t nx, int64_t ny, uint32_t flags) {
+    std::call_once(g_tune_once, load_table);
+    (void)ny;
+    const int r = rx > ry ? rx : ry;
+    if (g_forced != -2) return fits(g_forced, is_double, rx, ry) ? g_forced : -1;
+    int cfg = -2;
+    for (const TuneRow &t : g_tune) {
+        if ((t.is_double != 0) == is_double && r >= t.rlo && r <= t.rhi) {
+            cfg = t.cfg;
+            break;
+        }
+    }
+    if (cfg == -2) {
+        // defaults: fuse while the y halo is small against the row chunk
+        if (r > 24 && !(flags & DIPAMD_FORCE_FUSION)) return -1;
+        cfg = is_double ? 0 : (nx >= 256 ? 1 : 0);
+    }
+    if (cfg < 0) {
+        if (!(flags & DIPAMD_FORCE_FUSION)) return -1;
+        cfg = is_double ? 0 : 1;
+    }
+    if (!fits(cfg, is_double, rx, ry)) {
+        for (int c = 0; c < kNumFusedConfigs; ++c)
+            if (fits(c, is_double, rx, ry)) return c;
+        return -1;
+    }
+    return cfg;
+}
+
+}  // namespace dipamd

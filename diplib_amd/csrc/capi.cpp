@@ -1,0 +1,695 @@
+// C ABI of the Gaussian FIR path: argument checks, tap generation, the pass
+// planner (the MI355X counterpart of dip::Framework::Separable) and launches.
+//
+// Reference behaviour restated here:
+//   dip::Gauss / GaussFIR           derivative.cpp:62-86, gauss.cpp:238-286
+//   HalfGaussianSize / taps         gauss.cpp:38-146, numeric.h:69-76
+//   SeparableConvolution            convolution.cpp:319-421 (filter internalisation :67-162,
+//                                   border :355-368, process mask :370-387)
+//   Framework::Separable            framework_separable.cpp:35-431 (pass order by input
+//                                   stride :146-163, buffer-type intermediates :186-206,
+//                                   clamp_cast on the last write :384-418)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "dipamd.h"
+#include "dipamd_internal.hpp"
+#include "sep_common.hpp"
+#include "sep_kernels.hpp"
+
+namespace dipamd {
+
+namespace {
+thread_local std::string g_last_error;
+}  // namespace
+
+int set_error(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+const char *last_error() { return g_last_error.c_str(); }
+
+size_t dt_size(int dt) {
+    switch (dt) {
+        case DIPAMD_DT_UINT8:
+        case DIPAMD_DT_SINT8:
+        case DIPAMD_DT_BIN: return 1;
+        case DIPAMD_DT_UINT16:
+        case DIPAMD_DT_SINT16: return 2;
+        case DIPAMD_DT_UINT32:
+        case DIPAMD_DT_SINT32:
+        case DIPAMD_DT_SFLOAT: return 4;
+        case DIPAMD_DT_DFLOAT: return 8;
+        default: return 0;
+    }
+}
+
+// DataType::SuggestFlex (datatype.cpp:109-124), real types.
+int suggest_flex(int dt) {
+    return (dt == DIPAMD_DT_UINT32 || dt == DIPAMD_DT_SINT32 || dt == DIPAMD_DT_DFLOAT) ? DIPAMD_DT_DFLOAT
+                                                                                      : DIPAMD_DT_SFLOAT;
+}
+
+// ---------------------------------------------------------------------------
+// Taps
+// ---------------------------------------------------------------------------
+int gauss_radius(double sigma, int order, double truncation, bool is_double) {
+    if (truncation <= 0) truncation = 3;
+    const double eps = is_double ? std::numeric_limits<double>::epsilon()
+                                 : static_cast<double>(std::numeric_limits<float>::epsilon());
+    const double max_trunc = std::sqrt(-2.0 * std::log(eps));
+    truncation = std::min(truncation, max_trunc);
+    truncation += 0.5 * static_cast<double>(order);
+    const double v = std::ceil(truncation * sigma);
+    if (!(v > 0)) return 0;
+    if (v > 2147483647.0) return 2147483647;
+    return static_cast<int>(v);
+}
+
+// Half Gaussian of n = r+1 taps, x = 0 LAST (gauss.cpp:53-146).
+std::vector<double> half_gaussian_n(double sigma, int order, int n) {
+    std::vector<double> f(static_cast<size_t>(n));
+    const int r0 = n - 1;
+    const double sigma2 = sigma * sigma;
+    const double pi = 3.14159265358979323846264338327950288;
+    switch (order) {
+        case 0: {
+            const double factor = -0.5 / sigma2;
+            double normalization = 0;
+            f[r0] = 1.0;
+            for (int rr = 1; rr < n; rr++) {
+                const double rad = rr;
+                const double g = std::exp(factor * (rad * rad));
+                f[r0 - rr] = g;
+                normalization += g;
+            }
+            normalization = 1.0 / (normalization * 2 + 1);
+            for (int rr = 0; rr < n; rr++) f[rr] *= normalization;
+            break;
+        }
+        case 1: {
+            const double factor = -0.5 / sigma2;
+            double moment = 0.0;
+            f[r0] = 0.0;
+            for (int rr = 1; rr < n; rr++) {
+                const double rad = rr;
+                const double g = rad * std::exp(factor * (rad * rad));
+                f[r0 - rr] = g;
+                moment += rad * g;
+            }
+            const double normalization = 1.0 / (2.0 * moment);
+            for (int rr = 0; rr < n - 1; rr++) f[rr] *= normalization;
+            break;
+        }
+        case 2: {
+            const double norm = 1.0 / (std::sqrt(2.0 * pi) * sigma * sigma2);
+            double mean = 0.0;
+            f[r0] = -norm;
+            for (int rr = 1; rr < n; rr++) {
+                const double rad = rr;
+                const double sr2 = rad * rad / sigma2;
+                const double g = (sr2 - 1.0) * norm * std::exp(-0.5 * sr2);
+                f[r0 - rr] = g;
+                mean += g;
+            }
+            mean = (mean * 2.0 + f[r0]) / (static_cast<double>(r0) * 2.0 + 1.0);
+            f[r0] -= mean;
+            double moment = 0.0;
+            for (int rr = 1; rr < n; rr++) {
+                const double rad = rr;
+                f[r0 - rr] -= mean;
+                moment += rad * rad * f[r0 - rr];
+            }
+            const double normalization = 1.0 / moment;
+            for (int rr = 0; rr < n; rr++) f[rr] *= normalization;
+            break;
+        }
+        case 3: {
+            const double norm = 1.0 / (std::sqrt(2.0 * pi) * sigma * sigma2 * sigma2);
+            f[r0] = 0.0;
+            double moment = 0.0;
+            for (int rr = 1; rr < n; rr++) {
+                const double rad = rr;
+                const double rr2 = rad * rad;
+                const double sr2 = rr2 / sigma2;
+                const double g = norm * std::exp(-0.5 * sr2) * (rad * (3.0 - sr2));
+                f[r0 - rr] = g;
+                moment += g * rr2 * rad;
+            }
+            const double normalization = 3.0 / moment;
+            for (int rr = 0; rr < n; rr++) f[rr] *= normalization;
+            break;
+        }
+        default: break;
+    }
+    return f;
+}
+
+// MakeHalfGaussian (gauss.cpp:151-165).
+std::vector<double> half_gaussian(double sigma, int order, double truncation, bool is_double) {
+    if (sigma == 0.0) return {1.0};
+    int n = 1 + gauss_radius(sigma, order, truncation, is_double);
+    if (order > 2 && n < 2) n = 2;
+    return half_gaussian_n(sigma, order, n);
+}
+
+// ---------------------------------------------------------------------------
+// Filter internalisation (convolution.cpp:67-162).
+// ---------------------------------------------------------------------------
+int intern_filter(const double *taps, int n, int sym, int origin_in, InternFilter &F) {
+    if (sym == DIPAMD_SYM_CONJ) sym = DIPAMD_SYM_EVEN;
+    if (sym == DIPAMD_SYM_D_CONJ) sym = DIPAMD_SYM_D_EVEN;
+    if (sym < 0 || sym > DIPAMD_SYM_D_CONJ) return set_error(DIPAMD_E_PARAMETER, "Symmetry string not recognized");
+    F.sym = sym;
+    F.data.assign(taps, taps + n);
+    std::reverse(F.data.begin(), F.data.end());
+    F.size = n;
+    F.origin = 0;
+    F.border = 0;
+    F.shift = 0;
+    if (n == 0) return DIPAMD_OK;
+    if (sym == DIPAMD_SYM_EVEN || sym == DIPAMD_SYM_ODD) F.size = 2 * n - 1;
+    if (sym == DIPAMD_SYM_D_EVEN || sym == DIPAMD_SYM_D_ODD) F.size = 2 * n;
+    int origin = origin_in < 0 ? F.size / 2 : origin_in;
+    if (origin >= F.size) return set_error(DIPAMD_E_PARAMETER, "Origin outside of filter");
+    F.origin = F.size - origin - 1;
+    F.border = std::max(F.origin, F.size - F.origin - 1);
+    F.shift = (sym == DIPAMD_SYM_GENERAL) ? F.origin : F.origin - (n - 1);
+    return DIPAMD_OK;
+}
+
+bool meaningless(const InternFilter &F, bool use_double) {
+    if (F.size == 0) return true;
+    if (F.size == 1) return use_double ? (F.data[0] == 1.0) : (static_cast<float>(F.data[0]) == 1.0f);
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// Device scratch (stream-ordered; the default pool keeps it cached).
+// ---------------------------------------------------------------------------
+namespace {
+std::once_flag g_pool_once[64];
+}
+
+hipError_t scratch_alloc(void **p, size_t bytes, hipStream_t stream) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (dev >= 0 && dev < 64) {
+        std::call_once(g_pool_once[dev], [dev]() {
+            hipMemPool_t pool;
+            if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+                uint64_t thr = UINT64_MAX;
+                (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+            }
+        });
+    }
+    return hipMallocAsync(p, bytes ? bytes : 256, stream);
+}
+
+void scratch_free(void *p, hipStream_t stream) {
+    if (p) (void)hipFreeAsync(p, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Views and geometry helpers
+// ---------------------------------------------------------------------------
+void byte_range(const View &v, int ndim, uintptr_t *lo, uintptr_t *hi) {
+    const int64_t es = static_cast<int64_t>(dt_size(v.dt));
+    int64_t a = 0, b = 0;
+    for (int d = 0; d < ndim; ++d) {
+        const int64_t ext = (v.sizes[d] - 1) * v.strides[d];
+        if (ext < 0) a += ext; else b += ext;
+    }
+    const uintptr_t base = reinterpret_cast<uintptr_t>(v.p);
+    *lo = base + static_cast<uintptr_t>(a * es);
+    *hi = base + static_cast<uintptr_t>(b * es + es);
+}
+
+bool views_overlap(const View &x, const View &y, int ndim) {
+    uintptr_t a0, a1, b0, b1;
+    byte_range(x, ndim, &a0, &a1);
+    byte_range(y, ndim, &b0, &b1);
+    return a0 < b1 && b0 < a1;
+}
+
+View contiguous_view(void *p, int dt, int ndim, const int64_t *sizes) {
+    View v;
+    v.p = p;
+    v.dt = dt;
+    int64_t s = 1;
+    for (int d = 0; d < ndim; ++d) {
+        v.sizes[d] = sizes[d];
+        v.strides[d] = s;
+        s *= sizes[d];
+    }
+    return v;
+}
+
+static void fill_outer(Outer &o, const View &src, const View &dst, int ndim, int skip0, int skip1) {
+    // remaining dims, ascending |input stride| for locality
+    int dims[DIPAMD_MAX_DIMS];
+    int n = 0;
+    for (int d = 0; d < ndim; ++d)
+        if (d != skip0 && d != skip1 && src.sizes[d] > 1) dims[n++] = d;
+    std::sort(dims, dims + n, [&](int a, int b) { return std::llabs(src.strides[a]) < std::llabs(src.strides[b]); });
+    o.n = n;
+    o.pad_ = 0;
+    for (int i = 0; i < kMaxOuter; ++i) {
+        o.size[i] = 1;
+        o.sin[i] = 0;
+        o.sout[i] = 0;
+    }
+    for (int i = 0; i < n; ++i) {
+        o.size[i] = src.sizes[dims[i]];
+        o.sin[i] = src.strides[dims[i]];
+        o.sout[i] = dst.strides[dims[i]];
+    }
+}
+
+static int64_t product_except(const View &v, int ndim, int skip0, int skip1) {
+    int64_t p = 1;
+    for (int d = 0; d < ndim; ++d)
+        if (d != skip0 && d != skip1) p *= v.sizes[d];
+    return p;
+}
+
+template <typename TB>
+static TB const_value(int bc) {
+    switch (bc) {
+        case DIPAMD_BC_ADD_MAX_VALUE: return std::numeric_limits<TB>::max();
+        case DIPAMD_BC_ADD_MIN_VALUE: return std::numeric_limits<TB>::lowest();
+        default: return TB(0);
+    }
+}
+
+bool bc_supported(int bc) {
+    return bc == DIPAMD_BC_SYMMETRIC_MIRROR || bc == DIPAMD_BC_ASYMMETRIC_MIRROR || bc == DIPAMD_BC_PERIODIC ||
+           bc == DIPAMD_BC_ASYMMETRIC_PERIODIC || bc == DIPAMD_BC_ADD_ZEROS || bc == DIPAMD_BC_ADD_MAX_VALUE ||
+           bc == DIPAMD_BC_ADD_MIN_VALUE || bc == DIPAMD_BC_ZERO_ORDER_EXTRAPOLATE;
+}
+
+// ---------------------------------------------------------------------------
+// Single-axis and fused passes
+// ---------------------------------------------------------------------------
+static int64_t pick_chunk(int64_t len, int64_t base, int granule, int border, int64_t target_blocks) {
+    int64_t want = std::max<int64_t>(1, (target_blocks + base - 1) / std::max<int64_t>(base, 1));
+    int64_t chunk = (len + want - 1) / want;
+    chunk = ((chunk + granule - 1) / granule) * granule;
+    int64_t minc = ((4 * static_cast<int64_t>(border) + granule - 1) / granule) * granule;
+    chunk = std::max<int64_t>(chunk, std::max<int64_t>(granule, minc));
+    int64_t full = ((len + granule - 1) / granule) * granule;
+    return std::min(chunk, full);
+}
+
+template <typename TB>
+static int run_single(const View &src, const View &dst, int ndim, int d, const InternFilter &F, int bc, bool fast,
+                      hipStream_t stream) {
+    if (F.border > kLineCap - 1 || F.data.size() > static_cast<size_t>(kLineCap))
+        return set_error(DIPAMD_E_UNSUPPORTED, "filter longer than the kernels' tap capacity");
+    PassArgs<TB> a;
+    std::memset(&a, 0, sizeof(a));
+    a.in = src.p;
+    a.out = dst.p;
+    a.n = src.sizes[d];
+    a.sin = src.strides[d];
+    a.sout = dst.strides[d];
+    a.border = F.border;
+    a.ntaps = static_cast<int32_t>(F.data.size());
+    a.shift = F.shift;
+    a.bc = bc;
+    a.cval = const_value<TB>(bc);
+    for (size_t k = 0; k < F.data.size(); ++k) a.taps[k] = static_cast<TB>(F.data[k]);
+    const InType ti = (src.dt == DIPAMD_DT_UINT8) ? IN_U8 : IN_BUF;
+    const bool is_double = sizeof(TB) == 8;
+    // line kernel when the filter axis has the smallest |stride| of the source
+    int lane = -1;
+    for (int k = 0; k < ndim; ++k) {
+        if (k == d || src.sizes[k] <= 1) continue;
+        if (lane < 0 || std::llabs(src.strides[k]) < std::llabs(src.strides[lane])) lane = k;
+    }
+    hipError_t err;
+    if (lane < 0 || std::llabs(src.strides[d]) <= std::llabs(src.strides[lane])) {
+        fill_outer(a.outer, src, dst, ndim, d, -1);
+        a.nlines = product_except(src, ndim, d, -1);
+        if (line_lds_bytes(F.border, is_double) > 65536)
+            return set_error(DIPAMD_E_UNSUPPORTED, "filter border too large for the line kernel");
+        err = launch_line(ti, is_double, F.sym, fast, &a, a.n, a.nlines, F.border, stream);
+    } else {
+        a.e = src.sizes[lane];
+        a.ein = src.strides[lane];
+        a.eout = dst.strides[lane];
+        fill_outer(a.outer, src, dst, ndim, d, lane);
+        const int64_t nouter = product_except(src, ndim, d, lane);
+        const int bs = a.e >= 256 ? 256 : 64;
+        const int64_t ntiles = (a.e + bs - 1) / bs;
+        a.chunk = pick_chunk(a.n, ntiles * nouter, 8, F.border, 2048);
+        const int64_t nchunks = (a.n + a.chunk - 1) / a.chunk;
+        err = launch_sweep(ti, is_double, F.sym, fast, &a, a.e, nchunks, nouter, stream);
+    }
+    if (err != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(err));
+    return DIPAMD_OK;
+}
+
+template <typename TB>
+static int run_fused(const View &src, const View &dst, int ndim, int dx, int dy, const InternFilter &Fx,
+                     const InternFilter &Fy, int bcx, int bcy, bool fast, int cfg, hipStream_t stream) {
+    FusedArgs<TB> a;
+    std::memset(&a, 0, sizeof(a));
+    a.in = src.p;
+    a.out = dst.p;
+    a.nx = src.sizes[dx];
+    a.ny = src.sizes[dy];
+    a.sxin = src.strides[dx];
+    a.syin = src.strides[dy];
+    a.sxout = dst.strides[dx];
+    a.syout = dst.strides[dy];
+    a.rx = Fx.border;
+    a.ry = Fy.border;
+    a.bcx = bcx;
+    a.bcy = bcy;
+    a.cvalx = const_value<TB>(bcx);
+    a.cvaly = const_value<TB>(bcy);
+    for (size_t k = 0; k < Fx.data.size(); ++k) a.tx[k] = static_cast<TB>(Fx.data[k]);
+    for (size_t k = 0; k < Fy.data.size(); ++k) a.ty[k] = static_cast<TB>(Fy.data[k]);
+    fill_outer(a.outer, src, dst, ndim, dx, dy);
+    const int64_t nouter = product_except(src, ndim, dx, dy);
+    const int txc = kFusedConfigs[cfg].txc;
+    const int64_t ntiles = (a.nx + txc - 1) / txc;
+    a.chunk = pick_chunk(a.ny, ntiles * nouter, kFusedStep, Fy.border, 2048);
+    const int64_t nchunks = (a.ny + a.chunk - 1) / a.chunk;
+    const InType ti = (src.dt == DIPAMD_DT_UINT8) ? IN_U8 : IN_BUF;
+    hipError_t err = launch_fused(ti, sizeof(TB) == 8, cfg, Fx.sym, Fy.sym, fast, &a, a.nx, nchunks, nouter,
+                                  Fx.border, Fy.border, stream);
+    if (err != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(err));
+    return DIPAMD_OK;
+}
+
+static int run_cast_in(const View &src, const View &dst, int ndim, bool to_double, hipStream_t stream) {
+    CastGeo g;
+    std::memset(&g, 0, sizeof(g));
+    g.ndim = ndim;
+    g.total = 1;
+    for (int d = 0; d < ndim; ++d) {
+        g.size[d] = src.sizes[d];
+        g.sin[d] = src.strides[d];
+        g.sout[d] = dst.strides[d];
+        g.total *= src.sizes[d];
+    }
+    hipError_t err = launch_cast_in(src.dt, to_double, src.p, dst.p, &g, g.total, stream);
+    if (err != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("cast launch: ") + hipGetErrorString(err));
+    return DIPAMD_OK;
+}
+
+static int run_cast_out(const View &src, const View &dst, int ndim, hipStream_t stream) {
+    CastGeo g;
+    std::memset(&g, 0, sizeof(g));
+    g.ndim = ndim;
+    g.total = 1;
+    for (int d = 0; d < ndim; ++d) {
+        g.size[d] = src.sizes[d];
+        g.sin[d] = src.strides[d];
+        g.sout[d] = dst.strides[d];
+        g.total *= src.sizes[d];
+    }
+    hipError_t err = launch_cast_out(src.dt == DIPAMD_DT_DFLOAT, dst.dt, src.p, dst.p, &g, g.total, stream);
+    if (err != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("cast launch: ") + hipGetErrorString(err));
+    return DIPAMD_OK;
+}
+
+// Framework::Separable over device views.  `filters[d]` is the internal filter of
+// dim d; `process[d]` the final process mask (already cleared for size-1 dims
+// and meaningless filters).
+int separable_device(const View &in, const View &out, int ndim, const InternFilter *const *filters,
+                     const int *bc, const bool *process, uint32_t flags, hipStream_t stream) {
+    const int buf_dt = suggest_flex(in.dt);
+    const bool is_double = buf_dt == DIPAMD_DT_DFLOAT;
+    const bool fast = (flags & DIPAMD_FAST) != 0;
+    // pass order: processed dims sorted by (signed) input stride, stable (sortIndices)
+    std::vector<int> order;
+    for (int d = 0; d < ndim; ++d)
+        if (process[d] && in.sizes[d] > 1) order.push_back(d);
+    for (size_t i = 1; i < order.size(); ++i) {
+        int e = order[i];
+        size_t j = i;
+        while (j > 0 && in.strides[order[j - 1]] > in.strides[e]) {
+            order[j] = order[j - 1];
+            --j;
+        }
+        order[j] = e;
+    }
+    for (int d : order) {
+        if (!bc_supported(bc[d])) return set_error(DIPAMD_E_UNSUPPORTED, "boundary condition not supported");
+    }
+    std::vector<void *> scratch;
+    auto alloc = [&](int dt, View *v) -> int {
+        int64_t n = 1;
+        for (int d = 0; d < ndim; ++d) n *= in.sizes[d];
+        void *p = nullptr;
+        hipError_t e = scratch_alloc(&p, static_cast<size_t>(n) * dt_size(dt), stream);
+        if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("scratch allocation: ") + hipGetErrorString(e));
+        scratch.push_back(p);
+        *v = contiguous_view(p, dt, ndim, in.sizes);
+        return DIPAMD_OK;
+    };
+    auto release = [&]() {
+        for (void *p : scratch) scratch_free(p, stream);
+    };
+    int st = DIPAMD_OK;
+    if (order.empty()) {
+        // output.Copy( input ) with conversion
+        if (views_overlap(in, out, ndim) && in.p == out.p && in.dt == out.dt) return DIPAMD_OK;
+        View t;
+        if ((st = alloc(buf_dt, &t)) != DIPAMD_OK) return st;
+        if ((st = run_cast_in(in, t, ndim, is_double, stream)) == DIPAMD_OK) st = run_cast_out(t, out, ndim, stream);
+        release();
+        return st;
+    }
+
+    // Build the pass list: optionally fuse the first two dims.
+    struct PassPlan {
+        bool fused;
+        int d0, d1;
+    };
+    std::vector<PassPlan> passes;
+    size_t i0 = 0;
+    if (!(flags & DIPAMD_NO_FUSION) && order.size() >= 2) {
+        const InternFilter &Fx = *filters[order[0]];
+        const InternFilter &Fy = *filters[order[1]];
+        const bool symok = (Fx.sym == DIPAMD_SYM_EVEN || Fx.sym == DIPAMD_SYM_ODD) &&
+                           (Fy.sym == DIPAMD_SYM_EVEN || Fy.sym == DIPAMD_SYM_ODD) && Fx.shift == 0 && Fy.shift == 0;
+        const bool fits = Fx.border <= kFusedCap - 1 && Fy.border <= kFusedCap - 1;
+        if (symok && fits) {
+            const int cfg = choose_fused_config(is_double, Fx.border, Fy.border, in.sizes[order[0]], in.sizes[order[1]],
+                                                flags);
+            if (cfg >= 0) {
+                passes.push_back({true, order[0], order[1]});
+                i0 = 2;
+            }
+        }
+    }
+    for (size_t i = i0; i < order.size(); ++i) passes.push_back({false, order[i], -1});
+
+    // Source of the first pass: the input, converted first if its type has no kernel.
+    View src = in;
+    if (!(in.dt == DIPAMD_DT_UINT8 && !is_double) && in.dt != buf_dt) {
+        View t;
+        if ((st = alloc(buf_dt, &t)) != DIPAMD_OK) return st;
+        if ((st = run_cast_in(in, t, ndim, is_double, stream)) != DIPAMD_OK) {
+            release();
+            return st;
+        }
+        src = t;
+    }
+    // Destination of the last pass: the output if it holds the buffer type, else a scratch.
+    const bool out_is_buf = out.dt == buf_dt;
+    View final_dst = out;
+    if (!out_is_buf) {
+        if ((st = alloc(buf_dt, &final_dst)) != DIPAMD_OK) return st;
+    }
+    View s1, s2;
+    bool have_s1 = false, have_s2 = false;
+    const size_t m = passes.size();
+    for (size_t k = 0; k < m && st == DIPAMD_OK; ++k) {
+        // choose a destination that does not overlap the pass's source
+        View dst;
+        const bool last = k + 1 == m;
+        const size_t from_end = m - 1 - k;
+        bool want_final = (from_end % 2) == 0;
+        if (want_final && !views_overlap(final_dst, src, ndim)) {
+            dst = final_dst;
+        } else {
+            if (!have_s1) {
+                if ((st = alloc(buf_dt, &s1)) != DIPAMD_OK) break;
+                have_s1 = true;
+            }
+            if (!views_overlap(s1, src, ndim)) {
+                dst = s1;
+            } else {
+                if (!have_s2) {
+                    if ((st = alloc(buf_dt, &s2)) != DIPAMD_OK) break;
+                    have_s2 = true;
+                }
+                dst = s2;
+            }
+        }
+        const PassPlan &pp = passes[k];
+        if (pp.fused) {
+            const int cfg = choose_fused_config(is_double, filters[pp.d0]->border, filters[pp.d1]->border,
+                                                src.sizes[pp.d0], src.sizes[pp.d1], flags);
+            st = is_double ? run_fused<double>(src, dst, ndim, pp.d0, pp.d1, *filters[pp.d0], *filters[pp.d1],
+                                               bc[pp.d0], bc[pp.d1], fast, cfg, stream)
+                           : run_fused<float>(src, dst, ndim, pp.d0, pp.d1, *filters[pp.d0], *filters[pp.d1],
+                                              bc[pp.d0], bc[pp.d1], fast, cfg, stream);
+        } else {
+            st = is_double ? run_single<double>(src, dst, ndim, pp.d0, *filters[pp.d0], bc[pp.d0], fast, stream)
+                           : run_single<float>(src, dst, ndim, pp.d0, *filters[pp.d0], bc[pp.d0], fast, stream);
+        }
+        if (last && st == DIPAMD_OK && dst.p != final_dst.p) {
+            // the last pass could not write the final view directly (it overlapped the source)
+            st = run_cast_out(dst, final_dst, ndim, stream);
+        }
+        src = dst;
+    }
+    if (st == DIPAMD_OK && !out_is_buf) st = run_cast_out(final_dst, out, ndim, stream);
+    release();
+    return st;
+}
+
+// ---------------------------------------------------------------------------
+// Argument checks shared by the entry points
+// ---------------------------------------------------------------------------
+int check_image(const dipamd_image *im, const char *what) {
+    if (!im) return set_error(DIPAMD_E_PARAMETER, std::string(what) + ": null image");
+    if (!im->origin) return set_error(DIPAMD_E_NOT_FORGED, "Image is not forged");
+    if (im->ndim < 1 || im->ndim > DIPAMD_MAX_DIMS)
+        return set_error(DIPAMD_E_PARAMETER, "Dimensionality not supported");
+    if (dt_size(im->dtype) == 0) return set_error(DIPAMD_E_PARAMETER, "Data type not supported");
+    for (int d = 0; d < im->ndim; ++d)
+        if (im->sizes[d] < 1) return set_error(DIPAMD_E_PARAMETER, "Sizes must be positive");
+    return DIPAMD_OK;
+}
+
+View to_view(const dipamd_image *im) {
+    View v;
+    v.p = im->origin;
+    v.dt = im->dtype;
+    for (int d = 0; d < DIPAMD_MAX_DIMS; ++d) {
+        v.sizes[d] = d < im->ndim ? im->sizes[d] : 1;
+        v.strides[d] = d < im->ndim ? im->strides[d] : 0;
+    }
+    return v;
+}
+
+int broadcast_bc(const int32_t *bc, int nbc, int ndim, int *out) {
+    if (nbc != 0 && nbc != 1 && nbc != ndim) return set_error(DIPAMD_E_ARRAY_LENGTH, "Array parameter has the wrong number of elements");
+    for (int d = 0; d < ndim; ++d) {
+        const int b = nbc == 0 ? DIPAMD_BC_SYMMETRIC_MIRROR : (nbc == 1 ? bc[0] : bc[d]);
+        if (b < 0 || b > 12) return set_error(DIPAMD_E_PARAMETER, "Boundary condition not recognized");
+        out[d] = b;
+    }
+    return DIPAMD_OK;
+}
+
+}  // namespace dipamd
+
+using namespace dipamd;
+
+extern "C" {
+
+const char *dipamd_last_error(void) { return last_error(); }
+const char *dipamd_version(void) { return "dipamd 0.1 (DIPlib 3.6.1 Gauss FIR path, gfx950)"; }
+
+int dipamd_gauss_radius(double sigma, int32_t order, double truncation, int32_t is_double) {
+    return gauss_radius(sigma, order, truncation, is_double != 0);
+}
+
+int dipamd_make_half_gaussian(double sigma, int32_t order, double truncation, int32_t is_double, double *taps,
+                              int32_t capacity) {
+    if (order < 0 || order > 3) return set_error(DIPAMD_E_ORDER, "Gaussian FIR filter not implemented for order > 3");
+    std::vector<double> f = half_gaussian(sigma, order, truncation, is_double != 0);
+    if (static_cast<int>(f.size()) > capacity) return set_error(DIPAMD_E_PARAMETER, "capacity too small");
+    std::memcpy(taps, f.data(), f.size() * sizeof(double));
+    return static_cast<int>(f.size());
+}
+
+int dipamd_separable_convolution(const dipamd_image *in, const dipamd_image *out, const dipamd_filter *filters,
+                                 int32_t nfilters, const int32_t *bc, int32_t nbc, const uint8_t *process,
+                                 uint32_t flags, void *stream) {
+    int st;
+    if ((st = check_image(in, "in")) != DIPAMD_OK) return st;
+    if ((st = check_image(out, "out")) != DIPAMD_OK) return st;
+    const int nd = in->ndim;
+    if (out->ndim != nd) return set_error(DIPAMD_E_SIZES, "Sizes don't match");
+    for (int d = 0; d < nd; ++d)
+        if (out->sizes[d] != in->sizes[d]) return set_error(DIPAMD_E_SIZES, "Sizes don't match");
+    if (nfilters != 1 && nfilters != nd)
+        return set_error(DIPAMD_E_ARRAY_LENGTH, "Array parameter has the wrong number of elements");
+    int bcs[DIPAMD_MAX_DIMS];
+    if ((st = broadcast_bc(bc, nbc, nd, bcs)) != DIPAMD_OK) return st;
+    const bool use_double = suggest_flex(in->dtype) == DIPAMD_DT_DFLOAT;
+    std::vector<InternFilter> F(static_cast<size_t>(nfilters));
+    for (int i = 0; i < nfilters; ++i) {
+        if (filters[i].ntaps < 0 || (filters[i].ntaps > 0 && !filters[i].taps))
+            return set_error(DIPAMD_E_PARAMETER, "bad filter");
+        if ((st = intern_filter(filters[i].taps, filters[i].ntaps, filters[i].symmetry, filters[i].origin, F[i])) !=
+            DIPAMD_OK)
+            return st;
+    }
+    const InternFilter *per_dim[DIPAMD_MAX_DIMS];
+    bool proc[DIPAMD_MAX_DIMS];
+    for (int d = 0; d < nd; ++d) {
+        per_dim[d] = nfilters == 1 ? &F[0] : &F[static_cast<size_t>(d)];
+        proc[d] = process ? process[d] != 0 : true;
+        if (nfilters == 1) {
+            if (meaningless(F[0], use_double)) proc[d] = false;
+        } else if (in->sizes[d] <= 1 || meaningless(F[static_cast<size_t>(d)], use_double)) {
+            proc[d] = false;
+        }
+        if (in->sizes[d] == 1) proc[d] = false;
+    }
+    return separable_device(to_view(in), to_view(out), nd, per_dim, bcs, proc, flags,
+                            static_cast<hipStream_t>(stream));
+}
+
+int dipamd_gauss_fir(const dipamd_image *in, const dipamd_image *out, const double *sigmas, int32_t nsigmas,
+                     const int32_t *orders, int32_t norders, const int32_t *bc, int32_t nbc, double truncation,
+                     uint32_t flags, void *stream) {
+    int st;
+    if ((st = check_image(in, "in")) != DIPAMD_OK) return st;
+    const int nd = in->ndim;
+    if (nsigmas != 1 && nsigmas != nd)
+        return set_error(DIPAMD_E_ARRAY_LENGTH, "Array parameter has the wrong number of elements");
+    if (norders != 0 && norders != 1 && norders != nd)
+        return set_error(DIPAMD_E_ARRAY_LENGTH, "Array parameter has the wrong number of elements");
+    const bool compute_double = in->dtype == DIPAMD_DT_DFLOAT;
+    std::vector<dipamd_filter> fl(static_cast<size_t>(nd));
+    std::vector<std::vector<double>> taps(static_cast<size_t>(nd));
+    uint8_t proc[DIPAMD_MAX_DIMS];
+    for (int d = 0; d < nd; ++d) {
+        const double s = nsigmas == 1 ? sigmas[0] : sigmas[d];
+        const int o = norders == 0 ? 0 : (norders == 1 ? orders[0] : orders[d]);
+        fl[d] = dipamd_filter{nullptr, 0, -1, DIPAMD_SYM_EVEN, 0};
+        proc[d] = 0;
+        if (s > 0.0 && in->sizes[d] > 1) {
+            if (o < 0 || o > 3) return set_error(DIPAMD_E_ORDER, "Gaussian FIR filter not implemented for order > 3");
+            const int r = gauss_radius(s, o, truncation, compute_double);
+            if (r > DIPAMD_MAX_RADIUS) return set_error(DIPAMD_E_UNSUPPORTED, "Gaussian radius above DIPAMD_MAX_RADIUS");
+            taps[d] = half_gaussian(s, o, truncation, compute_double);
+            fl[d].taps = taps[d].data();
+            fl[d].ntaps = static_cast<int32_t>(taps[d].size());
+            fl[d].symmetry = (o & 1) ? DIPAMD_SYM_ODD : DIPAMD_SYM_EVEN;
+            proc[d] = 1;
+        }
+    }
+    return dipamd_separable_convolution(in, out, fl.data(), nd, bc, nbc, proc, flags, stream);
+}
+
+}  // extern "C"

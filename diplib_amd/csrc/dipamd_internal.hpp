@@ -1,0 +1,59 @@
+// Internal host-side declarations shared by the C ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "dipamd.h"
+
+namespace dipamd {
+
+// A strided device view; dims beyond ndim have size 1.
+struct View {
+    void *p;
+    int dt;
+    int64_t sizes[DIPAMD_MAX_DIMS];
+    int64_t strides[DIPAMD_MAX_DIMS];
+};
+
+// InternOneDimensionalFilter (convolution.cpp:67-162), real part: `data` holds the
+// taps reversed (correlation order), `border` the boundary extension, `shift` where
+// the tap loop starts relative to the output sample.
+struct InternFilter {
+    std::vector<double> data;
+    int size = 0;
+    int origin = 0;
+    int border = 0;
+    int shift = 0;
+    int sym = DIPAMD_SYM_GENERAL;
+};
+
+int set_error(int code, const std::string &msg);
+const char *last_error();
+size_t dt_size(int dt);
+int suggest_flex(int dt);
+int gauss_radius(double sigma, int order, double truncation, bool is_double);
+std::vector<double> half_gaussian_n(double sigma, int order, int n);
+std::vector<double> half_gaussian(double sigma, int order, double truncation, bool is_double);
+int intern_filter(const double *taps, int n, int sym, int origin_in, InternFilter &F);
+bool meaningless(const InternFilter &F, bool use_double);
+bool bc_supported(int bc);
+int check_image(const dipamd_image *im, const char *what);
+View to_view(const dipamd_image *im);
+View contiguous_view(void *p, int dt, int ndim, const int64_t *sizes);
+bool views_overlap(const View &x, const View &y, int ndim);
+int broadcast_bc(const int32_t *bc, int nbc, int ndim, int *out);
+hipError_t scratch_alloc(void **p, size_t bytes, hipStream_t stream);
+void scratch_free(void *p, hipStream_t stream);
+
+int separable_device(const View &in, const View &out, int ndim, const InternFilter *const *filters, const int *bc,
+                     const bool *process, uint32_t flags, hipStream_t stream);
+
+// Tile-shape choice for the fused 2-D kernel (autotune.cpp): a config index, or -1
+// to run one pass per axis instead.
+int choose_fused_config(bool is_double, int rx, int ry, int64_t nx, int64_t ny, uint32_t flags);
+
+}  // namespace dipamd

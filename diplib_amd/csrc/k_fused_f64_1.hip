@@ -1,0 +1,16 @@
+// Fused 2-D kernel instantiations: input double, buffer double, fast=true.
+#include "k_fused_common.hpp"
+
+namespace dipamd {
+
+hipError_t fused_entry_f64_1(int cfg, int sx, int sy, const void *args, int64_t nx, int64_t nc, int64_t no,
+                                   int rx, int ry, hipStream_t st) {
+    const auto &a = *static_cast<const FusedArgs<double> *>(args);
+    switch (cfg) {
+        case 0: return fused_sym<double, double, 64, 4, true>(sx, sy, a, nx, nc, no, rx, ry, st);
+        case 1: return fused_sym<double, double, 128, 8, true>(sx, sy, a, nx, nc, no, rx, ry, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace dipamd

@@ -1,0 +1,159 @@
+// Shared device/host definitions for the separable FIR kernels (gfx950).
+//
+// Sample loads convert to the buffer type exactly as dip::detail::CopyBuffer does
+// (copy_buffer.cpp:44-121: a static_cast for these types; bin -> 0/1); the
+// boundary extension of a line follows copy_buffer.cpp:515-629 in closed form:
+// an out-of-range coordinate j maps to a source index plus a sign, or to a
+// constant.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dipamd.h"
+
+namespace dipamd {
+
+constexpr int kMaxOuter = DIPAMD_MAX_DIMS;  // outer (non-processed) dims of a pass
+constexpr int kLineCap = 256;               // taps per filter in the line / sweep kernels
+constexpr int kFusedCap = 65;               // taps per axis in the fused 2-D kernel (radius <= 64)
+constexpr int kBlock = 256;                 // threads per workgroup (4 wave64)
+
+enum Sym : int { SYM_GENERAL = 0, SYM_EVEN = 1, SYM_ODD = 2, SYM_D_EVEN = 4, SYM_D_ODD = 5 };
+
+struct bin_t {
+    uint8_t v;
+};
+
+template <typename T>
+__device__ __forceinline__ T dev_load(const T *p) {
+    return *p;
+}
+
+template <typename TB, typename TI>
+__device__ __forceinline__ TB to_buf(TI v) {
+    return static_cast<TB>(v);
+}
+template <>
+__device__ __forceinline__ float to_buf<float, bin_t>(bin_t v) {
+    return v.v ? 1.0f : 0.0f;
+}
+template <>
+__device__ __forceinline__ double to_buf<double, bin_t>(bin_t v) {
+    return v.v ? 1.0 : 0.0;
+}
+
+// Boundary map of coordinate j (outside [0, n)) for a line of n samples.
+// Returns the source index; *mul is +1 or -1; *is_const is set for the constant
+// conditions (value in the pass arguments).
+__host__ __device__ __forceinline__ int64_t bc_map(int64_t j, int64_t n, int bc, int *mul, bool *is_const) {
+    *mul = 1;
+    *is_const = false;
+    switch (bc) {
+        case DIPAMD_BC_SYMMETRIC_MIRROR:
+        case DIPAMD_BC_ASYMMETRIC_MIRROR: {
+            if (n == 1) {
+                if (bc == DIPAMD_BC_ASYMMETRIC_MIRROR) *mul = -1;
+                return 0;
+            }
+            int64_t P = 2 * (n - 1);
+            int64_t p = j % P;
+            if (p < 0) p += P;
+            int64_t idx = p < n ? p : P - p;
+            if (bc == DIPAMD_BC_ASYMMETRIC_MIRROR) {
+                int64_t m = j < 0 ? -j : j - (n - 1);
+                int64_t c = (m - 1) / (n - 1) + 1;  // edge reflections crossed
+                *mul = (c & 1) ? -1 : 1;
+            }
+            return idx;
+        }
+        case DIPAMD_BC_PERIODIC:
+        case DIPAMD_BC_ASYMMETRIC_PERIODIC: {
+            int64_t q = j >= 0 ? j / n : -((-j - 1) / n) - 1;
+            if (bc == DIPAMD_BC_ASYMMETRIC_PERIODIC && (q & 1)) *mul = -1;
+            if (n == 1 && bc == DIPAMD_BC_ASYMMETRIC_PERIODIC) *mul = -1;
+            return j - q * n;
+        }
+        case DIPAMD_BC_ZERO_ORDER_EXTRAPOLATE:
+            return j < 0 ? 0 : n - 1;
+        default:
+            *is_const = true;
+            return 0;
+    }
+}
+
+// Padded LDS index: one spare word per 32, so lanes reading with a stride of
+// P (a power of two <= 32) words hit distinct banks.
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 5); }
+__host__ __device__ constexpr int lpad_size(int n) { return n + (n >> 5) + 1; }
+
+// Geometry of the dims that are neither the filter axis nor the lane axis.
+struct Outer {
+    int32_t n;
+    int32_t pad_;
+    int64_t size[kMaxOuter];
+    int64_t sin[kMaxOuter];
+    int64_t sout[kMaxOuter];
+};
+
+__device__ __forceinline__ void outer_offsets(const Outer &o, int64_t idx, int64_t *oin, int64_t *oout) {
+    int64_t a = 0, b = 0;
+    for (int d = 0; d < o.n; ++d) {
+        int64_t c = idx % o.size[d];
+        idx /= o.size[d];
+        a += c * o.sin[d];
+        b += c * o.sout[d];
+    }
+    *oin = a;
+    *oout = b;
+}
+
+// One pass along one axis (line kernel and sweep kernel).
+template <typename TB>
+struct PassArgs {
+    const void *in;
+    void *out;
+    int64_t n;        // samples along the filter axis
+    int64_t sin;      // input stride along the filter axis
+    int64_t sout;     // output stride along the filter axis
+    int64_t e;        // sweep kernel: samples along the lane axis
+    int64_t ein;      // lane-axis strides
+    int64_t eout;
+    int64_t nlines;   // line kernel: number of lines (product of the other sizes)
+    int64_t chunk;    // sweep kernel: outputs along the filter axis per workgroup
+    Outer outer;
+    int32_t border;   // samples of boundary extension on each side
+    int32_t ntaps;    // stored taps (data size)
+    int32_t shift;    // GENERAL: first tap reads x[i - shift]
+    int32_t bc;
+    TB cval;          // constant boundary value
+    TB taps[kLineCap];
+};
+
+// The fused x+y kernel of a 2-D plane (or each plane of a stack of them).
+template <typename TB>
+struct FusedArgs {
+    const void *in;
+    void *out;
+    int64_t nx, ny;   // sizes along x (lane / x-filter axis) and y (sweep axis)
+    int64_t sxin, syin, sxout, syout;
+    int64_t chunk;    // output rows per workgroup
+    Outer outer;
+    int32_t rx, ry;
+    int32_t bcx, bcy;
+    TB cvalx, cvaly;
+    TB tx[kFusedCap];
+    TB ty[kFusedCap];
+};
+
+// Geometry of an element-wise conversion pass.
+struct CastGeo {
+    int32_t ndim;
+    int32_t pad_;
+    int64_t size[DIPAMD_MAX_DIMS];
+    int64_t sin[DIPAMD_MAX_DIMS];
+    int64_t sout[DIPAMD_MAX_DIMS];
+    int64_t total;
+};
+
+}  // namespace dipamd

@@ -1,0 +1,41 @@
+// Host-side launchers of the separable FIR kernels (defined in sep_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "sep_common.hpp"
+
+namespace dipamd {
+
+// Input sample type of a kernel's first pass; other input types are converted to
+// the buffer type by launch_cast_in first.
+enum InType : int { IN_U8 = 0, IN_BUF = 1 };
+
+// Fused 2-D tile configurations: columns per workgroup and outputs per lane.
+struct FusedConfig {
+    int txc;
+    int p;
+};
+constexpr int kNumFusedConfigs = 3;
+constexpr FusedConfig kFusedConfigs[kNumFusedConfigs] = {{64, 4}, {128, 8}, {256, 16}};
+
+// LDS bytes of the line kernel for a given border.
+size_t line_lds_bytes(int border, bool is_double);
+// LDS bytes of the fused kernel for a given config and radii.
+size_t fused_lds_bytes(int cfg, int rx, int ry, bool is_double);
+// Rows per workgroup step of the fused kernel.
+constexpr int kFusedStep = 16;
+
+hipError_t launch_line(InType ti, bool is_double, int sym, bool fast, const void *args, int64_t n, int64_t nlines,
+                       int border, hipStream_t stream);
+hipError_t launch_sweep(InType ti, bool is_double, int sym, bool fast, const void *args, int64_t e, int64_t nchunks,
+                        int64_t nouter, hipStream_t stream);
+hipError_t launch_fused(InType ti, bool is_double, int cfg, int symx, int symy, bool fast, const void *args,
+                        int64_t nx, int64_t nchunks, int64_t nouter, int rx, int ry, hipStream_t stream);
+
+hipError_t launch_cast_in(int dt_in, bool to_double, const void *in, void *out, const void *geo, int64_t total,
+                          hipStream_t stream);
+hipError_t launch_cast_out(bool from_double, int dt_out, const void *in, void *out, const void *geo, int64_t total,
+                           hipStream_t stream);
+
+}  // namespace dipamd

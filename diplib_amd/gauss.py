@@ -1,0 +1,212 @@
+"""PyDIP-style front end of the Gaussian FIR path on PyTorch-ROCm device tensors.
+
+Mirrors PyDIP's `dip.Gauss` / `dip.GaussFIR` / `dip.SeparableConvolution`
+(pydip/src/filtering.cpp:121-136) with DIPlib's conventions:
+
+* axis order: a tensor of shape (..., H, W) is a dip image of sizes (W, H, ...);
+  sigma / order / boundary lists are in dip order (x first), as in PyDIP
+  (pydip/src/image.cpp:142-146 reverses the axes);
+* output type: `SuggestFlex(in)` -- sfloat, or dfloat for 32-bit integer and dfloat
+  input (datatype.cpp:109-124); a caller-supplied `out` keeps its own type and is
+  written with clamp_cast, like a protected dip::Image;
+* method "best" resolves as BestGaussMethod (derivative.cpp:35-58).
+
+Device tensors are processed in place on the current HIP stream; numpy arrays are
+copied to the GPU and back (the copies are not part of the kernel path).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import DipamdFilter, DipamdImage, check, lib
+
+_TORCH_DT = {
+    torch.uint8: _lib.DT_UINT8, torch.int8: _lib.DT_SINT8, torch.int16: _lib.DT_SINT16, torch.int32: _lib.DT_SINT32,
+    torch.float32: _lib.DT_SFLOAT, torch.float64: _lib.DT_DFLOAT, torch.bool: _lib.DT_BIN,
+}
+for _name, _code in (("uint16", _lib.DT_UINT16), ("uint32", _lib.DT_UINT32)):
+    if hasattr(torch, _name):
+        _TORCH_DT[getattr(torch, _name)] = _code
+
+
+def _flex_torch(dt):
+    if dt in (torch.float64, torch.int32) or (hasattr(torch, "uint32") and dt == torch.uint32):
+        return torch.float64
+    return torch.float32
+
+
+def image_struct(t):
+    """DipamdImage describing torch tensor `t` (dip axis order: reversed)."""
+    if t.dtype not in _TORCH_DT:
+        raise TypeError(f"data type {t.dtype} not supported")
+    if t.dim() < 1 or t.dim() > _lib.MAX_DIMS:
+        raise ValueError("dimensionality not supported")
+    im = DipamdImage()
+    im.origin = ctypes.c_void_p(t.data_ptr())
+    im.dtype = _TORCH_DT[t.dtype]
+    im.ndim = t.dim()
+    for d in range(t.dim()):
+        ax = t.dim() - 1 - d
+        im.sizes[d] = t.shape[ax]
+        im.strides[d] = t.stride(ax)
+    return im
+
+
+def _stream_handle(device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _as_list(v):
+    if v is None:
+        return []
+    if isinstance(v, (str, bytes)):
+        return [v]
+    if hasattr(v, "__len__"):
+        return list(v)
+    return [v]
+
+
+def _bc_codes(bc):
+    out = []
+    for b in _as_list(bc):
+        b = b.decode() if isinstance(b, bytes) else b
+        if b not in _lib.BC_CODES:
+            raise _lib.DipError(_lib.E_PARAMETER, "Boundary condition not recognized: " + str(b))
+        out.append(_lib.BC_CODES[b])
+    return out
+
+
+def best_gauss_method(sigmas, orders):
+    """BestGaussMethod (derivative.cpp:35-58)."""
+    if any(o > 3 for o in orders):
+        return "ft"
+    if any(0.0 < s < 0.8 for s in sigmas):
+        return "ft"
+    if any(s > 10 for s in sigmas):
+        return "iir"
+    return "fir"
+
+
+def _to_device(x):
+    """(device tensor, was_numpy)."""
+    if isinstance(x, torch.Tensor):
+        if not x.is_cuda:
+            return x.cuda(), True
+        return x, False
+    a = np.asarray(x)
+    if a.dtype == np.uint16:
+        t = torch.from_numpy(a.astype(np.int32)).cuda()  # widened: same values, same output type (sfloat)
+        return t.to(torch.uint16) if hasattr(torch, "uint16") else t, True
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda(), True
+
+
+def _prepare_out(x, out, default_dtype):
+    if out is None:
+        return torch.empty(x.shape, dtype=default_dtype, device=x.device)
+    if tuple(out.shape) != tuple(x.shape):
+        raise _lib.DipError(_lib.E_SIZES, "Image is protected")  # sizes of a protected out cannot change
+    return out
+
+
+def GaussFIR(in_, sigmas=(1.0,), derivativeOrder=(0,), boundaryCondition=(), truncation=3.0, out=None, fast=False,
+             fusion=None):
+    """dip::GaussFIR (linear.h:300-318).  Returns the output tensor (or numpy array
+    for numpy input).  `fast=True` allows FMA (not bit-identical to the reference)."""
+    x, was_np = _to_device(in_)
+    sig = [float(s) for s in _as_list(sigmas)]
+    ords = [int(o) for o in _as_list(derivativeOrder)]
+    nd = x.dim()
+    if len(sig) not in (1, nd) or len(ords) not in (0, 1, nd):
+        raise _lib.DipError(_lib.E_ARRAY_LENGTH, "Array parameter has the wrong number of elements")
+    bcs = _bc_codes(boundaryCondition)
+    y = _prepare_out(x, out, _flex_torch(x.dtype))
+    L = lib()
+    a_in, a_out = image_struct(x), image_struct(y)
+    s_arr = (ctypes.c_double * max(1, len(sig)))(*sig)
+    o_arr = (ctypes.c_int32 * max(1, len(ords)))(*ords)
+    b_arr = (ctypes.c_int32 * max(1, len(bcs)))(*bcs)
+    flags = _lib.FAST if fast else _lib.EXACT
+    if fusion is False:
+        flags |= _lib.NO_FUSION
+    elif fusion is True:
+        flags |= _lib.FORCE_FUSION
+    with torch.cuda.device(x.device):
+        check(L.dipamd_gauss_fir(ctypes.byref(a_in), ctypes.byref(a_out), s_arr, len(sig), o_arr, len(ords), b_arr,
+                                 len(bcs), float(truncation), flags, _stream_handle(x.device)))
+    if was_np and out is None:
+        return y.cpu().numpy()
+    return y
+
+
+def Gauss(in_, sigmas=(1.0,), derivativeOrder=(0,), method="best", boundaryCondition=(), truncation=3.0, out=None,
+          fast=False):
+    """dip::Gauss (linear.h:452-472; derivative.cpp:62-86).  Only the FIR method is
+    part of this library: "best" resolving to FT or IIR raises NotImplementedError."""
+    m = method[5:] if method[:5] == "gauss" else method
+    sig = [float(s) for s in _as_list(sigmas)]
+    ords = [int(o) for o in _as_list(derivativeOrder)]
+    if m == "best":
+        m = best_gauss_method(sig, ords)
+    if m in ("FIR", "fir"):
+        return GaussFIR(in_, sig, ords, boundaryCondition, truncation, out=out, fast=fast)
+    if m in ("FT", "ft", "IIR", "iir"):
+        raise NotImplementedError(f"dip::Gauss method '{m}' is outside this library (FIR only); pass method='FIR'")
+    raise _lib.DipError(_lib.E_PARAMETER, "Invalid flag: " + method)
+
+
+class OneDimensionalFilter:
+    """dip::OneDimensionalFilter (linear.h:84-90), real filters."""
+
+    def __init__(self, filter, origin=-1, symmetry="general"):
+        self.filter = [float(v) for v in filter]
+        self.origin = int(origin)
+        self.symmetry = symmetry
+
+
+def SeparableConvolution(in_, filter, boundaryCondition=(), process=(), out=None, fast=False):
+    """dip::SeparableConvolution (linear.h:114-130) with real 1-D filters."""
+    x, was_np = _to_device(in_)
+    filters = filter if isinstance(filter, (list, tuple)) else [filter]
+    nd = x.dim()
+    if len(filters) not in (1, nd):
+        raise _lib.DipError(_lib.E_ARRAY_LENGTH, "Array parameter has the wrong number of elements")
+    keep = []
+    farr = (DipamdFilter * len(filters))()
+    for i, f in enumerate(filters):
+        if f.symmetry not in _lib.SYM_CODES:
+            raise _lib.DipError(_lib.E_PARAMETER, "Symmetry string not recognized: " + f.symmetry)
+        taps = (ctypes.c_double * max(1, len(f.filter)))(*f.filter)
+        keep.append(taps)
+        farr[i].taps = ctypes.cast(taps, ctypes.POINTER(ctypes.c_double))
+        farr[i].ntaps = len(f.filter)
+        farr[i].origin = f.origin
+        farr[i].symmetry = _lib.SYM_CODES[f.symmetry]
+    proc = list(process) if process is not None else []
+    if len(proc) not in (0, nd):
+        raise _lib.DipError(_lib.E_ARRAY_LENGTH, "Array parameter has the wrong number of elements")
+    p_arr = (ctypes.c_uint8 * nd)(*[1 if p else 0 for p in proc]) if proc else None
+    bcs = _bc_codes(boundaryCondition)
+    b_arr = (ctypes.c_int32 * max(1, len(bcs)))(*bcs)
+    y = _prepare_out(x, out, _flex_torch(x.dtype))
+    a_in, a_out = image_struct(x), image_struct(y)
+    with torch.cuda.device(x.device):
+        check(lib().dipamd_separable_convolution(ctypes.byref(a_in), ctypes.byref(a_out), farr, len(filters), b_arr,
+                                                 len(bcs), p_arr, _lib.FAST if fast else _lib.EXACT,
+                                                 _stream_handle(x.device)))
+    if was_np and out is None:
+        return y.cpu().numpy()
+    return y
+
+
+def MakeHalfGaussian(sigma, derivativeOrder=0, truncation=3.0, dataType="SFLOAT"):
+    """dip::MakeHalfGaussian (generation.h:638-653): r+1 taps, x = 0 last."""
+    buf = (ctypes.c_double * 4096)()
+    n = check(lib().dipamd_make_half_gaussian(float(sigma), int(derivativeOrder), float(truncation),
+                                              1 if dataType.upper() == "DFLOAT" else 0, buf, 4096))
+    return [buf[i] for i in range(n)]
+
+
+def gauss_radius(sigma, order=0, truncation=3.0, is_double=False):
+    return lib().dipamd_gauss_radius(float(sigma), int(order), float(truncation), int(is_double))

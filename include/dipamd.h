@@ -1,0 +1,178 @@
+/*
+ * dipamd.h -- C ABI of the MI355X-native Gaussian FIR path (DIPlib-compatible).
+ *
+ * Every entry point works on DEVICE-resident images (HIP device pointers), is
+ * stream-ordered (asynchronous on `stream`, a hipStream_t or NULL for the null
+ * stream), returns 0 on success or a negative DIPAMD_E_* code, and leaves a
+ * thread-local message for dipamd_last_error().  Plain pointers and sizes only.
+ *
+ * Axis convention is DIPlib's: dim 0 is the fastest-varying axis (x), strides
+ * are in samples, sigmas / orders / boundary conditions are per dim in that order
+ * (or a single value broadcast to every dim, as dip::ArrayUseParameter does).
+ */
+#ifndef DIPAMD_H
+#define DIPAMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DIPAMD_MAX_DIMS 8
+/* Largest 1-D filter half size (radius) the kernels take; sigma <= ~84 at the
+ * default truncation of 3. */
+#define DIPAMD_MAX_RADIUS 255
+
+/* Sample types (dip::DataType, datatype.h), real types only. */
+enum {
+    DIPAMD_DT_UINT8 = 0,
+    DIPAMD_DT_UINT16 = 1,
+    DIPAMD_DT_UINT32 = 2,
+    DIPAMD_DT_SINT8 = 3,
+    DIPAMD_DT_SINT16 = 4,
+    DIPAMD_DT_SINT32 = 5,
+    DIPAMD_DT_SFLOAT = 6,
+    DIPAMD_DT_DFLOAT = 7,
+    DIPAMD_DT_BIN = 8
+};
+
+/* Boundary conditions (dip::BoundaryCondition, boundary.h:65-80, same order). */
+enum {
+    DIPAMD_BC_SYMMETRIC_MIRROR = 0,   /* "mirror", "default", "" */
+    DIPAMD_BC_ASYMMETRIC_MIRROR = 1,  /* "asym mirror" */
+    DIPAMD_BC_ANTISYMMETRIC_REFLECT = 2, /* not supported: returns DIPAMD_E_UNSUPPORTED */
+    DIPAMD_BC_PERIODIC = 3,           /* "periodic" */
+    DIPAMD_BC_ASYMMETRIC_PERIODIC = 4,/* "asym periodic" */
+    DIPAMD_BC_ADD_ZEROS = 5,          /* "add zeros" */
+    DIPAMD_BC_ADD_MAX_VALUE = 6,      /* "add max" */
+    DIPAMD_BC_ADD_MIN_VALUE = 7,      /* "add min" */
+    DIPAMD_BC_ZERO_ORDER_EXTRAPOLATE = 8 /* "zero order"; 9-12 not supported */
+};
+
+/* Filter symmetry (convolution.cpp:40-48 FilterSymmetry, real filters). */
+enum {
+    DIPAMD_SYM_GENERAL = 0,
+    DIPAMD_SYM_EVEN = 1,
+    DIPAMD_SYM_ODD = 2,
+    DIPAMD_SYM_CONJ = 3,   /* real filter: same as EVEN */
+    DIPAMD_SYM_D_EVEN = 4,
+    DIPAMD_SYM_D_ODD = 5,
+    DIPAMD_SYM_D_CONJ = 6  /* real filter: same as D_EVEN */
+};
+
+/* Flags. */
+enum {
+    DIPAMD_EXACT = 0,          /* default: the reference's arithmetic, bit-identical output */
+    DIPAMD_FAST = 1,           /* fused multiply-add in the tap loop (not bit-identical) */
+    DIPAMD_NO_FUSION = 2,      /* one pass per axis (no fused 2-D kernel) */
+    DIPAMD_FORCE_FUSION = 4    /* fused 2-D kernel wherever it applies (ignores the tuner) */
+};
+
+/* Error codes. */
+enum {
+    DIPAMD_OK = 0,
+    DIPAMD_E_PARAMETER = -1,     /* dip::ParameterError */
+    DIPAMD_E_NOT_FORGED = -2,    /* E::IMAGE_NOT_FORGED */
+    DIPAMD_E_ARRAY_LENGTH = -3,  /* E::ARRAY_PARAMETER_WRONG_LENGTH */
+    DIPAMD_E_SIZES = -4,         /* output sizes differ from the input's */
+    DIPAMD_E_UNSUPPORTED = -5,   /* boundary condition / type / radius not supported */
+    DIPAMD_E_HIP = -6,           /* a HIP runtime call failed */
+    DIPAMD_E_ORDER = -7          /* "Gaussian FIR filter not implemented for order > 3" */
+};
+
+/* A strided image view (the dip::Image subset the path uses: image.h:458-491). */
+typedef struct dipamd_image {
+    void *origin;                       /* device pointer to the first pixel */
+    int32_t dtype;                      /* DIPAMD_DT_* */
+    int32_t ndim;                       /* 1..DIPAMD_MAX_DIMS */
+    int64_t sizes[DIPAMD_MAX_DIMS];     /* dim 0 = x */
+    int64_t strides[DIPAMD_MAX_DIMS];   /* in samples, may be negative */
+} dipamd_image;
+
+/* A real 1-D filter (dip::OneDimensionalFilter, linear.h:84-90). `taps` is HOST
+ * memory, in the user's (convolution) order; origin -1 = centre. */
+typedef struct dipamd_filter {
+    const double *taps;
+    int32_t ntaps;
+    int32_t origin;
+    int32_t symmetry;                   /* DIPAMD_SYM_* */
+    int32_t reserved;
+} dipamd_filter;
+
+/* Replaces dip::GaussFIR (include/diplib/linear.h:300-318; src/linear/gauss.cpp:238-286)
+ * and, with method "FIR", dip::Gauss (linear.h:452-460; derivative.cpp:62-86).
+ * out must have the sizes of in; its dtype is the caller's choice (dip::Gauss picks
+ * SuggestFlex(in): sfloat, or dfloat for 32-bit integer and dfloat input); a
+ * different dtype is written with clamp_cast as for a protected dip::Image.
+ * out may alias in (same view).  nsigmas, norders, nbc: 1 or in->ndim (nbc 0 = mirror,
+ * norders 0 = all zero). */
+int dipamd_gauss_fir(const dipamd_image *in, const dipamd_image *out, const double *sigmas, int32_t nsigmas,
+                     const int32_t *orders, int32_t norders, const int32_t *bc, int32_t nbc, double truncation,
+                     uint32_t flags, void *stream);
+
+/* Replaces dip::SeparableConvolution (include/diplib/linear.h:114-130;
+ * src/linear/convolution.cpp:319-421) for real filters.  nfilters: 1 or ndim.
+ * process: NULL (all) or ndim bytes. */
+int dipamd_separable_convolution(const dipamd_image *in, const dipamd_image *out, const dipamd_filter *filters,
+                                 int32_t nfilters, const int32_t *bc, int32_t nbc, const uint8_t *process,
+                                 uint32_t flags, void *stream);
+
+/* Replaces dip::MakeHalfGaussian (include/diplib/generation.h:638-653;
+ * src/linear/gauss.cpp:151-165): writes r+1 host doubles, x = 0 LAST, returns the
+ * count (or a negative error code). */
+int dipamd_make_half_gaussian(double sigma, int32_t order, double truncation, int32_t is_double, double *taps,
+                              int32_t capacity);
+
+/* ---- batched form (no reference counterpart: a per-image loop of dip::Gauss) ----
+ * Many 2-D images of uneven sizes and per-image sigmas in ONE persistent kernel that
+ * pulls tiles from a work queue.  A plan fixes the device pointers, sizes, sigmas and
+ * tile shape; running it is graph-capturable (no allocation, no host sync). */
+typedef struct dipamd_batch_image {
+    const void *in;          /* device pointer, dtype in_dtype */
+    void *out;               /* device pointer, dtype out_dtype (sfloat or dfloat) */
+    int32_t width, height;   /* dip sizes (x, y) */
+    int64_t in_stride_y;     /* samples between rows (x stride is 1) */
+    int64_t out_stride_y;
+    double sigma_x, sigma_y;
+    int32_t order_x, order_y;
+} dipamd_batch_image;
+
+typedef struct dipamd_batch_plan dipamd_batch_plan;
+
+/* tile_config: -1 = autotuned choice, else an index < dipamd_batch_num_configs(). */
+int dipamd_batch_create(dipamd_batch_plan **plan, const dipamd_batch_image *images, int32_t n, int32_t in_dtype,
+                        int32_t out_dtype, const int32_t *bc, int32_t nbc, double truncation, uint32_t flags,
+                        int32_t tile_config, void *stream);
+int dipamd_batch_run(dipamd_batch_plan *plan, void *stream);
+/* Captures dipamd_batch_run into a hipGraph owned by the plan (once), then launches it. */
+int dipamd_batch_run_graph(dipamd_batch_plan *plan, void *stream);
+int dipamd_batch_destroy(dipamd_batch_plan *plan);
+int dipamd_batch_num_configs(void);
+/* Number of work-queue tiles and the config index of a plan. */
+int dipamd_batch_info(const dipamd_batch_plan *plan, int64_t *ntiles, int32_t *config);
+
+/* ---- backward (no reference counterpart: the adjoint of dip::Gauss FIR) ----
+ * grad_in = J^T grad_out, the exact adjoint of the boundary-extended separable
+ * convolution (flipped taps, halo contributions folded back by the boundary
+ * condition); grad_sigma[d] (DEVICE doubles, ndim of them, overwritten) =
+ * <grad_out, d out / d sigma_d>, differentiating the normalised discrete taps at a
+ * fixed radius.  x and grad_out are sfloat or dfloat, grad_in has the dtype of
+ * grad_out; either output may be NULL (origin NULL) to skip it. */
+int dipamd_gauss_fir_backward(const dipamd_image *x, const dipamd_image *grad_out, const dipamd_image *grad_in,
+                              double *grad_sigma, const double *sigmas, int32_t nsigmas, const int32_t *orders,
+                              int32_t norders, const int32_t *bc, int32_t nbc, double truncation, uint32_t flags,
+                              void *stream);
+
+/* Radius of the tap array used for (sigma, order) as GaussFIR computes it (gauss.cpp:38-51). */
+int dipamd_gauss_radius(double sigma, int32_t order, double truncation, int32_t is_double);
+
+/* Thread-local message of the last failed call on this thread. */
+const char *dipamd_last_error(void);
+/* Library version string. */
+const char *dipamd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DIPAMD_H */

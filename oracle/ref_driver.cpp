@@ -1,0 +1,146 @@
+// C entry points around the reference library's own dip::Gauss, for the test
+// suite and the CPU baseline only.  Compiled against the reference sources by
+// oracle/ref.mk into oracle/_ref/libdipref.so.  The product never links this.
+//
+// Data-type codes are the ones of include/dipamd.h (DIPAMD_DT_*).
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "diplib.h"
+#include "diplib/linear.h"
+#include "diplib/generation.h"
+#include "diplib/multithreading.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+dip::DataType ToDip( int code ) {
+   switch( code ) {
+      case 0: return dip::DT_UINT8;
+      case 1: return dip::DT_UINT16;
+      case 2: return dip::DT_UINT32;
+      case 3: return dip::DT_SINT8;
+      case 4: return dip::DT_SINT16;
+      case 5: return dip::DT_SINT32;
+      case 6: return dip::DT_SFLOAT;
+      case 7: return dip::DT_DFLOAT;
+      case 8: return dip::DT_BIN;
+      default: throw std::runtime_error( "unknown dtype code" );
+   }
+}
+
+dip::Image Wrap( void const* ptr, int dtype, int ndim, int64_t const* sizes ) {
+   dip::UnsignedArray sz( static_cast< dip::uint >( ndim ));
+   for( int ii = 0; ii < ndim; ++ii ) {
+      sz[ static_cast< dip::uint >( ii ) ] = static_cast< dip::uint >( sizes[ ii ] );
+   }
+   void* p = const_cast< void* >( ptr );
+   return dip::Image( dip::NonOwnedRefToDataSegment( p ), p, ToDip( dtype ), sz );
+}
+
+} // namespace
+
+extern "C" {
+
+const char* dipref_last_error() { return g_err.c_str(); }
+
+// dip::Gauss( in, out, sigmas, orders, method, bc, truncation ) on host buffers with
+// normal strides (dim 0 contiguous).  `out` is wrapped as a protected image of
+// `out_dtype`, so the reference writes into it in place.  `bcs` holds `nbc` C strings
+// (0, 1 or ndim of them).  nthreads == 0 leaves the reference's default.
+int dipref_gauss( void const* in, int in_dtype, void* out, int out_dtype, int ndim, int64_t const* sizes,
+                  double const* sigmas, int nsigmas, int const* orders, int norders,
+                  char const* const* bcs, int nbc, double truncation, char const* method, int nthreads ) {
+   try {
+      dip::Image imgIn = Wrap( in, in_dtype, ndim, sizes );
+      dip::Image imgOut = Wrap( out, out_dtype, ndim, sizes );
+      imgOut.Protect();
+      dip::FloatArray s( static_cast< dip::uint >( nsigmas ));
+      for( int ii = 0; ii < nsigmas; ++ii ) { s[ static_cast< dip::uint >( ii ) ] = sigmas[ ii ]; }
+      dip::UnsignedArray o( static_cast< dip::uint >( norders ));
+      for( int ii = 0; ii < norders; ++ii ) { o[ static_cast< dip::uint >( ii ) ] = static_cast< dip::uint >( orders[ ii ] ); }
+      dip::StringArray bc;
+      for( int ii = 0; ii < nbc; ++ii ) { bc.emplace_back( bcs[ ii ] ); }
+      if( nthreads > 0 ) {
+         dip::SetNumberOfThreads( static_cast< dip::uint >( nthreads ));
+      }
+      dip::Gauss( imgIn, imgOut, s, o, method, bc, truncation );
+      return 0;
+   } catch( std::exception const& e ) {
+      g_err = e.what();
+      return 1;
+   }
+}
+
+// In-place form: `buf` is both the input and the output (same dtype, which must be
+// sfloat or dfloat).
+int dipref_gauss_inplace( void* buf, int dtype, int ndim, int64_t const* sizes, double const* sigmas, int nsigmas,
+                          int const* orders, int norders, char const* bc, double truncation ) {
+   try {
+      dip::Image img = Wrap( buf, dtype, ndim, sizes );
+      img.Protect();
+      dip::FloatArray s( static_cast< dip::uint >( nsigmas ));
+      for( int ii = 0; ii < nsigmas; ++ii ) { s[ static_cast< dip::uint >( ii ) ] = sigmas[ ii ]; }
+      dip::UnsignedArray o( static_cast< dip::uint >( norders ));
+      for( int ii = 0; ii < norders; ++ii ) { o[ static_cast< dip::uint >( ii ) ] = static_cast< dip::uint >( orders[ ii ] ); }
+      dip::Gauss( img, img, s, o, "FIR", { bc }, truncation );
+      return 0;
+   } catch( std::exception const& e ) {
+      g_err = e.what();
+      return 1;
+   }
+}
+
+// dip::MakeHalfGaussian: writes up to `cap` taps (x = 0 last) and returns the count,
+// or -1 on error.  is_double selects the dfloat truncation limit.
+int dipref_half_gaussian( double sigma, int order, double truncation, int is_double, double* taps, int cap ) {
+   try {
+      std::vector< dip::dfloat > f = dip::MakeHalfGaussian( sigma, static_cast< dip::uint >( order ), truncation,
+                                                            is_double ? dip::DT_DFLOAT : dip::DT_SFLOAT );
+      int n = static_cast< int >( f.size() );
+      if( n > cap ) { g_err = "capacity"; return -1; }
+      std::memcpy( taps, f.data(), f.size() * sizeof( double ));
+      return n;
+   } catch( std::exception const& e ) {
+      g_err = e.what();
+      return -1;
+   }
+}
+
+// dip::SeparableConvolution with one 1-D filter per dimension (or one for all), given as
+// full tap arrays with a symmetry string each ("general", "even", "odd", "d-even", "d-odd")
+// and origin -1 (centre).
+int dipref_separable( void const* in, int in_dtype, void* out, int out_dtype, int ndim, int64_t const* sizes,
+                      double const* const* taps, int const* ntaps, char const* const* symmetry, int nfilters,
+                      char const* const* bcs, int nbc, unsigned char const* process ) {
+   try {
+      dip::Image imgIn = Wrap( in, in_dtype, ndim, sizes );
+      dip::Image imgOut = Wrap( out, out_dtype, ndim, sizes );
+      imgOut.Protect();
+      dip::OneDimensionalFilterArray fa( static_cast< dip::uint >( nfilters ));
+      for( int ii = 0; ii < nfilters; ++ii ) {
+         auto& f = fa[ static_cast< dip::uint >( ii ) ];
+         f.filter.assign( taps[ ii ], taps[ ii ] + ntaps[ ii ] );
+         f.symmetry = symmetry[ ii ];
+      }
+      dip::StringArray bc;
+      for( int ii = 0; ii < nbc; ++ii ) { bc.emplace_back( bcs[ ii ] ); }
+      dip::BooleanArray proc;
+      if( process ) {
+         proc.resize( static_cast< dip::uint >( ndim ));
+         for( int ii = 0; ii < ndim; ++ii ) { proc[ static_cast< dip::uint >( ii ) ] = process[ ii ] != 0; }
+      }
+      dip::SeparableConvolution( imgIn, imgOut, fa, bc, proc );
+      return 0;
+   } catch( std::exception const& e ) {
+      g_err = e.what();
+      return 1;
+   }
+}
+
+int dipref_max_threads() { return static_cast< int >( dip::GetNumberOfThreads() ); }
+
+} // extern "C"

@@ -1,0 +1,93 @@
+"""Deterministic synthetic inputs and the parity case lists shared by the golden
+fixture generator (tests/golden/make_golden.py), the tests and the benchmark.
+
+Images are numpy arrays in numpy axis order (..., y, x); sigma / order / boundary
+lists are in dip axis order (x first), as dip::Gauss takes them.
+"""
+import numpy as np
+
+
+def make_input(shape, dtype, seed, impulses=0.0):
+    """Seeded input: uniform [0,1) for float types, uniform integers over the type's
+    range (0..255 for uint8) for integer types, optional sparse impulses of amplitude 10
+    (fraction `impulses` of the pixels, seeded positions)."""
+    rng = np.random.default_rng(seed)
+    dt = np.dtype(dtype)
+    if dt.kind == "f":
+        x = rng.random(shape, dtype=np.float64).astype(dt) if dt == np.float64 else rng.random(shape, dtype=np.float32)
+    elif dt == np.bool_:
+        x = rng.random(shape) < 0.5
+    else:
+        info = np.iinfo(dt)
+        lo, hi = (0, 256) if dt == np.uint8 else (max(info.min, -1000), min(info.max, 1000) + 1)
+        x = rng.integers(lo, hi, size=shape, dtype=np.int64).astype(dt)
+    if impulses > 0:
+        n = x.size
+        k = int(round(impulses * n))
+        idx = rng.choice(n, size=k, replace=False)
+        flat = x.reshape(-1)
+        flat[idx] = flat[idx] + dt.type(10)
+    return x
+
+
+# Small full-output cases (pinned by fixtures written by the reference itself).
+# (name, numpy shape, dtype, sigmas(dip order), orders, boundary list, out dtype or None)
+SMALL_CASES = [
+    ("f32_64x48_s2_o0_mirror", (48, 64), "float32", [2.0], [0], ["mirror"], None),
+    ("f32_512x512_s2_o0_mirror", (512, 512), "float32", [2.0], [0], ["mirror"], None),
+    ("u8_64x48_s1_o10_mirror", (48, 64), "uint8", [1.0], [1, 0], ["mirror"], None),
+    ("u8_64x48_s1_o02_mirror", (48, 64), "uint8", [1.0], [0, 2], ["mirror"], None),
+    ("u8_97x61_s4_o00_periodic", (61, 97), "uint8", [4.0], [0, 0], ["periodic"], None),
+    ("f32_37x29x11_s221_mirror", (11, 29, 37), "float32", [2.0, 2.0, 1.0], [0], ["mirror"], None),
+    ("f32_37x29x11_s221_periodic", (11, 29, 37), "float32", [2.0, 2.0, 1.0], [0], ["periodic"], None),
+    ("f64_37x29x11_s221_mirror", (11, 29, 37), "float64", [2.0, 2.0, 1.0], [0], ["mirror"], None),
+    ("f64_33x20_s3_o12_periodic", (20, 33), "float64", [3.0], [1, 2], ["periodic"], None),
+    ("f32_130x70_s16_o00_mirror", (70, 130), "float32", [16.0], [0, 0], ["mirror"], None),
+    ("f32_70x130_s8_o21_mirror", (130, 70), "float32", [8.0], [2, 1], ["mirror"], None),
+    ("f32_41x23_s3_o3_mirror", (23, 41), "float32", [3.0], [3, 0], ["mirror"], None),
+    ("f32_50x40_s05_mirror", (40, 50), "float32", [0.5], [0], ["mirror"], None),
+    ("f32_50x40_aniso_mixedbc", (40, 50), "float32", [1.5, 2.5], [0, 1], ["periodic", "mirror"], None),
+    ("f32_50x40_sigma0_x", (40, 50), "float32", [0.0, 2.0], [0], ["mirror"], None),
+    ("f32_64x48_asym_mirror", (48, 64), "float32", [2.0], [1], ["asym mirror"], None),
+    ("f32_64x48_asym_periodic", (48, 64), "float32", [2.0], [0, 1], ["asym periodic"], None),
+    ("f32_64x48_add_zeros", (48, 64), "float32", [2.0], [0], ["add zeros"], None),
+    ("f32_64x48_zero_order", (48, 64), "float32", [3.0], [0], ["zero order"], None),
+    ("f32_64x48_add_max", (48, 64), "float32", [1.0], [0], ["add max"], None),
+    ("u16_45x31_s2", (31, 45), "uint16", [2.0], [0], ["mirror"], None),
+    ("s16_45x31_s2_o1", (31, 45), "int16", [2.0], [1], ["mirror"], None),
+    ("s8_45x31_s2", (31, 45), "int8", [2.0], [0], ["periodic"], None),
+    ("s32_45x31_s2", (31, 45), "int32", [2.0], [0], ["mirror"], None),
+    ("u32_45x31_s2", (31, 45), "uint32", [2.0], [1], ["mirror"], None),
+    ("bin_45x31_s2", (31, 45), "bool", [2.0], [0], ["mirror"], None),
+    ("f32_1d_300_s5", (300,), "float32", [5.0], [0], ["mirror"], None),
+    ("f32_4d_9x8x7x6", (6, 7, 8, 9), "float32", [1.0, 1.5, 1.0, 2.0], [0], ["mirror"], None),
+    ("f32_out_u8_64x48", (48, 64), "float32", [2.0], [0], ["mirror"], "uint8"),
+    ("u8_out_u16_64x48", (48, 64), "uint8", [2.0], [0], ["mirror"], "uint16"),
+    ("f32_out_f64_64x48", (48, 64), "float32", [2.0], [1], ["mirror"], "float64"),
+    ("f64_out_f32_64x48", (48, 64), "float64", [2.0], [0], ["periodic"], "float32"),
+    ("f32_out_s16_64x48", (48, 64), "float32", [1.0], [1], ["mirror"], "int16"),
+]
+# Bounce cases: lines shorter than the kernel radius (mirror bounces, periodic wraps
+# several times).  Widths 2..7 along x, heights 2..5 along y.
+for w in range(2, 8):
+    for h in (2, 3, 5):
+        for bc in ("mirror", "periodic", "asym mirror"):
+            SMALL_CASES.append((f"bounce_{w}x{h}_{bc.replace(' ', '_')}", (h, w), "float32", [3.0, 2.0], [0, 1],
+                                [bc], None))
+# width-1 lines (dim skipped) and single pixels
+SMALL_CASES.append(("f32_1x40_s2", (40, 1), "float32", [2.0], [0], ["mirror"], None))
+SMALL_CASES.append(("f32_1x1_s2", (1, 1), "float32", [2.0], [0], ["mirror"], None))
+
+TAP_SIGMAS = [0.5, 0.8, 1.0, 1.5, 2.0, 3.0, 4.0, 5.0, 8.0, 16.0, 31.7]
+
+# Full-size workloads: (name, numpy shape, dtype, seed, impulses, sigmas, orders, bc, out dtype)
+LARGE_CASES = []
+for s in (1.0, 4.0, 16.0):
+    for o in ((0, 0), (1, 0), (0, 2)):
+        LARGE_CASES.append((f"u8_8192sq_s{int(s)}_o{o[0]}{o[1]}", (8192, 8192), "uint8", 2, 0.0, [s], list(o),
+                            ["mirror"], None))
+for bc in ("mirror", "periodic"):
+    LARGE_CASES.append((f"f32_512x512x256_s221_{bc}", (256, 512, 512), "float32", 3, 0.01, [2.0, 2.0, 1.0], [0],
+                        [bc], None))
+LARGE_CASES.append(("f64_256x256x128_s221_mirror", (128, 256, 256), "float64", 4, 0.01, [2.0, 2.0, 1.0], [0],
+                    ["mirror"], None))

@@ -1,0 +1,96 @@
+"""Writes the golden fixtures under tests/golden/ from the REFERENCE library itself
+(oracle/_ref/libdipref.so, built from the reference sources by oracle/ref.mk).
+
+    make -f oracle/ref.mk -j8 && python tests/golden/make_golden.py
+
+Outputs (all small):
+  taps.npz           MakeHalfGaussian(sigma, order, 3, sfloat/dfloat) for TAP_SIGMAS x orders 0..3
+  small_cases.npz    input and dip::Gauss output of every case in tests/cases.py:SMALL_CASES
+  large_cases.json   SHA-256 of the exact output bytes + sampled pixels for LARGE_CASES
+                     (inputs regenerated from their seeds by tests/cases.py:make_input)
+  jacobian.npz       J^T w for dip::Gauss on small images, J assembled column by column
+                     from the reference's impulse responses (dfloat), w ~ N(0,1) seeded
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+import cases  # noqa: E402
+import oracle_ctypes as oc  # noqa: E402
+
+
+def sample_idx(n, k=64, seed=7):
+    rng = np.random.default_rng(seed)
+    return np.unique(np.concatenate([np.arange(min(n, 8)), n - 1 - np.arange(min(n, 8)), rng.integers(0, n, k)]))
+
+
+def main():
+    assert oc.ref_available(), "build oracle/_ref/libdipref.so first (make -f oracle/ref.mk)"
+    # taps
+    taps = {}
+    for s in cases.TAP_SIGMAS:
+        for o in range(4):
+            for dd in (0, 1):
+                taps[f"s{s}_o{o}_d{dd}"] = oc.half_gaussian(s, o, 3.0, bool(dd), use_ref=True)
+    np.savez_compressed(os.path.join(HERE, "taps.npz"), **taps)
+
+    # small full-output cases
+    arrs = {}
+    for i, (name, shape, dt, sig, ords, bc, odt) in enumerate(cases.SMALL_CASES):
+        x = cases.make_input(shape, dt, seed=1000 + i)
+        y = oc.gauss_fir(x, sig, ords, bc, 3.0, out_dtype=odt, nthreads=1, use_ref=True)
+        arrs[name + "__in"] = x
+        arrs[name + "__out"] = y
+    np.savez_compressed(os.path.join(HERE, "small_cases.npz"), **arrs)
+
+    # large cases: hashes of the exact output bytes + sampled pixels
+    large = {}
+    for (name, shape, dt, seed, imp, sig, ords, bc, odt) in cases.LARGE_CASES:
+        x = cases.make_input(shape, dt, seed, imp)
+        y = oc.gauss_fir(x, sig, ords, bc, 3.0, out_dtype=odt, nthreads=8, use_ref=True)
+        flat = y.reshape(-1)
+        idx = sample_idx(flat.size)
+        large[name] = {
+            "input_sha256": hashlib.sha256(x.tobytes()).hexdigest(),
+            "output_sha256": hashlib.sha256(y.tobytes()).hexdigest(),
+            "output_dtype": str(y.dtype),
+            "sample_index": idx.tolist(),
+            "sample_value": [float(v) for v in flat[idx]],
+            "sum": float(flat.astype(np.float64).sum()),
+        }
+        print(name, large[name]["output_sha256"][:16], flush=True)
+    with open(os.path.join(HERE, "large_cases.json"), "w") as f:
+        json.dump(large, f, indent=1)
+
+    # Jacobian-transpose products from impulse responses
+    jac = {}
+    jcases = [("o0_mirror_64", (64, 64), [3.0], [0], "mirror"),
+              ("o1_mirror_64", (64, 64), [3.0], [1, 0], "mirror"),
+              ("o0_periodic_64", (64, 64), [3.0], [0], "periodic"),
+              ("o1_periodic_64", (64, 64), [3.0], [0, 1], "periodic"),
+              ("o0_mirror_3d12", (12, 12, 12), [1.5], [0], "mirror")]
+    for k, (name, shape, sig, ords, bc) in enumerate(jcases):
+        w = np.random.default_rng(500 + k).standard_normal(shape)
+        n = int(np.prod(shape))
+        jtw = np.zeros(n)
+        e = np.zeros(shape)
+        ef = e.reshape(-1)
+        for j in range(n):
+            ef[j] = 1.0
+            resp = oc.gauss_fir(e, sig, ords, [bc], 3.0, use_ref=True)
+            jtw[j] = float((resp * w).sum())
+            ef[j] = 0.0
+        jac[name + "__w"] = w
+        jac[name + "__jtw"] = jtw.reshape(shape)
+        jac[name + "__meta"] = np.array(json.dumps({"shape": shape, "sigmas": sig, "orders": ords, "bc": bc}))
+        print("jacobian", name, flush=True)
+    np.savez_compressed(os.path.join(HERE, "jacobian.npz"), **jac)
+
+
+if __name__ == "__main__":
+    main()

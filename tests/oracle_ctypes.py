@@ -1,0 +1,161 @@
+"""ctypes bindings for the CPU oracle (oracle/liborc.so) and, where it was built,
+the reference library wrapper (oracle/_ref/libdipref.so).  Test infrastructure:
+imported by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg only.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORC_PATH = os.path.join(ROOT, "oracle", "liborc.so")
+REF_PATH = os.path.join(ROOT, "oracle", "_ref", "libdipref.so")
+
+# data-type codes shared by the oracle, the reference wrapper and include/dipamd.h
+DT = {np.dtype(np.uint8): 0, np.dtype(np.uint16): 1, np.dtype(np.uint32): 2,
+      np.dtype(np.int8): 3, np.dtype(np.int16): 4, np.dtype(np.int32): 5,
+      np.dtype(np.float32): 6, np.dtype(np.float64): 7, np.dtype(np.bool_): 8}
+BC = {"": 0, "default": 0, "mirror": 0, "asym mirror": 1, "antisym reflect": 2, "periodic": 3,
+      "asym periodic": 4, "add zeros": 5, "add max": 6, "add min": 7, "zero order": 8}
+SYM = {"general": 0, "": 0, "even": 1, "odd": 2, "conj": 3, "d-even": 4, "d-odd": 5, "d-conj": 6}
+
+_orc = None
+_ref = None
+
+
+def orc():
+    global _orc
+    if _orc is None:
+        lib = ctypes.CDLL(ORC_PATH)
+        P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        lib.orc_gauss_radius.argtypes = [D, I, D, I]
+        lib.orc_half_gaussian.argtypes = [D, I, D, I, P, I]
+        lib.orc_half_gaussian_n.argtypes = [D, I, I, P]
+        lib.orc_gauss_fir.argtypes = [P, I, P, I, I, P, P, I, P, I, P, I, D, I]
+        lib.orc_separable.argtypes = [P, I, P, I, I, P, P, P, P, P, I, P, I, P, I]
+        lib.orc_bc_map.argtypes = [ctypes.c_long, ctypes.c_long, I, ctypes.POINTER(I)]
+        lib.orc_bc_map.restype = ctypes.c_long
+        _orc = lib
+    return _orc
+
+
+def ref_available():
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        lib = ctypes.CDLL(REF_PATH)
+        P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        lib.dipref_gauss.argtypes = [P, I, P, I, I, P, P, I, P, I, P, I, D, ctypes.c_char_p, I]
+        lib.dipref_gauss_inplace.argtypes = [P, I, I, P, P, I, P, I, ctypes.c_char_p, D]
+        lib.dipref_half_gaussian.argtypes = [D, I, D, I, P, I]
+        lib.dipref_separable.argtypes = [P, I, P, I, I, P, P, P, P, I, P, I, P]
+        lib.dipref_last_error.restype = ctypes.c_char_p
+        _ref = lib
+    return _ref
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _sizes(arr):
+    # numpy (..., H, W) -> dip sizes (W, H, ...): dim 0 is the contiguous axis
+    return np.ascontiguousarray(arr.shape[::-1], dtype=np.int64)
+
+
+def _flex(dt):
+    dt = np.dtype(dt)
+    return np.dtype(np.float64) if dt in (np.dtype(np.uint32), np.dtype(np.int32), np.dtype(np.float64)) \
+        else np.dtype(np.float32)
+
+
+def _as_list(v, n):
+    v = list(v) if hasattr(v, "__len__") else [v]
+    return v
+
+
+def half_gaussian(sigma, order, truncation=3.0, is_double=False, use_ref=False):
+    buf = np.zeros(4096, np.float64)
+    if use_ref:
+        n = ref().dipref_half_gaussian(sigma, order, truncation, int(is_double), _ptr(buf), 4096)
+    else:
+        n = orc().orc_half_gaussian(sigma, order, truncation, int(is_double), _ptr(buf), 4096)
+    if n < 0:
+        raise ValueError("half_gaussian failed")
+    return buf[:n].copy()
+
+
+def gauss_radius(sigma, order, truncation=3.0, is_double=False):
+    return orc().orc_gauss_radius(sigma, order, truncation, int(is_double))
+
+
+def gauss_fir(x, sigmas, orders=(0,), bc=("mirror",), truncation=3.0, out_dtype=None, nthreads=1,
+              use_ref=False):
+    """dip::Gauss(x, sigmas, orders, "FIR", bc, truncation) on a C-contiguous numpy array.
+    sigmas / orders / bc are in dip axis order (dim 0 = last numpy axis)."""
+    x = np.ascontiguousarray(x)
+    out_dtype = np.dtype(out_dtype) if out_dtype is not None else _flex(x.dtype)
+    out = np.zeros(x.shape, out_dtype)
+    sizes = _sizes(x)
+    nd = x.ndim
+    sig = np.ascontiguousarray(list(sigmas) if hasattr(sigmas, "__len__") else [sigmas], np.float64)
+    ords = np.ascontiguousarray(list(orders) if hasattr(orders, "__len__") else [orders], np.int32)
+    bcl = list(bc) if not isinstance(bc, str) else [bc]
+    if use_ref:
+        arr = (ctypes.c_char_p * max(1, len(bcl)))(*[b.encode() for b in bcl])
+        st = ref().dipref_gauss(_ptr(x), DT[x.dtype], _ptr(out), DT[out_dtype], nd, _ptr(sizes),
+                                _ptr(sig), len(sig), _ptr(ords), len(ords), arr, len(bcl), truncation,
+                                b"FIR", nthreads)
+        if st != 0:
+            raise RuntimeError(ref().dipref_last_error().decode())
+    else:
+        bci = np.ascontiguousarray([BC[b] for b in bcl], np.int32)
+        st = orc().orc_gauss_fir(_ptr(x), DT[x.dtype], _ptr(out), DT[out_dtype], nd, _ptr(sizes),
+                                 _ptr(sig), len(sig), _ptr(ords), len(ords), _ptr(bci), len(bcl),
+                                 truncation, nthreads)
+        if st != 0:
+            raise RuntimeError(f"oracle gauss_fir failed ({st})")
+    return out
+
+
+def separable(x, filters, symmetries, bc=("mirror",), process=None, out_dtype=None, origins=None,
+              nthreads=1, use_ref=False):
+    """dip::SeparableConvolution with real 1-D filters given as tap lists (user order)."""
+    x = np.ascontiguousarray(x)
+    out_dtype = np.dtype(out_dtype) if out_dtype is not None else _flex(x.dtype)
+    out = np.zeros(x.shape, out_dtype)
+    sizes = _sizes(x)
+    nd = x.ndim
+    taps = [np.ascontiguousarray(f, np.float64) for f in filters]
+    tptrs = (ctypes.c_void_p * len(taps))(*[t.ctypes.data for t in taps])
+    ntaps = np.ascontiguousarray([len(t) for t in taps], np.int32)
+    bcl = list(bc) if not isinstance(bc, str) else [bc]
+    proc = None if process is None else np.ascontiguousarray(process, np.uint8)
+    if use_ref:
+        syms = (ctypes.c_char_p * len(taps))(*[s.encode() for s in symmetries])
+        arr = (ctypes.c_char_p * max(1, len(bcl)))(*[b.encode() for b in bcl])
+        st = ref().dipref_separable(_ptr(x), DT[x.dtype], _ptr(out), DT[out_dtype], nd, _ptr(sizes),
+                                    tptrs, _ptr(ntaps), syms, len(taps), arr, len(bcl),
+                                    None if proc is None else _ptr(proc))
+        if st != 0:
+            raise RuntimeError(ref().dipref_last_error().decode())
+    else:
+        syms = np.ascontiguousarray([SYM[s] for s in symmetries], np.int32)
+        org = None if origins is None else np.ascontiguousarray(origins, np.int32)
+        bci = np.ascontiguousarray([BC[b] for b in bcl], np.int32)
+        st = orc().orc_separable(_ptr(x), DT[x.dtype], _ptr(out), DT[out_dtype], nd, _ptr(sizes),
+                                 tptrs, _ptr(ntaps), _ptr(syms), None if org is None else _ptr(org),
+                                 len(taps), _ptr(bci), len(bcl), None if proc is None else _ptr(proc),
+                                 nthreads)
+        if st != 0:
+            raise RuntimeError(f"oracle separable failed ({st})")
+    return out
+
+
+def bc_map(j, n, bc):
+    s = ctypes.c_int(0)
+    idx = orc().orc_bc_map(j, n, BC[bc] if isinstance(bc, str) else bc, ctypes.byref(s))
+    return idx, s.value
