@@ -75,6 +75,7 @@ def lib():
                                           D, U32, I32, P]
         L.dipamd_batch_run.argtypes = [P, P]
         L.dipamd_batch_run_graph.argtypes = [P, P]
+        L.dipamd_batch_run_per_image.argtypes = [P, P]
         L.dipamd_batch_destroy.argtypes = [P]
         L.dipamd_batch_info.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(I32)]
     if hasattr(L, "dipamd_gauss_fir_backward"):
@@ -92,6 +93,6 @@ def check(status):
 
 EXPORTED_SYMBOLS = [
     "dipamd_gauss_fir", "dipamd_separable_convolution", "dipamd_make_half_gaussian", "dipamd_batch_create",
-    "dipamd_batch_run", "dipamd_batch_run_graph", "dipamd_batch_destroy", "dipamd_batch_num_configs",
+    "dipamd_batch_run", "dipamd_batch_run_graph", "dipamd_batch_run_per_image", "dipamd_batch_destroy", "dipamd_batch_num_configs",
     "dipamd_batch_info", "dipamd_gauss_fir_backward", "dipamd_gauss_radius", "dipamd_last_error", "dipamd_version",
 ]

@@ -1,0 +1,15 @@
+// Launchers of the batched persistent kernel (k_batched.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "sep_kernels.hpp"
+
+namespace dipamd {
+
+hipError_t launch_batched(InType ti, bool is_double, int cfg, bool fast, const void *args, int grid, size_t lds,
+                          hipStream_t stream);
+int batched_occupancy(InType ti, bool is_double, int cfg, bool fast, size_t lds);
+hipError_t launch_reset_counter(int32_t *counter, hipStream_t stream);
+
+}  // namespace dipamd

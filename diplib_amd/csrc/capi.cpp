@@ -28,7 +28,49 @@ namespace dipamd {
 
 namespace {
 thread_local std::string g_last_error;
+
+// Kernel trace: HIP events recorded on the launch stream around every kernel the
+// planner issues, for per-kernel timing inside a caller's timed region.
+struct TraceRec {
+    char name[48];
+    int64_t bytes;
+    hipEvent_t a, b;
+};
+std::atomic<bool> g_trace_on{false};
+std::mutex g_trace_mu;
+std::vector<TraceRec> g_trace;
+std::vector<hipEvent_t> g_event_pool;
+
+hipEvent_t pooled_event() {
+    if (!g_event_pool.empty()) {
+        hipEvent_t e = g_event_pool.back();
+        g_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
 }  // namespace
+
+TraceScope::TraceScope(const char *name, int64_t bytes, hipStream_t stream) : stream_(stream), index_(-1) {
+    if (!g_trace_on.load(std::memory_order_relaxed)) return;
+    std::lock_guard<std::mutex> lk(g_trace_mu);
+    TraceRec r;
+    std::snprintf(r.name, sizeof(r.name), "%s", name);
+    r.bytes = bytes;
+    r.a = pooled_event();
+    r.b = pooled_event();
+    (void)hipEventRecord(r.a, stream);
+    g_trace.push_back(r);
+    index_ = static_cast<int>(g_trace.size()) - 1;
+}
+
+TraceScope::~TraceScope() {
+    if (index_ < 0) return;
+    std::lock_guard<std::mutex> lk(g_trace_mu);
+    if (index_ < static_cast<int>(g_trace.size())) (void)hipEventRecord(g_trace[static_cast<size_t>(index_)].b, stream_);
+}
 
 int set_error(int code, const std::string &msg) {
     g_last_error = msg;

@@ -31,6 +31,19 @@ struct InternFilter {
     int sym = DIPAMD_SYM_GENERAL;
 };
 
+// Records HIP events around a kernel launch while dipamd_trace_enable(1) is on.
+class TraceScope {
+  public:
+    TraceScope(const char *name, int64_t bytes, hipStream_t stream);
+    ~TraceScope();
+    TraceScope(const TraceScope &) = delete;
+    TraceScope &operator=(const TraceScope &) = delete;
+
+  private:
+    hipStream_t stream_;
+    int index_;
+};
+
 int set_error(int code, const std::string &msg);
 const char *last_error();
 size_t dt_size(int dt);

@@ -146,6 +146,35 @@ struct FusedArgs {
     TB ty[kFusedCap];
 };
 
+// Batched form: per-image parameters, work items and kernel arguments.
+template <typename TB>
+struct BatchImage {
+    const void *in;
+    TB *out;
+    int64_t nx, ny, syin, syout;
+    int32_t rx, ry;
+    int32_t tap_offset;  // into the taps table: rx+1 x taps then ry+1 y taps
+    int32_t pad_;
+};
+
+struct BatchItem {
+    int32_t image;
+    int32_t tile;
+    int32_t z0;
+    int32_t z1;
+};
+
+template <typename TB>
+struct BatchArgs {
+    const BatchImage<TB> *images;
+    const BatchItem *items;
+    const TB *taps;
+    int32_t *counter;
+    int32_t nitems;
+    int32_t bcx, bcy;
+    TB cvalx, cvaly;
+};
+
 // Geometry of an element-wise conversion pass.
 struct CastGeo {
     int32_t ndim;

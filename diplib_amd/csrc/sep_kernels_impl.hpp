@@ -238,15 +238,28 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const PassArgs<TB> a, int64_t 
 //            y stage = TXC columns x (256/TXC) groups, P rows each;  S = 16.
 // Next step's input rows are prefetched into registers before the compute.
 // ---------------------------------------------------------------------------
+// One (column tile, row chunk) of a 2-D plane: the body of the fused kernel,
+// shared by the plain launch (one workgroup per tile) and the batched persistent
+// kernel (tiles pulled from a work queue).  `lds` holds S*pitchS staging words
+// followed by RC*pitchR ring words; `tx`/`ty` point at rx+1 / ry+1 taps.
+template <typename TB>
+struct Tile {
+    const void *in;   // plane origin
+    TB *out;
+    int64_t nx, ny;
+    int64_t sxin, syin, sxout, syout;
+    int64_t c0, z0, zend;  // first column, first and end output rows
+    int rx, ry, bcx, bcy;
+    TB cvalx, cvaly;
+};
+
 template <typename TI, typename TB, int TXC, int P, int SX, int SY, bool FAST>
-__global__ __launch_bounds__(kBlock) void k_fused2d(const FusedArgs<TB> a, int64_t ntiles, int64_t nchunks) {
+__device__ __forceinline__ void fused_tile(const Tile<TB> &t, TB *lds, const TB *tx, const TB *ty) {
     constexpr int S = 16;
     constexpr int XT = TXC / P;            // x-stage lanes per row (16)
     static_assert(XT * S == kBlock, "tile shape");
     constexpr int NLD = (TXC + 2 * (kFusedCap - 1) + 63) / 64;  // loads per staged row per lane (max)
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    TB *lds = reinterpret_cast<TB *>(smem_raw);
-    const int Bx = a.rx, By = a.ry;
+    const int Bx = t.rx, By = t.ry;
     const int Wx = TXC + 2 * Bx;
     const int pitchS = lpad_size(Wx);
     const int pitchR = lpad_size(TXC);
@@ -254,24 +267,14 @@ __global__ __launch_bounds__(kBlock) void k_fused2d(const FusedArgs<TB> a, int64
     const int RC = (A + 1) * S;
     TB *stg = lds;
     TB *ring = lds + S * pitchS;
-
-    const int64_t bid = blockIdx.x;
-    const int64_t tile = bid % ntiles;
-    const int64_t rest = bid / ntiles;
-    const int64_t chunk = rest % nchunks;
-    const int64_t outer = rest / nchunks;
-    int64_t oin, oout;
-    outer_offsets(a.outer, outer, &oin, &oout);
-    const TI *src = static_cast<const TI *>(a.in) + oin;
-    TB *dst = static_cast<TB *>(a.out) + oout;
-    const int64_t c0 = tile * TXC;
-    const int64_t z0 = chunk * a.chunk;
-    const int64_t zend = min(z0 + a.chunk, a.ny);
+    const TI *src = static_cast<const TI *>(t.in);
+    TB *dst = t.out;
+    const int64_t c0 = t.c0, z0 = t.z0, zend = t.zend;
     const int nsteps = static_cast<int>((zend - z0 + S - 1) / S);
     const int M = A + nsteps;
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
-    const bool xinterior = (c0 - Bx >= 0) && (c0 + TXC + Bx <= a.nx);
+    const bool xinterior = (c0 - Bx >= 0) && (c0 + TXC + Bx <= t.nx);
     const int nld = (Wx + 63) / 64;
 
     // --- staged-row loads: wave w loads rows w, w+4, w+8, w+12 of each step ---
@@ -283,22 +286,22 @@ __global__ __launch_bounds__(kBlock) void k_fused2d(const FusedArgs<TB> a, int64
             int mul = 1;
             bool isc = false;
             int64_t sr = j;
-            if (j < 0 || j >= a.ny) sr = bc_map(j, a.ny, a.bcy, &mul, &isc);
-            const TI *srow = src + sr * a.syin;
+            if (j < 0 || j >= t.ny) sr = bc_map(j, t.ny, t.bcy, &mul, &isc);
+            const TI *srow = src + sr * t.syin;
             if (isc) continue;  // constant row: filled after the x stage
             if (xinterior) {
-                const TI *s0 = srow + (c0 - Bx) * a.sxin;
+                const TI *s0 = srow + (c0 - Bx) * t.sxin;
 #pragma unroll
                 for (int i = 0; i < NLD; ++i) {
                     const int col = lane + 64 * i;
-                    if (i < nld && col < Wx) pre[rr][i] = to_buf<TB>(s0[col * a.sxin]);
+                    if (i < nld && col < Wx) pre[rr][i] = to_buf<TB>(s0[col * t.sxin]);
                 }
             } else {
 #pragma unroll
                 for (int i = 0; i < NLD; ++i) {
                     const int col = lane + 64 * i;
                     if (i < nld && col < Wx)
-                        pre[rr][i] = load_ext<TI, TB>(srow, a.sxin, c0 - Bx + col, a.nx, a.bcx, a.cvalx);
+                        pre[rr][i] = load_ext<TI, TB>(srow, t.sxin, c0 - Bx + col, t.nx, t.bcx, t.cvalx);
                 }
             }
         }
@@ -325,16 +328,16 @@ __global__ __launch_bounds__(kBlock) void k_fused2d(const FusedArgs<TB> a, int64
             const int64_t j = z0 - By + static_cast<int64_t>(m) * S + xr;
             int mul = 1;
             bool isc = false;
-            if (j < 0 || j >= a.ny) bc_map(j, a.ny, a.bcy, &mul, &isc);
+            if (j < 0 || j >= t.ny) bc_map(j, t.ny, t.bcy, &mul, &isc);
             TB *rrow = ring + ((m * S + xr) % RC) * pitchR;
             if (isc) {
 #pragma unroll
-                for (int p = 0; p < P; ++p) rrow[lpad(xl * P + p)] = a.cvaly;
+                for (int p = 0; p < P; ++p) rrow[lpad(xl * P + p)] = t.cvaly;
             } else {
                 const TB *srow = stg + xr * pitchS;
                 const int base = Bx + xl * P;
                 TB acc[P];
-                fir_window<TB, P, SX, FAST>(a.tx, Bx + 1, 0, [&](int q) { return srow[lpad(base + q)]; }, acc);
+                fir_window<TB, P, SX, FAST>(tx, Bx + 1, 0, [&](int q) { return srow[lpad(base + q)]; }, acc);
 #pragma unroll
                 for (int p = 0; p < P; ++p) rrow[lpad(xl * P + p)] = mul < 0 ? -acc[p] : acc[p];
             }
@@ -348,16 +351,98 @@ __global__ __launch_bounds__(kBlock) void k_fused2d(const FusedArgs<TB> a, int64
             const int64_t c = c0 + col;
             const int rel0 = n * S + grp * P + By;  // ring-relative row of output p = 0
             TB acc[P];
-            fir_window<TB, P, SY, FAST>(a.ty, By + 1, 0,
+            fir_window<TB, P, SY, FAST>(ty, By + 1, 0,
                                         [&](int q) { return ring[((rel0 + q) % RC) * pitchR + lpad(col)]; }, acc);
             const int64_t zs = z0 + static_cast<int64_t>(n) * S + grp * P;
-            if (c < a.nx) {
+            if (c < t.nx) {
 #pragma unroll
                 for (int p = 0; p < P; ++p)
-                    if (zs + p < zend) dst[(zs + p) * a.syout + c * a.sxout] = acc[p];
+                    if (zs + p < zend) dst[(zs + p) * t.syout + c * t.sxout] = acc[p];
             }
         }
     }
+}
+
+template <typename TI, typename TB, int TXC, int P, int SX, int SY, bool FAST>
+__global__ __launch_bounds__(kBlock) void k_fused2d(const FusedArgs<TB> a, int64_t ntiles, int64_t nchunks) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    TB *lds = reinterpret_cast<TB *>(smem_raw);
+    const int64_t bid = blockIdx.x;
+    const int64_t tile = bid % ntiles;
+    const int64_t rest = bid / ntiles;
+    const int64_t chunk = rest % nchunks;
+    const int64_t outer = rest / nchunks;
+    int64_t oin, oout;
+    outer_offsets(a.outer, outer, &oin, &oout);
+    Tile<TB> t;
+    t.in = static_cast<const TI *>(a.in) + oin;
+    t.out = static_cast<TB *>(a.out) + oout;
+    t.nx = a.nx;
+    t.ny = a.ny;
+    t.sxin = a.sxin;
+    t.syin = a.syin;
+    t.sxout = a.sxout;
+    t.syout = a.syout;
+    t.c0 = tile * TXC;
+    t.z0 = chunk * a.chunk;
+    t.zend = min(t.z0 + a.chunk, a.ny);
+    t.rx = a.rx;
+    t.ry = a.ry;
+    t.bcx = a.bcx;
+    t.bcy = a.bcy;
+    t.cvalx = a.cvalx;
+    t.cvaly = a.cvaly;
+    fused_tile<TI, TB, TXC, P, SX, SY, FAST>(t, lds, a.tx, a.ty);
+}
+
+// ---------------------------------------------------------------------------
+// K4: batched persistent form.  One launch covers a list of 2-D images of
+// different sizes and sigmas: each workgroup repeatedly takes the next work item
+// (image, column tile, row chunk) from a global counter, copies that image's taps
+// to LDS and runs the fused tile.  Items are ordered largest-cost first.  The
+// loop ends when the counter passes the item count, so every wave exits.
+// ---------------------------------------------------------------------------
+template <typename TI, typename TB, int TXC, int P, bool FAST>
+__global__ __launch_bounds__(kBlock) void k_batched(const BatchArgs<TB> a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ int s_item;
+    __shared__ TB s_taps[2 * kFusedCap];
+    TB *lds = reinterpret_cast<TB *>(smem_raw);
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(a.counter, 1);
+        __syncthreads();
+        const int item = s_item;
+        if (item >= a.nitems) break;  // uniform: every wave of the workgroup leaves
+        const BatchItem it = a.items[item];
+        const BatchImage<TB> im = a.images[it.image];
+        const int ntx = im.rx + 1, nty = im.ry + 1;
+        for (int k = threadIdx.x; k < ntx + nty; k += blockDim.x) s_taps[k] = a.taps[im.tap_offset + k];
+        __syncthreads();  // taps visible; s_item read by all before it is rewritten
+        Tile<TB> t;
+        t.in = im.in;
+        t.out = im.out;
+        t.nx = im.nx;
+        t.ny = im.ny;
+        t.sxin = 1;
+        t.syin = im.syin;
+        t.sxout = 1;
+        t.syout = im.syout;
+        t.c0 = static_cast<int64_t>(it.tile) * TXC;
+        t.z0 = it.z0;
+        t.zend = it.z1;
+        t.rx = im.rx;
+        t.ry = im.ry;
+        t.bcx = a.bcx;
+        t.bcy = a.bcy;
+        t.cvalx = a.cvalx;
+        t.cvaly = a.cvaly;
+        fused_tile<TI, TB, TXC, P, SYM_EVEN, SYM_EVEN, FAST>(t, lds, s_taps, s_taps + ntx);
+        __syncthreads();  // LDS (ring, taps) free before the next item
+    }
+}
+
+__global__ void k_reset_counter(int32_t *counter) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
 }
 
 // ---------------------------------------------------------------------------

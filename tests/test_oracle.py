@@ -1,0 +1,176 @@
+"""Pins the CPU oracle (oracle/gauss_oracle.c) before it is trusted as the checker.
+
+* taps: bit-identical to the reference's own MakeHalfGaussian (golden fixtures),
+  plus the moment invariants of gauss.cpp:496-531 that fix the formulas;
+* boundary extension: the oracle's closed-form map against np.pad's definitions
+  (reflect = dip "mirror", wrap = "periodic") and the pointer walks via a delta filter;
+* pipeline: bit-identical to the reference's dip::Gauss outputs (golden fixtures);
+  a dense FP64 operator built from the taps and the boundary map reproduces it;
+  the result does not depend on the number of threads (gauss.cpp:547-569).
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import cases
+import oracle_ctypes as oc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TAPS = np.load(os.path.join(HERE, "golden", "taps.npz"))
+GOLD = np.load(os.path.join(HERE, "golden", "small_cases.npz"))
+
+
+@pytest.mark.parametrize("sigma", cases.TAP_SIGMAS)
+@pytest.mark.parametrize("order", [0, 1, 2, 3])
+@pytest.mark.parametrize("is_double", [0, 1])
+def test_taps_match_reference(sigma, order, is_double):
+    got = oc.half_gaussian(sigma, order, 3.0, bool(is_double))
+    exp = TAPS[f"s{sigma}_o{order}_d{is_double}"]
+    assert np.array_equal(got, exp)
+
+
+def test_radius_table():
+    # radii of the reference's MakeHalfGaussian at truncation 3 (orders 0/1/2)
+    table = {1: (3, 4, 4), 2: (6, 7, 8), 3: (9, 11, 12), 4: (12, 14, 16), 8: (24, 28, 32), 16: (48, 56, 64)}
+    for s, rs in table.items():
+        assert tuple(oc.gauss_radius(s, o) for o in range(3)) == rs
+    assert oc.gauss_radius(0.5, 0) == 2
+    # truncation is capped at sqrt(-2 ln eps): 5.646 (sfloat), 8.49 (dfloat)
+    assert oc.gauss_radius(1.0, 0, 100.0, False) == 6
+    assert oc.gauss_radius(1.0, 0, 100.0, True) == 9
+
+
+@pytest.mark.parametrize("sigma", [1.0, 2.0, 3.0, 5.0])
+def test_tap_invariants(sigma):
+    # order 0: sums to 1
+    f0 = oc.half_gaussian(sigma, 0)
+    full0 = np.concatenate([f0, f0[-2::-1]])
+    assert abs(full0.sum() - 1.0) < 1e-12
+    # order 1 (correlation with x[i+k] - x[i-k]): a ramp has derivative exactly 1
+    f1 = oc.half_gaussian(sigma, 1)[::-1]  # f1[k] = tap at offset k
+    k = np.arange(len(f1))
+    assert abs((f1 * 2 * k).sum() - 1.0) < 1e-12
+    # order 2: sums to 0 and a parabola x^2 has second derivative 2
+    f2 = oc.half_gaussian(sigma, 2)[::-1]
+    k = np.arange(len(f2))
+    assert abs(f2[0] + 2 * f2[1:].sum()) < 1e-12
+    assert abs((f2 * 2 * k * k).sum() - 2.0) < 1e-12
+    # order 3: a cubic x^3 has third derivative 6
+    f3 = oc.half_gaussian(sigma, 3)[::-1]
+    k = np.arange(len(f3))
+    assert abs((f3 * 2 * k ** 3).sum() - 6.0) < 1e-9
+
+
+def _np_ext(n, bc, j):
+    """Reference definition of the extension via np.pad on an index ramp."""
+    pad = abs(j) + n + 2
+    idx = np.arange(n)
+    if bc == "mirror":
+        e = np.pad(idx, pad, mode="reflect")
+    elif bc == "periodic":
+        e = np.pad(idx, pad, mode="wrap")
+    else:
+        raise ValueError
+    return int(e[pad + j])
+
+
+@pytest.mark.parametrize("bc", ["mirror", "periodic"])
+@pytest.mark.parametrize("n", [2, 3, 5, 8])
+def test_bc_map_closed_form(bc, n):
+    for j in range(-40, 40 + n):
+        idx, s = oc.bc_map(j, n, bc)
+        assert idx == _np_ext(n, bc, j), (j, n)
+        assert s == 1
+
+
+@pytest.mark.parametrize("bc", ["mirror", "asym mirror", "periodic", "asym periodic", "zero order", "add zeros"])
+@pytest.mark.parametrize("n", [2, 3, 4, 7])
+def test_bc_walk_matches_closed_form(bc, n):
+    """The oracle's restated pointer walks (copy_buffer.cpp) against its closed form,
+    probed with a GENERAL filter that picks the single sample x[i+k]."""
+    x = (np.arange(n, dtype=np.float64) + 1.0) * 10.0
+    for k in list(range(-3 * n, -n)) + list(range(n, 3 * n)):
+        # a general filter of length 2|k|+1 with a single 1 at the far end reads x[i + k]
+        L = 2 * abs(k) + 1
+        taps = np.zeros(L)
+        taps[0 if k > 0 else L - 1] = 1.0  # user (convolution) order: reversed internally
+        y = oc.separable(x[None, :], [taps, [1.0]], ["general", "general"], bc=[bc], process=[1, 0])
+        for i in range(n):
+            j = i + k
+            idx, s = oc.bc_map(j, n, bc)
+            exp = 0.0 if idx < 0 else s * x[idx]
+            assert y[0, i] == exp, (bc, n, k, i, y[0, i], exp)
+
+
+def _dense_1d(n, taps_half, odd, bc):
+    """Dense FP64 operator of one filter pass along a line of n samples."""
+    r = len(taps_half) - 1
+    f = taps_half[::-1]  # f[k], k = 0..r
+    M = np.zeros((n, n))
+    for i in range(n):
+        for k in range(-r, r + 1):
+            c = f[abs(k)] * (np.sign(k) if odd else 1.0)
+            idx, s = oc.bc_map(i + k, n, bc)
+            if idx >= 0:
+                M[i, idx] += s * c
+    return M
+
+
+@pytest.mark.parametrize("bc", ["mirror", "periodic", "asym mirror"])
+@pytest.mark.parametrize("shape,sig,ords", [((6, 9), 2.0, (1, 0)), ((13, 5), 3.0, (0, 2)), ((4, 4), 1.0, (0, 0))])
+def test_dense_operator(bc, shape, sig, ords):
+    x = np.random.default_rng(3).random(shape)
+    y = oc.gauss_fir(x, [sig], list(ords), [bc])
+    H, W = shape
+    Mx = _dense_1d(W, oc.half_gaussian(sig, ords[0], is_double=True), ords[0] % 2 == 1, bc)
+    My = _dense_1d(H, oc.half_gaussian(sig, ords[1], is_double=True), ords[1] % 2 == 1, bc)
+    exp = My @ x @ Mx.T
+    assert np.allclose(y, exp, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("case", cases.SMALL_CASES, ids=[c[0] for c in cases.SMALL_CASES])
+def test_oracle_matches_reference_outputs(case):
+    name, shape, dt, sig, ords, bc, odt = case
+    x = GOLD[name + "__in"]
+    exp = GOLD[name + "__out"]
+    got = oc.gauss_fir(x, sig, ords, bc, 3.0, out_dtype=odt)
+    assert got.dtype == exp.dtype
+    assert np.array_equal(got.view(np.uint8), exp.view(np.uint8))
+
+
+def test_thread_independence():
+    x = cases.make_input((59, 192, 256), "float64", seed=5)
+    a = oc.gauss_fir(x, [2.0], [0], ["mirror"], nthreads=1)
+    b = oc.gauss_fir(x, [2.0], [0], ["mirror"], nthreads=4)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.slow
+def test_oracle_large_hash():
+    with open(os.path.join(HERE, "golden", "large_cases.json")) as f:
+        large = json.load(f)
+    name = "u8_8192sq_s1_o10"
+    spec = [c for c in cases.LARGE_CASES if c[0] == name][0]
+    _, shape, dt, seed, imp, sig, ords, bc, odt = spec
+    x = cases.make_input(shape, dt, seed, imp)
+    y = oc.gauss_fir(x, sig, ords, bc, nthreads=8)
+    assert hashlib.sha256(y.tobytes()).hexdigest() == large[name]["output_sha256"]
+
+
+def test_jacobian_fixture_consistent():
+    """J^T w from the reference's impulse responses equals the dense operator's
+    transpose applied to w (FP64), which pins the adjoint the backward must match."""
+    jac = np.load(os.path.join(HERE, "golden", "jacobian.npz"))
+    meta = json.loads(str(jac["o1_mirror_64__meta"]))
+    w = jac["o1_mirror_64__w"]
+    jtw = jac["o1_mirror_64__jtw"]
+    H, W = meta["shape"]
+    sig = meta["sigmas"][0]
+    ox, oy = meta["orders"]
+    Mx = _dense_1d(W, oc.half_gaussian(sig, ox, is_double=True), ox % 2 == 1, meta["bc"])
+    My = _dense_1d(H, oc.half_gaussian(sig, oy, is_double=True), oy % 2 == 1, meta["bc"])
+    exp = My.T @ w @ Mx
+    assert np.allclose(jtw, exp, rtol=1e-10, atol=1e-12)
