@@ -42,6 +42,10 @@ class DipamdBatchImage(ctypes.Structure):
                 ("order_y", ctypes.c_int32)]
 
 
+class DipamdTraceRecord(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 48), ("ms", ctypes.c_double), ("bytes", ctypes.c_int64)]
+
+
 class DipError(RuntimeError):
     """Error raised by the library (dip::ParameterError / dip::RunTimeError counterpart)."""
 
@@ -69,6 +73,8 @@ def lib():
     L.dipamd_make_half_gaussian.argtypes = [D, I32, D, I32, P, I32]
     L.dipamd_gauss_radius.argtypes = [D, I32, D, I32]
     L.dipamd_last_error.restype = ctypes.c_char_p
+    L.dipamd_trace_enable.argtypes = [I32]
+    L.dipamd_trace_read.argtypes = [ctypes.POINTER(DipamdTraceRecord), I32]
     L.dipamd_version.restype = ctypes.c_char_p
     if hasattr(L, "dipamd_batch_create"):
         L.dipamd_batch_create.argtypes = [ctypes.POINTER(P), ctypes.POINTER(DipamdBatchImage), I32, I32, I32, P, I32,
@@ -95,4 +101,5 @@ EXPORTED_SYMBOLS = [
     "dipamd_gauss_fir", "dipamd_separable_convolution", "dipamd_make_half_gaussian", "dipamd_batch_create",
     "dipamd_batch_run", "dipamd_batch_run_graph", "dipamd_batch_run_per_image", "dipamd_batch_destroy", "dipamd_batch_num_configs",
     "dipamd_batch_info", "dipamd_gauss_fir_backward", "dipamd_gauss_radius", "dipamd_last_error", "dipamd_version",
+    "dipamd_trace_enable", "dipamd_trace_read",
 ]

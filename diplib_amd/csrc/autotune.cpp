@@ -60,10 +60,21 @@ void load_table() {
 bool fits(int cfg, bool is_double, int rx, int ry) {
     if (cfg < 0 || cfg >= kNumFusedConfigs) return false;
     if (is_double && cfg == 2) return false;
-    return fused_lds_bytes(cfg, rx, ry, is_double) <= 65536;
+    return fused_lds_bytes(cfg, rx, ry, is_double) <= lds_limit_bytes();
 }
 
 }  // namespace
+
+size_t lds_limit_bytes() {
+    static size_t limit = [] {
+        int dev = 0, v = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+            v = 65536;
+        return static_cast<size_t>(v < 65536 ? v : 65536);
+    }();
+    return limit;
+}
 
 int choose_fused_config(bool is_double, int rx, int ry, int64_t nx, int64_t ny, uint32_t flags) {
     std::call_once(g_tune_once, load_table);

@@ -94,6 +94,19 @@ int build_plan(dipamd_batch_plan *P, const dipamd_batch_image *images, int n) {
         rmax_x = std::max(rmax_x, rx);
         rmax_y = std::max(rmax_y, ry);
     }
+    // a tile config whose LDS (staging + ring for the largest radii) fits; the
+    // requested one when it does
+    const size_t limit = lds_limit_bytes();
+    if (fused_lds_bytes(P->cfg, rmax_x, rmax_y, sizeof(TB) == 8) > limit) {
+        int fit = -1;
+        for (int c = kNumFusedConfigs - 1; c >= 0; --c)
+            if (!(sizeof(TB) == 8 && c == 2) && fused_lds_bytes(c, rmax_x, rmax_y, sizeof(TB) == 8) <= limit) {
+                fit = c;
+                break;
+            }
+        if (fit < 0) return set_error(DIPAMD_E_UNSUPPORTED, "batched form: tile does not fit in LDS");
+        P->cfg = fit;
+    }
     // work items, costliest images first (longest-processing-time order)
     const int txc = kFusedConfigs[P->cfg].txc;
     std::vector<int> order(static_cast<size_t>(n));
@@ -116,7 +129,6 @@ int build_plan(dipamd_batch_plan *P, const dipamd_batch_image *images, int n) {
     }
     P->nitems = static_cast<int64_t>(items.size());
     P->lds = fused_lds_bytes(P->cfg, rmax_x, rmax_y, sizeof(TB) == 8);
-    if (P->lds > 65536) return set_error(DIPAMD_E_UNSUPPORTED, "batched form: tile does not fit in LDS");
     // device tables
     hipError_t e;
     if ((e = hipMalloc(&P->d_images, ims.size() * sizeof(BatchImage<TB>))) != hipSuccess ||

@@ -15,6 +15,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <atomic>
+#include <cstdio>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -316,6 +318,12 @@ static void fill_outer(Outer &o, const View &src, const View &dst, int ndim, int
     }
 }
 
+static int64_t npx_of(const View &v, int ndim) {
+    int64_t p = 1;
+    for (int d = 0; d < ndim; ++d) p *= v.sizes[d];
+    return p;
+}
+
 static int64_t product_except(const View &v, int ndim, int skip0, int skip1) {
     int64_t p = 1;
     for (int d = 0; d < ndim; ++d)
@@ -383,6 +391,7 @@ static int run_single(const View &src, const View &dst, int ndim, int d, const I
         a.nlines = product_except(src, ndim, d, -1);
         if (line_lds_bytes(F.border, is_double) > 65536)
             return set_error(DIPAMD_E_UNSUPPORTED, "filter border too large for the line kernel");
+        TraceScope ts("line", npx_of(src, ndim) * static_cast<int64_t>(dt_size(src.dt) + sizeof(TB)), stream);
         err = launch_line(ti, is_double, F.sym, fast, &a, a.n, a.nlines, F.border, stream);
     } else {
         a.e = src.sizes[lane];
@@ -394,6 +403,7 @@ static int run_single(const View &src, const View &dst, int ndim, int d, const I
         const int64_t ntiles = (a.e + bs - 1) / bs;
         a.chunk = pick_chunk(a.n, ntiles * nouter, 8, F.border, 2048);
         const int64_t nchunks = (a.n + a.chunk - 1) / a.chunk;
+        TraceScope ts("sweep", npx_of(src, ndim) * static_cast<int64_t>(dt_size(src.dt) + sizeof(TB)), stream);
         err = launch_sweep(ti, is_double, F.sym, fast, &a, a.e, nchunks, nouter, stream);
     }
     if (err != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(err));
@@ -428,6 +438,7 @@ static int run_fused(const View &src, const View &dst, int ndim, int dx, int dy,
     a.chunk = pick_chunk(a.ny, ntiles * nouter, kFusedStep, Fy.border, 2048);
     const int64_t nchunks = (a.ny + a.chunk - 1) / a.chunk;
     const InType ti = (src.dt == DIPAMD_DT_UINT8) ? IN_U8 : IN_BUF;
+    TraceScope ts("fused2d", npx_of(src, ndim) * static_cast<int64_t>(dt_size(src.dt) + sizeof(TB)), stream);
     hipError_t err = launch_fused(ti, sizeof(TB) == 8, cfg, Fx.sym, Fy.sym, fast, &a, a.nx, nchunks, nouter,
                                   Fx.border, Fy.border, stream);
     if (err != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(err));
@@ -445,6 +456,7 @@ static int run_cast_in(const View &src, const View &dst, int ndim, bool to_doubl
         g.sout[d] = dst.strides[d];
         g.total *= src.sizes[d];
     }
+    TraceScope ts("cast_in", g.total * static_cast<int64_t>(dt_size(src.dt) + dt_size(dst.dt)), stream);
     hipError_t err = launch_cast_in(src.dt, to_double, src.p, dst.p, &g, g.total, stream);
     if (err != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("cast launch: ") + hipGetErrorString(err));
     return DIPAMD_OK;
@@ -461,6 +473,7 @@ static int run_cast_out(const View &src, const View &dst, int ndim, hipStream_t 
         g.sout[d] = dst.strides[d];
         g.total *= src.sizes[d];
     }
+    TraceScope ts("cast_out", g.total * static_cast<int64_t>(dt_size(src.dt) + dt_size(dst.dt)), stream);
     hipError_t err = launch_cast_out(src.dt == DIPAMD_DT_DFLOAT, dst.dt, src.p, dst.p, &g, g.total, stream);
     if (err != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("cast launch: ") + hipGetErrorString(err));
     return DIPAMD_OK;
@@ -647,6 +660,33 @@ using namespace dipamd;
 extern "C" {
 
 const char *dipamd_last_error(void) { return last_error(); }
+
+int dipamd_trace_enable(int32_t on) {
+    std::lock_guard<std::mutex> lk(g_trace_mu);
+    for (TraceRec &r : g_trace) {
+        g_event_pool.push_back(r.a);
+        g_event_pool.push_back(r.b);
+    }
+    g_trace.clear();
+    g_trace_on.store(on != 0);
+    return DIPAMD_OK;
+}
+
+int dipamd_trace_read(dipamd_trace_record *records, int32_t capacity) {
+    std::lock_guard<std::mutex> lk(g_trace_mu);
+    int n = 0;
+    for (TraceRec &r : g_trace) {
+        if (n >= capacity) break;
+        (void)hipEventSynchronize(r.b);
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) ms = -1.f;
+        std::snprintf(records[n].name, sizeof(records[n].name), "%s", r.name);
+        records[n].ms = ms;
+        records[n].bytes = r.bytes;
+        ++n;
+    }
+    return n;
+}
 const char *dipamd_version(void) { return "dipamd 0.1 (DIPlib 3.6.1 Gauss FIR path, gfx950)"; }
 
 int dipamd_gauss_radius(double sigma, int32_t order, double truncation, int32_t is_double) {

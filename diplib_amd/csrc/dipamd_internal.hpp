@@ -62,6 +62,10 @@ int broadcast_bc(const int32_t *bc, int nbc, int ndim, int *out);
 hipError_t scratch_alloc(void **p, size_t bytes, hipStream_t stream);
 void scratch_free(void *p, hipStream_t stream);
 
+// Dynamic LDS a workgroup may use on this device (64 KiB unless the device reports
+// less); kernels are sized against it.
+size_t lds_limit_bytes();
+
 int separable_device(const View &in, const View &out, int ndim, const InternFilter *const *filters, const int *bc,
                      const bool *process, uint32_t flags, hipStream_t stream);
 

@@ -4,6 +4,11 @@
 
 namespace dipamd {
 
+// Clears the work-queue counter with a kernel (graph-safe: no memset node).
+__global__ void k_reset_counter(int32_t *counter) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+}
+
 namespace {
 
 template <typename TI, typename TB, int TXC, int P, bool FAST>

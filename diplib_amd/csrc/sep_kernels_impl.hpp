@@ -441,9 +441,6 @@ __global__ __launch_bounds__(kBlock) void k_batched(const BatchArgs<TB> a) {
     }
 }
 
-__global__ void k_reset_counter(int32_t *counter) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
-}
 
 // ---------------------------------------------------------------------------
 // Element-wise conversions: input -> buffer type (contiguous), buffer type ->

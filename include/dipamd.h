@@ -170,6 +170,18 @@ int dipamd_gauss_fir_backward(const dipamd_image *x, const dipamd_image *grad_ou
 /* Radius of the tap array used for (sigma, order) as GaussFIR computes it (gauss.cpp:38-51). */
 int dipamd_gauss_radius(double sigma, int32_t order, double truncation, int32_t is_double);
 
+/* Kernel trace (no reference counterpart; the reference times with dip::testing::Timer,
+ * testing.h:221-265): while enabled, every kernel the library launches is bracketed by
+ * HIP events on its launch stream.  dipamd_trace_enable(1) clears the record list. */
+typedef struct dipamd_trace_record {
+    char name[48];     /* "fused2d", "line", "sweep", "cast_in", "cast_out", ... */
+    double ms;         /* event-timed duration of the launch */
+    int64_t bytes;     /* algorithmic bytes of the launch: samples read + written once */
+} dipamd_trace_record;
+int dipamd_trace_enable(int32_t on);
+/* Waits for the recorded events and copies up to `capacity` records; returns the count. */
+int dipamd_trace_read(dipamd_trace_record *records, int32_t capacity);
+
 /* Thread-local message of the last failed call on this thread. */
 const char *dipamd_last_error(void);
 /* Library version string. */
