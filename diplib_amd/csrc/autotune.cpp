@@ -65,6 +65,32 @@ bool fits(int cfg, bool is_double, int rx, int ry) {
 
 }  // namespace
 
+int fuse_max_radius() {
+    static int r = [] {
+        const char *v = std::getenv("DIPAMD_FUSE_MAX_RADIUS");
+        return v ? std::atoi(v) : 24;
+    }();
+    return r;
+}
+
+int ysweep_kind() {
+    static int kind = [] {
+        const char *v = std::getenv("DIPAMD_YSWEEP");
+        if (!v) return 0;
+        std::string s(v);
+        return s == "ring" ? 1 : (s == "global" ? 2 : 0);
+    }();
+    return kind;
+}
+
+int choose_ysweep_config(bool is_double, int ry) {
+    std::call_once(g_tune_once, load_table);
+    if (g_forced == -1) return -1;
+    const int pref = is_double ? 0 : 1;
+    if (fits(pref, is_double, 0, ry)) return pref;
+    return fits(0, is_double, 0, ry) ? 0 : -1;
+}
+
 size_t lds_limit_bytes() {
     static size_t limit = [] {
         int dev = 0, v = 0;
@@ -89,9 +115,12 @@ int choose_fused_config(bool is_double, int rx, int ry, int64_t nx, int64_t ny, 
         }
     }
     if (cfg == -2) {
-        // defaults: fuse while the y halo is small against the row chunk
-        if (r > 24 && !(flags & DIPAMD_FORCE_FUSION)) return -1;
-        cfg = is_double ? 0 : (nx >= 256 ? 1 : 0);
+        // defaults: fuse short filters (HBM-bound: the fused pass saves the intermediate
+        // round trip); long filters are VALU-bound and run as a line pass plus a
+        // column-tile sweep, which recompute nothing.  The 128-column tile while its
+        // ring fits comfortably, else 64 columns.
+        if (r > fuse_max_radius() && !(flags & DIPAMD_FORCE_FUSION)) return -1;
+        cfg = (!is_double && r <= 16 && nx >= 256) ? 1 : 0;
     }
     if (cfg < 0) {
         if (!(flags & DIPAMD_FORCE_FUSION)) return -1;

@@ -353,11 +353,16 @@ static int64_t pick_chunk(int64_t len, int64_t base, int granule, int border, in
     int64_t want = std::max<int64_t>(1, (target_blocks + base - 1) / std::max<int64_t>(base, 1));
     int64_t chunk = (len + want - 1) / want;
     chunk = ((chunk + granule - 1) / granule) * granule;
-    int64_t minc = ((4 * static_cast<int64_t>(border) + granule - 1) / granule) * granule;
+    // the halo (2*border rows) is re-read by each chunk: keep it under ~1/8 of the chunk
+    int64_t minc = ((16 * static_cast<int64_t>(border) + granule - 1) / granule) * granule;
     chunk = std::max<int64_t>(chunk, std::max<int64_t>(granule, minc));
     int64_t full = ((len + granule - 1) / granule) * granule;
     return std::min(chunk, full);
 }
+
+template <typename TB>
+static int run_fused(const View &src, const View &dst, int ndim, int dx, int dy, const InternFilter &Fx,
+                     const InternFilter &Fy, int bcx, int bcy, bool fast, int cfg, hipStream_t stream);
 
 template <typename TB>
 static int run_single(const View &src, const View &dst, int ndim, int d, const InternFilter &F, int bc, bool fast,
@@ -394,6 +399,36 @@ static int run_single(const View &src, const View &dst, int ndim, int d, const I
         TraceScope ts("line", npx_of(src, ndim) * static_cast<int64_t>(dt_size(src.dt) + sizeof(TB)), stream);
         err = launch_line(ti, is_double, F.sym, fast, &a, a.n, a.nlines, F.border, stream);
     } else {
+        // strided axis: the LDS-ring sweep (the fused kernel with a copy x stage) when
+        // the lane axis is contiguous and the window fits; else the global sweep
+        const bool ring_ok = (F.sym == DIPAMD_SYM_EVEN || F.sym == DIPAMD_SYM_ODD) && F.shift == 0 &&
+                             F.border <= kFusedCap - 1 && src.strides[lane] == 1 &&
+                             src.sizes[lane] < (int64_t(1) << 31) && src.sizes[d] < (int64_t(1) << 31);
+        const int kind = ysweep_kind();  // 0 column tiles (default), 1 LDS ring, 2 global sweep
+        const bool lane_ok = src.strides[lane] == 1 && src.sizes[d] < (int64_t(1) << 31);
+        const int rows = colsweep_rows(F.border, is_double);
+        if (kind == 0 && lane_ok && rows > 0) {
+            a.e = src.sizes[lane];
+            a.ein = src.strides[lane];
+            a.eout = dst.strides[lane];
+            fill_outer(a.outer, src, dst, ndim, d, lane);
+            const int64_t nouter = product_except(src, ndim, d, lane);
+            a.chunk = rows;
+            const int64_t nchunks = (a.n + rows - 1) / rows;
+            TraceScope ts("colsweep", npx_of(src, ndim) * static_cast<int64_t>(dt_size(src.dt) + sizeof(TB)), stream);
+            err = launch_colsweep(ti, is_double, F.sym, fast, &a, a.e, nchunks, nouter, stream);
+            if (err != hipSuccess)
+                return set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(err));
+            return DIPAMD_OK;
+        }
+        const int cfg = (ring_ok && kind != 2) ? choose_ysweep_config(is_double, F.border) : -1;
+        if (cfg >= 0) {
+            InternFilter ident;
+            ident.data = {1.0};
+            ident.size = 1;
+            ident.sym = SYM_NONE;
+            return run_fused<TB>(src, dst, ndim, lane, d, ident, F, DIPAMD_BC_SYMMETRIC_MIRROR, bc, fast, cfg, stream);
+        }
         a.e = src.sizes[lane];
         a.ein = src.strides[lane];
         a.eout = dst.strides[lane];
@@ -438,7 +473,8 @@ static int run_fused(const View &src, const View &dst, int ndim, int dx, int dy,
     a.chunk = pick_chunk(a.ny, ntiles * nouter, kFusedStep, Fy.border, 2048);
     const int64_t nchunks = (a.ny + a.chunk - 1) / a.chunk;
     const InType ti = (src.dt == DIPAMD_DT_UINT8) ? IN_U8 : IN_BUF;
-    TraceScope ts("fused2d", npx_of(src, ndim) * static_cast<int64_t>(dt_size(src.dt) + sizeof(TB)), stream);
+    TraceScope ts(Fx.sym == SYM_NONE ? "ringsweep" : "fused2d",
+                  npx_of(src, ndim) * static_cast<int64_t>(dt_size(src.dt) + sizeof(TB)), stream);
     hipError_t err = launch_fused(ti, sizeof(TB) == 8, cfg, Fx.sym, Fy.sym, fast, &a, a.nx, nchunks, nouter,
                                   Fx.border, Fy.border, stream);
     if (err != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(err));
@@ -540,7 +576,10 @@ int separable_device(const View &in, const View &out, int ndim, const InternFilt
         const InternFilter &Fy = *filters[order[1]];
         const bool symok = (Fx.sym == DIPAMD_SYM_EVEN || Fx.sym == DIPAMD_SYM_ODD) &&
                            (Fy.sym == DIPAMD_SYM_EVEN || Fy.sym == DIPAMD_SYM_ODD) && Fx.shift == 0 && Fy.shift == 0;
-        const bool fits = Fx.border <= kFusedCap - 1 && Fy.border <= kFusedCap - 1;
+        // the fused kernel keeps column offsets (x * x-stride) and row indices in 32 bits
+        const bool fits = Fx.border <= kFusedCap - 1 && Fy.border <= kFusedCap - 1 &&
+                          in.sizes[order[0]] * std::llabs(in.strides[order[0]]) < (int64_t(1) << 31) &&
+                          in.sizes[order[1]] < (int64_t(1) << 31);
         if (symok && fits) {
             const int cfg = choose_fused_config(is_double, Fx.border, Fy.border, in.sizes[order[0]], in.sizes[order[1]],
                                                 flags);

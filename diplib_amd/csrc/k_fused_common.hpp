@@ -14,7 +14,7 @@ hipError_t fused_t(const FusedArgs<TB> &a, int64_t nx, int64_t nchunks, int64_t 
     const int S = kFusedStep;
     const int A = (2 * ry + S - 1) / S;
     const size_t lds =
-        (static_cast<size_t>(S) * lpad_size(TXC + 2 * rx) + static_cast<size_t>((A + 1) * S) * lpad_size(TXC)) *
+        (static_cast<size_t>(S) * fused_pitch(TXC + 2 * rx, P) + static_cast<size_t>((A + 1) * S) * fused_pitch(TXC, P)) *
         sizeof(TB);
     hipLaunchKernelGGL((k_fused2d<TI, TB, TXC, P, SX, SY, FAST>), dim3(static_cast<unsigned>(nblk)), dim3(kBlock),
                        lds, st, a, ntiles, nchunks);
@@ -28,6 +28,8 @@ hipError_t fused_sym(int sx, int sy, const FusedArgs<TB> &a, int64_t nx, int64_t
     if (sx == SYM_EVEN && sy == SYM_ODD) return fused_t<TI, TB, TXC, P, SYM_EVEN, SYM_ODD, FAST>(a, nx, nc, no, rx, ry, st);
     if (sx == SYM_ODD && sy == SYM_EVEN) return fused_t<TI, TB, TXC, P, SYM_ODD, SYM_EVEN, FAST>(a, nx, nc, no, rx, ry, st);
     if (sx == SYM_ODD && sy == SYM_ODD) return fused_t<TI, TB, TXC, P, SYM_ODD, SYM_ODD, FAST>(a, nx, nc, no, rx, ry, st);
+    if (sx == SYM_NONE && sy == SYM_EVEN) return fused_t<TI, TB, TXC, P, SYM_NONE, SYM_EVEN, FAST>(a, nx, nc, no, rx, ry, st);
+    if (sx == SYM_NONE && sy == SYM_ODD) return fused_t<TI, TB, TXC, P, SYM_NONE, SYM_ODD, FAST>(a, nx, nc, no, rx, ry, st);
     return hipErrorInvalidValue;
 }
 

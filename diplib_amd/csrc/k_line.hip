@@ -14,7 +14,8 @@ size_t fused_lds_bytes(int cfg, int rx, int ry, bool is_double) {
     const int S = kFusedStep;
     const int A = (2 * ry + S - 1) / S;
     const int RC = (A + 1) * S;
-    const size_t words = static_cast<size_t>(S) * lpad_size(c.txc + 2 * rx) + static_cast<size_t>(RC) * lpad_size(c.txc);
+    const size_t words = static_cast<size_t>(S) * fused_pitch(c.txc + 2 * rx, c.p) +
+                         static_cast<size_t>(RC) * fused_pitch(c.txc, c.p);
     return words * (is_double ? 8 : 4);
 }
 

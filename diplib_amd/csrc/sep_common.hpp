@@ -19,7 +19,9 @@ constexpr int kLineCap = 256;               // taps per filter in the line / swe
 constexpr int kFusedCap = 65;               // taps per axis in the fused 2-D kernel (radius <= 64)
 constexpr int kBlock = 256;                 // threads per workgroup (4 wave64)
 
-enum Sym : int { SYM_GENERAL = 0, SYM_EVEN = 1, SYM_ODD = 2, SYM_D_EVEN = 4, SYM_D_ODD = 5 };
+// SYM_NONE: no filter along that axis (the fused kernel's x stage copies rows into
+// the ring, making it a y-only sweep).
+enum Sym : int { SYM_GENERAL = 0, SYM_EVEN = 1, SYM_ODD = 2, SYM_D_EVEN = 4, SYM_D_ODD = 5, SYM_NONE = 7 };
 
 struct bin_t {
     uint8_t v;

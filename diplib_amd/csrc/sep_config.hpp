@@ -5,5 +5,7 @@ namespace dipamd {
 constexpr int kLinePF = 8;  // outputs per lane, line kernel (float)
 constexpr int kLinePD = 4;  // outputs per lane, line kernel (double)
 constexpr int kSweepP = 8;  // outputs per thread and step, sweep kernel
+constexpr int kColPF = 8;   // outputs per thread and step, column-tile sweep (float)
+constexpr int kColPD = 4;   // outputs per thread and step, column-tile sweep (double)
 
 }  // namespace dipamd

@@ -30,6 +30,10 @@ hipError_t launch_line(InType ti, bool is_double, int sym, bool fast, const void
                        int border, hipStream_t stream);
 hipError_t launch_sweep(InType ti, bool is_double, int sym, bool fast, const void *args, int64_t e, int64_t nchunks,
                         int64_t nouter, hipStream_t stream);
+// Column-tile sweep: rows per block for a border (0 when the block does not fit).
+int colsweep_rows(int border, bool is_double);
+hipError_t launch_colsweep(InType ti, bool is_double, int sym, bool fast, const void *args, int64_t e,
+                           int64_t nchunks, int64_t nouter, hipStream_t stream);
 hipError_t launch_fused(InType ti, bool is_double, int cfg, int symx, int symy, bool fast, const void *args,
                         int64_t nx, int64_t nchunks, int64_t nouter, int rx, int ry, hipStream_t stream);
 

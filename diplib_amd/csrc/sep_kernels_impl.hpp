@@ -15,7 +15,8 @@
 // The window: a lane computes P consecutive outputs.  Right-hand samples x[i+p+k]
 // live in a P-slot register ring indexed (p+k) mod P, left-hand ones in a second
 // ring indexed (p-k) mod P; the tap loop is unrolled by P so every slot index is a
-// compile-time constant, and each tap step loads just one new sample per side.
+// compile-time constant, and each tap step reads one new sample per side through
+// a cursor of the sample source (LDS row, LDS ring column, or global column).
 #pragma once
 
 #include "sep_common.hpp"
@@ -36,18 +37,141 @@ __device__ __forceinline__ float mac<float, true>(float acc, float f, float t) {
     return __builtin_fmaf(f, t, acc);
 }
 
-// P outputs from a sample accessor get(q): q = 0 is output p = 0's centre sample.
-// `shift` moves the window: get(q) is asked for sample i + q - shift (GENERAL: the
-// reversed origin; the symmetric forms: origin - (ntaps - 1), zero for a centred filter).
-template <typename TB, int P, int SYM, bool FAST, typename Get0>
-__device__ __forceinline__ void fir_window(const TB *taps, int ntaps, int shift, Get0 get0, TB (&acc)[P]) {
-    auto get = [&](int q) { return get0(q - shift); };
+// Boundary map of coordinate j outside [0, n), packed as (source index << 1) | negate,
+// or -1 for a constant condition.  32-bit and out of line: it runs for edge samples
+// only, and returning by value keeps the call free of stack memory.
+static __device__ __noinline__ int bc_fold_packed(int j, int n, int bc) {
+    switch (bc) {
+        case DIPAMD_BC_SYMMETRIC_MIRROR:
+        case DIPAMD_BC_ASYMMETRIC_MIRROR: {
+            if (n == 1) return bc == DIPAMD_BC_ASYMMETRIC_MIRROR ? 1 : 0;
+            const int P = 2 * (n - 1);
+            int p = j % P;
+            if (p < 0) p += P;
+            const int idx = p < n ? p : P - p;
+            int neg = 0;
+            if (bc == DIPAMD_BC_ASYMMETRIC_MIRROR) {
+                const int m = j < 0 ? -j : j - (n - 1);
+                neg = ((m - 1) / (n - 1) + 1) & 1;  // odd number of edge reflections crossed
+            }
+            return (idx << 1) | neg;
+        }
+        case DIPAMD_BC_PERIODIC:
+        case DIPAMD_BC_ASYMMETRIC_PERIODIC: {
+            const int q = j >= 0 ? j / n : -((-j - 1) / n) - 1;
+            const int neg = (bc == DIPAMD_BC_ASYMMETRIC_PERIODIC && ((q & 1) || n == 1)) ? 1 : 0;
+            return ((j - q * n) << 1) | neg;
+        }
+        case DIPAMD_BC_ZERO_ORDER_EXTRAPOLATE:
+            return (j < 0 ? 0 : n - 1) << 1;
+        default:
+            return -1;
+    }
+}
+
+// Source index (-1: constant) and sign (*mul = +1 / -1) of coordinate j.
+__device__ __forceinline__ int bc_fold(int j, int n, int bc, int *mul) {
+    const int v = bc_fold_packed(j, n, bc);
+    *mul = (v > 0 && (v & 1)) ? -1 : 1;
+    return v < 0 ? -1 : (v >> 1);
+}
+
+// Sample j of a line of n samples at `base` with stride `st`, boundary-extended.
+template <typename TI, typename TB>
+__device__ __forceinline__ TB fetch_ext(const TI *base, int64_t st, int j, int n, int bc, TB cval) {
+    if (static_cast<unsigned>(j) < static_cast<unsigned>(n)) return to_buf<TB>(base[static_cast<int64_t>(j) * st]);
+    int mul;
+    const int s = bc_fold(j, n, bc, &mul);
+    if (s < 0) return cval;
+    const TB v = to_buf<TB>(base[static_cast<int64_t>(s) * st]);
+    return mul < 0 ? -v : v;
+}
+
+// ---------------------------------------------------------------------------
+// Window sources: at(q) reads sample q (q = 0: output p = 0's centre);
+// rbegin(q)/rnext() stream q, q+1, ...; lbegin(q)/lnext() stream q, q-1, ...
+// ---------------------------------------------------------------------------
+template <typename TB>
+struct LdsRowSrc {  // a staged row in LDS, padded index
+    const TB *row;
+    int base, ri, li;
+    __device__ TB at(int q) const { return row[lpad(base + q)]; }
+    __device__ void rbegin(int q) { ri = base + q; }
+    __device__ TB rnext() { return row[lpad(ri++)]; }
+    __device__ void lbegin(int q) { li = base + q; }
+    __device__ TB lnext() { return row[lpad(li--)]; }
+};
+
+template <typename TB>
+struct RingSrc {  // one column of the LDS ring; rows wrap modulo rc
+    const TB *col;  // ring + lpad(column)
+    int pitch, rc, span, r0, ro, lo;  // span = rc * pitch; r0 = row of q = 0
+    __device__ int wrap(int row) const {
+        row += row < 0 ? rc : 0;
+        row -= row >= rc ? rc : 0;
+        return row;
+    }
+    __device__ TB at(int q) const { return col[wrap(r0 + q) * pitch]; }
+    __device__ void rbegin(int q) { ro = wrap(r0 + q) * pitch; }
+    __device__ TB rnext() {
+        const TB v = col[ro];
+        ro += pitch;
+        ro = ro == span ? 0 : ro;
+        return v;
+    }
+    __device__ void lbegin(int q) { lo = wrap(r0 + q) * pitch; }
+    __device__ TB lnext() {
+        const TB v = col[lo];
+        lo = lo == 0 ? span - pitch : lo - pitch;
+        return v;
+    }
+};
+
+template <typename TI, typename TB>
+struct GColSrc {  // a column in global memory, all samples in range
+    const TI *p;
+    int64_t st;
+    const TI *rp, *lp;
+    __device__ TB at(int q) const { return to_buf<TB>(p[q * st]); }
+    __device__ void rbegin(int q) { rp = p + q * st; }
+    __device__ TB rnext() {
+        const TB v = to_buf<TB>(*rp);
+        rp += st;
+        return v;
+    }
+    __device__ void lbegin(int q) { lp = p + q * st; }
+    __device__ TB lnext() {
+        const TB v = to_buf<TB>(*lp);
+        lp -= st;
+        return v;
+    }
+};
+
+template <typename TI, typename TB>
+struct GColExtSrc {  // a column in global memory near its ends: boundary-extended
+    const TI *line;
+    int64_t st;
+    int z, n, bc;
+    TB cval;
+    int ri, li;
+    __device__ TB at(int q) const { return fetch_ext<TI, TB>(line, st, z + q, n, bc, cval); }
+    __device__ void rbegin(int q) { ri = z + q; }
+    __device__ TB rnext() { return fetch_ext<TI, TB>(line, st, ri++, n, bc, cval); }
+    __device__ void lbegin(int q) { li = z + q; }
+    __device__ TB lnext() { return fetch_ext<TI, TB>(line, st, li--, n, bc, cval); }
+};
+
+// P outputs from a window source.  `shift` moves the window: sample q is read as
+// q - shift (GENERAL: the reversed origin; the symmetric forms: origin - (ntaps-1),
+// zero for a centred filter).
+template <typename TB, int P, int SYM, bool FAST, typename Src>
+__device__ __forceinline__ void fir_window(const TB *taps, int ntaps, int shift, Src &s, TB (&acc)[P]) {
     if constexpr (SYM == SYM_EVEN || SYM == SYM_ODD) {
         const int r = ntaps - 1;
         TB rw[P], lw[P];
         TB w[P + 2];
 #pragma unroll
-        for (int q = 0; q < P + 2; ++q) w[q] = get(q - 1);
+        for (int q = 0; q < P + 2; ++q) w[q] = s.at(q - 1 - shift);
         const TB f0 = taps[0];
 #pragma unroll
         for (int p = 0; p < P; ++p) {
@@ -55,24 +179,37 @@ __device__ __forceinline__ void fir_window(const TB *taps, int ntaps, int shift,
             rw[(p + 1) % P] = w[p + 2];
             lw[(p - 1 + P) % P] = w[p];
         }
-        for (int kb = 1; kb <= r; kb += P) {
+        s.rbegin(P + 1 - shift);
+        s.lbegin(-2 - shift);
+        // tap step k (slot indices fold to constants once the u loops unroll)
+        auto step = [&](const int u, const int k) {
+            const TB f = taps[k];
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                const TB a = rw[(p + 1 + u) % P];
+                const TB b = lw[((p - 1 - u) % P + P) % P];
+                const TB t = (SYM == SYM_EVEN) ? (a + b) : (a - b);
+                acc[p] = mac<TB, FAST>(acc[p], f, t);
+            }
+        };
+        auto advance = [&](const int u) {
+            rw[(1 + u) % P] = s.rnext();
+            lw[((P - 2 - u) % P + P) % P] = s.lnext();
+        };
+        int kb = 1;
+        for (; kb + P <= r; kb += P) {  // whole blocks: every step also reads the next pair
 #pragma unroll
             for (int u = 0; u < P; ++u) {
-                const int k = kb + u;
-                if (k <= r) {
-                    const TB f = taps[k];
+                step(u, kb + u);
+                advance(u);
+            }
+        }
 #pragma unroll
-                    for (int p = 0; p < P; ++p) {
-                        const TB a = rw[(p + 1 + u) % P];
-                        const TB b = lw[((p - 1 - u) % P + P) % P];
-                        const TB t = (SYM == SYM_EVEN) ? (a + b) : (a - b);
-                        acc[p] = mac<TB, FAST>(acc[p], f, t);
-                    }
-                    if (k < r) {
-                        rw[(1 + u) % P] = get(k + P);
-                        lw[((P - 2 - u) % P + P) % P] = get(-k - 1);
-                    }
-                }
+        for (int u = 0; u < P; ++u) {  // the last (partial) block
+            const int k = kb + u;
+            if (k <= r) {
+                step(u, k);
+                if (k < r) advance(u);
             }
         }
     } else if constexpr (SYM == SYM_GENERAL) {
@@ -80,18 +217,28 @@ __device__ __forceinline__ void fir_window(const TB *taps, int ntaps, int shift,
 #pragma unroll
         for (int p = 0; p < P; ++p) {
             acc[p] = TB(0);
-            win[p] = get(p);
+            win[p] = s.at(p - shift);
         }
-        for (int kb = 0; kb < ntaps; kb += P) {
+        s.rbegin(P - shift);
+        auto step = [&](const int u, const int k) {
+            const TB f = taps[k];
+#pragma unroll
+            for (int p = 0; p < P; ++p) acc[p] = mac<TB, FAST>(acc[p], f, win[(p + u) % P]);
+        };
+        int kb = 0;
+        for (; kb + P < ntaps; kb += P) {
 #pragma unroll
             for (int u = 0; u < P; ++u) {
-                const int k = kb + u;
-                if (k < ntaps) {
-                    const TB f = taps[k];
+                step(u, kb + u);
+                win[u % P] = s.rnext();
+            }
+        }
 #pragma unroll
-                    for (int p = 0; p < P; ++p) acc[p] = mac<TB, FAST>(acc[p], f, win[(p + u) % P]);
-                    if (k + 1 < ntaps) win[u % P] = get(k + P);
-                }
+        for (int u = 0; u < P; ++u) {
+            const int k = kb + u;
+            if (k < ntaps) {
+                step(u, k);
+                if (k + 1 < ntaps) win[u % P] = s.rnext();
             }
         }
     } else {  // D_EVEN / D_ODD: s += f_k*(x[i+k] +- x[i-k-1]), k = 0..ntaps-1
@@ -99,43 +246,42 @@ __device__ __forceinline__ void fir_window(const TB *taps, int ntaps, int shift,
 #pragma unroll
         for (int p = 0; p < P; ++p) {
             acc[p] = TB(0);
-            rw[p] = get(p);
-            lw[(p - 1 + P) % P] = get(p - 1);
+            rw[p] = s.at(p - shift);
+            lw[(p - 1 + P) % P] = s.at(p - 1 - shift);
         }
-        for (int kb = 0; kb < ntaps; kb += P) {
+        s.rbegin(P - shift);
+        s.lbegin(-2 - shift);
+        auto step = [&](const int u, const int k) {
+            const TB f = taps[k];
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                const TB a = rw[(p + u) % P];
+                const TB b = lw[((p - 1 - u) % P + P) % P];
+                const TB t = (SYM == SYM_D_EVEN) ? (a + b) : (a - b);
+                acc[p] = mac<TB, FAST>(acc[p], f, t);
+            }
+        };
+        auto advance = [&](const int u) {
+            rw[u % P] = s.rnext();
+            lw[((P - 2 - u) % P + P) % P] = s.lnext();
+        };
+        int kb = 0;
+        for (; kb + P < ntaps; kb += P) {
 #pragma unroll
             for (int u = 0; u < P; ++u) {
-                const int k = kb + u;
-                if (k < ntaps) {
-                    const TB f = taps[k];
+                step(u, kb + u);
+                advance(u);
+            }
+        }
 #pragma unroll
-                    for (int p = 0; p < P; ++p) {
-                        const TB a = rw[(p + u) % P];
-                        const TB b = lw[((p - 1 - u) % P + P) % P];
-                        const TB t = (SYM == SYM_D_EVEN) ? (a + b) : (a - b);
-                        acc[p] = mac<TB, FAST>(acc[p], f, t);
-                    }
-                    if (k + 1 < ntaps) {
-                        rw[u % P] = get(k + P);
-                        lw[((P - 2 - u) % P + P) % P] = get(-k - 2);
-                    }
-                }
+        for (int u = 0; u < P; ++u) {
+            const int k = kb + u;
+            if (k < ntaps) {
+                step(u, k);
+                if (k + 1 < ntaps) advance(u);
             }
         }
     }
-}
-
-// Sample j of a line of n samples starting at `src` with stride `st`, extended by
-// the boundary condition.
-template <typename TI, typename TB>
-__device__ __forceinline__ TB load_ext(const TI *src, int64_t st, int64_t j, int64_t n, int bc, TB cval) {
-    if (j >= 0 && j < n) return to_buf<TB>(src[j * st]);
-    int mul;
-    bool isc;
-    int64_t s = bc_map(j, n, bc, &mul, &isc);
-    if (isc) return cval;
-    TB v = to_buf<TB>(src[s * st]);
-    return mul < 0 ? -v : v;
 }
 
 // ---------------------------------------------------------------------------
@@ -154,43 +300,49 @@ __global__ __launch_bounds__(kBlock) void k_line(const PassArgs<TB> a, int64_t n
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t seg = blockIdx.x % nseg;
     const int64_t line = (blockIdx.x / nseg) * 4 + wave;
-    const int64_t x0 = seg * TX;
+    const int x0 = static_cast<int>(seg) * TX;
+    const int n = static_cast<int>(a.n);
     TB *row = lds + wave * pitch;
     int64_t oin = 0, oout = 0;
     const bool active = line < a.nlines;
     if (active) {
         outer_offsets(a.outer, line, &oin, &oout);
         const TI *src = static_cast<const TI *>(a.in) + oin;
-        const bool interior = (x0 - B >= 0) && (x0 + TX + B <= a.n);
-        if (interior) {
-            const TI *s0 = src + (x0 - B) * a.sin;
-            for (int i = lane; i < W; i += 64) row[lpad(i)] = to_buf<TB>(s0[i * a.sin]);
+        const bool interior = (x0 - B >= 0) && (x0 + TX + B <= n);
+        if (interior && a.sin == 1) {
+            const TI *s0 = src + (x0 - B);
+            for (int i = lane; i < W; i += 64) row[lpad(i)] = to_buf<TB>(s0[i]);
         } else {
-            for (int i = lane; i < W; i += 64) row[lpad(i)] = load_ext<TI, TB>(src, a.sin, x0 - B + i, a.n, a.bc, a.cval);
+            for (int i = lane; i < W; i += 64) row[lpad(i)] = fetch_ext<TI, TB>(src, a.sin, x0 - B + i, n, a.bc, a.cval);
         }
     }
     __syncthreads();
     if (!active) return;
-    const int base = B + lane * P;
+    LdsRowSrc<TB> s{row, B + lane * P, 0, 0};
     TB acc[P];
-    fir_window<TB, P, SYM, FAST>(a.taps, a.ntaps, a.shift, [&](int q) { return row[lpad(base + q)]; }, acc);
+    fir_window<TB, P, SYM, FAST>(a.taps, a.ntaps, a.shift, s, acc);
     TB *dst = static_cast<TB *>(a.out) + oout;
-    const int64_t xs = x0 + lane * P;
-    if (a.sout == 1 && xs + P <= a.n) {
+    const int xs = x0 + lane * P;
+    if (a.sout == 1 && xs + P <= n) {
+        TB *d = dst + xs;
 #pragma unroll
-        for (int p = 0; p < P; ++p) dst[xs + p] = acc[p];
+        for (int p = 0; p < P; ++p) d[p] = acc[p];
     } else {
+        TB *d = dst + static_cast<int64_t>(xs) * a.sout;
 #pragma unroll
-        for (int p = 0; p < P; ++p)
-            if (xs + p < a.n) dst[(xs + p) * a.sout] = acc[p];
+        for (int p = 0; p < P; ++p) {
+            if (xs + p < n) *d = acc[p];
+            d += a.sout;
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
 // K2: sweep along a strided axis.  One thread per position of the lane axis
 // (coalesced across the wave), each running down a chunk of the filter axis P
-// outputs at a time with the register window reading global memory (the window
-// rows are re-read from L1/L2, HBM sees each row once).  No LDS, no barriers.
+// outputs at a time with the register window streaming from global memory (the
+// window rows are re-read from L1/L2; HBM sees each row once).  No LDS, no
+// barriers.
 // ---------------------------------------------------------------------------
 template <typename TI, typename TB, int P, int SYM, bool FAST>
 __global__ __launch_bounds__(kBlock) void k_sweep(const PassArgs<TB> a, int64_t ntiles, int64_t nchunks) {
@@ -205,25 +357,106 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const PassArgs<TB> a, int64_t 
     outer_offsets(a.outer, outer, &oin, &oout);
     const TI *src = static_cast<const TI *>(a.in) + oin + c * a.ein;
     TB *dst = static_cast<TB *>(a.out) + oout + c * a.eout;
-    const int64_t z0 = chunk * a.chunk;
-    const int64_t zend = min(z0 + a.chunk, a.n);
-    const int64_t n = a.n;
+    const int n = static_cast<int>(a.n);
+    const int z0 = static_cast<int>(chunk * a.chunk);
+    const int zend = static_cast<int>(min(chunk * a.chunk + a.chunk, a.n));
     const int64_t sin = a.sin;
-    for (int64_t zb = z0; zb < zend; zb += P) {
+    for (int zb = z0; zb < zend; zb += P) {
         TB acc[P];
-        const bool interior = (zb - a.border >= 0) && (zb + P + a.border <= n);
-        if (interior) {
-            const TI *s0 = src + zb * sin;
-            fir_window<TB, P, SYM, FAST>(a.taps, a.ntaps, a.shift,
-                                         [&](int q) { return to_buf<TB>(s0[q * sin]); }, acc);
+        if (zb - a.border >= 0 && zb + P + a.border <= n) {
+            GColSrc<TI, TB> s{src + zb * sin, sin, nullptr, nullptr};
+            fir_window<TB, P, SYM, FAST>(a.taps, a.ntaps, a.shift, s, acc);
         } else {
-            fir_window<TB, P, SYM, FAST>(a.taps, a.ntaps, a.shift,
-                                         [&](int q) { return load_ext<TI, TB>(src, sin, zb + q, n, a.bc, a.cval); },
-                                         acc);
+            GColExtSrc<TI, TB> s{src, sin, zb, n, a.bc, a.cval, 0, 0};
+            fir_window<TB, P, SYM, FAST>(a.taps, a.ntaps, a.shift, s, acc);
+        }
+        TB *d = dst + static_cast<int64_t>(zb) * a.sout;
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            if (zb + p < zend) *d = acc[p];
+            d += a.sout;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K2T: column-tile sweep along a strided axis.  A workgroup stages a block of 64
+// lane-axis positions x (R + 2*border) filter-axis samples in LDS, column-major
+// with an odd pitch (lanes = columns hit distinct banks), then each thread runs
+// the window down its column for R/4 outputs, P at a time, reading LDS at
+// compile-time offsets.  One barrier per block; stores are coalesced along the
+// lane axis.  Used for long filters, where the ring of the fused kernel is large.
+// ---------------------------------------------------------------------------
+template <typename TB>
+struct LdsColSrc {  // one staged column in LDS, unit stride
+    const TB *col;
+    int base, ri, li;
+    __device__ TB at(int q) const { return col[base + q]; }
+    __device__ void rbegin(int q) { ri = base + q; }
+    __device__ TB rnext() { return col[ri++]; }
+    __device__ void lbegin(int q) { li = base + q; }
+    __device__ TB lnext() { return col[li--]; }
+};
+
+template <typename TI, typename TB, int P, int SYM, bool FAST>
+__global__ __launch_bounds__(kBlock) void k_colsweep(const PassArgs<TB> a, int64_t ntiles, int64_t nchunks) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    TB *lds = reinterpret_cast<TB *>(smem_raw);
+    const int64_t bid = blockIdx.x;
+    const int64_t tile = bid % ntiles;
+    const int64_t rest = bid / ntiles;
+    const int64_t chunk = rest % nchunks;
+    const int64_t outer = rest / nchunks;
+    int64_t oin, oout;
+    outer_offsets(a.outer, outer, &oin, &oout);
+    const int B = a.border;
+    const int R = static_cast<int>(a.chunk);  // outputs per column in this block (multiple of 4*P)
+    const int H = R + 2 * B;                   // staged samples per column
+    const int cpitch = H | 1;
+    const int n = static_cast<int>(a.n);
+    const int z0 = static_cast<int>(chunk) * R;
+    const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    const int64_t c = tile * 64 + lane;
+    const bool cok = c < a.e;
+    const TI *src = static_cast<const TI *>(a.in) + oin + (cok ? c : 0) * a.ein;
+    TB *col = lds + lane * cpitch;
+    // stage: wave `grp` loads samples grp, grp+4, ... of every column (rows are
+    // coalesced across the lanes), eight loads in flight before their LDS stores
+    const bool interior = z0 - B >= 0 && z0 + R + B <= n;
+    for (int h0 = grp; h0 < H; h0 += 32) {
+        TB v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int h = h0 + 4 * i;
+            if (h < H) {
+                const int j = z0 - B + h;
+                v[i] = interior ? to_buf<TB>(src[static_cast<int64_t>(j) * a.sin])
+                                : fetch_ext<TI, TB>(src, a.sin, j, n, a.bc, a.cval);
+            }
         }
 #pragma unroll
-        for (int p = 0; p < P; ++p)
-            if (zb + p < zend) dst[(zb + p) * a.sout] = acc[p];
+        for (int i = 0; i < 8; ++i) {
+            const int h = h0 + 4 * i;
+            if (h < H) col[h] = v[i];
+        }
+    }
+    __syncthreads();
+    if (!cok) return;
+    TB *dst = static_cast<TB *>(a.out) + oout + c * a.eout;
+    const int per = R / 4;
+    const int zend = min(z0 + R, n);
+    for (int blk = 0; blk < per; blk += P) {
+        const int zs = z0 + grp * per + blk;
+        if (zs >= zend) break;
+        LdsColSrc<TB> s{col, B + grp * per + blk, 0, 0};
+        TB acc[P];
+        fir_window<TB, P, SYM, FAST>(a.taps, a.ntaps, a.shift, s, acc);
+        TB *d = dst + static_cast<int64_t>(zs) * a.sout;
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            if (zs + p < zend) *d = acc[p];
+            d += a.sout;
+        }
     }
 }
 
@@ -236,22 +469,28 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const PassArgs<TB> a, int64_t 
 // in the buffer type, so the result equals the reference's two passes.
 //   threads: x stage = 16 rows x (TXC/P) lanes, P outputs each;
 //            y stage = TXC columns x (256/TXC) groups, P rows each;  S = 16.
-// Next step's input rows are prefetched into registers before the compute.
+// Next step's input rows are prefetched into registers during the compute.
+// Column offsets of the staged row (boundary-folded) are computed once per tile.
 // ---------------------------------------------------------------------------
-// One (column tile, row chunk) of a 2-D plane: the body of the fused kernel,
-// shared by the plain launch (one workgroup per tile) and the batched persistent
-// kernel (tiles pulled from a work queue).  `lds` holds S*pitchS staging words
-// followed by RC*pitchR ring words; `tx`/`ty` point at rx+1 / ry+1 taps.
 template <typename TB>
 struct Tile {
     const void *in;   // plane origin
     TB *out;
-    int64_t nx, ny;
+    int nx, ny;
     int64_t sxin, syin, sxout, syout;
-    int64_t c0, z0, zend;  // first column, first and end output rows
+    int c0, z0, zend;  // first column, first and end output rows
     int rx, ry, bcx, bcy;
     TB cvalx, cvaly;
 };
+
+// LDS row pitch (words) of a staged or ring row: padded, and congruent to P/2
+// modulo P.  Sixteen lanes reading with a stride of P words cover four runs of
+// P/2... banks; a row offset of P/2 (mod P) moves the next row onto the runs in between,
+// so the two rows a half-wave reads hit disjoint banks.
+__host__ __device__ constexpr int fused_pitch(int n, int P) {
+    return ((lpad_size(n) - P / 2 + P - 1) / P) * P + P / 2;
+}
+__host__ __device__ constexpr int fused_ring_rows(int ry) { return ((2 * ry + 15) / 16 + 1) * 16; }
 
 template <typename TI, typename TB, int TXC, int P, int SX, int SY, bool FAST>
 __device__ __forceinline__ void fused_tile(const Tile<TB> &t, TB *lds, const TB *tx, const TB *ty) {
@@ -261,47 +500,57 @@ __device__ __forceinline__ void fused_tile(const Tile<TB> &t, TB *lds, const TB 
     constexpr int NLD = (TXC + 2 * (kFusedCap - 1) + 63) / 64;  // loads per staged row per lane (max)
     const int Bx = t.rx, By = t.ry;
     const int Wx = TXC + 2 * Bx;
-    const int pitchS = lpad_size(Wx);
-    const int pitchR = lpad_size(TXC);
+    const int pitchS = fused_pitch(Wx, P);
+    const int pitchR = fused_pitch(TXC, P);
     const int A = (2 * By + S - 1) / S;
     const int RC = (A + 1) * S;
     TB *stg = lds;
     TB *ring = lds + S * pitchS;
     const TI *src = static_cast<const TI *>(t.in);
-    TB *dst = t.out;
-    const int64_t c0 = t.c0, z0 = t.z0, zend = t.zend;
-    const int nsteps = static_cast<int>((zend - z0 + S - 1) / S);
+    const int c0 = t.c0, z0 = t.z0, zend = t.zend;
+    const int nsteps = (zend - z0 + S - 1) / S;
     const int M = A + nsteps;
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
-    const bool xinterior = (c0 - Bx >= 0) && (c0 + TXC + Bx <= t.nx);
     const int nld = (Wx + 63) / 64;
+
+    // boundary-folded column offsets of this lane's staged samples (fixed per tile)
+    int cofs[NLD];
+    int cmul[NLD];  // +1, -1, or 0 for the constant condition
+#pragma unroll
+    for (int i = 0; i < NLD; ++i) {
+        const int col = lane + 64 * i;
+        int x = c0 - Bx + col;
+        int m = 1;
+        if (col < Wx && (x < 0 || x >= t.nx)) {
+            x = bc_fold(x, t.nx, t.bcx, &m);
+            if (x < 0) {
+                x = 0;
+                m = 0;
+            }
+        }
+        if (col >= Wx) x = 0;
+        cofs[i] = x * static_cast<int>(t.sxin);
+        cmul[i] = m;
+    }
 
     // --- staged-row loads: wave w loads rows w, w+4, w+8, w+12 of each step ---
     TB pre[4][NLD];
     auto load_step = [&](int m) {
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
-            const int64_t j = z0 - By + static_cast<int64_t>(m) * S + wave + 4 * rr;  // extended row
-            int mul = 1;
-            bool isc = false;
-            int64_t sr = j;
-            if (j < 0 || j >= t.ny) sr = bc_map(j, t.ny, t.bcy, &mul, &isc);
-            const TI *srow = src + sr * t.syin;
-            if (isc) continue;  // constant row: filled after the x stage
-            if (xinterior) {
-                const TI *s0 = srow + (c0 - Bx) * t.sxin;
+            int j = z0 - By + m * S + wave + 4 * rr;  // extended row (wave-uniform)
+            if (j < 0 || j >= t.ny) {
+                int mul;
+                j = bc_fold(j, t.ny, t.bcy, &mul);  // sign / constant applied after the x stage
+                if (j < 0) j = 0;
+            }
+            const TI *rowp = src + static_cast<int64_t>(j) * t.syin;
 #pragma unroll
-                for (int i = 0; i < NLD; ++i) {
-                    const int col = lane + 64 * i;
-                    if (i < nld && col < Wx) pre[rr][i] = to_buf<TB>(s0[col * t.sxin]);
-                }
-            } else {
-#pragma unroll
-                for (int i = 0; i < NLD; ++i) {
-                    const int col = lane + 64 * i;
-                    if (i < nld && col < Wx)
-                        pre[rr][i] = load_ext<TI, TB>(srow, t.sxin, c0 - Bx + col, t.nx, t.bcx, t.cvalx);
+            for (int i = 0; i < NLD; ++i) {
+                if (i < nld) {
+                    const TB v = to_buf<TB>(rowp[cofs[i]]);
+                    pre[rr][i] = cmul[i] > 0 ? v : (cmul[i] < 0 ? -v : t.cvalx);
                 }
             }
         }
@@ -325,19 +574,23 @@ __device__ __forceinline__ void fused_tile(const Tile<TB> &t, TB *lds, const TB 
         // --- x stage: row xr of this step, outputs xl*P .. xl*P+P-1 ---
         {
             const int xr = tid / XT, xl = tid % XT;
-            const int64_t j = z0 - By + static_cast<int64_t>(m) * S + xr;
+            const int j = z0 - By + m * S + xr;
             int mul = 1;
             bool isc = false;
-            if (j < 0 || j >= t.ny) bc_map(j, t.ny, t.bcy, &mul, &isc);
+            if (j < 0 || j >= t.ny) isc = bc_fold(j, t.ny, t.bcy, &mul) < 0;
             TB *rrow = ring + ((m * S + xr) % RC) * pitchR;
             if (isc) {
 #pragma unroll
                 for (int p = 0; p < P; ++p) rrow[lpad(xl * P + p)] = t.cvaly;
             } else {
-                const TB *srow = stg + xr * pitchS;
-                const int base = Bx + xl * P;
+                LdsRowSrc<TB> s{stg + xr * pitchS, Bx + xl * P, 0, 0};
                 TB acc[P];
-                fir_window<TB, P, SX, FAST>(tx, Bx + 1, 0, [&](int q) { return srow[lpad(base + q)]; }, acc);
+                if constexpr (SX == SYM_NONE) {
+#pragma unroll
+                    for (int p = 0; p < P; ++p) acc[p] = s.at(p);
+                } else {
+                    fir_window<TB, P, SX, FAST>(tx, Bx + 1, 0, s, acc);
+                }
 #pragma unroll
                 for (int p = 0; p < P; ++p) rrow[lpad(xl * P + p)] = mul < 0 ? -acc[p] : acc[p];
             }
@@ -348,16 +601,19 @@ __device__ __forceinline__ void fused_tile(const Tile<TB> &t, TB *lds, const TB 
         const int n = m - A;
         if (n >= 0) {
             const int col = tid % TXC, grp = tid / TXC;
-            const int64_t c = c0 + col;
+            const int c = c0 + col;
             const int rel0 = n * S + grp * P + By;  // ring-relative row of output p = 0
+            RingSrc<TB> s{ring + lpad(col), pitchR, RC, RC * pitchR, rel0 % RC, 0, 0};
             TB acc[P];
-            fir_window<TB, P, SY, FAST>(ty, By + 1, 0,
-                                        [&](int q) { return ring[((rel0 + q) % RC) * pitchR + lpad(col)]; }, acc);
-            const int64_t zs = z0 + static_cast<int64_t>(n) * S + grp * P;
+            fir_window<TB, P, SY, FAST>(ty, By + 1, 0, s, acc);
+            const int zs = z0 + n * S + grp * P;
             if (c < t.nx) {
+                TB *d = t.out + static_cast<int64_t>(zs) * t.syout + static_cast<int64_t>(c) * t.sxout;
 #pragma unroll
-                for (int p = 0; p < P; ++p)
-                    if (zs + p < zend) dst[(zs + p) * t.syout + c * t.sxout] = acc[p];
+                for (int p = 0; p < P; ++p) {
+                    if (zs + p < zend) *d = acc[p];
+                    d += t.syout;
+                }
             }
         }
     }
@@ -377,15 +633,15 @@ __global__ __launch_bounds__(kBlock) void k_fused2d(const FusedArgs<TB> a, int64
     Tile<TB> t;
     t.in = static_cast<const TI *>(a.in) + oin;
     t.out = static_cast<TB *>(a.out) + oout;
-    t.nx = a.nx;
-    t.ny = a.ny;
+    t.nx = static_cast<int>(a.nx);
+    t.ny = static_cast<int>(a.ny);
     t.sxin = a.sxin;
     t.syin = a.syin;
     t.sxout = a.sxout;
     t.syout = a.syout;
-    t.c0 = tile * TXC;
-    t.z0 = chunk * a.chunk;
-    t.zend = min(t.z0 + a.chunk, a.ny);
+    t.c0 = static_cast<int>(tile * TXC);
+    t.z0 = static_cast<int>(chunk * a.chunk);
+    t.zend = static_cast<int>(min(chunk * a.chunk + a.chunk, a.ny));
     t.rx = a.rx;
     t.ry = a.ry;
     t.bcx = a.bcx;
@@ -421,13 +677,13 @@ __global__ __launch_bounds__(kBlock) void k_batched(const BatchArgs<TB> a) {
         Tile<TB> t;
         t.in = im.in;
         t.out = im.out;
-        t.nx = im.nx;
-        t.ny = im.ny;
+        t.nx = static_cast<int>(im.nx);
+        t.ny = static_cast<int>(im.ny);
         t.sxin = 1;
         t.syin = im.syin;
         t.sxout = 1;
         t.syout = im.syout;
-        t.c0 = static_cast<int64_t>(it.tile) * TXC;
+        t.c0 = it.tile * TXC;
         t.z0 = it.z0;
         t.zend = it.z1;
         t.rx = im.rx;
