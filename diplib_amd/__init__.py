@@ -5,6 +5,7 @@ include/dipamd.h); this package is its PyDIP-style front end on PyTorch-ROCm
 device tensors.  Importing the package does not touch the GPU.
 """
 from ._lib import DipError, lib  # noqa: F401
+from .autograd import gauss_fir_autograd, gauss_fir_backward  # noqa: F401
 from .batch import GaussBatch  # noqa: F401
 from .gauss import (Gauss, GaussFIR, MakeHalfGaussian, OneDimensionalFilter,  # noqa: F401
                     SeparableConvolution, best_gauss_method, gauss_radius, image_struct)

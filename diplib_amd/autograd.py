@@ -1,0 +1,110 @@
+"""Differentiable dip::Gauss (FIR) on PyTorch-ROCm: a torch.autograd.Function whose
+forward is the library's exact drop-in and whose backward is the library's adjoint
+kernel path (C ABI: dipamd_gauss_fir_backward).
+
+    y = gauss_fir_autograd(x, sigma, orders=(0,), bc=("mirror",), truncation=3.0, sigma_dims=None)
+
+`sigma` is a tensor: a scalar (one sigma for the dims in `sigma_dims`, all dims by
+default; the other dims are not filtered) or one value per dip dim.  Dims are in
+dip order (dim 0 = the last tensor axis).  dL/dx is the exact adjoint of the
+boundary-extended separable convolution; dL/dsigma differentiates the normalised
+discrete taps with the radius held at its forward value (the radius is a step
+function of sigma, so the derivative is taken on the piece the forward used).
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+from ._lib import check, lib
+from .gauss import GaussFIR, _bc_codes, _flex_torch, _stream_handle, image_struct
+
+
+def _per_dim_sigmas(sigma, nd, sigma_dims):
+    s = sigma.detach().double().cpu().reshape(-1).tolist()
+    if len(s) == 1:
+        dims = range(nd) if sigma_dims is None else sigma_dims
+        out = [0.0] * nd
+        for d in dims:
+            out[d] = s[0]
+        return out
+    if len(s) != nd:
+        raise _lib.DipError(_lib.E_ARRAY_LENGTH, "sigma must have one value or one per dimension")
+    return s
+
+
+class _GaussFIRFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, sigma, orders, bc, truncation, sigma_dims, fast):
+        nd = x.dim()
+        sig = _per_dim_sigmas(sigma, nd, sigma_dims)
+        y = GaussFIR(x, sig, list(orders), list(bc), truncation, fast=fast)
+        ctx.save_for_backward(x, sigma)
+        ctx.cfg = (sig, list(orders), list(bc), float(truncation), sigma_dims, fast)
+        return y
+
+    @staticmethod
+    def backward(ctx, g):
+        x, sigma = ctx.saved_tensors
+        sig, orders, bc, truncation, sigma_dims, fast = ctx.cfg
+        nd = x.dim()
+        want_dx = ctx.needs_input_grad[0]
+        want_ds = ctx.needs_input_grad[1]
+        g = g.to(_flex_torch(x.dtype))
+        dx = torch.empty(x.shape, dtype=_flex_torch(x.dtype), device=x.device) if want_dx else None
+        ds = torch.zeros(nd, dtype=torch.float64, device=x.device) if want_ds else None
+        a_x, a_g = image_struct(x), image_struct(g)
+        a_dx = image_struct(dx) if want_dx else _lib.DipamdImage()
+        s_arr = (ctypes.c_double * nd)(*sig)
+        o_arr = (ctypes.c_int32 * max(1, len(orders)))(*orders)
+        bcs = _bc_codes(bc)
+        b_arr = (ctypes.c_int32 * max(1, len(bcs)))(*bcs)
+        with torch.cuda.device(x.device):
+            check(lib().dipamd_gauss_fir_backward(
+                ctypes.byref(a_x), ctypes.byref(a_g), ctypes.byref(a_dx),
+                ctypes.c_void_p(ds.data_ptr()) if want_ds else None, s_arr, nd, o_arr, len(orders), b_arr, len(bcs),
+                truncation, _lib.FAST if fast else _lib.EXACT, _stream_handle(x.device)))
+        dsigma = None
+        if want_ds:
+            if sigma.numel() == 1:
+                dims = range(nd) if sigma_dims is None else sigma_dims
+                dsigma = ds[list(dims)].sum().to(sigma.dtype).reshape(sigma.shape)
+            else:
+                dsigma = ds.to(sigma.dtype).reshape(sigma.shape)
+        if want_dx:
+            dx = dx.to(x.dtype)
+        return dx, dsigma, None, None, None, None, None
+
+
+def gauss_fir_autograd(x, sigma, orders=(0,), bc=("mirror",), truncation=3.0, sigma_dims=None, fast=False):
+    """Differentiable GaussFIR: gradients with respect to `x` (float tensors) and `sigma`."""
+    if not torch.is_tensor(sigma):
+        sigma = torch.tensor(float(sigma), dtype=torch.float64, device=x.device)
+    orders = [int(o) for o in (orders if hasattr(orders, "__len__") else [orders])]
+    bc = [bc] if isinstance(bc, str) else list(bc)
+    dims = None if sigma_dims is None else tuple(int(d) for d in sigma_dims)
+    return _GaussFIRFunction.apply(x, sigma, tuple(orders), tuple(bc), float(truncation), dims, bool(fast))
+
+
+def gauss_fir_backward(x, grad_out, sigmas, orders=(0,), bc=("mirror",), truncation=3.0, fast=False,
+                       want_dx=True, want_dsigma=True):
+    """Direct call of the backward kernels: returns (dL/dx, dL/dsigma per dip dim)."""
+    nd = x.dim()
+    sig = [float(s) for s in (sigmas if hasattr(sigmas, "__len__") else [sigmas])]
+    if len(sig) == 1:
+        sig = sig * nd
+    orders = [int(o) for o in (orders if hasattr(orders, "__len__") else [orders])]
+    bcs = _bc_codes(bc)
+    dx = torch.empty(x.shape, dtype=_flex_torch(x.dtype), device=x.device) if want_dx else None
+    ds = torch.zeros(nd, dtype=torch.float64, device=x.device) if want_dsigma else None
+    a_x, a_g = image_struct(x), image_struct(grad_out)
+    a_dx = image_struct(dx) if want_dx else _lib.DipamdImage()
+    s_arr = (ctypes.c_double * nd)(*sig)
+    o_arr = (ctypes.c_int32 * max(1, len(orders)))(*orders)
+    b_arr = (ctypes.c_int32 * max(1, len(bcs)))(*bcs)
+    with torch.cuda.device(x.device):
+        check(lib().dipamd_gauss_fir_backward(
+            ctypes.byref(a_x), ctypes.byref(a_g), ctypes.byref(a_dx),
+            ctypes.c_void_p(ds.data_ptr()) if want_dsigma else None, s_arr, nd, o_arr, len(orders), b_arr, len(bcs),
+            float(truncation), _lib.FAST if fast else _lib.EXACT, _stream_handle(x.device)))
+    return dx, ds

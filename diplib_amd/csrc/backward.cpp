@@ -1,0 +1,269 @@
+// Backward form of the Gaussian FIR path: the exact adjoint of the boundary-
+// extended separable convolution and the derivative with respect to each sigma.
+//
+// With the passes in forward order i = 1..m (dims d_1..d_m, ascending input stride
+// as Framework::Separable runs them, framework_separable.cpp:146-163):
+//     v_0 = x,  v_i = K_i v_{i-1}            (forward intermediates)
+//     u_m = g,  u_{i-1} = K_i^T u_i          (adjoint chain; u_0 = dL/dx)
+//     dL/dsigma_{d_i} = < u_i , K'_i v_{i-1} >
+// where K'_i filters with d(taps)/d(sigma) of the normalised discrete half Gaussian
+// at the radius the forward pass used (gauss_taps.hpp), and K_i^T is
+//   - periodic / asym periodic: the same boundary condition with reversed taps;
+//   - mirror / asym mirror / zero order: reversed taps on the zero-padded signal
+//     plus the halo terms folded back through the boundary map (k_adjoint_fold);
+//   - constant conditions: reversed taps on the zero-padded signal.
+// Reversed taps of an even filter are the filter itself, of an odd one its negative.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dipamd.h"
+#include "dipamd_internal.hpp"
+#include "gauss_taps.hpp"
+#include "sep_common.hpp"
+#include "sep_kernels.hpp"
+
+namespace dipamd {
+
+hipError_t launch_adjoint_fold(bool is_double, const void *args, int64_t nlines, hipStream_t stream);
+hipError_t launch_dot(bool is_double, const void *a, const void *geo, const void *b, double *partials, double *out,
+                      hipStream_t stream);
+hipError_t launch_fill_doubles(double *p, int n, double v, hipStream_t stream);
+
+namespace {
+
+bool contiguous(const View &v, int ndim) {
+    int64_t s = 1;
+    for (int d = 0; d < ndim; ++d) {
+        if (v.sizes[d] > 1 && v.strides[d] != s) return false;
+        s *= v.sizes[d];
+    }
+    return true;
+}
+
+template <typename TB>
+int fold_pass(const View &g, const View &out, int ndim, int d, const InternFilter &FT, int bc, hipStream_t stream) {
+    PassArgs<TB> a;
+    std::memset(&a, 0, sizeof(a));
+    a.in = g.p;
+    a.out = out.p;
+    a.n = g.sizes[d];
+    a.sin = g.strides[d];
+    a.sout = out.strides[d];
+    a.border = FT.border;
+    a.shift = FT.sym == DIPAMD_SYM_ODD ? 1 : 0;
+    a.bc = bc;
+    for (size_t k = 0; k < FT.data.size() && k < static_cast<size_t>(kLineCap); ++k) a.taps[k] = static_cast<TB>(FT.data[k]);
+    // lines: every other dim
+    int dims[DIPAMD_MAX_DIMS];
+    int n = 0;
+    int64_t nlines = 1;
+    for (int k = 0; k < ndim; ++k) {
+        if (k == d) continue;
+        nlines *= g.sizes[k];
+        if (g.sizes[k] > 1) dims[n++] = k;
+    }
+    a.outer.n = n;
+    for (int i = 0; i < kMaxOuter; ++i) {
+        a.outer.size[i] = 1;
+        a.outer.sin[i] = 0;
+        a.outer.sout[i] = 0;
+    }
+    for (int i = 0; i < n; ++i) {
+        a.outer.size[i] = g.sizes[dims[i]];
+        a.outer.sin[i] = g.strides[dims[i]];
+        a.outer.sout[i] = out.strides[dims[i]];
+    }
+    a.nlines = nlines;
+    TraceScope ts("adjoint_fold", 0, stream);
+    hipError_t e = launch_adjoint_fold(sizeof(TB) == 8, &a, nlines, stream);
+    if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("fold launch: ") + hipGetErrorString(e));
+    return DIPAMD_OK;
+}
+
+// dst = K^T src along dim d (K: filter F with boundary condition bc).
+int adjoint_pass(const View &src, const View &dst, int ndim, int d, const InternFilter &F, int bc, bool is_double,
+                 bool fast, hipStream_t stream) {
+    InternFilter FT = F;
+    if (F.sym == DIPAMD_SYM_ODD)
+        for (size_t k = 1; k < FT.data.size(); ++k) FT.data[k] = -FT.data[k];
+    const bool periodic = bc == DIPAMD_BC_PERIODIC || bc == DIPAMD_BC_ASYMMETRIC_PERIODIC;
+    const bool folds = bc == DIPAMD_BC_SYMMETRIC_MIRROR || bc == DIPAMD_BC_ASYMMETRIC_MIRROR ||
+                       bc == DIPAMD_BC_ZERO_ORDER_EXTRAPOLATE;
+    int st = run_pass(src, dst, ndim, d, FT, periodic ? bc : DIPAMD_BC_ADD_ZEROS, is_double, fast, stream);
+    if (st != DIPAMD_OK || !folds) return st;
+    return is_double ? fold_pass<double>(src, dst, ndim, d, FT, bc, stream)
+                     : fold_pass<float>(src, dst, ndim, d, FT, bc, stream);
+}
+
+}  // namespace
+}  // namespace dipamd
+
+using namespace dipamd;
+
+extern "C" int dipamd_gauss_fir_backward(const dipamd_image *x, const dipamd_image *grad_out,
+                                         const dipamd_image *grad_in, double *grad_sigma, const double *sigmas,
+                                         int32_t nsigmas, const int32_t *orders, int32_t norders, const int32_t *bc,
+                                         int32_t nbc, double truncation, uint32_t flags, void *stream_) {
+    hipStream_t stream = static_cast<hipStream_t>(stream_);
+    int st;
+    if ((st = check_image(x, "x")) != DIPAMD_OK) return st;
+    if ((st = check_image(grad_out, "grad_out")) != DIPAMD_OK) return st;
+    const int nd = x->ndim;
+    const bool want_dx = grad_in && grad_in->origin;
+    const bool want_ds = grad_sigma != nullptr;
+    if (want_dx && (st = check_image(grad_in, "grad_in")) != DIPAMD_OK) return st;
+    if (grad_out->ndim != nd || (want_dx && grad_in->ndim != nd))
+        return set_error(DIPAMD_E_SIZES, "Sizes don't match");
+    for (int d = 0; d < nd; ++d) {
+        if (grad_out->sizes[d] != x->sizes[d] || (want_dx && grad_in->sizes[d] != x->sizes[d]))
+            return set_error(DIPAMD_E_SIZES, "Sizes don't match");
+    }
+    if (want_dx && (views_overlap(to_view(grad_in), to_view(grad_out), nd) || views_overlap(to_view(grad_in), to_view(x), nd)))
+        return set_error(DIPAMD_E_PARAMETER, "grad_in must not overlap grad_out or x");
+    const int buf_dt = suggest_flex(x->dtype);
+    const bool is_double = buf_dt == DIPAMD_DT_DFLOAT;
+    if (grad_out->dtype != buf_dt || (want_dx && grad_in->dtype != buf_dt))
+        return set_error(DIPAMD_E_PARAMETER, "gradients must have the output type of the forward pass");
+    if (nsigmas != 1 && nsigmas != nd) return set_error(DIPAMD_E_ARRAY_LENGTH, "Array parameter has the wrong number of elements");
+    if (norders != 0 && norders != 1 && norders != nd)
+        return set_error(DIPAMD_E_ARRAY_LENGTH, "Array parameter has the wrong number of elements");
+    int bcs[DIPAMD_MAX_DIMS];
+    if ((st = broadcast_bc(bc, nbc, nd, bcs)) != DIPAMD_OK) return st;
+    const bool fast = (flags & DIPAMD_FAST) != 0;
+    const bool compute_double = x->dtype == DIPAMD_DT_DFLOAT;
+
+    // filters (GaussFIR, gauss.cpp:238-286) and their sigma derivatives
+    std::vector<InternFilter> F(static_cast<size_t>(nd)), dF(static_cast<size_t>(nd));
+    std::vector<int> order;
+    for (int d = 0; d < nd; ++d) {
+        const double s = nsigmas == 1 ? sigmas[0] : sigmas[d];
+        const int o = norders == 0 ? 0 : (norders == 1 ? orders[0] : orders[d]);
+        if (!(s > 0.0) || x->sizes[d] <= 1) continue;
+        if (o < 0 || o > 3) return set_error(DIPAMD_E_ORDER, "Gaussian FIR filter not implemented for order > 3");
+        if (!bc_supported(bcs[d])) return set_error(DIPAMD_E_UNSUPPORTED, "boundary condition not supported");
+        std::vector<double> h = half_gaussian(s, o, truncation, compute_double);
+        if (static_cast<int>(h.size()) - 1 > DIPAMD_MAX_RADIUS)
+            return set_error(DIPAMD_E_UNSUPPORTED, "Gaussian radius above DIPAMD_MAX_RADIUS");
+        const int sym = (o & 1) ? DIPAMD_SYM_ODD : DIPAMD_SYM_EVEN;
+        if ((st = intern_filter(h.data(), static_cast<int>(h.size()), sym, -1, F[d])) != DIPAMD_OK) return st;
+        std::vector<double> dh = half_gaussian_dsigma(s, o, static_cast<int>(h.size()));
+        if ((st = intern_filter(dh.data(), static_cast<int>(dh.size()), sym, -1, dF[d])) != DIPAMD_OK) return st;
+        if (meaningless(F[d], is_double)) continue;
+        order.push_back(d);
+    }
+    const View xv = to_view(x);
+    for (size_t i = 1; i < order.size(); ++i) {  // stable sort by input stride
+        const int e = order[i];
+        size_t j = i;
+        while (j > 0 && xv.strides[order[j - 1]] > xv.strides[e]) {
+            order[j] = order[j - 1];
+            --j;
+        }
+        order[j] = e;
+    }
+    const int m = static_cast<int>(order.size());
+
+    std::vector<void *> scratch;
+    auto alloc = [&](View *v) -> int {
+        int64_t n = 1;
+        for (int d = 0; d < nd; ++d) n *= x->sizes[d];
+        void *p = nullptr;
+        hipError_t e = scratch_alloc(&p, static_cast<size_t>(n) * dt_size(buf_dt), stream);
+        if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("scratch allocation: ") + hipGetErrorString(e));
+        scratch.push_back(p);
+        *v = contiguous_view(p, buf_dt, nd, xv.sizes);
+        return DIPAMD_OK;
+    };
+    auto finish = [&](int status) {
+        for (void *p : scratch) scratch_free(p, stream);
+        return status;
+    };
+
+    if (want_ds) {
+        hipError_t e = launch_fill_doubles(grad_sigma, nd, 0.0, stream);
+        if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("fill: ") + hipGetErrorString(e));
+    }
+    const View gv = to_view(grad_out);
+    if (m == 0) {
+        // identity forward: dL/dx = g
+        if (want_dx) return cast_from_buffer(gv, to_view(grad_in), nd, stream);
+        return DIPAMD_OK;
+    }
+
+    // the input as a kernel source (uint8 or the buffer type); others are converted
+    View x0 = xv;
+    if (!(x0.dt == DIPAMD_DT_UINT8 && !is_double) && x0.dt != buf_dt) {
+        View t;
+        if ((st = alloc(&t)) != DIPAMD_OK) return finish(st);
+        if ((st = cast_to_buffer(x0, t, nd, is_double, stream)) != DIPAMD_OK) return finish(st);
+        x0 = t;
+    }
+
+    // forward intermediates v_1..v_{m-1} (only needed for the sigma derivatives)
+    std::vector<View> v(static_cast<size_t>(m));
+    v[0] = x0;
+    if (want_ds) {
+        for (int i = 1; i < m; ++i) {
+            if ((st = alloc(&v[static_cast<size_t>(i)])) != DIPAMD_OK) return finish(st);
+            const int d = order[static_cast<size_t>(i - 1)];
+            if ((st = run_pass(v[static_cast<size_t>(i - 1)], v[static_cast<size_t>(i)], nd, d, F[d], bcs[d],
+                               is_double, fast, stream)) != DIPAMD_OK)
+                return finish(st);
+        }
+    }
+
+    // adjoint chain u_m = g, u_{i-1} = K_i^T u_i
+    std::vector<View> u(static_cast<size_t>(m + 1));
+    u[static_cast<size_t>(m)] = gv;
+    View w;
+    bool have_w = false;
+    double *partials = nullptr;
+    if (want_ds) {
+        hipError_t e = scratch_alloc(reinterpret_cast<void **>(&partials), 1024 * sizeof(double), stream);
+        if (e != hipSuccess) return finish(set_error(DIPAMD_E_HIP, "scratch allocation failed"));
+        scratch.push_back(partials);
+    }
+    for (int i = m; i >= 1; --i) {
+        const int d = order[static_cast<size_t>(i - 1)];
+        if (want_ds) {
+            // dL/dsigma_d = < u_i , K'_i v_{i-1} >
+            if (!have_w) {
+                if ((st = alloc(&w)) != DIPAMD_OK) return finish(st);
+                have_w = true;
+            }
+            if ((st = run_pass(v[static_cast<size_t>(i - 1)], w, nd, d, dF[d], bcs[d], is_double, fast, stream)) !=
+                DIPAMD_OK)
+                return finish(st);
+            const View &ui = u[static_cast<size_t>(i)];
+            CastGeo geo;
+            std::memset(&geo, 0, sizeof(geo));
+            geo.total = 1;
+            for (int k = 0; k < nd; ++k) geo.total *= ui.sizes[k];
+            if (!contiguous(ui, nd)) {
+                geo.ndim = nd;
+                for (int k = 0; k < nd; ++k) {
+                    geo.size[k] = ui.sizes[k];
+                    geo.sin[k] = ui.strides[k];
+                }
+            }
+            TraceScope ts("dot", geo.total * static_cast<int64_t>(2 * dt_size(buf_dt)), stream);
+            hipError_t e = launch_dot(is_double, ui.p, &geo, w.p, partials, grad_sigma + d, stream);
+            if (e != hipSuccess) return finish(set_error(DIPAMD_E_HIP, std::string("dot: ") + hipGetErrorString(e)));
+        }
+        if (i == 1 && !want_dx) break;
+        View dst;
+        if (i == 1) {
+            dst = to_view(grad_in);
+        } else if ((st = alloc(&dst)) != DIPAMD_OK) {
+            return finish(st);
+        }
+        if ((st = adjoint_pass(u[static_cast<size_t>(i)], dst, nd, d, F[d], bcs[d], is_double, fast, stream)) !=
+            DIPAMD_OK)
+            return finish(st);
+        u[static_cast<size_t>(i - 1)] = dst;
+    }
+    return finish(DIPAMD_OK);
+}

@@ -23,6 +23,7 @@
 
 #include "dipamd.h"
 #include "dipamd_internal.hpp"
+#include "gauss_taps.hpp"
 #include "sep_common.hpp"
 #include "sep_kernels.hpp"
 
@@ -118,84 +119,8 @@ int gauss_radius(double sigma, int order, double truncation, bool is_double) {
     return static_cast<int>(v);
 }
 
-// Half Gaussian of n = r+1 taps, x = 0 LAST (gauss.cpp:53-146).
-std::vector<double> half_gaussian_n(double sigma, int order, int n) {
-    std::vector<double> f(static_cast<size_t>(n));
-    const int r0 = n - 1;
-    const double sigma2 = sigma * sigma;
-    const double pi = 3.14159265358979323846264338327950288;
-    switch (order) {
-        case 0: {
-            const double factor = -0.5 / sigma2;
-            double normalization = 0;
-            f[r0] = 1.0;
-            for (int rr = 1; rr < n; rr++) {
-                const double rad = rr;
-                const double g = std::exp(factor * (rad * rad));
-                f[r0 - rr] = g;
-                normalization += g;
-            }
-            normalization = 1.0 / (normalization * 2 + 1);
-            for (int rr = 0; rr < n; rr++) f[rr] *= normalization;
-            break;
-        }
-        case 1: {
-            const double factor = -0.5 / sigma2;
-            double moment = 0.0;
-            f[r0] = 0.0;
-            for (int rr = 1; rr < n; rr++) {
-                const double rad = rr;
-                const double g = rad * std::exp(factor * (rad * rad));
-                f[r0 - rr] = g;
-                moment += rad * g;
-            }
-            const double normalization = 1.0 / (2.0 * moment);
-            for (int rr = 0; rr < n - 1; rr++) f[rr] *= normalization;
-            break;
-        }
-        case 2: {
-            const double norm = 1.0 / (std::sqrt(2.0 * pi) * sigma * sigma2);
-            double mean = 0.0;
-            f[r0] = -norm;
-            for (int rr = 1; rr < n; rr++) {
-                const double rad = rr;
-                const double sr2 = rad * rad / sigma2;
-                const double g = (sr2 - 1.0) * norm * std::exp(-0.5 * sr2);
-                f[r0 - rr] = g;
-                mean += g;
-            }
-            mean = (mean * 2.0 + f[r0]) / (static_cast<double>(r0) * 2.0 + 1.0);
-            f[r0] -= mean;
-            double moment = 0.0;
-            for (int rr = 1; rr < n; rr++) {
-                const double rad = rr;
-                f[r0 - rr] -= mean;
-                moment += rad * rad * f[r0 - rr];
-            }
-            const double normalization = 1.0 / moment;
-            for (int rr = 0; rr < n; rr++) f[rr] *= normalization;
-            break;
-        }
-        case 3: {
-            const double norm = 1.0 / (std::sqrt(2.0 * pi) * sigma * sigma2 * sigma2);
-            f[r0] = 0.0;
-            double moment = 0.0;
-            for (int rr = 1; rr < n; rr++) {
-                const double rad = rr;
-                const double rr2 = rad * rad;
-                const double sr2 = rr2 / sigma2;
-                const double g = norm * std::exp(-0.5 * sr2) * (rad * (3.0 - sr2));
-                f[r0 - rr] = g;
-                moment += g * rr2 * rad;
-            }
-            const double normalization = 3.0 / moment;
-            for (int rr = 0; rr < n; rr++) f[rr] *= normalization;
-            break;
-        }
-        default: break;
-    }
-    return f;
-}
+// Half Gaussian of n = r+1 taps, x = 0 LAST (gauss.cpp:53-146); see gauss_taps.hpp.
+std::vector<double> half_gaussian_n(double sigma, int order, int n) { return half_gaussian_t<double>(sigma, order, n); }
 
 // MakeHalfGaussian (gauss.cpp:151-165).
 std::vector<double> half_gaussian(double sigma, int order, double truncation, bool is_double) {
@@ -655,6 +580,21 @@ int separable_device(const View &in, const View &out, int ndim, const InternFilt
     if (st == DIPAMD_OK && !out_is_buf) st = run_cast_out(final_dst, out, ndim, stream);
     release();
     return st;
+}
+
+// One filter pass along dim d from `src` (buffer type or uint8) into `dst` (buffer type).
+int run_pass(const View &src, const View &dst, int ndim, int d, const InternFilter &F, int bc, bool is_double,
+             bool fast, hipStream_t stream) {
+    return is_double ? run_single<double>(src, dst, ndim, d, F, bc, fast, stream)
+                     : run_single<float>(src, dst, ndim, d, F, bc, fast, stream);
+}
+
+// Element-wise conversion passes (exposed for the backward form).
+int cast_to_buffer(const View &src, const View &dst, int ndim, bool to_double, hipStream_t stream) {
+    return run_cast_in(src, dst, ndim, to_double, stream);
+}
+int cast_from_buffer(const View &src, const View &dst, int ndim, hipStream_t stream) {
+    return run_cast_out(src, dst, ndim, stream);
 }
 
 // ---------------------------------------------------------------------------

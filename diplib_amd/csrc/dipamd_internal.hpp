@@ -66,6 +66,11 @@ void scratch_free(void *p, hipStream_t stream);
 // less); kernels are sized against it.
 size_t lds_limit_bytes();
 
+int run_pass(const View &src, const View &dst, int ndim, int d, const InternFilter &F, int bc, bool is_double,
+             bool fast, hipStream_t stream);
+int cast_to_buffer(const View &src, const View &dst, int ndim, bool to_double, hipStream_t stream);
+int cast_from_buffer(const View &src, const View &dst, int ndim, hipStream_t stream);
+
 int separable_device(const View &in, const View &out, int ndim, const InternFilter *const *filters, const int *bc,
                      const bool *process, uint32_t flags, hipStream_t stream);
 

@@ -159,3 +159,43 @@ def bc_map(j, n, bc):
     s = ctypes.c_int(0)
     idx = orc().orc_bc_map(j, n, BC[bc] if isinstance(bc, str) else bc, ctypes.byref(s))
     return idx, s.value
+
+
+def half_gaussian_n(sigma, order, n):
+    """Taps at a FIXED half size n (x = 0 last): the oracle's tap formula with the
+    radius frozen, for finite differences in sigma."""
+    buf = np.zeros(n, np.float64)
+    if orc().orc_half_gaussian_n(sigma, order, n, _ptr(buf)) != n:
+        raise ValueError("half_gaussian_n failed")
+    return buf
+
+
+def gauss_fixed_radius(x, sigmas, orders, bc, radii):
+    """dip::Gauss FIR in FP64 with the per-dim radii held fixed (dip dim order)."""
+    nd = x.ndim
+    taps, syms, proc = [], [], []
+    for d in range(nd):
+        s = sigmas[d]
+        o = orders[d]
+        if s > 0 and x.shape[nd - 1 - d] > 1:
+            taps.append(half_gaussian_n(s, o, radii[d] + 1))
+            syms.append("odd" if o % 2 else "even")
+            proc.append(1)
+        else:
+            taps.append(np.array([1.0]))
+            syms.append("general")
+            proc.append(0)
+    return separable(np.asarray(x, np.float64), taps, syms, bc=bc, process=proc)
+
+
+def jacobian_t(shape, sigmas, orders, bc, w, truncation=3.0):
+    """J^T w for dip::Gauss FIR (FP64), J assembled from impulse responses."""
+    n = int(np.prod(shape))
+    out = np.zeros(n)
+    e = np.zeros(shape)
+    ef = e.reshape(-1)
+    for j in range(n):
+        ef[j] = 1.0
+        out[j] = float((gauss_fir(e, sigmas, orders, bc, truncation) * w).sum())
+        ef[j] = 0.0
+    return out.reshape(shape)

@@ -1,0 +1,102 @@
+"""Backward form: dL/dx against the Jacobian assembled from the REFERENCE's impulse
+responses (tests/golden/jacobian.npz) and from the oracle's, and dL/dsigma against
+FP64 central differences of the oracle with the radius held fixed."""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle_ctypes as oc
+import diplib_amd as dip
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+JAC = np.load(os.path.join(HERE, "golden", "jacobian.npz"))
+JCASES = sorted({k.split("__")[0] for k in JAC.keys()})
+
+
+@pytest.mark.parametrize("name", JCASES)
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_grad_x_vs_reference_jacobian(cuda, name, dtype):
+    meta = json.loads(str(JAC[name + "__meta"]))
+    w = JAC[name + "__w"]
+    jtw = JAC[name + "__jtw"]
+    x = torch.rand(tuple(meta["shape"]), dtype=dtype, device=cuda)  # the map is linear in x
+    g = torch.from_numpy(w).to(cuda).to(dtype)
+    dx, _ = dip.gauss_fir_backward(x, g, meta["sigmas"], meta["orders"], [meta["bc"]], want_dsigma=False)
+    err = np.abs(dx.double().cpu().numpy() - jtw).max() / np.abs(jtw).max()
+    assert err < (1e-12 if dtype == torch.float64 else 2e-6), err
+
+
+SMALL = [((5, 7), [3.0, 2.0], [0, 1]), ((3, 9), [1.5, 2.5], [2, 0]), ((2, 2), [1.0, 1.0], [1, 1]),
+         ((11, 4), [0.7, 3.3], [0, 0]), ((4, 5, 3), [1.0, 2.0, 1.5], [0, 1, 0])]
+BCS = ["mirror", "periodic", "asym mirror", "asym periodic", "zero order", "add zeros"]
+
+
+@pytest.mark.parametrize("bc", BCS)
+@pytest.mark.parametrize("case", SMALL, ids=[str(c[0]) for c in SMALL])
+def test_grad_x_bounce_and_boundaries(cuda, bc, case):
+    shape, sig, ords = case
+    rng = np.random.default_rng(3)
+    w = rng.standard_normal(shape)
+    exp = oc.jacobian_t(shape, sig, ords, [bc], w)
+    x = torch.rand(shape, dtype=torch.float64, device=cuda)
+    dx, _ = dip.gauss_fir_backward(x, torch.from_numpy(w).to(cuda), sig, ords, [bc], want_dsigma=False)
+    np.testing.assert_allclose(dx.cpu().numpy(), exp, rtol=1e-11, atol=1e-12)
+
+
+@pytest.mark.parametrize("bc", ["mirror", "periodic", "asym mirror", "zero order"])
+@pytest.mark.parametrize("ords", [[0, 0], [1, 0], [0, 2], [3, 1]])
+def test_grad_sigma_vs_central_differences(cuda, bc, ords):
+    shape = (40, 52)
+    sig = [2.2, 3.1]
+    rng = np.random.default_rng(11)
+    x = rng.random(shape)
+    w = rng.standard_normal(shape)
+    _, ds = dip.gauss_fir_backward(torch.from_numpy(x).to(cuda), torch.from_numpy(w).to(cuda), sig, ords, [bc],
+                                   want_dx=False)
+    ds = ds.cpu().numpy()
+    radii = [oc.gauss_radius(s, o, 3.0, True) for s, o in zip(sig, ords)]
+    h = 1e-5
+    for d in range(2):
+        sp = list(sig)
+        sm = list(sig)
+        sp[d] += h
+        sm[d] -= h
+        lp = (oc.gauss_fixed_radius(x, sp, ords, [bc], radii) * w).sum()
+        lm = (oc.gauss_fixed_radius(x, sm, ords, [bc], radii) * w).sum()
+        fd = (lp - lm) / (2 * h)
+        assert abs(ds[d] - fd) <= 1e-6 * max(1.0, abs(fd)), (d, ds[d], fd)
+
+
+def test_autograd_gradcheck(cuda):
+    x = torch.rand((9, 13), dtype=torch.float64, device=cuda, requires_grad=True)
+    sigma = torch.tensor(1.9, dtype=torch.float64, device=cuda, requires_grad=True)  # radius 6 stays fixed under +-eps
+
+    def f(x_, s_):
+        return dip.gauss_fir_autograd(x_, s_, [1, 0], ["mirror"], 3.0)
+
+    assert torch.autograd.gradcheck(f, (x, sigma), eps=1e-6, atol=1e-7, rtol=1e-5)
+
+
+def test_autograd_batch_sigma_dims(cuda):
+    """A stack of 2-D images (dip dims x, y filtered, dim 2 = the stack) in FP32
+    against FP64: the FP32 gradient error is reported and bounded."""
+    g = torch.Generator(device=cuda)
+    g.manual_seed(0)
+    x = torch.rand((4, 200, 180), generator=g, device=cuda, dtype=torch.float32)
+    w = torch.randn((4, 200, 180), generator=g, device=cuda, dtype=torch.float32)
+    res = {}
+    for dt in (torch.float32, torch.float64):
+        xr = x.to(dt).clone().requires_grad_(True)
+        s = torch.tensor(3.0, dtype=torch.float64, device=cuda, requires_grad=True)
+        y = dip.gauss_fir_autograd(xr, s, [0], ["mirror"], 3.0, sigma_dims=(0, 1))
+        (y * w.to(dt)).sum().backward()
+        res[dt] = (xr.grad.double(), s.grad.double())
+        # the stack dim is not filtered: each image is independent
+    gx32, gs32 = res[torch.float32]
+    gx64, gs64 = res[torch.float64]
+    assert (gx32 - gx64).abs().max().item() / gx64.abs().max().item() < 1e-5
+    assert abs(gs32.item() - gs64.item()) / abs(gs64.item()) < 1e-5
