@@ -1,0 +1,393 @@
+/*
+ * diplib_amd.hpp -- C++ drop-in for DIPlib's Gaussian FIR path on MI355X.
+ *
+ * Keeps the reference signatures so existing callers compile unchanged:
+ *   dip::Gauss                 (include/diplib/linear.h:452-472)
+ *   dip::GaussFIR              (include/diplib/linear.h:300-318)
+ *   dip::SeparableConvolution  (include/diplib/linear.h:114-130)
+ *   dip::MakeHalfGaussian      (include/diplib/generation.h:638-653)
+ * over the subset of dip::Image the path uses (include/diplib/library/image.h):
+ * sizes, strides in samples (dim 0 fastest), data type, a shared data segment,
+ * Forge / ReForge / Strip / Protect, and the external-data constructor
+ * (image.h:458-491).  Pixel data live in DEVICE memory; Forge allocates with
+ * hipMalloc.  Calls run asynchronously on the stream set with dipamd::SetStream
+ * (the null stream by default), through the C ABI of include/dipamd.h.
+ *
+ * Output rules follow Framework::Separable (framework_separable.cpp:126): `out` is
+ * reforged to the input's sizes and SuggestFlex(in) type unless it is protected,
+ * in which case it keeps its type (written with clamp_cast) and must already have
+ * the input's sizes ("Image is protected").  `out` may be `in` (in-place).
+ * Errors are thrown as dip::ParameterError / dip::RunTimeError.
+ *
+ * Header-only C++14; link libdipamd.so and include ROCm's hip_runtime_api.h path.
+ */
+#ifndef DIPLIB_AMD_HPP
+#define DIPLIB_AMD_HPP
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "dipamd.h"
+
+namespace dipamd {
+inline hipStream_t &CurrentStream() {
+    thread_local hipStream_t s = nullptr;
+    return s;
+}
+/* Stream for the dip:: calls of this thread. */
+inline void SetStream(hipStream_t s) { CurrentStream() = s; }
+}  // namespace dipamd
+
+namespace dip {
+
+using uint = std::size_t;
+using sint = std::ptrdiff_t;
+using dfloat = double;
+using sfloat = float;
+using String = std::string;
+using StringArray = std::vector<String>;
+using FloatArray = std::vector<dfloat>;
+using UnsignedArray = std::vector<uint>;
+using IntegerArray = std::vector<sint>;
+using BooleanArray = std::vector<bool>;
+
+/* error.h:53-123 */
+class Error : public std::exception {
+  public:
+    explicit Error(std::string msg) : msg_(std::move(msg)) {}
+    const char *what() const noexcept override { return msg_.c_str(); }
+
+  private:
+    std::string msg_;
+};
+class ParameterError : public Error {
+  public:
+    using Error::Error;
+};
+class RunTimeError : public Error {
+  public:
+    using Error::Error;
+};
+
+/* The real sample types of datatype.h. */
+class DataType {
+  public:
+    enum class DT : int32_t {
+        UINT8 = DIPAMD_DT_UINT8, UINT16 = DIPAMD_DT_UINT16, UINT32 = DIPAMD_DT_UINT32, SINT8 = DIPAMD_DT_SINT8,
+        SINT16 = DIPAMD_DT_SINT16, SINT32 = DIPAMD_DT_SINT32, SFLOAT = DIPAMD_DT_SFLOAT, DFLOAT = DIPAMD_DT_DFLOAT,
+        BIN = DIPAMD_DT_BIN
+    };
+    constexpr DataType() : dt_(DT::SFLOAT) {}
+    constexpr DataType(DT dt) : dt_(dt) {}  // NOLINT
+    constexpr bool operator==(DataType o) const { return dt_ == o.dt_; }
+    constexpr bool operator!=(DataType o) const { return dt_ != o.dt_; }
+    constexpr int32_t Code() const { return static_cast<int32_t>(dt_); }
+    std::size_t SizeOf() const {
+        switch (dt_) {
+            case DT::UINT8: case DT::SINT8: case DT::BIN: return 1;
+            case DT::UINT16: case DT::SINT16: return 2;
+            case DT::DFLOAT: return 8;
+            default: return 4;
+        }
+    }
+    /* datatype.cpp:109-124 */
+    static DataType SuggestFlex(DataType t) {
+        return (t.dt_ == DT::UINT32 || t.dt_ == DT::SINT32 || t.dt_ == DT::DFLOAT) ? DataType(DT::DFLOAT)
+                                                                                  : DataType(DT::SFLOAT);
+    }
+
+  private:
+    DT dt_;
+};
+constexpr DataType DT_UINT8{DataType::DT::UINT8};
+constexpr DataType DT_UINT16{DataType::DT::UINT16};
+constexpr DataType DT_UINT32{DataType::DT::UINT32};
+constexpr DataType DT_SINT8{DataType::DT::SINT8};
+constexpr DataType DT_SINT16{DataType::DT::SINT16};
+constexpr DataType DT_SINT32{DataType::DT::SINT32};
+constexpr DataType DT_SFLOAT{DataType::DT::SFLOAT};
+constexpr DataType DT_DFLOAT{DataType::DT::DFLOAT};
+constexpr DataType DT_BIN{DataType::DT::BIN};
+
+using DataSegment = std::shared_ptr<void>;
+/* image.h:80-87: a shared pointer that does not own the data. */
+inline DataSegment NonOwnedRefToDataSegment(void *ptr) { return DataSegment(ptr, [](void *) {}); }
+inline DataSegment NonOwnedRefToDataSegment(void const *ptr) {
+    return DataSegment(const_cast<void *>(ptr), [](void *) {});
+}
+
+namespace Option {
+enum class AcceptDataTypeChange { DONT_ALLOW, DO_ALLOW };
+}
+
+/* The part of dip::Image the Gaussian path uses, with device-resident pixels. */
+class Image {
+  public:
+    Image() = default;
+    /* Forged image of the given sizes and type (normal strides), on the device. */
+    explicit Image(UnsignedArray sizes, uint tensorElems = 1, dip::DataType dt = DT_SFLOAT)
+        : dataType_(dt), sizes_(std::move(sizes)) {
+        if (tensorElems != 1) throw ParameterError("Tensor images are not supported by this library");
+        Forge();
+    }
+    /* Image around existing device data (image.h:458-491); no ownership if `data` says so. */
+    Image(DataSegment const &data, void *origin, dip::DataType dt, UnsignedArray sizes, IntegerArray strides = {})
+        : dataType_(dt), sizes_(std::move(sizes)), strides_(std::move(strides)), data_(data), origin_(origin) {
+        if (!data || !origin) throw ParameterError("Bad data pointer");
+        if (strides_.empty()) SetNormalStrides();
+        if (strides_.size() != sizes_.size()) throw ParameterError("Strides array size does not match image dimensionality");
+    }
+
+    bool IsForged() const { return origin_ != nullptr; }
+    uint Dimensionality() const { return sizes_.size(); }
+    UnsignedArray const &Sizes() const { return sizes_; }
+    uint Size(uint d) const { return sizes_.at(d); }
+    IntegerArray const &Strides() const { return strides_; }
+    sint Stride(uint d) const { return strides_.at(d); }
+    dip::DataType DataType() const { return dataType_; }
+    void *Origin() const { return origin_; }
+    uint NumberOfPixels() const {
+        uint n = 1;
+        for (uint s : sizes_) n *= s;
+        return n;
+    }
+    bool IsProtected() const { return protect_; }
+    bool Protect(bool set = true) {
+        bool old = protect_;
+        protect_ = set;
+        return old;
+    }
+    void SetSizes(UnsignedArray s) {
+        if (IsForged()) throw ParameterError("Image is forged");
+        sizes_ = std::move(s);
+    }
+    void SetDataType(dip::DataType dt) {
+        if (IsForged()) throw ParameterError("Image is forged");
+        dataType_ = dt;
+    }
+    /* Allocates device memory for the current sizes and type (normal strides). */
+    void Forge() {
+        if (IsForged()) return;
+        SetNormalStrides();
+        std::size_t bytes = NumberOfPixels() * dataType_.SizeOf();
+        void *p = nullptr;
+        if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) throw RunTimeError("hipMalloc failed");
+        data_ = DataSegment(p, [](void *q) { (void)hipFree(q); });
+        origin_ = p;
+    }
+    /* image.h ReForge: keep the data when sizes and type fit, else reallocate (or
+     * throw for a protected image; a protected image keeps its type). */
+    void ReForge(UnsignedArray const &sizes, uint tensorElems, dip::DataType dt,
+                 Option::AcceptDataTypeChange acceptDataTypeChange = Option::AcceptDataTypeChange::DONT_ALLOW) {
+        if (tensorElems != 1) throw ParameterError("Tensor images are not supported by this library");
+        if (IsForged()) {
+            if (sizes_ == sizes && (dataType_ == dt ||
+                                    (protect_ && acceptDataTypeChange == Option::AcceptDataTypeChange::DO_ALLOW)))
+                return;
+            if (protect_) throw ParameterError("Image is protected");
+            Strip();
+        }
+        sizes_ = sizes;
+        dataType_ = dt;
+        Forge();
+    }
+    void Strip() {
+        if (protect_) throw ParameterError("Image is protected");
+        data_.reset();
+        origin_ = nullptr;
+        strides_.clear();
+    }
+    /* Host <-> device copies (normal strides only). */
+    void CopyFromHost(void const *host) {
+        if (!IsForged()) throw ParameterError("Image is not forged");
+        if (hipMemcpy(origin_, host, NumberOfPixels() * dataType_.SizeOf(), hipMemcpyHostToDevice) != hipSuccess)
+            throw RunTimeError("hipMemcpy failed");
+    }
+    void CopyToHost(void *host) const {
+        if (!IsForged()) throw ParameterError("Image is not forged");
+        if (hipMemcpy(host, origin_, NumberOfPixels() * dataType_.SizeOf(), hipMemcpyDeviceToHost) != hipSuccess)
+            throw RunTimeError("hipMemcpy failed");
+    }
+    /* The C ABI view of this image. */
+    dipamd_image View() const {
+        dipamd_image v{};
+        v.origin = origin_;
+        v.dtype = dataType_.Code();
+        v.ndim = static_cast<int32_t>(sizes_.size());
+        for (uint d = 0; d < sizes_.size() && d < DIPAMD_MAX_DIMS; ++d) {
+            v.sizes[d] = static_cast<int64_t>(sizes_[d]);
+            v.strides[d] = static_cast<int64_t>(strides_[d]);
+        }
+        return v;
+    }
+
+  private:
+    void SetNormalStrides() {
+        strides_.resize(sizes_.size());
+        sint s = 1;
+        for (uint d = 0; d < sizes_.size(); ++d) {
+            strides_[d] = s;
+            s *= static_cast<sint>(sizes_[d]);
+        }
+    }
+
+    dip::DataType dataType_ = DT_SFLOAT;
+    UnsignedArray sizes_;
+    IntegerArray strides_;
+    DataSegment data_;
+    void *origin_ = nullptr;
+    bool protect_ = false;
+};
+
+/* linear.h:84-90 */
+struct OneDimensionalFilter {
+    std::vector<dfloat> filter;
+    sint origin = -1;
+    String symmetry;
+    bool isComplex = false;
+};
+using OneDimensionalFilterArray = std::vector<OneDimensionalFilter>;
+
+namespace detail {
+inline void Check(int status) {
+    if (status == DIPAMD_OK) return;
+    const char *m = dipamd_last_error();
+    if (status == DIPAMD_E_HIP) throw RunTimeError(m);
+    throw ParameterError(m);
+}
+inline int32_t BoundaryCode(String const &bc) {
+    static const char *names[] = {"mirror", "asym mirror", "antisym reflect", "periodic", "asym periodic",
+                                  "add zeros", "add max", "add min", "zero order", "first order",
+                                  "second order", "third order", "already expanded"};
+    if (bc.empty() || bc == "default") return DIPAMD_BC_SYMMETRIC_MIRROR;
+    for (int32_t i = 0; i < 13; ++i)
+        if (bc == names[i]) return i;
+    throw ParameterError("Boundary condition not recognized: " + bc);
+}
+inline std::vector<int32_t> BoundaryCodes(StringArray const &bc) {
+    std::vector<int32_t> out;
+    for (auto const &b : bc) out.push_back(BoundaryCode(b));
+    return out;
+}
+inline int32_t SymmetryCode(String const &s) {
+    if (s.empty() || s == "general") return DIPAMD_SYM_GENERAL;
+    if (s == "even") return DIPAMD_SYM_EVEN;
+    if (s == "odd") return DIPAMD_SYM_ODD;
+    if (s == "conj") return DIPAMD_SYM_CONJ;
+    if (s == "d-even") return DIPAMD_SYM_D_EVEN;
+    if (s == "d-odd") return DIPAMD_SYM_D_ODD;
+    if (s == "d-conj") return DIPAMD_SYM_D_CONJ;
+    throw ParameterError("Symmetry string not recognized: " + s);
+}
+/* ReForge `out` for the result of `in` (framework_separable.cpp:72-76, 126). */
+inline void PrepareOutput(Image const &in, Image &out) {
+    if (!in.IsForged()) throw ParameterError("Image is not forged");
+    const bool same = out.IsForged() && out.Origin() == in.Origin() && out.Sizes() == in.Sizes() &&
+                      out.Strides() == in.Strides() && out.DataType() == in.DataType();
+    if (same) {
+        if (out.DataType() != DataType::SuggestFlex(in.DataType()) && !out.IsProtected()) {
+            // in-place but the output type changes: a new data segment for out
+            Image fresh(in.Sizes(), 1, DataType::SuggestFlex(in.DataType()));
+            out = fresh;
+        }
+        return;
+    }
+    out.ReForge(in.Sizes(), 1, DataType::SuggestFlex(in.DataType()), Option::AcceptDataTypeChange::DO_ALLOW);
+}
+}  // namespace detail
+
+/* dip::SeparableConvolution (linear.h:114-130), real filters. */
+inline void SeparableConvolution(Image const &in, Image &out, OneDimensionalFilterArray const &filterArray,
+                                 StringArray const &boundaryCondition = {}, BooleanArray process = {}) {
+    if (!in.IsForged()) throw ParameterError("Image is not forged");
+    Image src = in;  // keeps the input's data alive if `out` is `in`
+    detail::PrepareOutput(in, out);
+    std::vector<dipamd_filter> f;
+    for (auto const &o : filterArray) {
+        if (o.isComplex) throw ParameterError("Complex filters are not supported by this library");
+        f.push_back(dipamd_filter{o.filter.data(), static_cast<int32_t>(o.filter.size()),
+                                  static_cast<int32_t>(o.origin), detail::SymmetryCode(o.symmetry), 0});
+    }
+    std::vector<int32_t> bc = detail::BoundaryCodes(boundaryCondition);
+    std::vector<uint8_t> proc(process.begin(), process.end());
+    dipamd_image a = src.View(), b = out.View();
+    detail::Check(dipamd_separable_convolution(&a, &b, f.data(), static_cast<int32_t>(f.size()), bc.data(),
+                                               static_cast<int32_t>(bc.size()), proc.empty() ? nullptr : proc.data(),
+                                               DIPAMD_EXACT, dipamd::CurrentStream()));
+}
+inline Image SeparableConvolution(Image const &in, OneDimensionalFilterArray const &filter,
+                                  StringArray const &boundaryCondition = {}, BooleanArray process = {}) {
+    Image out;
+    SeparableConvolution(in, out, filter, boundaryCondition, std::move(process));
+    return out;
+}
+
+/* dip::GaussFIR (linear.h:300-318). */
+inline void GaussFIR(Image const &in, Image &out, FloatArray sigmas = {1.0}, UnsignedArray derivativeOrder = {0},
+                     StringArray const &boundaryCondition = {}, dfloat truncation = 3) {
+    if (!in.IsForged()) throw ParameterError("Image is not forged");
+    Image src = in;
+    detail::PrepareOutput(in, out);
+    std::vector<int32_t> ord(derivativeOrder.begin(), derivativeOrder.end());
+    std::vector<int32_t> bc = detail::BoundaryCodes(boundaryCondition);
+    dipamd_image a = src.View(), b = out.View();
+    detail::Check(dipamd_gauss_fir(&a, &b, sigmas.data(), static_cast<int32_t>(sigmas.size()), ord.data(),
+                                   static_cast<int32_t>(ord.size()), bc.data(), static_cast<int32_t>(bc.size()),
+                                   truncation, DIPAMD_EXACT, dipamd::CurrentStream()));
+}
+inline Image GaussFIR(Image const &in, FloatArray sigmas = {1.0}, UnsignedArray derivativeOrder = {0},
+                      StringArray const &boundaryCondition = {}, dfloat truncation = 3) {
+    Image out;
+    GaussFIR(in, out, std::move(sigmas), std::move(derivativeOrder), boundaryCondition, truncation);
+    return out;
+}
+
+/* dip::Gauss (linear.h:452-472; dispatch derivative.cpp:35-86).  Only the FIR
+ * method is in this library: "best" resolving to FT or IIR throws. */
+inline void Gauss(Image const &in, Image &out, FloatArray sigmas = {1.0}, UnsignedArray derivativeOrder = {0},
+                  String const &c_method = "best", StringArray const &boundaryCondition = {}, dfloat truncation = 3) {
+    String method = c_method.substr(0, 5) == "gauss" ? c_method.substr(5) : c_method;
+    if (method == "best") {
+        method = "fir";
+        for (auto o : derivativeOrder)
+            if (o > 3) method = "ft";
+        for (auto s : sigmas)
+            if (s < 0.8 && s > 0.0) method = "ft";
+        if (method == "fir")
+            for (auto s : sigmas)
+                if (s > 10) method = "iir";
+    }
+    if (method == "FIR" || method == "fir") {
+        GaussFIR(in, out, std::move(sigmas), std::move(derivativeOrder), boundaryCondition, truncation);
+        return;
+    }
+    if (method == "FT" || method == "ft" || method == "IIR" || method == "iir")
+        throw ParameterError("dip::Gauss method \"" + method + "\" is not provided by this library (FIR only)");
+    throw ParameterError("Invalid flag: " + c_method);
+}
+inline Image Gauss(Image const &in, FloatArray sigmas = {1.0}, UnsignedArray derivativeOrder = {0},
+                   String const &method = "best", StringArray const &boundaryCondition = {}, dfloat truncation = 3) {
+    Image out;
+    Gauss(in, out, std::move(sigmas), std::move(derivativeOrder), method, boundaryCondition, truncation);
+    return out;
+}
+
+/* dip::MakeHalfGaussian (generation.h:638-653; gauss.cpp:151-165). */
+inline std::vector<dfloat> MakeHalfGaussian(dfloat sigma, uint derivativeOrder = 0, dfloat truncation = 3,
+                                            DataType dt = DT_DFLOAT) {
+    std::vector<dfloat> f(4096);
+    const int n = dipamd_make_half_gaussian(sigma, static_cast<int32_t>(derivativeOrder), truncation,
+                                            dt == DT_DFLOAT ? 1 : 0, f.data(), static_cast<int32_t>(f.size()));
+    if (n < 0) detail::Check(n);
+    f.resize(static_cast<std::size_t>(n));
+    return f;
+}
+
+}  // namespace dip
+
+#endif  // DIPLIB_AMD_HPP

@@ -1,0 +1,125 @@
+// Exercises the C++ drop-in (include/diplib_amd.hpp) the way a DIPlib caller
+// would: dip::Gauss / GaussFIR / SeparableConvolution on device images, value-
+// returning and in/out forms, a protected output, in-place, and the error paths.
+// Usage: test_dip_shim <in.f32> <W> <H> <expected.f32> <sigma>
+//   in.f32 / expected.f32: raw little-endian float32, W*H samples (dip order: x fastest).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <vector>
+
+#include "diplib_amd.hpp"
+
+static std::vector<float> read_f32(const char *path, size_t n) {
+    std::vector<float> v(n);
+    std::ifstream f(path, std::ios::binary);
+    f.read(reinterpret_cast<char *>(v.data()), static_cast<std::streamsize>(n * sizeof(float)));
+    if (!f) throw std::runtime_error(std::string("cannot read ") + path);
+    return v;
+}
+
+static int fails = 0;
+#define CHECK(cond, what)                                              \
+    do {                                                               \
+        if (!(cond)) {                                                 \
+            std::cout << "FAIL: " << what << std::endl;                \
+            ++fails;                                                   \
+        } else {                                                       \
+            std::cout << "ok: " << what << std::endl;                  \
+        }                                                              \
+    } while (0)
+
+int main(int argc, char **argv) {
+    if (argc < 6) {
+        std::cerr << "usage: test_dip_shim in.f32 W H expected.f32 sigma\n";
+        return 2;
+    }
+    const size_t W = std::stoul(argv[2]), H = std::stoul(argv[3]);
+    const double sigma = std::stod(argv[5]);
+    std::vector<float> in = read_f32(argv[1], W * H);
+    std::vector<float> exp = read_f32(argv[4], W * H);
+    std::vector<float> got(W * H);
+    try {
+        dip::Image img({W, H}, 1, dip::DT_SFLOAT);
+        img.CopyFromHost(in.data());
+        // value-returning form
+        dip::Image out = dip::Gauss(img, {sigma}, {0}, "best", {"mirror"}, 3);
+        (void)hipDeviceSynchronize();
+        out.CopyToHost(got.data());
+        CHECK(out.DataType() == dip::DT_SFLOAT && out.Sizes() == img.Sizes(), "output forged as sfloat, same sizes");
+        CHECK(std::memcmp(got.data(), exp.data(), W * H * sizeof(float)) == 0, "dip::Gauss bit-identical to the reference");
+        // in/out form into an existing output, GaussFIR
+        dip::Image out2;
+        dip::GaussFIR(img, out2, {sigma}, {0}, {"mirror"}, 3);
+        (void)hipDeviceSynchronize();
+        out2.CopyToHost(got.data());
+        CHECK(std::memcmp(got.data(), exp.data(), W * H * sizeof(float)) == 0, "dip::GaussFIR bit-identical");
+        // SeparableConvolution with the half Gaussian, even symmetry
+        std::vector<double> h = dip::MakeHalfGaussian(sigma, 0, 3, dip::DT_SFLOAT);
+        dip::OneDimensionalFilterArray fa(1);
+        fa[0].filter = h;
+        fa[0].symmetry = "even";
+        dip::Image out3 = dip::SeparableConvolution(img, fa, {"mirror"});
+        (void)hipDeviceSynchronize();
+        out3.CopyToHost(got.data());
+        CHECK(std::memcmp(got.data(), exp.data(), W * H * sizeof(float)) == 0,
+              "dip::SeparableConvolution(MakeHalfGaussian) bit-identical");
+        // in-place
+        dip::Image inplace({W, H}, 1, dip::DT_SFLOAT);
+        inplace.CopyFromHost(in.data());
+        dip::Gauss(inplace, inplace, {sigma}, {0}, "FIR", {"mirror"}, 3);
+        (void)hipDeviceSynchronize();
+        inplace.CopyToHost(got.data());
+        CHECK(std::memcmp(got.data(), exp.data(), W * H * sizeof(float)) == 0, "in-place bit-identical");
+        // protected output of another type keeps its type (clamp_cast)
+        dip::Image prot({W, H}, 1, dip::DT_DFLOAT);
+        prot.Protect();
+        dip::Gauss(img, prot, {sigma}, {0}, "FIR", {"mirror"}, 3);
+        (void)hipDeviceSynchronize();
+        std::vector<double> gotd(W * H);
+        prot.CopyToHost(gotd.data());
+        bool same = true;
+        for (size_t i = 0; i < W * H; ++i) same &= gotd[i] == static_cast<double>(exp[i]);
+        CHECK(prot.DataType() == dip::DT_DFLOAT && same, "protected dfloat output keeps its type");
+        // protected output of the wrong size throws
+        dip::Image small({W / 2 + 1, H}, 1, dip::DT_SFLOAT);
+        small.Protect();
+        bool threw = false;
+        try {
+            dip::Gauss(img, small, {sigma});
+        } catch (dip::ParameterError const &e) {
+            threw = std::string(e.what()).find("protected") != std::string::npos;
+        }
+        CHECK(threw, "protected output of the wrong size throws \"Image is protected\"");
+        // bad boundary condition and order > 3 throw ParameterError
+        threw = false;
+        try {
+            dip::Gauss(img, {sigma}, {0}, "FIR", {"nonsense"});
+        } catch (dip::ParameterError const &) {
+            threw = true;
+        }
+        CHECK(threw, "unknown boundary condition throws");
+        threw = false;
+        try {
+            dip::GaussFIR(img, {sigma}, {4});
+        } catch (dip::ParameterError const &) {
+            threw = true;
+        }
+        CHECK(threw, "order > 3 throws");
+        threw = false;
+        try {
+            dip::Image unforged;
+            dip::Gauss(unforged, {sigma});
+        } catch (dip::ParameterError const &) {
+            threw = true;
+        }
+        CHECK(threw, "unforged input throws");
+    } catch (std::exception const &e) {
+        std::cout << "FAIL: exception " << e.what() << std::endl;
+        return 1;
+    }
+    std::cout << (fails ? "FAILED" : "PASSED") << std::endl;
+    return fails ? 1 : 0;
+}

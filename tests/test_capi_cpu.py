@@ -106,3 +106,15 @@ def test_best_method_resolution():
     assert best_gauss_method([12.0], [0]) == "iir"
     assert best_gauss_method([2.0], [4]) == "ft"
     assert best_gauss_method([0.0, 2.0], [0]) == "fir"
+
+
+def test_cpp_drop_in_header_compiles(tmp_path):
+    """include/diplib_amd.hpp compiles as plain C++14 with g++ (no GPU needed)."""
+    import subprocess
+    src = tmp_path / "t.cpp"
+    src.write_text('#include "diplib_amd.hpp"\n'
+                   'int main() { dip::Image a; dip::Image b = dip::Gauss(a, {2.0}); (void)b; return 0; }\n')
+    p = subprocess.run(["g++", "-std=c++14", "-fsyntax-only", "-D__HIP_PLATFORM_AMD__", "-I",
+                        os.path.join(ROOT, "include"), "-I", "/opt/rocm/include", str(src)],
+                       capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr
