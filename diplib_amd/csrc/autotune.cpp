@@ -73,12 +73,20 @@ int fuse_max_radius() {
     return r;
 }
 
+bool stream2d_enabled() {
+    static bool on = [] {
+        const char *v = std::getenv("DIPAMD_STREAM2D");
+        return !(v && v[0] == '0');
+    }();
+    return on;
+}
+
 int ysweep_kind() {
     static int kind = [] {
         const char *v = std::getenv("DIPAMD_YSWEEP");
-        if (!v) return 0;
+        if (!v) return -1;
         std::string s(v);
-        return s == "ring" ? 1 : (s == "global" ? 2 : 0);
+        return s == "col" ? 0 : (s == "ring" ? 1 : (s == "global" ? 2 : -1));
     }();
     return kind;
 }

@@ -25,6 +25,7 @@
 #include "dipamd_internal.hpp"
 #include "gauss_taps.hpp"
 #include "sep_common.hpp"
+#include "sep_config.hpp"
 #include "sep_kernels.hpp"
 
 namespace dipamd {
@@ -329,7 +330,11 @@ static int run_single(const View &src, const View &dst, int ndim, int d, const I
         const bool ring_ok = (F.sym == DIPAMD_SYM_EVEN || F.sym == DIPAMD_SYM_ODD) && F.shift == 0 &&
                              F.border <= kFusedCap - 1 && src.strides[lane] == 1 &&
                              src.sizes[lane] < (int64_t(1) << 31) && src.sizes[d] < (int64_t(1) << 31);
-        const int kind = ysweep_kind();  // 0 column tiles (default), 1 LDS ring, 2 global sweep
+        // 0 column tiles, 1 LDS ring, 2 global sweep; by default the column tiles for
+        // long filters (VALU-bound: the LDS tile keeps the window loads cheap) and the
+        // global sweep for short ones (HBM-bound: no staging phase, full occupancy)
+        int kind = ysweep_kind();
+        if (kind < 0) kind = F.border >= kColsweepMinBorder ? 0 : 2;
         const bool lane_ok = src.strides[lane] == 1 && src.sizes[d] < (int64_t(1) << 31);
         const int rows = colsweep_rows(F.border, is_double);
         if (kind == 0 && lane_ok && rows > 0) {
@@ -402,6 +407,59 @@ static int run_fused(const View &src, const View &dst, int ndim, int dx, int dy,
                   npx_of(src, ndim) * static_cast<int64_t>(dt_size(src.dt) + sizeof(TB)), stream);
     hipError_t err = launch_fused(ti, sizeof(TB) == 8, cfg, Fx.sym, Fy.sym, fast, &a, a.nx, nchunks, nouter,
                                   Fx.border, Fy.border, stream);
+    if (err != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(err));
+    return DIPAMD_OK;
+}
+
+// The register-streaming fused kernel for uint8 planes and radii <= 4 (stream2d.hpp).
+// Returns -100 when its conditions do not hold (the caller then uses the LDS kernel).
+static int run_stream2d(const View &src, const View &dst, int ndim, int dx, int dy, const InternFilter &Fx,
+                        const InternFilter &Fy, int bcx, int bcy, bool fast, hipStream_t stream) {
+    if (!stream2d_enabled()) return -100;
+    auto sym_ok = [](const InternFilter &F) {
+        return (F.sym == DIPAMD_SYM_EVEN || F.sym == DIPAMD_SYM_ODD) && F.shift == 0 && F.border <= kStreamRMaxHost;
+    };
+    if (src.dt != DIPAMD_DT_UINT8 || !sym_ok(Fx) || !sym_ok(Fy)) return -100;
+    if (src.strides[dx] != 1 || dst.strides[dx] != 1) return -100;
+    if (src.sizes[dx] >= (int64_t(1) << 30) || src.sizes[dy] >= (int64_t(1) << 30)) return -100;
+    if (src.sizes[dx] < 16 || src.sizes[dy] < 16) return -100;  // its boundary folds reflect once
+    // word loads need 4-byte aligned rows in every plane
+    if ((reinterpret_cast<uintptr_t>(src.p) & 3) || (src.strides[dy] & 3)) return -100;
+    for (int d = 0; d < ndim; ++d)
+        if (d != dx && d != dy && src.sizes[d] > 1 && (src.strides[d] & 3)) return -100;
+    StreamArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.in = static_cast<const uint8_t *>(src.p);
+    a.out = static_cast<float *>(dst.p);
+    a.nx = static_cast<int>(src.sizes[dx]);
+    a.ny = static_cast<int>(src.sizes[dy]);
+    a.syin = src.strides[dy];
+    a.syout = dst.strides[dy];
+    a.rx = Fx.border;
+    a.ry = Fy.border;
+    a.bcx = bcx;
+    a.bcy = bcy;
+    a.cvalx = const_value<float>(bcx);
+    a.cvaly = const_value<float>(bcy);
+    for (size_t k = 0; k < Fx.data.size(); ++k) a.tx[k] = static_cast<float>(Fx.data[k]);
+    for (size_t k = 0; k < Fy.data.size(); ++k) a.ty[k] = static_cast<float>(Fy.data[k]);
+    fill_outer(a.outer, src, dst, ndim, dx, dy);
+    bool aligned = (reinterpret_cast<uintptr_t>(dst.p) & 15) == 0 && (dst.strides[dy] & 3) == 0;
+    for (int d = 0; d < ndim; ++d)
+        if (d != dx && d != dy && dst.sizes[d] > 1 && (dst.strides[d] & 3)) aligned = false;
+    a.aligned_out = aligned ? 1 : 0;
+    const int64_t nouter = product_except(src, ndim, dx, dy);
+    a.nstrips = static_cast<int>((a.nx + 255) / 256);
+    // enough waves to fill the chip (~24 per CU), chunks of at least 32 rows
+    const int64_t target = 6144;
+    const int64_t base = static_cast<int64_t>(a.nstrips) * nouter;
+    int64_t ch = (static_cast<int64_t>(a.ny) * base + target - 1) / target;
+    ch = std::max<int64_t>(32, std::min<int64_t>(ch, a.ny));
+    a.chunk = static_cast<int>(ch);
+    a.nchunks = static_cast<int>((a.ny + ch - 1) / ch);
+    a.nwaves = base * a.nchunks;
+    TraceScope ts("stream2d", npx_of(src, ndim) * static_cast<int64_t>(1 + sizeof(float)), stream);
+    hipError_t err = launch_stream2d(Fx.sym, Fy.sym, fast, &a, stream);
     if (err != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(err));
     return DIPAMD_OK;
 }
@@ -561,12 +619,18 @@ int separable_device(const View &in, const View &out, int ndim, const InternFilt
         }
         const PassPlan &pp = passes[k];
         if (pp.fused) {
-            const int cfg = choose_fused_config(is_double, filters[pp.d0]->border, filters[pp.d1]->border,
-                                                src.sizes[pp.d0], src.sizes[pp.d1], flags);
-            st = is_double ? run_fused<double>(src, dst, ndim, pp.d0, pp.d1, *filters[pp.d0], *filters[pp.d1],
-                                               bc[pp.d0], bc[pp.d1], fast, cfg, stream)
-                           : run_fused<float>(src, dst, ndim, pp.d0, pp.d1, *filters[pp.d0], *filters[pp.d1],
-                                              bc[pp.d0], bc[pp.d1], fast, cfg, stream);
+            st = -100;
+            if (!is_double && !(flags & DIPAMD_FORCE_FUSION))
+                st = run_stream2d(src, dst, ndim, pp.d0, pp.d1, *filters[pp.d0], *filters[pp.d1], bc[pp.d0],
+                                  bc[pp.d1], fast, stream);
+            if (st == -100) {
+                const int cfg = choose_fused_config(is_double, filters[pp.d0]->border, filters[pp.d1]->border,
+                                                    src.sizes[pp.d0], src.sizes[pp.d1], flags);
+                st = is_double ? run_fused<double>(src, dst, ndim, pp.d0, pp.d1, *filters[pp.d0], *filters[pp.d1],
+                                                   bc[pp.d0], bc[pp.d1], fast, cfg, stream)
+                               : run_fused<float>(src, dst, ndim, pp.d0, pp.d1, *filters[pp.d0], *filters[pp.d1],
+                                                  bc[pp.d0], bc[pp.d1], fast, cfg, stream);
+            }
         } else {
             st = is_double ? run_single<double>(src, dst, ndim, pp.d0, *filters[pp.d0], bc[pp.d0], fast, stream)
                            : run_single<float>(src, dst, ndim, pp.d0, *filters[pp.d0], bc[pp.d0], fast, stream);

@@ -77,6 +77,8 @@ int separable_device(const View &in, const View &out, int ndim, const InternFilt
 // Tile-shape choice for the fused 2-D kernel (autotune.cpp): a config index, or -1
 // to run one pass per axis instead.
 int choose_fused_config(bool is_double, int rx, int ry, int64_t nx, int64_t ny, uint32_t flags);
+// Whether the register-streaming fused kernel may be used (DIPAMD_STREAM2D=0 disables).
+bool stream2d_enabled();
 // Kernel for a pass along a strided axis: 0 column tiles, 1 LDS ring, 2 global sweep
 // (DIPAMD_YSWEEP=col|ring|global; column tiles by default).
 int ysweep_kind();

@@ -24,7 +24,7 @@ hipError_t col_t(const PassArgs<TB> &a, int64_t e, int64_t nchunks, int64_t nout
     const int H = static_cast<int>(a.chunk) + 2 * a.border;
     const size_t lds = static_cast<size_t>(64) * (H | 1) * sizeof(TB);
     hipLaunchKernelGGL((k_colsweep<TI, TB, P, SYM, FAST>), dim3(static_cast<unsigned>(nblk)), dim3(kBlock), lds, st,
-                       a, ntiles, nchunks);
+                       a, ntiles, nchunks, static_cast<const TI *>(a.in), static_cast<TB *>(a.out));
     return hipGetLastError();
 }
 

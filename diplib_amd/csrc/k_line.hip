@@ -27,7 +27,7 @@ hipError_t line_t(const PassArgs<TB> &a, int64_t n, int64_t nlines, int border, 
     const int64_t nblk = nseg * ((nlines + 3) / 4);
     const size_t lds = static_cast<size_t>(4) * lpad_size(64 * P + 2 * border) * sizeof(TB);
     hipLaunchKernelGGL((k_line<TI, TB, P, SYM, FAST>), dim3(static_cast<unsigned>(nblk)), dim3(kBlock), lds, st, a,
-                       nseg);
+                       nseg, static_cast<const TI *>(a.in), static_cast<TB *>(a.out));
     return hipGetLastError();
 }
 

@@ -12,7 +12,7 @@ hipError_t sweep_t(const PassArgs<TB> &a, int64_t e, int64_t nchunks, int64_t no
     const int64_t ntiles = (e + bs - 1) / bs;
     const int64_t nblk = ntiles * nchunks * nouter;
     hipLaunchKernelGGL((k_sweep<TI, TB, P, SYM, FAST>), dim3(static_cast<unsigned>(nblk)), dim3(bs), 0, st, a, ntiles,
-                       nchunks);
+                       nchunks, static_cast<const TI *>(a.in), static_cast<TB *>(a.out));
     return hipGetLastError();
 }
 

@@ -148,6 +148,26 @@ struct FusedArgs {
     TB ty[kFusedCap];
 };
 
+// Register-streaming fused kernel (stream2d.hpp): radii up to kStreamRMax.
+constexpr int kStreamRMax = 4;
+constexpr int kStreamRMaxHost = kStreamRMax;
+
+struct StreamArgs {
+    const uint8_t *in;
+    float *out;
+    int nx, ny;
+    int64_t syin, syout;  // row strides in samples (x strides are 1)
+    int chunk;            // output rows per wave
+    int nstrips, nchunks;
+    int rx, ry, bcx, bcy;
+    float cvalx, cvaly;
+    int aligned_out;      // output rows 16-byte aligned: vector stores
+    int64_t nwaves;
+    float tx[kStreamRMax + 1];
+    float ty[kStreamRMax + 1];
+    Outer outer;          // stack of planes
+};
+
 // Batched form: per-image parameters, work items and kernel arguments.
 template <typename TB>
 struct BatchImage {

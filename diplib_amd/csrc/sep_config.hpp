@@ -7,5 +7,6 @@ constexpr int kLinePD = 4;  // outputs per lane, line kernel (double)
 constexpr int kSweepP = 8;  // outputs per thread and step, sweep kernel
 constexpr int kColPF = 8;   // outputs per thread and step, column-tile sweep (float)
 constexpr int kColPD = 4;   // outputs per thread and step, column-tile sweep (double)
+constexpr int kColsweepMinBorder = 16;  // shorter filters on strided axes use the global sweep
 
 }  // namespace dipamd

@@ -290,7 +290,8 @@ __device__ __forceinline__ void fir_window(const TB *taps, int ntaps, int shift,
 // is staged in LDS (converted to the buffer type, boundary folded).
 // ---------------------------------------------------------------------------
 template <typename TI, typename TB, int P, int SYM, bool FAST>
-__global__ __launch_bounds__(kBlock) void k_line(const PassArgs<TB> a, int64_t nseg) {
+__global__ __launch_bounds__(kBlock) void k_line(const PassArgs<TB> a, int64_t nseg, const TI *__restrict__ in,
+                                                  TB *__restrict__ out) {
     constexpr int TX = 64 * P;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     TB *lds = reinterpret_cast<TB *>(smem_raw);
@@ -307,7 +308,7 @@ __global__ __launch_bounds__(kBlock) void k_line(const PassArgs<TB> a, int64_t n
     const bool active = line < a.nlines;
     if (active) {
         outer_offsets(a.outer, line, &oin, &oout);
-        const TI *src = static_cast<const TI *>(a.in) + oin;
+        const TI *src = in + oin;
         const bool interior = (x0 - B >= 0) && (x0 + TX + B <= n);
         if (interior && a.sin == 1) {
             const TI *s0 = src + (x0 - B);
@@ -321,7 +322,7 @@ __global__ __launch_bounds__(kBlock) void k_line(const PassArgs<TB> a, int64_t n
     LdsRowSrc<TB> s{row, B + lane * P, 0, 0};
     TB acc[P];
     fir_window<TB, P, SYM, FAST>(a.taps, a.ntaps, a.shift, s, acc);
-    TB *dst = static_cast<TB *>(a.out) + oout;
+    TB *dst = out + oout;
     const int xs = x0 + lane * P;
     if (a.sout == 1 && xs + P <= n) {
         TB *d = dst + xs;
@@ -345,7 +346,8 @@ __global__ __launch_bounds__(kBlock) void k_line(const PassArgs<TB> a, int64_t n
 // barriers.
 // ---------------------------------------------------------------------------
 template <typename TI, typename TB, int P, int SYM, bool FAST>
-__global__ __launch_bounds__(kBlock) void k_sweep(const PassArgs<TB> a, int64_t ntiles, int64_t nchunks) {
+__global__ __launch_bounds__(kBlock) void k_sweep(const PassArgs<TB> a, int64_t ntiles, int64_t nchunks,
+                                                   const TI *__restrict__ in, TB *__restrict__ out) {
     const int64_t bid = blockIdx.x;
     const int64_t tile = bid % ntiles;
     const int64_t rest = bid / ntiles;
@@ -355,8 +357,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const PassArgs<TB> a, int64_t 
     if (c >= a.e) return;
     int64_t oin, oout;
     outer_offsets(a.outer, outer, &oin, &oout);
-    const TI *src = static_cast<const TI *>(a.in) + oin + c * a.ein;
-    TB *dst = static_cast<TB *>(a.out) + oout + c * a.eout;
+    const TI *src = in + oin + c * a.ein;
+    TB *dst = out + oout + c * a.eout;
     const int n = static_cast<int>(a.n);
     const int z0 = static_cast<int>(chunk * a.chunk);
     const int zend = static_cast<int>(min(chunk * a.chunk + a.chunk, a.n));
@@ -399,7 +401,8 @@ struct LdsColSrc {  // one staged column in LDS, unit stride
 };
 
 template <typename TI, typename TB, int P, int SYM, bool FAST>
-__global__ __launch_bounds__(kBlock) void k_colsweep(const PassArgs<TB> a, int64_t ntiles, int64_t nchunks) {
+__global__ __launch_bounds__(kBlock) void k_colsweep(const PassArgs<TB> a, int64_t ntiles, int64_t nchunks,
+                                                      const TI *__restrict__ in, TB *__restrict__ out) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     TB *lds = reinterpret_cast<TB *>(smem_raw);
     const int64_t bid = blockIdx.x;
@@ -418,7 +421,7 @@ __global__ __launch_bounds__(kBlock) void k_colsweep(const PassArgs<TB> a, int64
     const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
     const int64_t c = tile * 64 + lane;
     const bool cok = c < a.e;
-    const TI *src = static_cast<const TI *>(a.in) + oin + (cok ? c : 0) * a.ein;
+    const TI *src = in + oin + (cok ? c : 0) * a.ein;
     TB *col = lds + lane * cpitch;
     // stage: wave `grp` loads samples grp, grp+4, ... of every column (rows are
     // coalesced across the lanes), eight loads in flight before their LDS stores
@@ -442,7 +445,7 @@ __global__ __launch_bounds__(kBlock) void k_colsweep(const PassArgs<TB> a, int64
     }
     __syncthreads();
     if (!cok) return;
-    TB *dst = static_cast<TB *>(a.out) + oout + c * a.eout;
+    TB *dst = out + oout + c * a.eout;
     const int per = R / 4;
     const int zend = min(z0 + R, n);
     for (int blk = 0; blk < per; blk += P) {
@@ -493,7 +496,8 @@ __host__ __device__ constexpr int fused_pitch(int n, int P) {
 __host__ __device__ constexpr int fused_ring_rows(int ry) { return ((2 * ry + 15) / 16 + 1) * 16; }
 
 template <typename TI, typename TB, int TXC, int P, int SX, int SY, bool FAST>
-__device__ __forceinline__ void fused_tile(const Tile<TB> &t, TB *lds, const TB *tx, const TB *ty) {
+__device__ __forceinline__ void fused_tile(const Tile<TB> &t, const TI *__restrict__ src, TB *__restrict__ dst,
+                                           TB *lds, const TB *tx, const TB *ty) {
     constexpr int S = 16;
     constexpr int XT = TXC / P;            // x-stage lanes per row (16)
     static_assert(XT * S == kBlock, "tile shape");
@@ -506,7 +510,6 @@ __device__ __forceinline__ void fused_tile(const Tile<TB> &t, TB *lds, const TB 
     const int RC = (A + 1) * S;
     TB *stg = lds;
     TB *ring = lds + S * pitchS;
-    const TI *src = static_cast<const TI *>(t.in);
     const int c0 = t.c0, z0 = t.z0, zend = t.zend;
     const int nsteps = (zend - z0 + S - 1) / S;
     const int M = A + nsteps;
@@ -608,7 +611,7 @@ __device__ __forceinline__ void fused_tile(const Tile<TB> &t, TB *lds, const TB 
             fir_window<TB, P, SY, FAST>(ty, By + 1, 0, s, acc);
             const int zs = z0 + n * S + grp * P;
             if (c < t.nx) {
-                TB *d = t.out + static_cast<int64_t>(zs) * t.syout + static_cast<int64_t>(c) * t.sxout;
+                TB *d = dst + static_cast<int64_t>(zs) * t.syout + static_cast<int64_t>(c) * t.sxout;
 #pragma unroll
                 for (int p = 0; p < P; ++p) {
                     if (zs + p < zend) *d = acc[p];
@@ -648,7 +651,7 @@ __global__ __launch_bounds__(kBlock) void k_fused2d(const FusedArgs<TB> a, int64
     t.bcy = a.bcy;
     t.cvalx = a.cvalx;
     t.cvaly = a.cvaly;
-    fused_tile<TI, TB, TXC, P, SX, SY, FAST>(t, lds, a.tx, a.ty);
+    fused_tile<TI, TB, TXC, P, SX, SY, FAST>(t, static_cast<const TI *>(t.in), t.out, lds, a.tx, a.ty);
 }
 
 // ---------------------------------------------------------------------------
@@ -692,7 +695,8 @@ __global__ __launch_bounds__(kBlock) void k_batched(const BatchArgs<TB> a) {
         t.bcy = a.bcy;
         t.cvalx = a.cvalx;
         t.cvaly = a.cvaly;
-        fused_tile<TI, TB, TXC, P, SYM_EVEN, SYM_EVEN, FAST>(t, lds, s_taps, s_taps + ntx);
+        fused_tile<TI, TB, TXC, P, SYM_EVEN, SYM_EVEN, FAST>(t, static_cast<const TI *>(t.in), t.out, lds, s_taps,
+                                                             s_taps + ntx);
         __syncthreads();  // LDS (ring, taps) free before the next item
     }
 }

@@ -74,6 +74,16 @@ for w in range(2, 8):
         for bc in ("mirror", "periodic", "asym mirror"):
             SMALL_CASES.append((f"bounce_{w}x{h}_{bc.replace(' ', '_')}", (h, w), "float32", [3.0, 2.0], [0, 1],
                                 [bc], None))
+# uint8 planes with radius <= 4 and 4-sample-aligned rows (the register-streaming
+# kernel's domain): every boundary condition, partial strips, tiny planes, stacks
+for bc in ("mirror", "asym mirror", "periodic", "asym periodic", "add zeros", "add max", "add min", "zero order"):
+    SMALL_CASES.append((f"u8_64x40_s1_o01_{bc.replace(' ', '_')}", (40, 64), "uint8", [1.0], [0, 1], [bc], None))
+SMALL_CASES.append(("u8_260x21_s1_o20_mirror", (21, 260), "uint8", [1.0], [2, 0], ["mirror"], None))
+SMALL_CASES.append(("u8_12x9_s1_o11_periodic", (9, 12), "uint8", [1.0], [1, 1], ["periodic"], None))
+SMALL_CASES.append(("u8_8x5_s1_o00_mirror", (5, 8), "uint8", [1.0], [0, 0], ["mirror"], None))
+SMALL_CASES.append(("u8_stack3_64x40_s1", (3, 40, 64), "uint8", [1.0, 1.2, 0.0], [0, 0, 0], ["mirror"], None))
+SMALL_CASES.append(("u8_520x70_s07_o00_asym_mirror", (70, 520), "uint8", [0.7, 1.3], [0, 0], ["asym mirror"],
+                    None))
 # width-1 lines (dim skipped) and single pixels
 SMALL_CASES.append(("f32_1x40_s2", (40, 1), "float32", [2.0], [0], ["mirror"], None))
 SMALL_CASES.append(("f32_1x1_s2", (1, 1), "float32", [2.0], [0], ["mirror"], None))
