@@ -26,10 +26,12 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "dipamd.h"
@@ -387,6 +389,26 @@ inline std::vector<dfloat> MakeHalfGaussian(dfloat sigma, uint derivativeOrder =
     f.resize(static_cast<std::size_t>(n));
     return f;
 }
+
+/* dip::SetNumberOfThreads / GetNumberOfThreads (multithreading.h:65-74;
+ * multithreading.cpp:28-44).  The launch grid replaces the OpenMP team, so the value is
+ * only kept (per host thread, default = the host's hardware threads, 0 resets it, larger
+ * values are capped) and returned, as the reference keeps it. */
+namespace detail {
+inline uint DefaultThreadCount() {
+    const unsigned n = std::thread::hardware_concurrency();
+    return n ? static_cast<uint>(n) : 1;
+}
+inline uint &ThreadCount() {
+    static thread_local uint n = DefaultThreadCount();
+    return n;
+}
+}  // namespace detail
+inline void SetNumberOfThreads(uint nThreads) {
+    detail::ThreadCount() = nThreads == 0 ? detail::DefaultThreadCount()
+                                          : std::min(nThreads, detail::DefaultThreadCount());
+}
+inline uint GetNumberOfThreads() { return detail::ThreadCount(); }
 
 }  // namespace dip
 

@@ -41,6 +41,10 @@ int main(int argc, char **argv) {
     std::vector<float> in = read_f32(argv[1], W * H);
     std::vector<float> exp = read_f32(argv[4], W * H);
     std::vector<float> got(W * H);
+    dip::SetNumberOfThreads(1);
+    CHECK(dip::GetNumberOfThreads() == 1, "SetNumberOfThreads(1) kept");
+    dip::SetNumberOfThreads(0);
+    CHECK(dip::GetNumberOfThreads() >= 1, "SetNumberOfThreads(0) restores the default");
     try {
         dip::Image img({W, H}, 1, dip::DT_SFLOAT);
         img.CopyFromHost(in.data());
