@@ -55,7 +55,7 @@ def read_trace(L):
     cap = 1 << 16
     recs = (DipamdTraceRecord * cap)()
     n = L.dipamd_trace_read(recs, cap)
-    return [(recs[i].name.decode(), recs[i].ms, recs[i].bytes) for i in range(n)]
+    return [(recs[i].name.decode(), recs[i].ms, recs[i].bytes, recs[i].ops) for i in range(n)]
 
 
 def cpu_baseline(x_np, budget_s=25.0):
@@ -98,6 +98,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only")
+    ap.add_argument("--case", nargs=2, metavar=("SIGMA", "ORDERS"),
+                    help="profiling mode: run one case of the sweep (e.g. 16 00) --steps times, then a "
+                         "256 MiB device copy as the calibration kernel for the traffic counters")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; the multi-GPU (sharded) path is a design only "
@@ -117,6 +120,18 @@ def main():
 
     x_np = cases.make_input((SIZE, SIZE), "uint8", seed=2)
     x = torch.from_numpy(x_np).to(dev)
+    if a.case:
+        s, o = float(a.case[0]), [int(c) for c in a.case[1]]
+        y = torch.empty((SIZE, SIZE), dtype=torch.float32, device=dev)
+        for _ in range(a.steps):
+            dip.GaussFIR(x, [s], o, ["mirror"], 3.0, out=y)
+        src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev)  # 256 MiB read + 256 MiB written
+        dst = torch.empty_like(src)
+        for _ in range(a.steps):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"case sigma={s} orders={o}: {a.steps} calls, then {a.steps} copies of 256 MiB")
+        return
     outs = [torch.empty((SIZE, SIZE), dtype=torch.float32, device=dev) for _ in range(len(SIGMAS) * len(ORDERS))]
     combos = [(s, o) for s in SIGMAS for o in ORDERS]
 
@@ -143,18 +158,21 @@ def main():
 
     # dominant kernel over the timed region
     agg = {}
-    for name, ms, by in trace:
-        e = agg.setdefault(name, [0.0, 0, 0])
+    for name, ms, by, ops in trace:
+        e = agg.setdefault(name, [0.0, 0, 0, 0])
         e[0] += ms
         e[1] += 1
         e[2] += by
+        e[3] += ops
     kernels = {}
-    for name, (tot, cnt, by) in agg.items():
+    for name, (tot, cnt, by, ops) in agg.items():
         avg_ms = tot / cnt
         gbs = (by / cnt) / (avg_ms * 1e-3) / 1e9
         kernels[name] = {"launches": cnt, "total_ms": round(tot, 4), "avg_ms": round(avg_ms, 5),
                          "bytes_per_launch": by // cnt, "achieved_GBs": round(gbs, 1),
-                         "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+                         "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                         "ops_per_launch": ops // cnt,
+                         "valu_frac": round((ops / cnt) / (avg_ms * 1e-3) / VALU_PEAK_OPS, 4)}
     dom = max(agg.items(), key=lambda kv: kv[1][0])[0] if agg else None
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
@@ -169,7 +187,10 @@ def main():
         k = kernels[dom]
         roofline = {"bound": "hbm", "achieved": k["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": k["hbm_frac"], "traffic": traffic, "kernel": dom,
-                    "bytes_per_launch": k["bytes_per_launch"], "avg_launch_ms": k["avg_ms"]}
+                    "bytes_per_launch": k["bytes_per_launch"], "avg_launch_ms": k["avg_ms"],
+                    "valu": {"achieved": round(k["ops_per_launch"] / (k["avg_ms"] * 1e-3) / 1e12, 3),
+                             "peak": VALU_PEAK_OPS / 1e12, "unit": "T lane-ops/s", "frac": k["valu_frac"],
+                             "ops_per_launch": k["ops_per_launch"]}}
 
     # per-case rates (event-timed) and roofline fractions
     per_case = []
@@ -208,6 +229,7 @@ def main():
         extra.update(bench_volume(dip, torch, cases, dev))
         extra.update(bench_batched(dip, torch, np, dev, L))
         extra.update(bench_backward(dip, torch, dev))
+        extra.update(bench_fp32_vs_fp64(dip, torch, np, cases, dev))
     del outs
     cpu = None
     if not a.no_cpu:
@@ -260,6 +282,99 @@ def bench_volume(dip, torch, cases, dev):
     except Exception as e:  # report, do not hide
         out["volume_512x512x256_s221"] = {"error": repr(e)}
     return out
+
+
+def err_vs_fp64(got, ref64):
+    """FP32 result against an FP64 result: max|a-b| / max|b| and DIPlib's
+    MaximumRelativeError, max |a-b| / (|a|+|b|)/2 with 0 where both are 0
+    (error.cpp:77-109; ill-conditioned at zero crossings of derivative outputs)."""
+    import numpy as np
+    a = got.astype(np.float64)
+    d = np.abs(a - ref64)
+    den = 0.5 * (np.abs(a) + np.abs(ref64))
+    mre = float(np.max(np.where(den > 0, d / np.where(den > 0, den, 1.0), 0.0))) if d.size else 0.0
+    scale = float(np.max(np.abs(ref64))) if d.size else 0.0
+    return {"max_abs_over_max_ref": float(d.max() / scale) if scale > 0 else 0.0, "dip_max_rel_err": mre}
+
+
+def bench_fp32_vs_fp64(dip, torch, np, cases, dev):
+    """FP32 outputs (exact mode, so the reference's own sfloat results) against the
+    FP64 oracle, for every workload.  Large images are checked on bands of rows (or
+    planes) extended by the filter radius, whose centre rows do not see the band's
+    cut: the top band also covers the image edge's boundary extension."""
+    import oracle_ctypes as oc
+    res = {}
+    nthr = min(16, len(os.sched_getaffinity(0)))
+
+    def ref64(x, *args):
+        return oc.gauss_fir(x, *args, nthreads=nthr)
+
+    try:
+        band = 64
+        # 512x512 sfloat, sigma 2 (the reference's CPU-runnable case): whole image
+        x = cases.make_input((512, 512), "float32", 1)
+        y = dip.GaussFIR(torch.from_numpy(x).to(dev), [2.0], [0], ["mirror"], 3.0).cpu().numpy()
+        res["f32_512x512_s2"] = err_vs_fp64(y, ref64(x.astype(np.float64), [2.0], [0], ["mirror"], 3.0))
+        # 8192^2 uint8 sweep: a band at the top edge and one at the centre
+        xu = cases.make_input((SIZE, SIZE), "uint8", 2)
+        xd = torch.from_numpy(xu).to(dev)
+        for s in SIGMAS:
+            for o in ORDERS:
+                y = dip.GaussFIR(xd, [s], list(o), ["mirror"], 3.0)
+                r = radius(s, o[1])
+                worst = {"max_abs_over_max_ref": 0.0, "dip_max_rel_err": 0.0}
+                for y0 in (0, SIZE // 2):
+                    lo = max(0, y0 - r)
+                    ref = ref64(xu[lo:y0 + band + r].astype(np.float64), [s], list(o), ["mirror"], 3.0)
+                    e = err_vs_fp64(y[y0:y0 + band].cpu().numpy(), ref[y0 - lo:y0 - lo + band])
+                    worst = {k: max(worst[k], e[k]) for k in worst}
+                res[f"u8_8192sq_s{int(s)}_o{o[0]}{o[1]}"] = worst
+        del xd
+        # 3-D volume: a slab of planes at the centre and at the first plane
+        xv = cases.make_input((256, 512, 512), "float32", 3, 0.01)
+        xvd = torch.from_numpy(xv).to(dev)
+        rz = radius(1.0, 0)
+        for bc in ("mirror", "periodic"):
+            yv = dip.GaussFIR(xvd, [2.0, 2.0, 1.0], [0], [bc], 3.0)
+            worst = {"max_abs_over_max_ref": 0.0, "dip_max_rel_err": 0.0}
+            for z0 in ((0, 128) if bc == "mirror" else (128,)):
+                lo = max(0, z0 - rz)
+                ref = ref64(xv[lo:z0 + 8 + rz].astype(np.float64), [2.0, 2.0, 1.0], [0], [bc], 3.0)
+                e = err_vs_fp64(yv[z0:z0 + 8].cpu().numpy(), ref[z0 - lo:z0 - lo + 8])
+                worst = {k: max(worst[k], e[k]) for k in worst}
+            res[f"f32_512x512x256_s221_{bc}"] = worst
+        del xvd
+        # FP64 volume: the device's dfloat path against the FP64 oracle (whole volume)
+        x64 = cases.make_input((128, 256, 256), "float64", 4, 0.01)
+        y64 = dip.GaussFIR(torch.from_numpy(x64).to(dev), [2.0, 2.0, 1.0], [0], ["mirror"], 3.0).cpu().numpy()
+        res["f64_256x256x128_s221_mirror"] = err_vs_fp64(y64, ref64(x64, [2.0, 2.0, 1.0], [0], ["mirror"], 3.0))
+        # batched uneven form: the first 8 images of the benchmark's batch
+        rng = np.random.default_rng(1234)
+        ws, hs = rng.integers(64, 2049, 512), rng.integers(64, 2049, 512)
+        sig = np.exp(rng.uniform(np.log(0.5), np.log(8.0), 512))
+        ins = [torch.from_numpy(cases.make_input((int(h), int(w)), "float32", 50 + i)).to(dev)
+               for i, (w, h) in enumerate(zip(ws[:8], hs[:8]))]
+        outs = [torch.empty_like(t) for t in ins]
+        plan = dip.GaussBatch(ins, outs, [(float(s), float(s)) for s in sig[:8]], boundary="mirror")
+        plan.run()
+        torch.cuda.synchronize()
+        worst = {"max_abs_over_max_ref": 0.0, "dip_max_rel_err": 0.0}
+        for t, o, s in zip(ins, outs, sig[:8]):
+            e = err_vs_fp64(o.cpu().numpy(), ref64(t.cpu().numpy().astype(np.float64), [float(s)], [0],
+                                                          ["mirror"], 3.0))
+            worst = {k: max(worst[k], e[k]) for k in worst}
+        plan.close()
+        res["batched_uneven_first8"] = worst
+        # differentiation form, forward: one 1024^2 image, sigma 3, orders 0 and 1
+        x5 = cases.make_input((1024, 1024), "float32", 5)
+        for order in (0, 1):
+            y5 = dip.GaussFIR(torch.from_numpy(x5).to(dev), [3.0], [order, 0], ["mirror"], 3.0).cpu().numpy()
+            res[f"f32_1024sq_s3_o{order}0"] = err_vs_fp64(
+                y5, ref64(x5.astype(np.float64), [3.0], [order, 0], ["mirror"], 3.0))
+        log(f"FP32 vs FP64: {res}")
+    except Exception as e:  # report, do not hide
+        res["error"] = repr(e)
+    return {"fp32_vs_fp64": res}
 
 
 def bench_batched(dip, torch, np, dev, L):

@@ -43,7 +43,8 @@ class DipamdBatchImage(ctypes.Structure):
 
 
 class DipamdTraceRecord(ctypes.Structure):
-    _fields_ = [("name", ctypes.c_char * 48), ("ms", ctypes.c_double), ("bytes", ctypes.c_int64)]
+    _fields_ = [("name", ctypes.c_char * 48), ("ms", ctypes.c_double), ("bytes", ctypes.c_int64),
+                ("ops", ctypes.c_int64)]
 
 
 class DipError(RuntimeError):

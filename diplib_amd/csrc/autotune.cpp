@@ -81,6 +81,15 @@ bool stream2d_enabled() {
     return on;
 }
 
+int stream2d_target_waves() {
+    static int n = [] {
+        const char *v = std::getenv("DIPAMD_STREAM_WAVES");
+        const int k = v ? std::atoi(v) : 0;
+        return k > 0 ? k : 6144;
+    }();
+    return n;
+}
+
 int ysweep_kind() {
     static int kind = [] {
         const char *v = std::getenv("DIPAMD_YSWEEP");

@@ -37,7 +37,7 @@ thread_local std::string g_last_error;
 // planner issues, for per-kernel timing inside a caller's timed region.
 struct TraceRec {
     char name[48];
-    int64_t bytes;
+    int64_t bytes, ops;
     hipEvent_t a, b;
 };
 std::atomic<bool> g_trace_on{false};
@@ -57,12 +57,14 @@ hipEvent_t pooled_event() {
 }
 }  // namespace
 
-TraceScope::TraceScope(const char *name, int64_t bytes, hipStream_t stream) : stream_(stream), index_(-1) {
+TraceScope::TraceScope(const char *name, int64_t bytes, hipStream_t stream, int64_t ops)
+    : stream_(stream), index_(-1) {
     if (!g_trace_on.load(std::memory_order_relaxed)) return;
     std::lock_guard<std::mutex> lk(g_trace_mu);
     TraceRec r;
     std::snprintf(r.name, sizeof(r.name), "%s", name);
     r.bytes = bytes;
+    r.ops = ops;
     r.a = pooled_event();
     r.b = pooled_event();
     (void)hipEventRecord(r.a, stream);
@@ -160,6 +162,17 @@ bool meaningless(const InternFilter &F, bool use_double) {
     if (F.size == 0) return true;
     if (F.size == 1) return use_double ? (F.data[0] == 1.0) : (static_cast<float>(F.data[0]) == 1.0f);
     return false;
+}
+
+// Algorithmic VALU lane-ops per output sample of one filter (the trace's ops field):
+// symmetric forms, n stored taps: a pair add, a multiply and an add per pair plus the
+// centre multiply (fast: add + fma); general form, n taps: multiply + add each (fast: fma).
+int64_t filter_ops(const InternFilter &F, bool fast) {
+    const int64_t n = static_cast<int64_t>(F.data.size());
+    if (F.sym == SYM_NONE || n == 0) return 0;
+    if (F.sym == DIPAMD_SYM_GENERAL) return fast ? n : 2 * n - 1;
+    if (F.sym == DIPAMD_SYM_EVEN || F.sym == DIPAMD_SYM_ODD) return fast ? 2 * (n - 1) + 1 : 3 * (n - 1) + 1;
+    return fast ? 2 * n : 3 * n - 1;  // D_EVEN / D_ODD: n pairs, no centre
 }
 
 // ---------------------------------------------------------------------------
@@ -322,7 +335,8 @@ static int run_single(const View &src, const View &dst, int ndim, int d, const I
         a.nlines = product_except(src, ndim, d, -1);
         if (line_lds_bytes(F.border, is_double) > 65536)
             return set_error(DIPAMD_E_UNSUPPORTED, "filter border too large for the line kernel");
-        TraceScope ts("line", npx_of(src, ndim) * static_cast<int64_t>(dt_size(src.dt) + sizeof(TB)), stream);
+        TraceScope ts("line", npx_of(src, ndim) * static_cast<int64_t>(dt_size(src.dt) + sizeof(TB)), stream,
+                      npx_of(src, ndim) * filter_ops(F, fast));
         err = launch_line(ti, is_double, F.sym, fast, &a, a.n, a.nlines, F.border, stream);
     } else {
         // strided axis: the LDS-ring sweep (the fused kernel with a copy x stage) when
@@ -345,7 +359,8 @@ static int run_single(const View &src, const View &dst, int ndim, int d, const I
             const int64_t nouter = product_except(src, ndim, d, lane);
             a.chunk = rows;
             const int64_t nchunks = (a.n + rows - 1) / rows;
-            TraceScope ts("colsweep", npx_of(src, ndim) * static_cast<int64_t>(dt_size(src.dt) + sizeof(TB)), stream);
+            TraceScope ts("colsweep", npx_of(src, ndim) * static_cast<int64_t>(dt_size(src.dt) + sizeof(TB)), stream,
+                          npx_of(src, ndim) * filter_ops(F, fast));
             err = launch_colsweep(ti, is_double, F.sym, fast, &a, a.e, nchunks, nouter, stream);
             if (err != hipSuccess)
                 return set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(err));
@@ -368,7 +383,8 @@ static int run_single(const View &src, const View &dst, int ndim, int d, const I
         const int64_t ntiles = (a.e + bs - 1) / bs;
         a.chunk = pick_chunk(a.n, ntiles * nouter, 8, F.border, 2048);
         const int64_t nchunks = (a.n + a.chunk - 1) / a.chunk;
-        TraceScope ts("sweep", npx_of(src, ndim) * static_cast<int64_t>(dt_size(src.dt) + sizeof(TB)), stream);
+        TraceScope ts("sweep", npx_of(src, ndim) * static_cast<int64_t>(dt_size(src.dt) + sizeof(TB)), stream,
+                      npx_of(src, ndim) * filter_ops(F, fast));
         err = launch_sweep(ti, is_double, F.sym, fast, &a, a.e, nchunks, nouter, stream);
     }
     if (err != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(err));
@@ -404,7 +420,8 @@ static int run_fused(const View &src, const View &dst, int ndim, int dx, int dy,
     const int64_t nchunks = (a.ny + a.chunk - 1) / a.chunk;
     const InType ti = (src.dt == DIPAMD_DT_UINT8) ? IN_U8 : IN_BUF;
     TraceScope ts(Fx.sym == SYM_NONE ? "ringsweep" : "fused2d",
-                  npx_of(src, ndim) * static_cast<int64_t>(dt_size(src.dt) + sizeof(TB)), stream);
+                  npx_of(src, ndim) * static_cast<int64_t>(dt_size(src.dt) + sizeof(TB)), stream,
+                  npx_of(src, ndim) * (filter_ops(Fx, fast) + filter_ops(Fy, fast)));
     hipError_t err = launch_fused(ti, sizeof(TB) == 8, cfg, Fx.sym, Fy.sym, fast, &a, a.nx, nchunks, nouter,
                                   Fx.border, Fy.border, stream);
     if (err != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(err));
@@ -417,12 +434,14 @@ static int run_stream2d(const View &src, const View &dst, int ndim, int dx, int 
                         const InternFilter &Fy, int bcx, int bcy, bool fast, hipStream_t stream) {
     if (!stream2d_enabled()) return -100;
     auto sym_ok = [](const InternFilter &F) {
-        return (F.sym == DIPAMD_SYM_EVEN || F.sym == DIPAMD_SYM_ODD) && F.shift == 0 && F.border <= kStreamRMaxHost;
+        return (F.sym == DIPAMD_SYM_EVEN || F.sym == DIPAMD_SYM_ODD) && F.shift == 0 && F.border >= 1 &&
+               F.border <= kStreamRMaxHost && static_cast<int>(F.data.size()) == F.border + 1;
     };
     if (src.dt != DIPAMD_DT_UINT8 || !sym_ok(Fx) || !sym_ok(Fy)) return -100;
     if (src.strides[dx] != 1 || dst.strides[dx] != 1) return -100;
     if (src.sizes[dx] >= (int64_t(1) << 30) || src.sizes[dy] >= (int64_t(1) << 30)) return -100;
     if (src.sizes[dx] < 16 || src.sizes[dy] < 16) return -100;  // its boundary folds reflect once
+    if (src.sizes[dx] & 3) return -100;                          // whole 4-sample words per row
     // word loads need 4-byte aligned rows in every plane
     if ((reinterpret_cast<uintptr_t>(src.p) & 3) || (src.strides[dy] & 3)) return -100;
     for (int d = 0; d < ndim; ++d)
@@ -447,18 +466,41 @@ static int run_stream2d(const View &src, const View &dst, int ndim, int dx, int 
     bool aligned = (reinterpret_cast<uintptr_t>(dst.p) & 15) == 0 && (dst.strides[dy] & 3) == 0;
     for (int d = 0; d < ndim; ++d)
         if (d != dx && d != dy && dst.sizes[d] > 1 && (dst.strides[d] & 3)) aligned = false;
-    a.aligned_out = aligned ? 1 : 0;
+    if (!aligned) return -100;  // 16-byte stores of 4 outputs
+    a.aligned_out = 1;
+    // one-reflection row fold coefficients (fold_once in stream2d.hpp, as arithmetic)
+    const int ny = static_cast<int>(src.sizes[dy]);
+    switch (bcy) {
+        case DIPAMD_BC_SYMMETRIC_MIRROR:
+        case DIPAMD_BC_ASYMMETRIC_MIRROR:
+            a.ylo_m = -1, a.ylo_c = 0, a.yhi_m = -1, a.yhi_c = 2 * (ny - 1);
+            a.yflag = bcy == DIPAMD_BC_ASYMMETRIC_MIRROR ? 1 : 0;
+            break;
+        case DIPAMD_BC_PERIODIC:
+        case DIPAMD_BC_ASYMMETRIC_PERIODIC:
+            a.ylo_m = 1, a.ylo_c = ny, a.yhi_m = 1, a.yhi_c = -ny;
+            a.yflag = bcy == DIPAMD_BC_ASYMMETRIC_PERIODIC ? 1 : 0;
+            break;
+        case DIPAMD_BC_ZERO_ORDER_EXTRAPOLATE:
+            a.ylo_m = 0, a.ylo_c = 0, a.yhi_m = 0, a.yhi_c = ny - 1, a.yflag = 0;
+            break;
+        default:  // constant conditions: any valid row, value replaced
+            a.ylo_m = 0, a.ylo_c = 0, a.yhi_m = 0, a.yhi_c = 0, a.yflag = 2;
+            break;
+    }
     const int64_t nouter = product_except(src, ndim, dx, dy);
     a.nstrips = static_cast<int>((a.nx + 255) / 256);
     // enough waves to fill the chip (~24 per CU), chunks of at least 32 rows
-    const int64_t target = 6144;
+    const int64_t target = stream2d_target_waves();
     const int64_t base = static_cast<int64_t>(a.nstrips) * nouter;
     int64_t ch = (static_cast<int64_t>(a.ny) * base + target - 1) / target;
     ch = std::max<int64_t>(32, std::min<int64_t>(ch, a.ny));
     a.chunk = static_cast<int>(ch);
     a.nchunks = static_cast<int>((a.ny + ch - 1) / ch);
     a.nwaves = base * a.nchunks;
-    TraceScope ts("stream2d", npx_of(src, ndim) * static_cast<int64_t>(1 + sizeof(float)), stream);
+    a.nplanes = nouter;
+    TraceScope ts("stream2d", npx_of(src, ndim) * static_cast<int64_t>(1 + sizeof(float)), stream,
+                  npx_of(src, ndim) * (filter_ops(Fx, fast) + filter_ops(Fy, fast)));
     hipError_t err = launch_stream2d(Fx.sym, Fy.sym, fast, &a, stream);
     if (err != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(err));
     return DIPAMD_OK;
@@ -726,6 +768,7 @@ int dipamd_trace_read(dipamd_trace_record *records, int32_t capacity) {
         std::snprintf(records[n].name, sizeof(records[n].name), "%s", r.name);
         records[n].ms = ms;
         records[n].bytes = r.bytes;
+        records[n].ops = r.ops;
         ++n;
     }
     return n;

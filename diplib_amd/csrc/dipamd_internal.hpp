@@ -34,7 +34,7 @@ struct InternFilter {
 // Records HIP events around a kernel launch while dipamd_trace_enable(1) is on.
 class TraceScope {
   public:
-    TraceScope(const char *name, int64_t bytes, hipStream_t stream);
+    TraceScope(const char *name, int64_t bytes, hipStream_t stream, int64_t ops = 0);
     ~TraceScope();
     TraceScope(const TraceScope &) = delete;
     TraceScope &operator=(const TraceScope &) = delete;
@@ -79,6 +79,7 @@ int separable_device(const View &in, const View &out, int ndim, const InternFilt
 int choose_fused_config(bool is_double, int rx, int ry, int64_t nx, int64_t ny, uint32_t flags);
 // Whether the register-streaming fused kernel may be used (DIPAMD_STREAM2D=0 disables).
 bool stream2d_enabled();
+int stream2d_target_waves();  // waves per launch of the register-streaming kernel
 // Kernel for a pass along a strided axis: 0 column tiles, 1 LDS ring, 2 global sweep
 // (DIPAMD_YSWEEP=col|ring|global; column tiles by default).
 int ysweep_kind();

@@ -160,9 +160,13 @@ struct StreamArgs {
     int chunk;            // output rows per wave
     int nstrips, nchunks;
     int rx, ry, bcx, bcy;
+    // rows outside [0, ny) fold branch-free: z < 0 -> ylo_m*z + ylo_c, z >= ny ->
+    // yhi_m*z + yhi_c, with flag yflag (0 as is, 1 negate, 2 constant) -- one reflection
+    int ylo_m, ylo_c, yhi_m, yhi_c, yflag;
     float cvalx, cvaly;
     int aligned_out;      // output rows 16-byte aligned: vector stores
-    int64_t nwaves;
+    int64_t nwaves;       // nstrips * nchunks * nplanes
+    int64_t nplanes;      // product of the outer sizes
     float tx[kStreamRMax + 1];
     float ty[kStreamRMax + 1];
     Outer outer;          // stack of planes

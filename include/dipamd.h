@@ -177,6 +177,9 @@ typedef struct dipamd_trace_record {
     char name[48];     /* "fused2d", "line", "sweep", "cast_in", "cast_out", ... */
     double ms;         /* event-timed duration of the launch */
     int64_t bytes;     /* algorithmic bytes of the launch: samples read + written once */
+    int64_t ops;       /* algorithmic VALU lane-ops of the launch: per output sample and filtered
+                          axis, 3r+1 for a symmetric filter of radius r in exact mode (add, mul,
+                          add per tap pair and the centre multiply), 2r+1 in fast mode */
 } dipamd_trace_record;
 int dipamd_trace_enable(int32_t on);
 /* Waits for the recorded events and copies up to `capacity` records; returns the count. */
