@@ -15,6 +15,8 @@
 // Reversed taps of an even filter are the filter itself, of an odd one its negative.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -31,6 +33,11 @@ hipError_t launch_adjoint_fold(bool is_double, const void *args, int64_t nlines,
 hipError_t launch_dot(bool is_double, const void *a, const void *geo, const void *b, double *partials, double *out,
                       hipStream_t stream);
 hipError_t launch_fill_doubles(double *p, int n, double v, hipStream_t stream);
+hipError_t launch_bwd2d(int symx, int symy, int rc, const BwdArgs &a, int64_t nblocks, double *gx, double *gy,
+                        hipStream_t stream);
+size_t bwd2d_lds_bytes(int rc);
+int bwd2d_class(int rx, int ry);
+int bwd2d_tile_columns(int rc);
 
 namespace {
 
@@ -96,6 +103,114 @@ int adjoint_pass(const View &src, const View &dst, int ndim, int d, const Intern
     if (st != DIPAMD_OK || !folds) return st;
     return is_double ? fold_pass<double>(src, dst, ndim, d, FT, bc, stream)
                      : fold_pass<float>(src, dst, ndim, d, FT, bc, stream);
+}
+
+bool bwd_fused_enabled() {
+    static bool on = [] {
+        const char *v = std::getenv("DIPAMD_BWD_FUSED");
+        return !(v && v[0] == '0');
+    }();
+    return on;
+}
+
+int bwd_mode(int bc) {
+    switch (bc) {
+        case DIPAMD_BC_PERIODIC:
+        case DIPAMD_BC_ASYMMETRIC_PERIODIC: return kBwdPeriodic;
+        case DIPAMD_BC_SYMMETRIC_MIRROR:
+        case DIPAMD_BC_ASYMMETRIC_MIRROR:
+        case DIPAMD_BC_ZERO_ORDER_EXTRAPOLATE: return kBwdFold;
+        case DIPAMD_BC_ADD_ZEROS: return kBwdZero;
+        default: return -1;  // add max / min: the constant's halo enters dL/dsigma_y
+    }
+}
+
+// The fused 2-D backward (k_bwd2d.hip) when the forward filters dims 0 and 1 only,
+// in float, with unit x strides, symmetric filters of radius <= 20, no line shorter
+// than its filter, and a condition it supports.  Returns -100 when it does not apply.
+int fused_backward_2d(const dipamd_image *x, const dipamd_image *grad_out, const dipamd_image *grad_in,
+                      double *grad_sigma, const std::vector<InternFilter> &F, const std::vector<InternFilter> &dF,
+                      const std::vector<int> &order, const int *bcs, hipStream_t stream) {
+    if (!bwd_fused_enabled()) return -100;
+    const int nd = x->ndim;
+    if (order.size() != 2 || order[0] != 0 || order[1] != 1) return -100;
+    if (x->dtype != DIPAMD_DT_SFLOAT || grad_out->dtype != DIPAMD_DT_SFLOAT) return -100;
+    const bool want_dx = grad_in && grad_in->origin;
+    if (x->strides[0] != 1 || grad_out->strides[0] != 1 || (want_dx && grad_in->strides[0] != 1)) return -100;
+    for (int d = 0; d < 2; ++d) {
+        const InternFilter &f = F[static_cast<size_t>(d)];
+        if ((f.sym != DIPAMD_SYM_EVEN && f.sym != DIPAMD_SYM_ODD) || f.shift != 0 || f.border > 20) return -100;
+        if (bwd_mode(bcs[d]) < 0) return -100;
+        if (x->sizes[d] < 2 * f.border + 1 || x->sizes[d] >= (int64_t(1) << 30)) return -100;
+    }
+    if (nd - 2 > kMaxOuter) return -100;
+    BwdArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.x = static_cast<const float *>(x->origin);
+    a.g = static_cast<const float *>(grad_out->origin);
+    a.dx = want_dx ? static_cast<float *>(grad_in->origin) : nullptr;
+    a.want_dx = want_dx ? 1 : 0;
+    a.nx = static_cast<int>(x->sizes[0]);
+    a.ny = static_cast<int>(x->sizes[1]);
+    a.syx = x->strides[1];
+    a.syg = grad_out->strides[1];
+    a.sydx = want_dx ? grad_in->strides[1] : 0;
+    a.rx = F[0].border;
+    a.ry = F[1].border;
+    a.bcx = bcs[0];
+    a.bcy = bcs[1];
+    a.modex = bwd_mode(bcs[0]);
+    a.modey = bwd_mode(bcs[1]);
+    a.cvalx = 0.f;  // add zeros is the only constant condition taken here
+    int64_t nplanes = 1;
+    a.outer.n = 0;
+    for (int d = 2; d < nd; ++d) {
+        if (x->sizes[d] <= 1) continue;
+        const int k = a.outer.n++;
+        a.outer.size[k] = x->sizes[d];
+        a.outer.sin[k] = x->strides[d];
+        a.outer.sout[k] = want_dx ? grad_in->strides[d] : 0;
+        a.gplane[k] = grad_out->strides[d];
+        nplanes *= x->sizes[d];
+    }
+    auto put = [](float *dst, const InternFilter &f, bool adjoint) {
+        for (size_t k = 0; k < f.data.size(); ++k) {
+            const double v = (adjoint && f.sym == DIPAMD_SYM_ODD && k > 0) ? -f.data[k] : f.data[k];
+            dst[k] = static_cast<float>(v);
+        }
+    };
+    put(a.tx, F[0], false);
+    put(a.tdx, dF[0], false);
+    put(a.txa, F[0], true);
+    put(a.tya, F[1], true);
+    put(a.tdya, dF[1], true);
+    const int rc = bwd2d_class(a.rx, a.ry);  // compile-time window; taps stay zero beyond r
+    if (rc == 0) return -100;
+    const int txc = bwd2d_tile_columns(rc);
+    a.ntiles = (a.nx + txc - 1) / txc;
+    const int64_t base = static_cast<int64_t>(a.ntiles) * nplanes;
+    int64_t want = std::max<int64_t>(1, (2048 + base - 1) / base);
+    int64_t chunk = (a.ny + want - 1) / want;
+    chunk = std::max<int64_t>(kBwdStep, ((chunk + kBwdStep - 1) / kBwdStep) * kBwdStep);
+    a.chunk = static_cast<int>(chunk);
+    a.nchunks = static_cast<int>((a.ny + chunk - 1) / chunk);
+    const int64_t nblocks = base * a.nchunks;
+    if (nblocks >= (int64_t(1) << 31)) return -100;
+    if (bwd2d_lds_bytes(rc) > lds_limit_bytes()) return -100;
+    double *partials = nullptr;
+    hipError_t e = scratch_alloc(reinterpret_cast<void **>(&partials), static_cast<size_t>(2 * nblocks) * sizeof(double),
+                                 stream);
+    if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("scratch allocation: ") + hipGetErrorString(e));
+    a.partials = partials;
+    if (grad_sigma) e = launch_fill_doubles(grad_sigma, nd, 0.0, stream);
+    if (e == hipSuccess) {
+        TraceScope ts("bwd2d", static_cast<int64_t>(a.nx) * a.ny * nplanes * (want_dx ? 12 : 8), stream);
+        e = launch_bwd2d(F[0].sym, F[1].sym, rc, a, nblocks, grad_sigma ? grad_sigma + 0 : nullptr,
+                         grad_sigma ? grad_sigma + 1 : nullptr, stream);
+    }
+    scratch_free(partials, stream);
+    if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("backward launch: ") + hipGetErrorString(e));
+    return DIPAMD_OK;
 }
 
 }  // namespace
@@ -165,6 +280,8 @@ extern "C" int dipamd_gauss_fir_backward(const dipamd_image *x, const dipamd_ima
         order[j] = e;
     }
     const int m = static_cast<int>(order.size());
+    st = fused_backward_2d(x, grad_out, grad_in, grad_sigma, F, dF, order, bcs, stream);
+    if (st != -100) return st;
 
     std::vector<void *> scratch;
     auto alloc = [&](View *v) -> int {

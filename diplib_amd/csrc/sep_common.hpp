@@ -98,6 +98,31 @@ struct Outer {
     int64_t sout[kMaxOuter];
 };
 
+// Logical block index with each XCD owning a contiguous range: the hardware hands
+// consecutive workgroups to the 8 XCDs in turn, and each XCD has its own L2, so
+// blocks that share halo rows or columns (neighbours in the logical order) should
+// run on the same XCD.  A bijection of [0, gridDim.x).
+constexpr int kNumXcd = 8;
+__device__ __forceinline__ int64_t xcd_block() {
+    const int64_t bid = blockIdx.x, nblk = gridDim.x;
+    const int64_t per = nblk / kNumXcd, rem = nblk % kNumXcd;
+    const int64_t x = bid % kNumXcd, i = bid / kNumXcd;
+    return x * per + (x < rem ? x : rem) + i;
+}
+
+// Workgroup barrier ordering LDS accesses only.  __syncthreads() also waits for
+// every outstanding global load (its fence covers all address spaces), which would
+// drain the prefetched rows at each barrier; these kernels never read back global
+// data written by another thread of the block, so LDS ordering is all they need.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Wave index within the block as a wave-uniform (scalar) value.
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6)); }
+
 __device__ __forceinline__ void outer_offsets(const Outer &o, int64_t idx, int64_t *oin, int64_t *oout) {
     int64_t a = 0, b = 0;
     for (int d = 0; d < o.n; ++d) {
@@ -210,5 +235,34 @@ struct CastGeo {
     int64_t sout[DIPAMD_MAX_DIMS];
     int64_t total;
 };
+
+// Fused backward pass over stacks of 2-D planes (k_bwd2d.hip): x, dL/dy and dL/dx are
+// float with unit x stride; `outer` holds the plane sizes with the x (sin) and dL/dx
+// (sout) strides, gplane the dL/dy strides.  Taps are half filters, centre first.
+struct BwdArgs {
+    const float *x;
+    const float *g;
+    float *dx;
+    double *partials;  // two per block: <h, Ax' x> (sigma x), <h', Ax x> (sigma y)
+    int nx, ny;
+    int64_t syx, syg, sydx;  // row strides
+    int rx, ry;
+    int bcx, bcy;
+    int modex, modey;  // kBwdZero, kBwdPeriodic, kBwdFold
+    float cvalx;       // the forward x extension's constant
+    int chunk, nchunks, ntiles;
+    int want_dx;
+    Outer outer;
+    int64_t gplane[kMaxOuter];
+    float tx[kFusedCap], tdx[kFusedCap];    // forward x filter and its sigma derivative
+    float txa[kFusedCap];                   // adjoint x filter (odd: negated)
+    float tya[kFusedCap], tdya[kFusedCap];  // adjoint y filter and adjoint y derivative
+};
+// How an adjoint pass treats positions outside the line: dL/dy zero-padded (constant
+// conditions), dL/dy periodically extended (periodic conditions: self-adjoint with
+// reversed taps), or zero-padded plus the halo outputs folded back onto the pixels the
+// forward extension copied them from (mirror conditions, zero order).
+constexpr int kBwdZero = 0, kBwdPeriodic = 1, kBwdFold = 2;
+constexpr int kBwdStep = 16;   // rows per step
 
 }  // namespace dipamd

@@ -298,9 +298,10 @@ __global__ __launch_bounds__(kBlock) void k_line(const PassArgs<TB> a, int64_t n
     const int B = a.border;
     const int W = TX + 2 * B;
     const int pitch = lpad_size(W);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t seg = blockIdx.x % nseg;
-    const int64_t line = (blockIdx.x / nseg) * 4 + wave;
+    const int wave = wave_id(), lane = threadIdx.x & 63;
+    const int64_t bid = xcd_block();
+    const int64_t seg = bid % nseg;
+    const int64_t line = (bid / nseg) * 4 + wave;
     const int x0 = static_cast<int>(seg) * TX;
     const int n = static_cast<int>(a.n);
     TB *row = lds + wave * pitch;
@@ -317,7 +318,7 @@ __global__ __launch_bounds__(kBlock) void k_line(const PassArgs<TB> a, int64_t n
             for (int i = lane; i < W; i += 64) row[lpad(i)] = fetch_ext<TI, TB>(src, a.sin, x0 - B + i, n, a.bc, a.cval);
         }
     }
-    __syncthreads();
+    lds_barrier();
     if (!active) return;
     LdsRowSrc<TB> s{row, B + lane * P, 0, 0};
     TB acc[P];
@@ -348,11 +349,13 @@ __global__ __launch_bounds__(kBlock) void k_line(const PassArgs<TB> a, int64_t n
 template <typename TI, typename TB, int P, int SYM, bool FAST>
 __global__ __launch_bounds__(kBlock) void k_sweep(const PassArgs<TB> a, int64_t ntiles, int64_t nchunks,
                                                    const TI *__restrict__ in, TB *__restrict__ out) {
-    const int64_t bid = blockIdx.x;
-    const int64_t tile = bid % ntiles;
-    const int64_t rest = bid / ntiles;
-    const int64_t chunk = rest % nchunks;
-    const int64_t outer = rest / nchunks;
+    // chunk fastest: consecutive blocks (one XCD, one L2) go down a column of tiles
+    // and share the halo rows between chunks; columns share nothing
+    const int64_t bid = xcd_block();
+    const int64_t chunk = bid % nchunks;
+    const int64_t rest = bid / nchunks;
+    const int64_t tile = rest % ntiles;
+    const int64_t outer = rest / ntiles;
     const int64_t c = tile * blockDim.x + threadIdx.x;
     if (c >= a.e) return;
     int64_t oin, oout;
@@ -405,11 +408,13 @@ __global__ __launch_bounds__(kBlock) void k_colsweep(const PassArgs<TB> a, int64
                                                       const TI *__restrict__ in, TB *__restrict__ out) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     TB *lds = reinterpret_cast<TB *>(smem_raw);
-    const int64_t bid = blockIdx.x;
-    const int64_t tile = bid % ntiles;
-    const int64_t rest = bid / ntiles;
-    const int64_t chunk = rest % nchunks;
-    const int64_t outer = rest / nchunks;
+    // chunk fastest: consecutive blocks (one XCD, one L2) go down a column of tiles
+    // and share the halo rows between chunks; columns share nothing
+    const int64_t bid = xcd_block();
+    const int64_t chunk = bid % nchunks;
+    const int64_t rest = bid / nchunks;
+    const int64_t tile = rest % ntiles;
+    const int64_t outer = rest / ntiles;
     int64_t oin, oout;
     outer_offsets(a.outer, outer, &oin, &oout);
     const int B = a.border;
@@ -443,7 +448,7 @@ __global__ __launch_bounds__(kBlock) void k_colsweep(const PassArgs<TB> a, int64
             if (h < H) col[h] = v[i];
         }
     }
-    __syncthreads();
+    lds_barrier();
     if (!cok) return;
     TB *dst = out + oout + c * a.eout;
     const int per = R / 4;
@@ -514,7 +519,7 @@ __device__ __forceinline__ void fused_tile(const Tile<TB> &t, const TI *__restri
     const int nsteps = (zend - z0 + S - 1) / S;
     const int M = A + nsteps;
     const int tid = threadIdx.x;
-    const int wave = tid >> 6, lane = tid & 63;
+    const int wave = wave_id(), lane = tid & 63;
     const int nld = (Wx + 63) / 64;
 
     // boundary-folded column offsets of this lane's staged samples (fixed per tile)
@@ -538,7 +543,9 @@ __device__ __forceinline__ void fused_tile(const Tile<TB> &t, const TI *__restri
     }
 
     // --- staged-row loads: wave w loads rows w, w+4, w+8, w+12 of each step ---
-    TB pre[4][NLD];
+    // Raw samples, loaded unconditionally (offsets stay in the row) and converted only
+    // when they are written to LDS one step later, so the loads stay in flight.
+    TI pre[4][NLD];
     auto load_step = [&](int m) {
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
@@ -550,28 +557,26 @@ __device__ __forceinline__ void fused_tile(const Tile<TB> &t, const TI *__restri
             }
             const TI *rowp = src + static_cast<int64_t>(j) * t.syin;
 #pragma unroll
-            for (int i = 0; i < NLD; ++i) {
-                if (i < nld) {
-                    const TB v = to_buf<TB>(rowp[cofs[i]]);
-                    pre[rr][i] = cmul[i] > 0 ? v : (cmul[i] < 0 ? -v : t.cvalx);
-                }
-            }
+            for (int i = 0; i < NLD; ++i) pre[rr][i] = rowp[cofs[i]];
         }
     };
 
     load_step(0);
     for (int m = 0; m < M; ++m) {
-        __syncthreads();  // previous x stage done with `stg`
+        lds_barrier();  // previous x stage done with `stg`
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
             TB *srow = stg + (wave + 4 * rr) * pitchS;
 #pragma unroll
             for (int i = 0; i < NLD; ++i) {
                 const int col = lane + 64 * i;
-                if (i < nld && col < Wx) srow[lpad(col)] = pre[rr][i];
+                if (i < nld && col < Wx) {
+                    const TB v = to_buf<TB>(pre[rr][i]);
+                    srow[lpad(col)] = cmul[i] > 0 ? v : (cmul[i] < 0 ? -v : t.cvalx);
+                }
             }
         }
-        __syncthreads();  // `stg` full; previous y stage done with the ring slots we overwrite
+        lds_barrier();  // `stg` full; previous y stage done with the ring slots we overwrite
         if (m + 1 < M) load_step(m + 1);
 
         // --- x stage: row xr of this step, outputs xl*P .. xl*P+P-1 ---
@@ -580,7 +585,11 @@ __device__ __forceinline__ void fused_tile(const Tile<TB> &t, const TI *__restri
             const int j = z0 - By + m * S + xr;
             int mul = 1;
             bool isc = false;
-            if (j < 0 || j >= t.ny) isc = bc_fold(j, t.ny, t.bcy, &mul) < 0;
+            // rows outside the plane only in the first and last steps (a uniform test)
+            const int jstep = z0 - By + m * S;
+            if (jstep < 0 || jstep + S > t.ny) {
+                if (j < 0 || j >= t.ny) isc = bc_fold(j, t.ny, t.bcy, &mul) < 0;
+            }
             TB *rrow = ring + ((m * S + xr) % RC) * pitchR;
             if (isc) {
 #pragma unroll
@@ -598,7 +607,7 @@ __device__ __forceinline__ void fused_tile(const Tile<TB> &t, const TI *__restri
                 for (int p = 0; p < P; ++p) rrow[lpad(xl * P + p)] = mul < 0 ? -acc[p] : acc[p];
             }
         }
-        __syncthreads();  // ring rows of this step written
+        lds_barrier();  // ring rows of this step written
 
         // --- y stage: step n = m - A ---
         const int n = m - A;
@@ -626,7 +635,7 @@ template <typename TI, typename TB, int TXC, int P, int SX, int SY, bool FAST>
 __global__ __launch_bounds__(kBlock) void k_fused2d(const FusedArgs<TB> a, int64_t ntiles, int64_t nchunks) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     TB *lds = reinterpret_cast<TB *>(smem_raw);
-    const int64_t bid = blockIdx.x;
+    const int64_t bid = xcd_block();
     const int64_t tile = bid % ntiles;
     const int64_t rest = bid / ntiles;
     const int64_t chunk = rest % nchunks;
@@ -669,14 +678,14 @@ __global__ __launch_bounds__(kBlock) void k_batched(const BatchArgs<TB> a) {
     TB *lds = reinterpret_cast<TB *>(smem_raw);
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(a.counter, 1);
-        __syncthreads();
+        lds_barrier();
         const int item = s_item;
         if (item >= a.nitems) break;  // uniform: every wave of the workgroup leaves
         const BatchItem it = a.items[item];
         const BatchImage<TB> im = a.images[it.image];
         const int ntx = im.rx + 1, nty = im.ry + 1;
         for (int k = threadIdx.x; k < ntx + nty; k += blockDim.x) s_taps[k] = a.taps[im.tap_offset + k];
-        __syncthreads();  // taps visible; s_item read by all before it is rewritten
+        lds_barrier();  // taps visible; s_item read by all before it is rewritten
         Tile<TB> t;
         t.in = im.in;
         t.out = im.out;
@@ -697,7 +706,7 @@ __global__ __launch_bounds__(kBlock) void k_batched(const BatchArgs<TB> a) {
         t.cvaly = a.cvaly;
         fused_tile<TI, TB, TXC, P, SYM_EVEN, SYM_EVEN, FAST>(t, static_cast<const TI *>(t.in), t.out, lds, s_taps,
                                                              s_taps + ntx);
-        __syncthreads();  // LDS (ring, taps) free before the next item
+        lds_barrier();  // LDS (ring, taps) free before the next item
     }
 }
 

@@ -186,8 +186,7 @@ __global__ __launch_bounds__(kBlock) void k_stream2d(const StreamArgs a) {
     const int lane = threadIdx.x & 63;
     // the wave index through readfirstlane: everything derived from it (rows, row
     // folds, row pointers, loop bounds) is then known wave-uniform and lives in SGPRs
-    const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-    const int64_t gw = static_cast<int64_t>(blockIdx.x) * 4 + wv;
+    const int64_t gw = xcd_block() * 4 + wave_id();
     if (gw >= a.nwaves) return;
     const int strip = static_cast<int>(gw % a.nstrips);
     const int64_t rest = gw / a.nstrips;
