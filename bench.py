@@ -429,51 +429,63 @@ def bench_batched(dip, torch, np, dev, L):
 
 
 def bench_backward(dip, torch, dev):
+    """Forward and forward+backward through the torch.autograd.Function (sigma
+    requires grad; loss = sum(W * out) via out.backward(W)): a stack of 32 sfloat
+    1024x1024 images (sigma shared by x and y) and a 128^3 sfloat volume (sigma shared
+    by all three axes), derivative orders 0 and 1 along x.  x.grad is reset before
+    each backward so no gradient accumulation is timed."""
     out = {}
     if not hasattr(dip, "gauss_fir_autograd"):
         return out
-    try:
+
+    def run(shape, sdims, orders, seed):
         g = torch.Generator(device=dev)
-        g.manual_seed(5)
-        x = torch.rand((32, 1024, 1024), generator=g, device=dev, dtype=torch.float32)
-        w = torch.randn((32, 1024, 1024), generator=g, device=dev, dtype=torch.float32)
-        res = {}
-        for order in (0, 1):
-            sigma = torch.tensor(3.0, device=dev, dtype=torch.float64, requires_grad=True)
-            xr = x.clone().requires_grad_(True)
+        g.manual_seed(seed)
+        x = torch.rand(shape, generator=g, device=dev, dtype=torch.float32)
+        w = torch.randn(shape, generator=g, device=dev, dtype=torch.float32)
+        sigma = torch.tensor(3.0, device=dev, dtype=torch.float64, requires_grad=True)
+        xr = x.clone().requires_grad_(True)
 
-            def fwd():
-                return dip.gauss_fir_autograd(xr, sigma, [order, 0, 0], ["mirror"], 3.0, sigma_dims=(0, 1))
+        def fwd():
+            return dip.gauss_fir_autograd(xr, sigma, orders, ["mirror"], 3.0, sigma_dims=sdims)
 
-            def fwdbwd():
-                y = fwd()
-                y.backward(w)
+        def fwdbwd():
+            xr.grad = None
+            sigma.grad = None
+            fwd().backward(w)
 
-            for _ in range(2):
+        for _ in range(2):
+            fwd()
+            fwdbwd()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            with torch.no_grad():
                 fwd()
-                fwdbwd()
-            torch.cuda.synchronize()
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record()
-            for _ in range(5):
-                with torch.no_grad():
-                    fwd()
-            e1.record()
-            torch.cuda.synchronize()
-            ms_f = e0.elapsed_time(e1) / 5
-            e0.record()
-            for _ in range(5):
-                fwdbwd()
-            e1.record()
-            torch.cuda.synchronize()
-            ms_fb = e0.elapsed_time(e1) / 5
-            res[f"order{order}"] = {"fwd_ms": round(ms_f, 4), "fwd_bwd_ms": round(ms_fb, 4),
-                                    "ratio": round(ms_fb / ms_f, 3),
-                                    "fwd_GPix_s": round(x.numel() / (ms_f * 1e-3) / 1e9, 2)}
-        out["fwd_bwd_32x1024x1024_s3"] = res
-        log(f"fwd+bwd: {res}")
-    except Exception as e:
-        out["fwd_bwd_32x1024x1024_s3"] = {"error": repr(e)}
+        e1.record()
+        torch.cuda.synchronize()
+        ms_f = e0.elapsed_time(e1) / 5
+        e0.record()
+        for _ in range(5):
+            fwdbwd()
+        e1.record()
+        torch.cuda.synchronize()
+        ms_fb = e0.elapsed_time(e1) / 5
+        return {"fwd_ms": round(ms_f, 4), "fwd_bwd_ms": round(ms_fb, 4), "ratio": round(ms_fb / ms_f, 3),
+                "fwd_GPix_s": round(x.numel() / (ms_f * 1e-3) / 1e9, 2),
+                "fwd_bwd_GPix_s": round(x.numel() / (ms_fb * 1e-3) / 1e9, 2)}
+
+    for key, shape, sdims, nd in (("fwd_bwd_32x1024x1024_s3", (32, 1024, 1024), (0, 1), 3),
+                                  ("fwd_bwd_128x128x128_s3", (128, 128, 128), (0, 1, 2), 3)):
+        try:
+            res = {}
+            for order in (0, 1):
+                res[f"order{order}"] = run(shape, sdims, [order] + [0] * (nd - 1), 5)
+            out[key] = res
+            log(f"{key}: {res}")
+        except Exception as e:  # report, do not hide
+            out[key] = {"error": repr(e)}
     return out
 
 

@@ -349,13 +349,14 @@ __global__ __launch_bounds__(kBlock) void k_line(const PassArgs<TB> a, int64_t n
 template <typename TI, typename TB, int P, int SYM, bool FAST>
 __global__ __launch_bounds__(kBlock) void k_sweep(const PassArgs<TB> a, int64_t ntiles, int64_t nchunks,
                                                    const TI *__restrict__ in, TB *__restrict__ out) {
-    // chunk fastest: consecutive blocks (one XCD, one L2) go down a column of tiles
-    // and share the halo rows between chunks; columns share nothing
-    const int64_t bid = xcd_block();
-    const int64_t chunk = bid % nchunks;
-    const int64_t rest = bid / nchunks;
-    const int64_t tile = rest % ntiles;
-    const int64_t outer = rest / ntiles;
+    // tile fastest in hardware order: the blocks of neighbouring chunks of a column are
+    // ntiles apart, dispatched later (their shared halo rows are then L2 hits, not two
+    // concurrent misses) and, for ntiles a multiple of 8, on the same XCD
+    const int64_t bid = blockIdx.x;
+    const int64_t tile = bid % ntiles;
+    const int64_t rest = bid / ntiles;
+    const int64_t chunk = rest % nchunks;
+    const int64_t outer = rest / nchunks;
     const int64_t c = tile * blockDim.x + threadIdx.x;
     if (c >= a.e) return;
     int64_t oin, oout;
@@ -408,13 +409,14 @@ __global__ __launch_bounds__(kBlock) void k_colsweep(const PassArgs<TB> a, int64
                                                       const TI *__restrict__ in, TB *__restrict__ out) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     TB *lds = reinterpret_cast<TB *>(smem_raw);
-    // chunk fastest: consecutive blocks (one XCD, one L2) go down a column of tiles
-    // and share the halo rows between chunks; columns share nothing
-    const int64_t bid = xcd_block();
-    const int64_t chunk = bid % nchunks;
-    const int64_t rest = bid / nchunks;
-    const int64_t tile = rest % ntiles;
-    const int64_t outer = rest / ntiles;
+    // tile fastest in hardware order: the blocks of neighbouring chunks of a column are
+    // ntiles apart, dispatched later (their shared halo rows are then L2 hits, not two
+    // concurrent misses) and, for ntiles a multiple of 8, on the same XCD
+    const int64_t bid = blockIdx.x;
+    const int64_t tile = bid % ntiles;
+    const int64_t rest = bid / ntiles;
+    const int64_t chunk = rest % nchunks;
+    const int64_t outer = rest / nchunks;
     int64_t oin, oout;
     outer_offsets(a.outer, outer, &oin, &oout);
     const int B = a.border;
