@@ -433,7 +433,8 @@ def bench_backward(dip, torch, dev):
     requires grad; loss = sum(W * out) via out.backward(W)): a stack of 32 sfloat
     1024x1024 images (sigma shared by x and y) and a 128^3 sfloat volume (sigma shared
     by all three axes), derivative orders 0 and 1 along x.  x.grad is reset before
-    each backward so no gradient accumulation is timed."""
+    each backward so no gradient accumulation is timed; sigma's host value is passed
+    (sigma_host) so the device sigma is not read back (a stream wait) every call."""
     out = {}
     if not hasattr(dip, "gauss_fir_autograd"):
         return out
@@ -447,7 +448,7 @@ def bench_backward(dip, torch, dev):
         xr = x.clone().requires_grad_(True)
 
         def fwd():
-            return dip.gauss_fir_autograd(xr, sigma, orders, ["mirror"], 3.0, sigma_dims=sdims)
+            return dip.gauss_fir_autograd(xr, sigma, orders, ["mirror"], 3.0, sigma_dims=sdims, sigma_host=3.0)
 
         def fwdbwd():
             xr.grad = None

@@ -20,8 +20,15 @@ from ._lib import check, lib
 from .gauss import GaussFIR, _bc_codes, _flex_torch, _stream_handle, image_struct
 
 
-def _per_dim_sigmas(sigma, nd, sigma_dims):
-    s = sigma.detach().double().cpu().reshape(-1).tolist()
+def _per_dim_sigmas(sigma, nd, sigma_dims, sigma_host=None):
+    # the taps are computed on the host: a device-resident sigma is read back (which
+    # waits for the stream) unless the caller passes its value as sigma_host
+    if sigma_host is not None:
+        s = [float(v) for v in (sigma_host if hasattr(sigma_host, "__len__") else [sigma_host])]
+        if len(s) != sigma.numel():
+            raise _lib.DipError(_lib.E_ARRAY_LENGTH, "sigma_host must have as many values as sigma")
+    else:
+        s = sigma.detach().double().cpu().reshape(-1).tolist()
     if len(s) == 1:
         dims = range(nd) if sigma_dims is None else sigma_dims
         out = [0.0] * nd
@@ -35,9 +42,9 @@ def _per_dim_sigmas(sigma, nd, sigma_dims):
 
 class _GaussFIRFunction(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, x, sigma, orders, bc, truncation, sigma_dims, fast):
+    def forward(ctx, x, sigma, orders, bc, truncation, sigma_dims, fast, sigma_host):
         nd = x.dim()
-        sig = _per_dim_sigmas(sigma, nd, sigma_dims)
+        sig = _per_dim_sigmas(sigma, nd, sigma_dims, sigma_host)
         y = GaussFIR(x, sig, list(orders), list(bc), truncation, fast=fast)
         ctx.save_for_backward(x, sigma)
         ctx.cfg = (sig, list(orders), list(bc), float(truncation), sigma_dims, fast)
@@ -67,23 +74,32 @@ class _GaussFIRFunction(torch.autograd.Function):
         dsigma = None
         if want_ds:
             if sigma.numel() == 1:
-                dims = range(nd) if sigma_dims is None else sigma_dims
-                dsigma = ds[list(dims)].sum().to(sigma.dtype).reshape(sigma.shape)
+                dims = sorted(range(nd) if sigma_dims is None else sigma_dims)
+                if dims == list(range(dims[0], dims[-1] + 1)):
+                    sel = ds[dims[0]:dims[-1] + 1]  # a view: no index tensor to copy to the device
+                else:
+                    sel = ds[torch.tensor(dims, device=ds.device)]
+                dsigma = sel.sum().to(sigma.dtype).reshape(sigma.shape)
             else:
                 dsigma = ds.to(sigma.dtype).reshape(sigma.shape)
         if want_dx:
             dx = dx.to(x.dtype)
-        return dx, dsigma, None, None, None, None, None
+        return dx, dsigma, None, None, None, None, None, None
 
 
-def gauss_fir_autograd(x, sigma, orders=(0,), bc=("mirror",), truncation=3.0, sigma_dims=None, fast=False):
-    """Differentiable GaussFIR: gradients with respect to `x` (float tensors) and `sigma`."""
+def gauss_fir_autograd(x, sigma, orders=(0,), bc=("mirror",), truncation=3.0, sigma_dims=None, fast=False,
+                       sigma_host=None):
+    """Differentiable GaussFIR: gradients with respect to `x` (float tensors) and `sigma`.
+    `sigma_host` (optional): the value(s) of `sigma` on the host, which the caller
+    guarantees to match, so that a device-resident sigma is not read back each call."""
     if not torch.is_tensor(sigma):
         sigma = torch.tensor(float(sigma), dtype=torch.float64, device=x.device)
     orders = [int(o) for o in (orders if hasattr(orders, "__len__") else [orders])]
     bc = [bc] if isinstance(bc, str) else list(bc)
     dims = None if sigma_dims is None else tuple(int(d) for d in sigma_dims)
-    return _GaussFIRFunction.apply(x, sigma, tuple(orders), tuple(bc), float(truncation), dims, bool(fast))
+    host = None if sigma_host is None else tuple(float(v) for v in (sigma_host if hasattr(sigma_host, "__len__")
+                                                                     else [sigma_host]))
+    return _GaussFIRFunction.apply(x, sigma, tuple(orders), tuple(bc), float(truncation), dims, bool(fast), host)
 
 
 def gauss_fir_backward(x, grad_out, sigmas, orders=(0,), bc=("mirror",), truncation=3.0, fast=False,

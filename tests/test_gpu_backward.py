@@ -100,3 +100,23 @@ def test_autograd_batch_sigma_dims(cuda):
     gx64, gs64 = res[torch.float64]
     assert (gx32 - gx64).abs().max().item() / gx64.abs().max().item() < 1e-5
     assert abs(gs32.item() - gs64.item()) / abs(gs64.item()) < 1e-5
+
+
+def test_autograd_sigma_host(cuda):
+    """A device sigma read back each call, and the same with its host value passed:
+    identical outputs and gradients; an in-place update of sigma is seen."""
+    x = torch.rand((40, 50), dtype=torch.float32, device=cuda)
+    w = torch.randn((40, 50), dtype=torch.float32, device=cuda)
+    res = []
+    for host in (None, 2.0):
+        xr = x.clone().requires_grad_(True)
+        sigma = torch.tensor(2.0, dtype=torch.float64, device=cuda, requires_grad=True)
+        y = dip.gauss_fir_autograd(xr, sigma, [1, 0], ["mirror"], 3.0, sigma_host=host)
+        (y * w).sum().backward()
+        res.append((y.detach(), xr.grad, sigma.grad))
+    assert all(torch.equal(a, b) for a, b in zip(res[0], res[1]))
+    sigma = torch.tensor(2.0, dtype=torch.float64, device=cuda, requires_grad=True)
+    with torch.no_grad():
+        sigma.add_(1.0)
+    y2 = dip.gauss_fir_autograd(x, sigma, [0], ["mirror"], 3.0)
+    assert torch.equal(y2, dip.GaussFIR(x, [3.0], [0], ["mirror"], 3.0))
