@@ -125,53 +125,56 @@ int bwd_mode(int bc) {
     }
 }
 
-// The fused 2-D backward (k_bwd2d.hip) when the forward filters dims 0 and 1 only,
-// in float, with unit x strides, symmetric filters of radius <= 20, no line shorter
-// than its filter, and a condition it supports.  Returns -100 when it does not apply.
-int fused_backward_2d(const dipamd_image *x, const dipamd_image *grad_out, const dipamd_image *grad_in,
-                      double *grad_sigma, const std::vector<InternFilter> &F, const std::vector<InternFilter> &dF,
-                      const std::vector<int> &order, const int *bcs, hipStream_t stream) {
+// The fused 2-D backward (k_bwd2d.hip) for the passes along dims 0 and 1, on views:
+// x (the forward input), g (dL/d(output of those two passes)), dx (dL/dx, or p == null
+// for none), in float with unit x strides, symmetric filters of radius <= 20, no line
+// shorter than its filter and a condition it supports; dims >= 2 are stacked planes.
+// dL/dsigma along dims 0 and 1 is ADDED to gs0 / gs1 (device doubles, nullable).
+// Returns -100 when it does not apply.
+int fused_backward_2d(const View &x, const View &g, const View &dx, int nd, const InternFilter &F0,
+                      const InternFilter &F1, const InternFilter &dF0, const InternFilter &dF1, int bc0, int bc1,
+                      double *gs0, double *gs1, hipStream_t stream) {
     if (!bwd_fused_enabled()) return -100;
-    const int nd = x->ndim;
-    if (order.size() != 2 || order[0] != 0 || order[1] != 1) return -100;
-    if (x->dtype != DIPAMD_DT_SFLOAT || grad_out->dtype != DIPAMD_DT_SFLOAT) return -100;
-    const bool want_dx = grad_in && grad_in->origin;
-    if (x->strides[0] != 1 || grad_out->strides[0] != 1 || (want_dx && grad_in->strides[0] != 1)) return -100;
+    if (x.dt != DIPAMD_DT_SFLOAT || g.dt != DIPAMD_DT_SFLOAT) return -100;
+    const bool want_dx = dx.p != nullptr;
+    if (x.strides[0] != 1 || g.strides[0] != 1 || (want_dx && dx.strides[0] != 1)) return -100;
+    const InternFilter *Fs[2] = {&F0, &F1};
+    const int bcs[2] = {bc0, bc1};
     for (int d = 0; d < 2; ++d) {
-        const InternFilter &f = F[static_cast<size_t>(d)];
+        const InternFilter &f = *Fs[d];
         if ((f.sym != DIPAMD_SYM_EVEN && f.sym != DIPAMD_SYM_ODD) || f.shift != 0 || f.border > 20) return -100;
         if (bwd_mode(bcs[d]) < 0) return -100;
-        if (x->sizes[d] < 2 * f.border + 1 || x->sizes[d] >= (int64_t(1) << 30)) return -100;
+        if (x.sizes[d] < 2 * f.border + 1 || x.sizes[d] >= (int64_t(1) << 30)) return -100;
     }
     if (nd - 2 > kMaxOuter) return -100;
     BwdArgs a;
     std::memset(&a, 0, sizeof(a));
-    a.x = static_cast<const float *>(x->origin);
-    a.g = static_cast<const float *>(grad_out->origin);
-    a.dx = want_dx ? static_cast<float *>(grad_in->origin) : nullptr;
+    a.x = static_cast<const float *>(x.p);
+    a.g = static_cast<const float *>(g.p);
+    a.dx = want_dx ? static_cast<float *>(dx.p) : nullptr;
     a.want_dx = want_dx ? 1 : 0;
-    a.nx = static_cast<int>(x->sizes[0]);
-    a.ny = static_cast<int>(x->sizes[1]);
-    a.syx = x->strides[1];
-    a.syg = grad_out->strides[1];
-    a.sydx = want_dx ? grad_in->strides[1] : 0;
-    a.rx = F[0].border;
-    a.ry = F[1].border;
-    a.bcx = bcs[0];
-    a.bcy = bcs[1];
-    a.modex = bwd_mode(bcs[0]);
-    a.modey = bwd_mode(bcs[1]);
+    a.nx = static_cast<int>(x.sizes[0]);
+    a.ny = static_cast<int>(x.sizes[1]);
+    a.syx = x.strides[1];
+    a.syg = g.strides[1];
+    a.sydx = want_dx ? dx.strides[1] : 0;
+    a.rx = F0.border;
+    a.ry = F1.border;
+    a.bcx = bc0;
+    a.bcy = bc1;
+    a.modex = bwd_mode(bc0);
+    a.modey = bwd_mode(bc1);
     a.cvalx = 0.f;  // add zeros is the only constant condition taken here
     int64_t nplanes = 1;
     a.outer.n = 0;
     for (int d = 2; d < nd; ++d) {
-        if (x->sizes[d] <= 1) continue;
+        if (x.sizes[d] <= 1) continue;
         const int k = a.outer.n++;
-        a.outer.size[k] = x->sizes[d];
-        a.outer.sin[k] = x->strides[d];
-        a.outer.sout[k] = want_dx ? grad_in->strides[d] : 0;
-        a.gplane[k] = grad_out->strides[d];
-        nplanes *= x->sizes[d];
+        a.outer.size[k] = x.sizes[d];
+        a.outer.sin[k] = x.strides[d];
+        a.outer.sout[k] = want_dx ? dx.strides[d] : 0;
+        a.gplane[k] = g.strides[d];
+        nplanes *= x.sizes[d];
     }
     auto put = [](float *dst, const InternFilter &f, bool adjoint) {
         for (size_t k = 0; k < f.data.size(); ++k) {
@@ -179,11 +182,11 @@ int fused_backward_2d(const dipamd_image *x, const dipamd_image *grad_out, const
             dst[k] = static_cast<float>(v);
         }
     };
-    put(a.tx, F[0], false);
-    put(a.tdx, dF[0], false);
-    put(a.txa, F[0], true);
-    put(a.tya, F[1], true);
-    put(a.tdya, dF[1], true);
+    put(a.tx, F0, false);
+    put(a.tdx, dF0, false);
+    put(a.txa, F0, true);
+    put(a.tya, F1, true);
+    put(a.tdya, dF1, true);
     const int rc = bwd2d_class(a.rx, a.ry);  // compile-time window; taps stay zero beyond r
     if (rc == 0) return -100;
     const int txc = bwd2d_tile_columns(rc);
@@ -202,11 +205,9 @@ int fused_backward_2d(const dipamd_image *x, const dipamd_image *grad_out, const
                                  stream);
     if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("scratch allocation: ") + hipGetErrorString(e));
     a.partials = partials;
-    if (grad_sigma) e = launch_fill_doubles(grad_sigma, nd, 0.0, stream);
-    if (e == hipSuccess) {
+    {
         TraceScope ts("bwd2d", static_cast<int64_t>(a.nx) * a.ny * nplanes * (want_dx ? 12 : 8), stream);
-        e = launch_bwd2d(F[0].sym, F[1].sym, rc, a, nblocks, grad_sigma ? grad_sigma + 0 : nullptr,
-                         grad_sigma ? grad_sigma + 1 : nullptr, stream);
+        e = launch_bwd2d(F0.sym, F1.sym, rc, a, nblocks, gs0, gs1, stream);
     }
     scratch_free(partials, stream);
     if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("backward launch: ") + hipGetErrorString(e));
@@ -280,8 +281,21 @@ extern "C" int dipamd_gauss_fir_backward(const dipamd_image *x, const dipamd_ima
         order[j] = e;
     }
     const int m = static_cast<int>(order.size());
-    st = fused_backward_2d(x, grad_out, grad_in, grad_sigma, F, dF, order, bcs, stream);
-    if (st != -100) return st;
+    // dims 0 and 1 of a float image through the fused 2-D kernel: the whole backward
+    // when only they are filtered, else after the adjoint passes of the other dims
+    const bool fuse01 = m >= 2 && order[0] == 0 && order[1] == 1;
+    if (fuse01 && m == 2) {
+        if (want_ds) {
+            hipError_t e = launch_fill_doubles(grad_sigma, nd, 0.0, stream);
+            if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("fill: ") + hipGetErrorString(e));
+        }
+        View dxv;
+        std::memset(&dxv, 0, sizeof(dxv));
+        if (want_dx) dxv = to_view(grad_in);
+        st = fused_backward_2d(to_view(x), to_view(grad_out), dxv, nd, F[0], F[1], dF[0], dF[1], bcs[0], bcs[1],
+                               want_ds ? grad_sigma : nullptr, want_ds ? grad_sigma + 1 : nullptr, stream);
+        if (st != -100) return st;
+    }
 
     std::vector<void *> scratch;
     auto alloc = [&](View *v) -> int {
@@ -345,6 +359,15 @@ extern "C" int dipamd_gauss_fir_backward(const dipamd_image *x, const dipamd_ima
     }
     for (int i = m; i >= 1; --i) {
         const int d = order[static_cast<size_t>(i - 1)];
+        if (i == 2 && fuse01 && m > 2) {
+            // the remaining passes (dims 0 and 1) and their sigma derivatives in one kernel
+            View dxv;
+            std::memset(&dxv, 0, sizeof(dxv));
+            if (want_dx) dxv = to_view(grad_in);
+            st = fused_backward_2d(xv, u[2], dxv, nd, F[0], F[1], dF[0], dF[1], bcs[0], bcs[1],
+                                   want_ds ? grad_sigma : nullptr, want_ds ? grad_sigma + 1 : nullptr, stream);
+            if (st != -100) return finish(st);
+        }
         if (want_ds) {
             // dL/dsigma_d = < u_i , K'_i v_{i-1} >
             if (!have_w) {

@@ -81,3 +81,29 @@ def test_fused_backward_dx_only_and_dsigma_only(cuda):
     _, ds_c = dip.gauss_fir_backward(x, w, [2.5, 2.5, 0.0], [0, 0, 0], ["mirror"], want_dx=False)
     assert torch.equal(dx_a, dx_b)
     assert torch.equal(ds_a, ds_c)
+
+
+@pytest.mark.parametrize("bc", BCS)
+def test_fused_backward_3d_vs_fp64(cuda, bc):
+    """3-D: the z adjoint pass and sigma_z by passes, then dims x and y through the fused
+    kernel on the z adjoint of dL/dy."""
+    shape = (20, 30, 40)  # z, y, x
+    sig, ords = [2.0, 2.5, 1.5], [0, 1, 0]
+    g = torch.Generator(device=cuda)
+    g.manual_seed(9)
+    x = torch.rand(shape, generator=g, device=cuda, dtype=torch.float32)
+    w = torch.randn(shape, generator=g, device=cuda, dtype=torch.float32)
+    dx32, ds32 = dip.gauss_fir_backward(x, w, sig, ords, [bc])
+    dx64, ds64 = dip.gauss_fir_backward(x.double(), w.double(), sig, ords, [bc])
+    err = (dx32.double() - dx64).abs().max().item() / dx64.abs().max().item()
+    assert err < 2e-6, err
+    ds32, ds64 = ds32.cpu().numpy(), ds64.cpu().numpy()
+    xn, wn = x.double().cpu().numpy(), w.double().cpu().numpy()
+    radii = [oc.gauss_radius(s, o, 3.0, True) for s, o in zip(sig, ords)]
+    for d in range(3):
+        sp, sm = list(sig), list(sig)
+        sp[d] += 1e-4
+        sm[d] -= 1e-4
+        dyds = (oc.gauss_fixed_radius(xn, sp, ords, [bc], radii) - oc.gauss_fixed_radius(xn, sm, ords, [bc], radii)) / 2e-4
+        bound = np.linalg.norm(wn) * np.linalg.norm(dyds)
+        assert abs(ds32[d] - ds64[d]) <= 1e-6 * bound, (d, ds32[d], ds64[d], bound)
