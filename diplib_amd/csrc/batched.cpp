@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -20,6 +22,22 @@
 #include "sep_kernels.hpp"
 
 using namespace dipamd;
+
+namespace {
+// Row-chunk rule of the work items: max(minimum rows, rows per unit of y radius).
+// DIPAMD_BATCH_CHUNK="<min>,<per radius>" overrides it for experiments.
+std::pair<int, int> batch_chunk_rule() {
+    static std::pair<int, int> rule = [] {
+        std::pair<int, int> r{64, 8};
+        if (const char *v = std::getenv("DIPAMD_BATCH_CHUNK")) {
+            int a = 0, b = 0;
+            if (std::sscanf(v, "%d,%d", &a, &b) == 2 && a >= 16 && b >= 1) r = {a, b};
+        }
+        return r;
+    }();
+    return rule;
+}
+}  // namespace
 
 struct dipamd_batch_plan {
     int n = 0;
@@ -121,7 +139,7 @@ int build_plan(dipamd_batch_plan *P, const dipamd_batch_image *images, int n) {
         const auto &im = ims[static_cast<size_t>(i)];
         const int ntiles = static_cast<int>((im.nx + txc - 1) / txc);
         // row chunks: long enough that the y halo stays small, short enough to balance
-        int ch = std::max(64, ((8 * im.ry + 15) / 16) * 16);
+        int ch = std::max(batch_chunk_rule().first, ((batch_chunk_rule().second * im.ry + 15) / 16) * 16);
         ch = std::min<int>(ch, static_cast<int>(((im.ny + 15) / 16) * 16));
         for (int64_t z = 0; z < im.ny; z += ch)
             for (int t = 0; t < ntiles; ++t)

@@ -3,7 +3,21 @@
 
 #include "sep_config.hpp"
 
+#include <cstdlib>
+
 namespace dipamd {
+
+namespace {
+// waves per workgroup: eight, so that a workgroup computing beside one that stages
+// its tile still has two waves per SIMD to hide the LDS latency
+int colsweep_waves() {
+    static int nw = [] {
+        const char *v = std::getenv("DIPAMD_COLSWEEP_WAVES");
+        return (v && std::atoi(v) == 4) ? 4 : 8;
+    }();
+    return nw;
+}
+}  // namespace
 
 int colsweep_rows(int border, bool is_double) {
     // staged samples per column: R + 2*border, 64 columns, within the LDS limit
@@ -11,8 +25,8 @@ int colsweep_rows(int border, bool is_double) {
     const size_t es = is_double ? 8 : 4;
     const int hmax = static_cast<int>(65536 / (64 * es)) - 1;  // odd pitch <= hmax+1
     int R = hmax - 2 * border;
-    R = (R / (4 * P)) * (4 * P);
-    return R >= 4 * P ? R : 0;
+    R = (R / P) * P;
+    return R >= P ? R : 0;
 }
 
 namespace {
@@ -23,8 +37,13 @@ hipError_t col_t(const PassArgs<TB> &a, int64_t e, int64_t nchunks, int64_t nout
     const int64_t nblk = ntiles * nchunks * nouter;
     const int H = static_cast<int>(a.chunk) + 2 * a.border;
     const size_t lds = static_cast<size_t>(64) * (H | 1) * sizeof(TB);
-    hipLaunchKernelGGL((k_colsweep<TI, TB, P, SYM, FAST>), dim3(static_cast<unsigned>(nblk)), dim3(kBlock), lds, st,
-                       a, ntiles, nchunks, static_cast<const TI *>(a.in), static_cast<TB *>(a.out));
+    if (colsweep_waves() == 4) {
+        hipLaunchKernelGGL((k_colsweep<TI, TB, P, SYM, FAST, 4>), dim3(static_cast<unsigned>(nblk)), dim3(256), lds,
+                           st, a, ntiles, nchunks, static_cast<const TI *>(a.in), static_cast<TB *>(a.out));
+    } else {
+        hipLaunchKernelGGL((k_colsweep<TI, TB, P, SYM, FAST, 8>), dim3(static_cast<unsigned>(nblk)), dim3(512), lds,
+                           st, a, ntiles, nchunks, static_cast<const TI *>(a.in), static_cast<TB *>(a.out));
+    }
     return hipGetLastError();
 }
 

@@ -404,9 +404,9 @@ struct LdsColSrc {  // one staged column in LDS, unit stride
     __device__ TB lnext() { return col[li--]; }
 };
 
-template <typename TI, typename TB, int P, int SYM, bool FAST>
-__global__ __launch_bounds__(kBlock) void k_colsweep(const PassArgs<TB> a, int64_t ntiles, int64_t nchunks,
-                                                      const TI *__restrict__ in, TB *__restrict__ out) {
+template <typename TI, typename TB, int P, int SYM, bool FAST, int NW>
+__global__ __launch_bounds__(64 * NW) void k_colsweep(const PassArgs<TB> a, int64_t ntiles, int64_t nchunks,
+                                                       const TI *__restrict__ in, TB *__restrict__ out) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     TB *lds = reinterpret_cast<TB *>(smem_raw);
     // tile fastest in hardware order: the blocks of neighbouring chunks of a column are
@@ -420,7 +420,7 @@ __global__ __launch_bounds__(kBlock) void k_colsweep(const PassArgs<TB> a, int64
     int64_t oin, oout;
     outer_offsets(a.outer, outer, &oin, &oout);
     const int B = a.border;
-    const int R = static_cast<int>(a.chunk);  // outputs per column in this block (multiple of 4*P)
+    const int R = static_cast<int>(a.chunk);  // outputs per column in this block (multiple of P)
     const int H = R + 2 * B;                   // staged samples per column
     const int cpitch = H | 1;
     const int n = static_cast<int>(a.n);
@@ -430,14 +430,14 @@ __global__ __launch_bounds__(kBlock) void k_colsweep(const PassArgs<TB> a, int64
     const bool cok = c < a.e;
     const TI *src = in + oin + (cok ? c : 0) * a.ein;
     TB *col = lds + lane * cpitch;
-    // stage: wave `grp` loads samples grp, grp+4, ... of every column (rows are
+    // stage: wave `grp` loads samples grp, grp+NW, ... of every column (rows are
     // coalesced across the lanes), eight loads in flight before their LDS stores
     const bool interior = z0 - B >= 0 && z0 + R + B <= n;
-    for (int h0 = grp; h0 < H; h0 += 32) {
+    for (int h0 = grp; h0 < H; h0 += 8 * NW) {
         TB v[8];
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const int h = h0 + 4 * i;
+            const int h = h0 + NW * i;
             if (h < H) {
                 const int j = z0 - B + h;
                 v[i] = interior ? to_buf<TB>(src[static_cast<int64_t>(j) * a.sin])
@@ -446,19 +446,18 @@ __global__ __launch_bounds__(kBlock) void k_colsweep(const PassArgs<TB> a, int64
         }
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const int h = h0 + 4 * i;
+            const int h = h0 + NW * i;
             if (h < H) col[h] = v[i];
         }
     }
     lds_barrier();
     if (!cok) return;
     TB *dst = out + oout + c * a.eout;
-    const int per = R / 4;
     const int zend = min(z0 + R, n);
-    for (int blk = 0; blk < per; blk += P) {
-        const int zs = z0 + grp * per + blk;
+    for (int blk = grp * P; blk < R; blk += NW * P) {  // wave `grp` takes blocks of P rows grp, grp+NW, ...
+        const int zs = z0 + blk;
         if (zs >= zend) break;
-        LdsColSrc<TB> s{col, B + grp * per + blk, 0, 0};
+        LdsColSrc<TB> s{col, B + blk, 0, 0};
         TB acc[P];
         fir_window<TB, P, SYM, FAST>(a.taps, a.ntaps, a.shift, s, acc);
         TB *d = dst + static_cast<int64_t>(zs) * a.sout;
