@@ -613,7 +613,9 @@ __device__ __forceinline__ void fused_tile(const Tile<TB> &t, const TI *__restri
         // --- y stage: step n = m - A ---
         const int n = m - A;
         if (n >= 0) {
-            const int col = tid % TXC, grp = tid / TXC;
+            // a wave lies within one row group (TXC >= 64): taken from the wave index, the
+            // ring row offsets are wave-uniform and their wrap-around is scalar work
+            const int col = tid % TXC, grp = (wave * 64) / TXC;
             const int c = c0 + col;
             const int rel0 = n * S + grp * P + By;  // ring-relative row of output p = 0
             RingSrc<TB> s{ring + lpad(col), pitchR, RC, RC * pitchR, rel0 % RC, 0, 0};
