@@ -285,6 +285,12 @@ bool bc_supported(int bc) {
            bc == DIPAMD_BC_ADD_MIN_VALUE || bc == DIPAMD_BC_ZERO_ORDER_EXTRAPOLATE;
 }
 
+// Conditions whose extension is computed from the edge values (k_expand.hip).
+bool bc_value_based(int bc) {
+    return bc == DIPAMD_BC_ANTISYMMETRIC_REFLECT || bc == DIPAMD_BC_FIRST_ORDER_EXTRAPOLATE ||
+           bc == DIPAMD_BC_SECOND_ORDER_EXTRAPOLATE || bc == DIPAMD_BC_THIRD_ORDER_EXTRAPOLATE;
+}
+
 // ---------------------------------------------------------------------------
 // Single-axis and fused passes
 // ---------------------------------------------------------------------------
@@ -389,6 +395,64 @@ static int run_single(const View &src, const View &dst, int ndim, int d, const I
     }
     if (err != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(err));
     return DIPAMD_OK;
+}
+
+// A pass whose boundary condition is computed from the edge values: the lines are
+// written, extended by `border` samples each side, into a buffer-type scratch
+// (k_expand), filtered there (the interior outputs never read past the written
+// extension, so the condition given to the filter kernel is immaterial), and the
+// interior is copied to `dst`.  The reference extends its line buffer the same way
+// (CopyBuffer then ExpandBuffer, framework_separable.cpp:356-380).
+static int run_cast_out(const View &src, const View &dst, int ndim, hipStream_t stream);
+
+template <typename TB>
+static int run_extended_pass(const View &src, const View &dst, int ndim, int d, const InternFilter &F, int bc,
+                             bool fast, hipStream_t stream) {
+    const int B = F.border;
+    if (B == 0) return run_single<TB>(src, dst, ndim, d, F, DIPAMD_BC_ZERO_ORDER_EXTRAPOLATE, fast, stream);
+    const int buf_dt = sizeof(TB) == 8 ? DIPAMD_DT_DFLOAT : DIPAMD_DT_SFLOAT;
+    int64_t psz[DIPAMD_MAX_DIMS];
+    int64_t total = 1;
+    for (int k = 0; k < ndim; ++k) {
+        psz[k] = src.sizes[k] + (k == d ? 2 * static_cast<int64_t>(B) : 0);
+        total *= psz[k];
+    }
+    void *p1 = nullptr, *p2 = nullptr;
+    hipError_t e = scratch_alloc(&p1, static_cast<size_t>(total) * sizeof(TB), stream);
+    if (e == hipSuccess) e = scratch_alloc(&p2, static_cast<size_t>(total) * sizeof(TB), stream);
+    if (e != hipSuccess) {
+        if (p1) scratch_free(p1, stream);
+        return set_error(DIPAMD_E_HIP, std::string("scratch allocation: ") + hipGetErrorString(e));
+    }
+    const View t1 = contiguous_view(p1, buf_dt, ndim, psz);
+    const View t2 = contiguous_view(p2, buf_dt, ndim, psz);
+    CastGeo g;
+    std::memset(&g, 0, sizeof(g));
+    g.ndim = ndim;
+    g.total = total;
+    for (int k = 0; k < ndim; ++k) {
+        g.size[k] = psz[k];
+        g.sin[k] = src.strides[k];
+        g.sout[k] = t1.strides[k];
+    }
+    int st = DIPAMD_OK;
+    {
+        TraceScope ts("expand", total * static_cast<int64_t>(sizeof(TB)) +
+                                    npx_of(src, ndim) * static_cast<int64_t>(dt_size(src.dt)), stream);
+        e = launch_expand(src.dt == DIPAMD_DT_UINT8 ? IN_U8 : IN_BUF, sizeof(TB) == 8, src.p, t1.p, &g, d,
+                          static_cast<int>(src.sizes[d]), B, bc, stream);
+        if (e != hipSuccess) st = set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    }
+    if (st == DIPAMD_OK) st = run_single<TB>(t1, t2, ndim, d, F, DIPAMD_BC_ZERO_ORDER_EXTRAPOLATE, fast, stream);
+    if (st == DIPAMD_OK) {
+        View inner = t2;
+        inner.p = static_cast<TB *>(t2.p) + static_cast<int64_t>(B) * t2.strides[d];
+        for (int k = 0; k < ndim; ++k) inner.sizes[k] = src.sizes[k];
+        st = run_cast_out(inner, dst, ndim, stream);
+    }
+    scratch_free(p2, stream);
+    scratch_free(p1, stream);
+    return st;
 }
 
 template <typename TB>
@@ -562,7 +626,10 @@ int separable_device(const View &in, const View &out, int ndim, const InternFilt
         order[j] = e;
     }
     for (int d : order) {
-        if (!bc_supported(bc[d])) return set_error(DIPAMD_E_UNSUPPORTED, "boundary condition not supported");
+        if (bc[d] == DIPAMD_BC_ALREADY_EXPANDED)  // ExpandBuffer's default case (copy_buffer.cpp:745)
+            return set_error(DIPAMD_E_UNSUPPORTED, "Functionality has not (yet) been implemented");
+        if (!bc_supported(bc[d]) && !bc_value_based(bc[d]))
+            return set_error(DIPAMD_E_UNSUPPORTED, "boundary condition not supported");
     }
     std::vector<void *> scratch;
     auto alloc = [&](int dt, View *v) -> int {
@@ -605,7 +672,8 @@ int separable_device(const View &in, const View &out, int ndim, const InternFilt
         const bool fits = Fx.border <= kFusedCap - 1 && Fy.border <= kFusedCap - 1 &&
                           in.sizes[order[0]] * std::llabs(in.strides[order[0]]) < (int64_t(1) << 31) &&
                           in.sizes[order[1]] < (int64_t(1) << 31);
-        if (symok && fits) {
+        const bool index_bcs = !bc_value_based(bc[order[0]]) && !bc_value_based(bc[order[1]]);
+        if (symok && fits && index_bcs) {
             const int cfg = choose_fused_config(is_double, Fx.border, Fy.border, in.sizes[order[0]], in.sizes[order[1]],
                                                 flags);
             if (cfg >= 0) {
@@ -673,6 +741,10 @@ int separable_device(const View &in, const View &out, int ndim, const InternFilt
                                : run_fused<float>(src, dst, ndim, pp.d0, pp.d1, *filters[pp.d0], *filters[pp.d1],
                                                   bc[pp.d0], bc[pp.d1], fast, cfg, stream);
             }
+        } else if (bc_value_based(bc[pp.d0])) {
+            st = is_double
+                     ? run_extended_pass<double>(src, dst, ndim, pp.d0, *filters[pp.d0], bc[pp.d0], fast, stream)
+                     : run_extended_pass<float>(src, dst, ndim, pp.d0, *filters[pp.d0], bc[pp.d0], fast, stream);
         } else {
             st = is_double ? run_single<double>(src, dst, ndim, pp.d0, *filters[pp.d0], bc[pp.d0], fast, stream)
                            : run_single<float>(src, dst, ndim, pp.d0, *filters[pp.d0], bc[pp.d0], fast, stream);

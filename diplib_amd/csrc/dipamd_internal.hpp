@@ -54,6 +54,7 @@ std::vector<double> half_gaussian(double sigma, int order, double truncation, bo
 int intern_filter(const double *taps, int n, int sym, int origin_in, InternFilter &F);
 bool meaningless(const InternFilter &F, bool use_double);
 bool bc_supported(int bc);
+bool bc_value_based(int bc);
 int check_image(const dipamd_image *im, const char *what);
 View to_view(const dipamd_image *im);
 View contiguous_view(void *p, int dt, int ndim, const int64_t *sizes);

@@ -41,14 +41,23 @@ enum {
 enum {
     DIPAMD_BC_SYMMETRIC_MIRROR = 0,   /* "mirror", "default", "" */
     DIPAMD_BC_ASYMMETRIC_MIRROR = 1,  /* "asym mirror" */
-    DIPAMD_BC_ANTISYMMETRIC_REFLECT = 2, /* not supported: returns DIPAMD_E_UNSUPPORTED */
+    DIPAMD_BC_ANTISYMMETRIC_REFLECT = 2, /* "antisym reflect" */
     DIPAMD_BC_PERIODIC = 3,           /* "periodic" */
     DIPAMD_BC_ASYMMETRIC_PERIODIC = 4,/* "asym periodic" */
     DIPAMD_BC_ADD_ZEROS = 5,          /* "add zeros" */
     DIPAMD_BC_ADD_MAX_VALUE = 6,      /* "add max" */
     DIPAMD_BC_ADD_MIN_VALUE = 7,      /* "add min" */
-    DIPAMD_BC_ZERO_ORDER_EXTRAPOLATE = 8 /* "zero order"; 9-12 not supported */
+    DIPAMD_BC_ZERO_ORDER_EXTRAPOLATE = 8,   /* "zero order" */
+    DIPAMD_BC_FIRST_ORDER_EXTRAPOLATE = 9,  /* "first order" */
+    DIPAMD_BC_SECOND_ORDER_EXTRAPOLATE = 10, /* "second order" */
+    DIPAMD_BC_THIRD_ORDER_EXTRAPOLATE = 11, /* "third order" */
+    DIPAMD_BC_ALREADY_EXPANDED = 12         /* "already expanded": an error in the separable
+                                               framework, as in the reference (ExpandBuffer,
+                                               copy_buffer.cpp:745, has no case for it) */
 };
+/* Codes 2 and 9-11 are computed from the edge values: the plain (non-batched) calls take
+   them (one extra pass over the image per axis that uses one); the batched and backward
+   forms return DIPAMD_E_UNSUPPORTED for them. */
 
 /* Filter symmetry (convolution.cpp:40-48 FilterSymmetry, real filters). */
 enum {

@@ -19,8 +19,10 @@
  *
  * Codes (shared with include/dipamd.h):
  *   data types 0 u8, 1 u16, 2 u32, 3 s8, 4 s16, 5 s32, 6 sfloat, 7 dfloat, 8 bin
- *   boundary   0 mirror, 1 asym mirror, 3 periodic, 4 asym periodic, 5 add zeros,
- *              6 add max, 7 add min, 8 zero order (2, 9-12 are not restated)
+ *   boundary   0 mirror, 1 asym mirror, 2 antisym reflect, 3 periodic, 4 asym periodic,
+ *              5 add zeros, 6 add max, 7 add min, 8 zero order, 9 first order,
+ *              10 second order, 11 third order (12, "already expanded", is an error in
+ *              the separable framework: ExpandBuffer has no case for it)
  *   symmetry   0 general, 1 even, 2 odd, 3 conj(=even), 4 d-even, 5 d-odd, 6 d-conj(=d-even)
  */
 #include <float.h>
@@ -36,7 +38,8 @@
 
 enum { DT_U8, DT_U16, DT_U32, DT_S8, DT_S16, DT_S32, DT_SF, DT_DF, DT_BIN };
 enum { BC_MIRROR = 0, BC_ASYM_MIRROR = 1, BC_ANTISYM = 2, BC_PERIODIC = 3, BC_ASYM_PERIODIC = 4,
-       BC_ZEROS = 5, BC_MAXV = 6, BC_MINV = 7, BC_ZERO_ORDER = 8 };
+       BC_ZEROS = 5, BC_MAXV = 6, BC_MINV = 7, BC_ZERO_ORDER = 8, BC_FIRST = 9, BC_SECOND = 10,
+       BC_THIRD = 11 };
 enum { SY_GENERAL, SY_EVEN, SY_ODD, SY_CONJ, SY_D_EVEN, SY_D_ODD, SY_D_CONJ };
 
 #define ORC_MAX_DIMS 8
@@ -251,21 +254,90 @@ DEFINE_LINE_FILTER(line_filter_f, float, f)
 DEFINE_LINE_FILTER(line_filter_d, double, d)
 
 /* ---------------------------------------------------------------------------
- * Boundary extension of a line buffer (copy_buffer.cpp:360-379, 515-549,
- * 591-629, 631-747): the pointer walks of the reference, restated.
+ * Boundary extension of a line buffer (copy_buffer.cpp:360-379, 380-513,
+ * 515-549, 551-589, 591-629, 631-747): the pointer walks and the double-precision
+ * extrapolation polynomials of the reference, restated.  Extrapolated samples are
+ * computed in double and stored with a plain cast (clamp_cast to a float type).
  * ------------------------------------------------------------------------- */
 #define DEFINE_EXPAND(NAME, T, TMAX)                                                               \
+    static void NAME##_first(T *buf, long n, long left, long right) {                              \
+        /* copy_buffer.cpp:380-411: a line from the edge sample to 0 at the end of the */         \
+        /* border, by repeated subtraction */                                                      \
+        if (left > 0) {                                                                            \
+            double d0 = (double)buf[0], d1 = d0 / (double)(left + 1);                              \
+            for (long i = 1; i <= left; i++) { d0 -= d1; buf[-i] = (T)d0; }                         \
+        }                                                                                          \
+        if (right > 0) {                                                                           \
+            double d0 = (double)buf[n - 1], d1 = d0 / (double)(right + 1);                         \
+            for (long i = 0; i < right; i++) { d0 -= d1; buf[n + i] = (T)d0; }                      \
+        }                                                                                          \
+    }                                                                                              \
+    static void NAME##_poly(T *buf, long n, long border, int side, int third) {                    \
+        /* copy_buffer.cpp:413-457 (second order), 459-513 (third order); side 0 left */           \
+        if (border <= 0) return;                                                                   \
+        double b = (double)border + 1.0;                                                           \
+        double d0 = (double)(side == 0 ? buf[0] : buf[n - 1]);                                     \
+        double f1 = (double)(side == 0 ? buf[1] : buf[n - 2]);                                     \
+        double d1, d2, d3 = 0.0;                                                                   \
+        if (!third) {                                                                              \
+            d1 = (b - 1.0) / b * d0 - b / (b + 1.0) * f1;                                          \
+            d2 = -1.0 / b * d0 + 1.0 / (b + 1.0) * f1;                                             \
+        } else {                                                                                   \
+            double b12 = (b + 1) * (b + 1);                                                        \
+            d1 = -(2.0 * d0) / b + d0 - (b * b * f1) / b12;                                        \
+            d2 = (2.0 * b * f1) / b12 - (d0 * (2.0 * b - 1.0)) / (b * b);                          \
+            d3 = d0 / (b * b) - f1 / b12;                                                          \
+        }                                                                                          \
+        for (long i = 1; i <= border; i++) {                                                       \
+            double x = (double)i;                                                                  \
+            double v = third ? d0 + x * d1 + x * x * d2 + x * x * x * d3 : d0 + x * d1 + x * x * d2; \
+            if (side == 0) buf[-i] = (T)v; else buf[n - 1 + i] = (T)v;                             \
+        }                                                                                          \
+    }                                                                                              \
     static int NAME(T *buf, long n, long left, long right, int bc) {                               \
         if (left == 0 && right == 0) return 0;                                                     \
         int asym = (bc == BC_ASYM_MIRROR || bc == BC_ASYM_PERIODIC);                               \
         if ((bc == BC_MIRROR || bc == BC_ASYM_MIRROR || bc == BC_PERIODIC ||                       \
-             bc == BC_ASYM_PERIODIC) && n == 1) {                                                  \
+             bc == BC_ASYM_PERIODIC || bc == BC_ANTISYM) && n == 1) {                              \
             T v = asym ? -buf[0] : buf[0];                                                         \
             for (long i = 1; i <= left; i++) buf[-i] = v;                                          \
             for (long i = 1; i <= right; i++) buf[i] = v;                                          \
             return 0;                                                                              \
         }                                                                                          \
+        if ((bc == BC_SECOND || bc == BC_THIRD) && n == 1) bc = BC_FIRST; /* :791-808 */          \
         switch (bc) {                                                                              \
+            case BC_ANTISYM: {                                                                     \
+                /* copy_buffer.cpp:551-589: mirror around the edge value, bouncing */             \
+                long end = n - 1, in = 0, out = 0;                                                 \
+                int sign = -1;                                                                     \
+                double offset = (double)buf[out] - sign * (double)buf[in];                         \
+                for (long i = 0; i < left; i++) {                                                  \
+                    in -= sign; out -= 1;                                                          \
+                    buf[out] = (T)(offset + sign * (double)buf[in]);                               \
+                    if (in == end || in == 0) {                                                    \
+                        sign = -sign;                                                              \
+                        offset = (double)buf[out] - sign * (double)buf[in];                        \
+                    }                                                                              \
+                }                                                                                  \
+                in = end; out = end; sign = -1;                                                    \
+                offset = (double)buf[out] - sign * (double)buf[in];                                \
+                for (long i = 0; i < right; i++) {                                                 \
+                    in += sign; out += 1;                                                          \
+                    buf[out] = (T)(offset + sign * (double)buf[in]);                               \
+                    if (in == end || in == 0) {                                                    \
+                        sign = -sign;                                                              \
+                        offset = (double)buf[out] - sign * (double)buf[in];                        \
+                    }                                                                              \
+                }                                                                                  \
+                return 0;                                                                          \
+            }                                                                                      \
+            case BC_FIRST:                                                                         \
+                NAME##_first(buf, n, left, right);                                                 \
+                return 0;                                                                          \
+            case BC_SECOND: case BC_THIRD:                                                         \
+                NAME##_poly(buf, n, left, 0, bc == BC_THIRD);                                      \
+                NAME##_poly(buf, n, right, 1, bc == BC_THIRD);                                     \
+                return 0;                                                                          \
             case BC_MIRROR: case BC_ASYM_MIRROR: {                                                 \
                 long end = n - 1, in = 0, out = 0;                                                 \
                 int sign = -1;                                                                     \

@@ -88,6 +88,30 @@ SMALL_CASES.append(("u8_520x70_s07_o00_asym_mirror", (70, 520), "uint8", [0.7, 1
 SMALL_CASES.append(("f32_1x40_s2", (40, 1), "float32", [2.0], [0], ["mirror"], None))
 SMALL_CASES.append(("f32_1x1_s2", (1, 1), "float32", [2.0], [0], ["mirror"], None))
 
+# Boundary conditions whose extension is computed from the edge values rather than
+# copied (copy_buffer.cpp:380-513, 551-589): own fixture file (extrap_cases.npz),
+# same tuple layout as SMALL_CASES.  Widths below the radius make the antisymmetric
+# reflection bounce; width 1 takes the reference's fallbacks (constant, first order).
+VALUE_BCS = ("antisym reflect", "first order", "second order", "third order")
+EXTRAP_CASES = []
+for bc in VALUE_BCS:
+    t = bc.replace(" ", "_")
+    EXTRAP_CASES += [
+        (f"f32_64x48_s2_o00_{t}", (48, 64), "float32", [2.0], [0, 0], [bc], None),
+        (f"u8_61x37_s3_o10_{t}", (37, 61), "uint8", [3.0], [1, 0], [bc], None),
+        (f"u8_50x40_s2_o00_{t}_u8out", (40, 50), "uint8", [2.0], [0, 0], [bc], "uint8"),
+        (f"f64_33x20_s2_o02_{t}", (20, 33), "float64", [2.0], [0, 2], [bc], None),
+        (f"f32_37x29x11_s221_{t}", (11, 29, 37), "float32", [2.0, 2.0, 1.0], [0], [bc], None),
+        (f"f32_130x70_s8_o00_{t}", (70, 130), "float32", [8.0], [0, 0], [bc], None),
+    ]
+    for w in (1, 2, 3, 5):
+        EXTRAP_CASES.append((f"bounce_{w}x9_{t}", (9, w), "float32", [3.0, 2.0], [0, 1], [bc], None))
+EXTRAP_CASES.append(("f32_40x30_first_order_x_mirror_y", (30, 40), "float32", [2.0], [0, 1],
+                     ["first order", "mirror"], None))
+EXTRAP_CASES.append(("f32_40x30_periodic_x_third_order_y", (30, 40), "float32", [2.0], [1, 0],
+                     ["periodic", "third order"], None))
+EXTRAP_CASES.append(("s16_45x33_s2_antisym_reflect", (33, 45), "int16", [2.0], [0, 0], ["antisym reflect"], None))
+
 TAP_SIGMAS = [0.5, 0.8, 1.0, 1.5, 2.0, 3.0, 4.0, 5.0, 8.0, 16.0, 31.7]
 
 # Full-size workloads: (name, numpy shape, dtype, seed, impulses, sigmas, orders, bc, out dtype)

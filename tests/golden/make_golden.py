@@ -1,9 +1,11 @@
 """Writes the golden fixtures under tests/golden/ from the REFERENCE library itself
 (oracle/_ref/libdipref.so, built from the reference sources by oracle/ref.mk).
 
-    make -f oracle/ref.mk -j8 && python tests/golden/make_golden.py
+    make -f oracle/ref.mk -j8 && python tests/golden/make_golden.py [--only extrap]
 
 Outputs (all small):
+  extrap_cases.npz   input and dip::Gauss output of tests/cases.py:EXTRAP_CASES (the
+                     boundary conditions that extrapolate from the edge values)
   taps.npz           MakeHalfGaussian(sigma, order, 3, sfloat/dfloat) for TAP_SIGMAS x orders 0..3
   small_cases.npz    input and dip::Gauss output of every case in tests/cases.py:SMALL_CASES
   large_cases.json   SHA-256 of the exact output bytes + sampled pixels for LARGE_CASES
@@ -29,8 +31,21 @@ def sample_idx(n, k=64, seed=7):
     return np.unique(np.concatenate([np.arange(min(n, 8)), n - 1 - np.arange(min(n, 8)), rng.integers(0, n, k)]))
 
 
+def write_cases(case_list, fname, seed0):
+    arrs = {}
+    for i, (name, shape, dt, sig, ords, bc, odt) in enumerate(case_list):
+        x = cases.make_input(shape, dt, seed=seed0 + i)
+        y = oc.gauss_fir(x, sig, ords, bc, 3.0, out_dtype=odt, nthreads=1, use_ref=True)
+        arrs[name + "__in"] = x
+        arrs[name + "__out"] = y
+    np.savez_compressed(os.path.join(HERE, fname), **arrs)
+
+
 def main():
     assert oc.ref_available(), "build oracle/_ref/libdipref.so first (make -f oracle/ref.mk)"
+    write_cases(cases.EXTRAP_CASES, "extrap_cases.npz", 3000)
+    if "--only" in sys.argv and sys.argv[sys.argv.index("--only") + 1] == "extrap":
+        return
     # taps
     taps = {}
     for s in cases.TAP_SIGMAS:

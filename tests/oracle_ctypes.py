@@ -16,7 +16,8 @@ DT = {np.dtype(np.uint8): 0, np.dtype(np.uint16): 1, np.dtype(np.uint32): 2,
       np.dtype(np.int8): 3, np.dtype(np.int16): 4, np.dtype(np.int32): 5,
       np.dtype(np.float32): 6, np.dtype(np.float64): 7, np.dtype(np.bool_): 8}
 BC = {"": 0, "default": 0, "mirror": 0, "asym mirror": 1, "antisym reflect": 2, "periodic": 3,
-      "asym periodic": 4, "add zeros": 5, "add max": 6, "add min": 7, "zero order": 8}
+      "asym periodic": 4, "add zeros": 5, "add max": 6, "add min": 7, "zero order": 8,
+      "first order": 9, "second order": 10, "third order": 11}
 SYM = {"general": 0, "": 0, "even": 1, "odd": 2, "conj": 3, "d-even": 4, "d-odd": 5, "d-conj": 6}
 
 _orc = None

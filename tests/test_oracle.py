@@ -21,6 +21,7 @@ import oracle_ctypes as oc
 HERE = os.path.dirname(os.path.abspath(__file__))
 TAPS = np.load(os.path.join(HERE, "golden", "taps.npz"))
 GOLD = np.load(os.path.join(HERE, "golden", "small_cases.npz"))
+EXTRAP = np.load(os.path.join(HERE, "golden", "extrap_cases.npz"))
 
 
 @pytest.mark.parametrize("sigma", cases.TAP_SIGMAS)
@@ -174,3 +175,81 @@ def test_jacobian_fixture_consistent():
     My = _dense_1d(H, oc.half_gaussian(sig, oy, is_double=True), oy % 2 == 1, meta["bc"])
     exp = My.T @ w @ Mx
     assert np.allclose(jtw, exp, rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("case", cases.EXTRAP_CASES, ids=[c[0] for c in cases.EXTRAP_CASES])
+def test_oracle_matches_reference_extrapolation(case):
+    """Boundary conditions computed from the edge values (antisym reflect, first to
+    third order): the oracle against the reference's own outputs, bit for bit."""
+    name, shape, dt, sig, ords, bc, odt = case
+    x = EXTRAP[name + "__in"]
+    exp = EXTRAP[name + "__out"]
+    got = oc.gauss_fir(x, sig, ords, bc, 3.0, out_dtype=odt)
+    assert got.dtype == exp.dtype
+    assert np.array_equal(got.view(np.uint8), exp.view(np.uint8))
+
+
+def extension(line, border, bc):
+    """The extended line [-border, n + border) as the separable framework builds it,
+    read back through shift filters: a GENERAL filter of 2*border+1 taps (centre
+    origin, so the border is `border`) with a single 1 at tap j outputs
+    x[i + border - j]; output 0 with j = border + k is x[-k], output n-1 with
+    j = border - k is x[n-1+k]."""
+    n = line.size
+    left, right = [], []
+    for k in range(1, border + 1):
+        e = np.zeros(2 * border + 1)
+        e[border + k] = 1.0
+        left.append(oc.separable(line, [e], ["general"], [bc])[0])
+        e = np.zeros(2 * border + 1)
+        e[border - k] = 1.0
+        right.append(oc.separable(line, [e], ["general"], [bc])[n - 1])
+    return np.concatenate([left[::-1], line, right]).astype(np.float64)
+
+
+def test_extension_readback_is_the_mirror():
+    x = np.arange(1, 8, dtype=np.float32)
+    ext = extension(x, 3, "mirror")
+    assert np.array_equal(ext, [4, 3, 2, 1, 2, 3, 4, 5, 6, 7, 6, 5, 4])
+
+
+@pytest.mark.parametrize("border", [1, 3, 6])
+@pytest.mark.parametrize("n", [2, 4, 9])
+def test_extrapolation_closed_forms(border, n):
+    """The polynomials the reference documents (boundary.h:58-60, copy_buffer.cpp
+    comments), solved here independently as linear systems: first order falls
+    linearly from the edge value to 0 at distance border+1; second order passes
+    through the two edge values (t = 0, -1) and 0 at t = border+1; third order also
+    has zero slope there.  Antisymmetric reflection is 2 f(0) - f(k) before it bounces."""
+    rng = np.random.default_rng(n * 10 + border)
+    x = rng.random(n).astype(np.float32)
+    b = border + 1
+    t = np.arange(1, border + 1, dtype=np.float64)
+    for side in (0, 1):
+        xs = x if side == 0 else x[::-1]
+        f0, f1 = float(xs[0]), float(xs[1])
+        got_first = extension(x, border, "first order")
+        got_second = extension(x, border, "second order")
+        got_third = extension(x, border, "third order")
+        got_anti = extension(x, border, "antisym reflect")
+        pick = (lambda e: e[:border][::-1]) if side == 0 else (lambda e: e[n + border:])
+        exp_first = f0 * (1.0 - t / b)
+        A2 = np.array([[1, 0, 0], [1, -1, 1], [1, b, b * b]], np.float64)
+        c2 = np.linalg.solve(A2, [f0, f1, 0.0])
+        exp_second = c2[0] + c2[1] * t + c2[2] * t * t
+        A3 = np.array([[1, 0, 0, 0], [1, -1, 1, -1], [1, b, b * b, b ** 3], [0, 1, 2 * b, 3 * b * b]], np.float64)
+        c3 = np.linalg.solve(A3, [f0, f1, 0.0, 0.0])
+        exp_third = c3[0] + c3[1] * t + c3[2] * t * t + c3[3] * t ** 3
+        for got, exp in ((got_first, exp_first), (got_second, exp_second), (got_third, exp_third)):
+            assert np.allclose(pick(got), exp, rtol=1e-6, atol=1e-6), (side, pick(got), exp)
+        k = np.arange(1, min(border, n - 1) + 1)
+        assert np.allclose(pick(got_anti)[:k.size], 2 * f0 - xs[k].astype(np.float64), rtol=1e-6, atol=1e-6)
+
+
+def test_antisym_reflect_bounces():
+    """Past the far end the reflection turns around the value written there: on a
+    line of 2 samples a, b the left extension is 2a-b, then a-b+... (the walk of
+    copy_buffer.cpp:551-589), which for this input is an arithmetic progression."""
+    x = np.array([1.0, 3.0], np.float32)  # slope 2: the reflection continues it
+    ext = extension(x, 4, "antisym reflect")
+    assert np.array_equal(ext, [-7, -5, -3, -1, 1, 3, 5, 7, 9, 11])
