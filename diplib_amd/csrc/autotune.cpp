@@ -81,6 +81,14 @@ bool stream2d_enabled() {
     return on;
 }
 
+bool zsweep_enabled() {
+    static bool on = [] {
+        const char *v = std::getenv("DIPAMD_ZSWEEP");
+        return !(v && v[0] == '0');
+    }();
+    return on;
+}
+
 int stream2d_target_waves() {
     static int n = [] {
         const char *v = std::getenv("DIPAMD_STREAM_WAVES");

@@ -389,9 +389,14 @@ static int run_single(const View &src, const View &dst, int ndim, int d, const I
         const int64_t ntiles = (a.e + bs - 1) / bs;
         a.chunk = pick_chunk(a.n, ntiles * nouter, 8, F.border, 2048);
         const int64_t nchunks = (a.n + a.chunk - 1) / a.chunk;
-        TraceScope ts("sweep", npx_of(src, ndim) * static_cast<int64_t>(dt_size(src.dt) + sizeof(TB)), stream,
+        const bool zsw = zsweep_enabled() && (F.sym == DIPAMD_SYM_EVEN || F.sym == DIPAMD_SYM_ODD) && F.shift == 0 &&
+                         F.border >= 1 && F.border <= kZSweepMaxR &&
+                         static_cast<int>(F.data.size()) == F.border + 1;
+        TraceScope ts(zsw ? "zsweep" : "sweep",
+                      npx_of(src, ndim) * static_cast<int64_t>(dt_size(src.dt) + sizeof(TB)), stream,
                       npx_of(src, ndim) * filter_ops(F, fast));
-        err = launch_sweep(ti, is_double, F.sym, fast, &a, a.e, nchunks, nouter, stream);
+        err = zsw ? launch_zsweep(ti, is_double, F.border, F.sym, fast, &a, a.e, nchunks, nouter, stream)
+                  : launch_sweep(ti, is_double, F.sym, fast, &a, a.e, nchunks, nouter, stream);
     }
     if (err != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(err));
     return DIPAMD_OK;

@@ -80,6 +80,8 @@ int separable_device(const View &in, const View &out, int ndim, const InternFilt
 int choose_fused_config(bool is_double, int rx, int ry, int64_t nx, int64_t ny, uint32_t flags);
 // Whether the register-streaming fused kernel may be used (DIPAMD_STREAM2D=0 disables).
 bool stream2d_enabled();
+// DIPAMD_ZSWEEP=0 turns the register-window sweep off (experiments).
+bool zsweep_enabled();
 int stream2d_target_waves();  // waves per launch of the register-streaming kernel
 // Kernel for a pass along a strided axis: 0 column tiles, 1 LDS ring, 2 global sweep
 // (DIPAMD_YSWEEP=col|ring|global; column tiles by default).

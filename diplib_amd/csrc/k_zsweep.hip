@@ -1,0 +1,146 @@
+// K2R: sweep along a strided axis for short symmetric filters (radius R <= 8, a
+// compile-time constant).  One thread per position of the lane axis, as K2, but
+// the window is a register array that slides down the chunk: each step keeps the
+// 2R samples it shares with the next step, so every input sample is loaded once
+// per chunk, and the next step's P samples are loaded before this step's
+// arithmetic, so the loads are in flight while it runs.  Every window index is a
+// compile-time constant.  The per-output operation order is the reference's
+// (convolution.cpp:214-241): centre product first, then the tap pairs k = 1..R.
+#include "sep_kernels_impl.hpp"
+
+#include "sep_config.hpp"
+
+namespace dipamd {
+
+namespace {
+
+constexpr int kZP = 8;  // outputs per thread and step
+
+template <typename TI, typename TB, int R, int SYM, bool FAST>
+__global__ __launch_bounds__(kBlock) void k_zsweep(const PassArgs<TB> a, int64_t ntiles, int64_t nchunks,
+                                                    const TI *__restrict__ in, TB *__restrict__ out) {
+    constexpr int P = kZP;
+    constexpr int NW = P + 2 * R;
+    // tile fastest in hardware order, as K2: neighbouring chunks of a column are
+    // dispatched later and find their shared rows in L2
+    const int64_t bid = blockIdx.x;
+    const int64_t tile = bid % ntiles;
+    const int64_t rest = bid / ntiles;
+    const int64_t chunk = rest % nchunks;
+    const int64_t outer = rest / nchunks;
+    const int64_t c = tile * blockDim.x + threadIdx.x;
+    if (c >= a.e) return;
+    int64_t oin, oout;
+    outer_offsets(a.outer, outer, &oin, &oout);
+    const TI *src = in + oin + c * a.ein;
+    TB *dst = out + oout + c * a.eout;
+    const int n = static_cast<int>(a.n);
+    const int z0 = static_cast<int>(chunk * a.chunk);
+    const int zend = static_cast<int>(min(chunk * a.chunk + a.chunk, a.n));
+    const int64_t sin = a.sin;
+    TB f[R + 1];
+#pragma unroll
+    for (int k = 0; k <= R; ++k) f[k] = a.taps[k];
+
+    // sample j of the line, boundary-extended where the block touches an end
+    auto load = [&](int j, bool inside) -> TB {
+        return inside ? to_buf<TB>(src[static_cast<int64_t>(j) * sin]) : fetch_ext<TI, TB>(src, sin, j, n, a.bc, a.cval);
+    };
+    TB w[NW];
+    {
+        const bool inside = z0 - R >= 0 && z0 + P + R <= n;
+#pragma unroll
+        for (int q = 0; q < NW; ++q) w[q] = load(z0 - R + q, inside);
+    }
+    for (int zb = z0; zb < zend; zb += P) {
+        // next step's new samples: rows zb + P + R .. zb + 2P + R - 1
+        TB nx[P];
+        const int jn = zb + P + R;
+        const bool more = zb + P < zend;
+        if (more) {
+            const bool inside = jn + P <= n;
+#pragma unroll
+            for (int p = 0; p < P; ++p) nx[p] = load(jn + p, inside);
+        }
+        TB acc[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            acc[p] = f[0] * w[R + p];
+#pragma unroll
+            for (int k = 1; k <= R; ++k) {
+                const TB t = (SYM == SYM_EVEN) ? (w[R + p + k] + w[R + p - k]) : (w[R + p + k] - w[R + p - k]);
+                acc[p] = mac<TB, FAST>(acc[p], f[k], t);
+            }
+        }
+        TB *d = dst + static_cast<int64_t>(zb) * a.sout;
+        if (zb + P <= zend) {
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                *d = acc[p];
+                d += a.sout;
+            }
+        } else {
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                if (zb + p < zend) *d = acc[p];
+                d += a.sout;
+            }
+        }
+        if (more) {
+#pragma unroll
+            for (int q = 0; q < 2 * R; ++q) w[q] = w[q + P];
+#pragma unroll
+            for (int p = 0; p < P; ++p) w[2 * R + p] = nx[p];
+        }
+    }
+}
+
+template <typename TI, typename TB, int R, int SYM, bool FAST>
+hipError_t zsweep_t(const PassArgs<TB> &a, int64_t e, int64_t nchunks, int64_t nouter, hipStream_t st) {
+    const int bs = e >= 256 ? 256 : 64;
+    const int64_t ntiles = (e + bs - 1) / bs;
+    const int64_t nblk = ntiles * nchunks * nouter;
+    hipLaunchKernelGGL((k_zsweep<TI, TB, R, SYM, FAST>), dim3(static_cast<unsigned>(nblk)), dim3(bs), 0, st, a,
+                       ntiles, nchunks, static_cast<const TI *>(a.in), static_cast<TB *>(a.out));
+    return hipGetLastError();
+}
+
+template <typename TI, typename TB, int R>
+hipError_t zsweep_sym(int sym, bool fast, const PassArgs<TB> &a, int64_t e, int64_t nc, int64_t no, hipStream_t st) {
+    if (sym == SYM_EVEN)
+        return fast ? zsweep_t<TI, TB, R, SYM_EVEN, true>(a, e, nc, no, st)
+                    : zsweep_t<TI, TB, R, SYM_EVEN, false>(a, e, nc, no, st);
+    return fast ? zsweep_t<TI, TB, R, SYM_ODD, true>(a, e, nc, no, st)
+                : zsweep_t<TI, TB, R, SYM_ODD, false>(a, e, nc, no, st);
+}
+
+template <typename TI, typename TB>
+hipError_t zsweep_r(int r, int sym, bool fast, const PassArgs<TB> &a, int64_t e, int64_t nc, int64_t no,
+                    hipStream_t st) {
+    switch (r) {
+        case 1: return zsweep_sym<TI, TB, 1>(sym, fast, a, e, nc, no, st);
+        case 2: return zsweep_sym<TI, TB, 2>(sym, fast, a, e, nc, no, st);
+        case 3: return zsweep_sym<TI, TB, 3>(sym, fast, a, e, nc, no, st);
+        case 4: return zsweep_sym<TI, TB, 4>(sym, fast, a, e, nc, no, st);
+        case 5: return zsweep_sym<TI, TB, 5>(sym, fast, a, e, nc, no, st);
+        case 6: return zsweep_sym<TI, TB, 6>(sym, fast, a, e, nc, no, st);
+        case 7: return zsweep_sym<TI, TB, 7>(sym, fast, a, e, nc, no, st);
+        case 8: return zsweep_sym<TI, TB, 8>(sym, fast, a, e, nc, no, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+hipError_t launch_zsweep(InType ti, bool is_double, int r, int sym, bool fast, const void *args, int64_t e,
+                         int64_t nchunks, int64_t nouter, hipStream_t stream) {
+    if (r < 1 || r > kZSweepMaxR || (sym != SYM_EVEN && sym != SYM_ODD)) return hipErrorInvalidValue;
+    if (is_double)
+        return zsweep_r<double, double>(r, sym, fast, *static_cast<const PassArgs<double> *>(args), e, nchunks,
+                                        nouter, stream);
+    const auto &a = *static_cast<const PassArgs<float> *>(args);
+    return ti == IN_U8 ? zsweep_r<uint8_t, float>(r, sym, fast, a, e, nchunks, nouter, stream)
+                       : zsweep_r<float, float>(r, sym, fast, a, e, nchunks, nouter, stream);
+}
+
+}  // namespace dipamd

@@ -30,6 +30,9 @@ hipError_t launch_line(InType ti, bool is_double, int sym, bool fast, const void
                        int border, hipStream_t stream);
 hipError_t launch_sweep(InType ti, bool is_double, int sym, bool fast, const void *args, int64_t e, int64_t nchunks,
                         int64_t nouter, hipStream_t stream);
+// Register-window sweep for symmetric radii 1..kZSweepMaxR (k_zsweep.hip).
+hipError_t launch_zsweep(InType ti, bool is_double, int r, int sym, bool fast, const void *args, int64_t e,
+                         int64_t nchunks, int64_t nouter, hipStream_t stream);
 // Register-streaming fused kernel for uint8 planes (k_stream2d.hip).
 hipError_t launch_stream2d(int symx, int symy, bool fast, const void *args, hipStream_t stream);
 // Column-tile sweep: rows per block for a border (0 when the block does not fit).
