@@ -81,6 +81,14 @@ bool stream2d_enabled() {
     return on;
 }
 
+int sweep_target_blocks(int dflt) {
+    static int n = [] {
+        const char *v = std::getenv("DIPAMD_SWEEP_BLOCKS");
+        return v ? std::atoi(v) : 0;
+    }();
+    return n > 0 ? n : dflt;
+}
+
 bool zsweep_enabled() {
     static bool on = [] {
         const char *v = std::getenv("DIPAMD_ZSWEEP");

@@ -387,11 +387,13 @@ static int run_single(const View &src, const View &dst, int ndim, int d, const I
         const int64_t nouter = product_except(src, ndim, d, lane);
         const int bs = a.e >= 256 ? 256 : 64;
         const int64_t ntiles = (a.e + bs - 1) / bs;
-        a.chunk = pick_chunk(a.n, ntiles * nouter, 8, F.border, 2048);
-        const int64_t nchunks = (a.n + a.chunk - 1) / a.chunk;
         const bool zsw = zsweep_enabled() && (F.sym == DIPAMD_SYM_EVEN || F.sym == DIPAMD_SYM_ODD) && F.shift == 0 &&
                          F.border >= 1 && F.border <= kZSweepMaxR &&
                          static_cast<int>(F.data.size()) == F.border + 1;
+        // the register-window sweep gains from long chunks (no halo re-reads, long
+        // streams), so it aims for fewer workgroups
+        a.chunk = pick_chunk(a.n, ntiles * nouter, 8, F.border, sweep_target_blocks(zsw ? 1024 : 2048));
+        const int64_t nchunks = (a.n + a.chunk - 1) / a.chunk;
         TraceScope ts(zsw ? "zsweep" : "sweep",
                       npx_of(src, ndim) * static_cast<int64_t>(dt_size(src.dt) + sizeof(TB)), stream,
                       npx_of(src, ndim) * filter_ops(F, fast));

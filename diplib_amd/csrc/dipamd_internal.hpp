@@ -82,6 +82,9 @@ int choose_fused_config(bool is_double, int rx, int ry, int64_t nx, int64_t ny, 
 bool stream2d_enabled();
 // DIPAMD_ZSWEEP=0 turns the register-window sweep off (experiments).
 bool zsweep_enabled();
+// Workgroups a strided sweep aims for when it splits the filter axis into chunks:
+// `dflt`, or DIPAMD_SWEEP_BLOCKS when set (experiments).
+int sweep_target_blocks(int dflt);
 int stream2d_target_waves();  // waves per launch of the register-streaming kernel
 // Kernel for a pass along a strided axis: 0 column tiles, 1 LDS ring, 2 global sweep
 // (DIPAMD_YSWEEP=col|ring|global; column tiles by default).
