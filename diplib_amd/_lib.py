@@ -85,6 +85,8 @@ def lib():
         L.dipamd_batch_run_per_image.argtypes = [P, P]
         L.dipamd_batch_destroy.argtypes = [P]
         L.dipamd_batch_info.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(I32)]
+    L.dipamd_derivative.argtypes = [I32, IMG, IMG, I32, P, I32, P, I32, P, I32, D, U32, P]
+    L.dipamd_derivative_outputs.argtypes = [I32, I32]
     if hasattr(L, "dipamd_gauss_fir_backward"):
         L.dipamd_gauss_fir_backward.argtypes = [IMG, IMG, IMG, P, P, I32, P, I32, P, I32, D, U32, P]
     _lib = L
@@ -102,5 +104,7 @@ EXPORTED_SYMBOLS = [
     "dipamd_gauss_fir", "dipamd_separable_convolution", "dipamd_make_half_gaussian", "dipamd_batch_create",
     "dipamd_batch_run", "dipamd_batch_run_graph", "dipamd_batch_run_per_image", "dipamd_batch_destroy", "dipamd_batch_num_configs",
     "dipamd_batch_info", "dipamd_gauss_fir_backward", "dipamd_gauss_radius", "dipamd_last_error", "dipamd_version",
-    "dipamd_trace_enable", "dipamd_trace_read",
+    "dipamd_trace_enable", "dipamd_trace_read", "dipamd_derivative", "dipamd_derivative_outputs",
 ]
+
+FN_GRADIENT, FN_GRADIENT_MAGNITUDE, FN_HESSIAN, FN_LAPLACE = 0, 1, 2, 3

@@ -1,0 +1,107 @@
+"""The derivative family on the Gaussian FIR path, with PyDIP's names and arguments:
+dip.Derivative, dip.Gradient, dip.GradientMagnitude, dip.Hessian, dip.Laplace
+(derivative.cpp:88-418; linear.h:666, 697, 862, 904).
+
+Tensor results (Gradient: the partial derivatives in dim order; Hessian: the
+symmetric matrix as dip stores it, diagonal first, then the upper triangle column by
+column) come back as a tensor whose LAST axis holds the elements, as numpy shows a
+PyDIP tensor image; the elements are stored as planes, so each is a contiguous image.
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+from ._lib import check, lib
+from .gauss import (Gauss, _as_list, _bc_codes, _flex_torch, _stream_handle, _to_device, best_gauss_method,
+                    image_struct)
+
+
+def _fir_method(method, sigmas, orders):
+    m = method[5:] if method[:5] == "gauss" else method
+    if m == "best":
+        m = best_gauss_method(sigmas, orders)
+    if m in ("FIR", "fir"):
+        return
+    if m in ("FT", "ft", "IIR", "iir", "finitediff"):
+        raise NotImplementedError(f"method '{m}' is outside this library (Gaussian FIR only); pass method='FIR'")
+    raise _lib.DipError(_lib.E_PARAMETER, "Invalid flag: " + method)
+
+
+def _diff_dims(x, sigmas, process):
+    nd = x.dim()
+    sig = [float(s) for s in _as_list(sigmas)] or [1.0]
+    sig = sig * nd if len(sig) == 1 else sig
+    proc = [bool(p) for p in _as_list(process)] or [True]
+    proc = proc * nd if len(proc) == 1 else proc
+    if len(sig) != nd or len(proc) != nd:
+        raise _lib.DipError(_lib.E_ARRAY_LENGTH, "Array parameter has the wrong number of elements")
+    sizes = list(reversed(x.shape))  # dip order
+    return sig, proc, [d for d in range(nd) if proc[d] and sizes[d] > 1 and sig[d] > 0.0]
+
+
+def _run(function, x, outs, sig, proc, bc, truncation, fast):
+    nd = x.dim()
+    bcs = _bc_codes(bc)
+    imgs = (_lib.DipamdImage * len(outs))(*[image_struct(o) for o in outs])
+    s_arr = (ctypes.c_double * nd)(*sig)
+    p_arr = (ctypes.c_uint8 * nd)(*[1 if p else 0 for p in proc])
+    b_arr = (ctypes.c_int32 * max(1, len(bcs)))(*bcs)
+    a_in = image_struct(x)
+    with torch.cuda.device(x.device):
+        check(lib().dipamd_derivative(function, ctypes.byref(a_in), imgs, len(outs), s_arr, nd, b_arr, len(bcs),
+                                      p_arr, nd, float(truncation), _lib.FAST if fast else _lib.EXACT,
+                                      _stream_handle(x.device)))
+
+
+def _tensor_result(function, in_, sigmas, method, boundaryCondition, process, truncation, fast):
+    x, was_np = _to_device(in_)
+    sig, proc, dims = _diff_dims(x, sigmas, process)
+    if not dims:
+        raise _lib.DipError(_lib.E_PARAMETER, "Dimensionality not supported")
+    _fir_method(method, sig, [2] * x.dim() if function == _lib.FN_HESSIAN else [1] * x.dim())
+    n = check(lib().dipamd_derivative_outputs(function, len(dims)))
+    planes = torch.empty((n,) + tuple(x.shape), dtype=_flex_torch(x.dtype), device=x.device)
+    _run(function, x, [planes[i] for i in range(n)], sig, proc, boundaryCondition, truncation, fast)
+    y = planes.movedim(0, -1)
+    return y.cpu().numpy() if was_np else y
+
+
+def _scalar_result(function, in_, sigmas, method, boundaryCondition, process, truncation, out, fast):
+    x, was_np = _to_device(in_)
+    sig, proc, dims = _diff_dims(x, sigmas, process)
+    if not dims:
+        raise _lib.DipError(_lib.E_PARAMETER, "Dimensionality not supported")
+    _fir_method(method, sig, [2] * x.dim())
+    y = torch.empty(x.shape, dtype=_flex_torch(x.dtype), device=x.device) if out is None else out
+    _run(function, x, [y], sig, proc, boundaryCondition, truncation, fast)
+    return y.cpu().numpy() if (was_np and out is None) else y
+
+
+def Derivative(in_, derivativeOrder=(0,), sigmas=(1.0,), method="best", boundaryCondition=(), truncation=3.0,
+               out=None, fast=False):
+    """dip::Derivative (derivative.cpp:88-115), Gaussian methods: dip::Gauss."""
+    return Gauss(in_, sigmas, derivativeOrder, method, boundaryCondition, truncation, out=out, fast=fast)
+
+
+def Gradient(in_, sigmas=(1.0,), method="best", boundaryCondition=(), process=(), truncation=3.0, fast=False):
+    """dip::Gradient (derivative.cpp:142-172)."""
+    return _tensor_result(_lib.FN_GRADIENT, in_, sigmas, method, boundaryCondition, process, truncation, fast)
+
+
+def Hessian(in_, sigmas=(1.0,), method="best", boundaryCondition=(), process=(), truncation=3.0, fast=False):
+    """dip::Hessian (derivative.cpp:330-372)."""
+    return _tensor_result(_lib.FN_HESSIAN, in_, sigmas, method, boundaryCondition, process, truncation, fast)
+
+
+def GradientMagnitude(in_, sigmas=(1.0,), method="best", boundaryCondition=(), process=(), truncation=3.0, out=None,
+                      fast=False):
+    """dip::GradientMagnitude (derivative.cpp:174-210)."""
+    return _scalar_result(_lib.FN_GRADIENT_MAGNITUDE, in_, sigmas, method, boundaryCondition, process, truncation, out,
+                          fast)
+
+
+def Laplace(in_, sigmas=(1.0,), method="best", boundaryCondition=(), process=(), truncation=3.0, out=None,
+            fast=False):
+    """dip::Laplace (derivative.cpp:374-418), Gaussian methods."""
+    return _scalar_result(_lib.FN_LAPLACE, in_, sigmas, method, boundaryCondition, process, truncation, out, fast)

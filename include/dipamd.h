@@ -120,6 +120,28 @@ int dipamd_gauss_fir(const dipamd_image *in, const dipamd_image *out, const doub
                      const int32_t *orders, int32_t norders, const int32_t *bc, int32_t nbc, double truncation,
                      uint32_t flags, void *stream);
 
+/* The derivative family on the Gaussian FIR path (src/linear/derivative.cpp:120-418;
+ * include/diplib/linear.h:666, 697, 862, 904): dip::Gradient, dip::GradientMagnitude,
+ * dip::Hessian and dip::Laplace with method "FIR" (or "best" resolving to FIR), each Gaussian derivative
+ * computed as dipamd_gauss_fir computes it and combined in the reference's order and
+ * arithmetic type.  The differentiated dims are those with process != 0 (NULL: all),
+ * size > 1 and sigma > 0; sigmas apply to every dim (a dim that is not differentiated
+ * is still smoothed).  `outs` holds dipamd_derivative_outputs(function, n) images
+ * of the sizes of `in`, n = the number of differentiated dims: Gradient the partial
+ * derivatives in dim order; Hessian the symmetric matrix as dip stores it, the n
+ * diagonal elements first, then the upper triangle column by column; GradientMagnitude
+ * and Laplace one image, of type SuggestFlex(in). */
+enum {
+    DIPAMD_FN_GRADIENT = 0,
+    DIPAMD_FN_GRADIENT_MAGNITUDE = 1,
+    DIPAMD_FN_HESSIAN = 2,
+    DIPAMD_FN_LAPLACE = 3
+};
+int dipamd_derivative_outputs(int32_t function, int32_t ndiff);
+int dipamd_derivative(int32_t function, const dipamd_image *in, const dipamd_image *outs, int32_t nouts,
+                      const double *sigmas, int32_t nsigmas, const int32_t *bc, int32_t nbc, const uint8_t *process,
+                      int32_t nprocess, double truncation, uint32_t flags, void *stream);
+
 /* Replaces dip::SeparableConvolution (include/diplib/linear.h:114-130;
  * src/linear/convolution.cpp:319-421) for real filters.  nfilters: 1 or ndim.
  * process: NULL (all) or ndim bytes. */

@@ -379,6 +379,113 @@ inline Image Gauss(Image const &in, FloatArray sigmas = {1.0}, UnsignedArray der
     return out;
 }
 
+/* The derivative family (derivative.cpp:120-418; linear.h:666, 697, 862, 904) with the
+ * FIR method ("best" must resolve to it).  This shim has no tensor images: Gradient and
+ * Hessian return their tensor elements as an array of images, in dip's element order
+ * (Hessian: the diagonal first, then the upper triangle column by column). */
+using ImageArray = std::vector<Image>;
+namespace detail {
+inline void CheckFirMethod(String const &c_method, FloatArray const &sigmas) {
+    String method = c_method.substr(0, 5) == "gauss" ? c_method.substr(5) : c_method;
+    if (method == "best") {  // derivative orders <= 2 here: sigma decides (derivative.cpp:35-58)
+        method = "fir";
+        for (auto s : sigmas)
+            if (s < 0.8 && s > 0.0) method = "ft";
+        if (method == "fir")
+            for (auto s : sigmas)
+                if (s > 10) method = "iir";
+    }
+    if (method == "FIR" || method == "fir") return;
+    if (method == "FT" || method == "ft" || method == "IIR" || method == "iir" || method == "finitediff")
+        throw ParameterError("method \"" + method + "\" is not provided by this library (Gaussian FIR only)");
+    throw ParameterError("Invalid flag: " + c_method);
+}
+inline void RunDerivative(int32_t fn, Image const &in, std::vector<dipamd_image> &outs, FloatArray const &sigmas,
+                          StringArray const &boundaryCondition, BooleanArray const &process, dfloat truncation) {
+    std::vector<int32_t> bc = BoundaryCodes(boundaryCondition);
+    std::vector<uint8_t> proc(process.begin(), process.end());
+    dipamd_image a = in.View();
+    Check(dipamd_derivative(fn, &a, outs.data(), static_cast<int32_t>(outs.size()), sigmas.data(),
+                            static_cast<int32_t>(sigmas.size()), bc.data(), static_cast<int32_t>(bc.size()),
+                            proc.empty() ? nullptr : proc.data(), static_cast<int32_t>(proc.size()), truncation,
+                            DIPAMD_EXACT, dipamd::CurrentStream()));
+}
+inline int DifferentiatedDims(Image const &in, FloatArray const &sigmas, BooleanArray const &process) {
+    const uint nd = in.Dimensionality();
+    int n = 0;
+    for (uint d = 0; d < nd; ++d) {
+        const dfloat s = sigmas.empty() ? 1.0 : (sigmas.size() == 1 ? sigmas[0] : sigmas[d]);
+        const bool p = process.empty() ? true : (process.size() == 1 ? process[0] : process[d]);
+        if (p && in.Sizes()[d] > 1 && s > 0.0) ++n;
+    }
+    if (n < 1) throw ParameterError("Dimensionality not supported");
+    return n;
+}
+inline void TensorResult(int32_t fn, Image const &in, ImageArray &out, FloatArray const &sigmas,
+                         String const &method, StringArray const &bc, BooleanArray const &process,
+                         dfloat truncation) {
+    if (!in.IsForged()) throw ParameterError("Image is not forged");
+    CheckFirMethod(method, sigmas);
+    const int n = dipamd_derivative_outputs(fn, DifferentiatedDims(in, sigmas, process));
+    if (n < 0) Check(n);
+    out.clear();
+    std::vector<dipamd_image> views;
+    for (int k = 0; k < n; ++k) {
+        out.emplace_back(in.Sizes(), 1, DataType::SuggestFlex(in.DataType()));
+        views.push_back(out.back().View());
+    }
+    RunDerivative(fn, in, views, sigmas, bc, process, truncation);
+}
+inline void ScalarResult(int32_t fn, Image const &in, Image &out, FloatArray const &sigmas, String const &method,
+                         StringArray const &bc, BooleanArray const &process, dfloat truncation) {
+    if (!in.IsForged()) throw ParameterError("Image is not forged");
+    CheckFirMethod(method, sigmas);
+    DifferentiatedDims(in, sigmas, process);
+    Image src = in;
+    PrepareOutput(in, out);
+    std::vector<dipamd_image> views{out.View()};
+    RunDerivative(fn, src, views, sigmas, bc, process, truncation);
+}
+}  // namespace detail
+
+inline void Gradient(Image const &in, ImageArray &out, FloatArray sigmas = {1.0}, String const &method = "best",
+                     StringArray const &boundaryCondition = {}, BooleanArray process = {}, dfloat truncation = 3) {
+    detail::TensorResult(DIPAMD_FN_GRADIENT, in, out, sigmas, method, boundaryCondition, process, truncation);
+}
+inline void Hessian(Image const &in, ImageArray &out, FloatArray sigmas = {1.0}, String const &method = "best",
+                    StringArray const &boundaryCondition = {}, BooleanArray process = {}, dfloat truncation = 3) {
+    detail::TensorResult(DIPAMD_FN_HESSIAN, in, out, sigmas, method, boundaryCondition, process, truncation);
+}
+inline void GradientMagnitude(Image const &in, Image &out, FloatArray sigmas = {1.0}, String const &method = "best",
+                              StringArray const &boundaryCondition = {}, BooleanArray process = {},
+                              dfloat truncation = 3) {
+    detail::ScalarResult(DIPAMD_FN_GRADIENT_MAGNITUDE, in, out, sigmas, method, boundaryCondition, process,
+                         truncation);
+}
+inline Image GradientMagnitude(Image const &in, FloatArray sigmas = {1.0}, String const &method = "best",
+                               StringArray const &boundaryCondition = {}, BooleanArray process = {},
+                               dfloat truncation = 3) {
+    Image out;
+    GradientMagnitude(in, out, std::move(sigmas), method, boundaryCondition, std::move(process), truncation);
+    return out;
+}
+inline void Laplace(Image const &in, Image &out, FloatArray sigmas = {1.0}, String const &method = "best",
+                    StringArray const &boundaryCondition = {}, BooleanArray process = {}, dfloat truncation = 3) {
+    detail::ScalarResult(DIPAMD_FN_LAPLACE, in, out, sigmas, method, boundaryCondition, process, truncation);
+}
+inline Image Laplace(Image const &in, FloatArray sigmas = {1.0}, String const &method = "best",
+                     StringArray const &boundaryCondition = {}, BooleanArray process = {}, dfloat truncation = 3) {
+    Image out;
+    Laplace(in, out, std::move(sigmas), method, boundaryCondition, std::move(process), truncation);
+    return out;
+}
+/* dip::Derivative (derivative.cpp:88-115), Gaussian methods: dip::Gauss. */
+inline void Derivative(Image const &in, Image &out, UnsignedArray derivativeOrder, FloatArray sigmas = {1.0},
+                       String const &method = "best", StringArray const &boundaryCondition = {},
+                       dfloat truncation = 3) {
+    Gauss(in, out, std::move(sigmas), std::move(derivativeOrder), method, boundaryCondition, truncation);
+}
+
 /* dip::MakeHalfGaussian (generation.h:638-653; gauss.cpp:151-165). */
 inline std::vector<dfloat> MakeHalfGaussian(dfloat sigma, uint derivativeOrder = 0, dfloat truncation = 3,
                                             DataType dt = DT_DFLOAT) {

@@ -112,6 +112,19 @@ EXTRAP_CASES.append(("f32_40x30_periodic_x_third_order_y", (30, 40), "float32", 
                      ["periodic", "third order"], None))
 EXTRAP_CASES.append(("s16_45x33_s2_antisym_reflect", (33, 45), "int16", [2.0], [0, 0], ["antisym reflect"], None))
 
+# The derivative family (dip::Gradient, GradientMagnitude, Hessian, Laplace with the FIR
+# method): (name, function, numpy shape, dtype, sigmas (dip order), boundary list).
+DERIV_CASES = []
+for fn in ("gradient", "gradmag", "hessian", "laplace"):
+    DERIV_CASES += [
+        (f"{fn}_f32_64x48_s2", fn, (48, 64), "float32", [2.0], ["mirror"]),
+        (f"{fn}_u8_97x61_s15_periodic", fn, (61, 97), "uint8", [1.5], ["periodic"]),
+        (f"{fn}_f64_33x20_s3", fn, (20, 33), "float64", [3.0], ["mirror"]),
+        (f"{fn}_f32_37x29x11_s221", fn, (11, 29, 37), "float32", [2.0, 2.0, 1.0], ["mirror"]),
+        (f"{fn}_f32_1x40_s2", fn, (40, 1), "float32", [2.0], ["mirror"]),
+        (f"{fn}_f32_50x40_s0x3", fn, (40, 50), "float32", [0.0, 3.0], ["asym mirror"]),
+    ]
+
 TAP_SIGMAS = [0.5, 0.8, 1.0, 1.5, 2.0, 3.0, 4.0, 5.0, 8.0, 16.0, 31.7]
 
 # Full-size workloads: (name, numpy shape, dtype, seed, impulses, sigmas, orders, bc, out dtype)

@@ -77,6 +77,40 @@ int main(int argc, char **argv) {
         (void)hipDeviceSynchronize();
         inplace.CopyToHost(got.data());
         CHECK(std::memcmp(got.data(), exp.data(), W * H * sizeof(float)) == 0, "in-place bit-identical");
+        // the derivative family: elements equal the Gauss derivatives; GradientMagnitude and
+        // Laplace combine them in float, in the reference's order
+        {
+            auto host = [&](dip::Image const &im) {
+                std::vector<float> v(W * H);
+                (void)hipDeviceSynchronize();
+                im.CopyToHost(v.data());
+                return v;
+            };
+            std::vector<float> dx = host(dip::GaussFIR(img, {sigma}, {1, 0}, {"mirror"}, 3));
+            std::vector<float> dy = host(dip::GaussFIR(img, {sigma}, {0, 1}, {"mirror"}, 3));
+            std::vector<float> dxx = host(dip::GaussFIR(img, {sigma}, {2, 0}, {"mirror"}, 3));
+            std::vector<float> dyy = host(dip::GaussFIR(img, {sigma}, {0, 2}, {"mirror"}, 3));
+            std::vector<float> dxy = host(dip::GaussFIR(img, {sigma}, {1, 1}, {"mirror"}, 3));
+            dip::ImageArray g, hs;
+            dip::Gradient(img, g, {sigma}, "FIR", {"mirror"});
+            dip::Hessian(img, hs, {sigma}, "best", {"mirror"});
+            CHECK(g.size() == 2 && host(g[0]) == dx && host(g[1]) == dy, "dip::Gradient elements are the derivatives");
+            CHECK(hs.size() == 3 && host(hs[0]) == dxx && host(hs[1]) == dyy && host(hs[2]) == dxy,
+                  "dip::Hessian elements in dip's order");
+            std::vector<float> lap = host(dip::Laplace(img, {sigma}, "FIR", {"mirror"}));
+            std::vector<float> mag = host(dip::GradientMagnitude(img, {sigma}, "FIR", {"mirror"}));
+            bool lap_ok = true, mag_ok = true;
+            for (size_t i = 0; i < W * H; ++i) {
+                const float l = dxx[i] + dyy[i];
+                const float a = dx[i] * dx[i];
+                const float b = dy[i] * dy[i];
+                const float m = std::sqrt(a + b);
+                lap_ok &= std::memcmp(&l, &lap[i], 4) == 0;
+                mag_ok &= std::memcmp(&m, &mag[i], 4) == 0;
+            }
+            CHECK(lap_ok, "dip::Laplace = dxx + dyy in float");
+            CHECK(mag_ok, "dip::GradientMagnitude = sqrt(dx*dx + dy*dy) in float");
+        }
         // protected output of another type keeps its type (clamp_cast)
         dip::Image prot({W, H}, 1, dip::DT_DFLOAT);
         prot.Protect();

@@ -1,11 +1,14 @@
 """Writes the golden fixtures under tests/golden/ from the REFERENCE library itself
 (oracle/_ref/libdipref.so, built from the reference sources by oracle/ref.mk).
 
-    make -f oracle/ref.mk -j8 && python tests/golden/make_golden.py [--only extrap]
+    make -f oracle/ref.mk -j8 && python tests/golden/make_golden.py [--only extrap|deriv]
 
 Outputs (all small):
   extrap_cases.npz   input and dip::Gauss output of tests/cases.py:EXTRAP_CASES (the
                      boundary conditions that extrapolate from the edge values)
+  deriv_cases.npz    input and result (tensor elements on the first axis) of
+                     tests/cases.py:DERIV_CASES (dip::Gradient, GradientMagnitude,
+                     Hessian, Laplace with the FIR method)
   taps.npz           MakeHalfGaussian(sigma, order, 3, sfloat/dfloat) for TAP_SIGMAS x orders 0..3
   small_cases.npz    input and dip::Gauss output of every case in tests/cases.py:SMALL_CASES
   large_cases.json   SHA-256 of the exact output bytes + sampled pixels for LARGE_CASES
@@ -41,10 +44,23 @@ def write_cases(case_list, fname, seed0):
     np.savez_compressed(os.path.join(HERE, fname), **arrs)
 
 
+def write_derivative_cases():
+    arrs = {}
+    for i, (name, fn, shape, dt, sig, bc) in enumerate(cases.DERIV_CASES):
+        x = cases.make_input(shape, dt, seed=4000 + i)
+        arrs[name + "__in"] = x
+        arrs[name + "__out"] = oc.ref_derivative(fn, x, sig, bc)
+    np.savez_compressed(os.path.join(HERE, "deriv_cases.npz"), **arrs)
+
+
 def main():
     assert oc.ref_available(), "build oracle/_ref/libdipref.so first (make -f oracle/ref.mk)"
-    write_cases(cases.EXTRAP_CASES, "extrap_cases.npz", 3000)
-    if "--only" in sys.argv and sys.argv[sys.argv.index("--only") + 1] == "extrap":
+    only = sys.argv[sys.argv.index("--only") + 1] if "--only" in sys.argv else None
+    if only in (None, "extrap"):
+        write_cases(cases.EXTRAP_CASES, "extrap_cases.npz", 3000)
+    if only in (None, "deriv"):
+        write_derivative_cases()
+    if only is not None:
         return
     # taps
     taps = {}

@@ -54,6 +54,9 @@ def ref():
         lib.dipref_half_gaussian.argtypes = [D, I, D, I, P, I]
         lib.dipref_separable.argtypes = [P, I, P, I, I, P, P, P, P, I, P, I, P]
         lib.dipref_last_error.restype = ctypes.c_char_p
+        if hasattr(lib, "dipref_derivative"):
+            lib.dipref_derivative.argtypes = [ctypes.c_char_p, P, I, P, I, I, ctypes.POINTER(I), I, P, P, I, P, I,
+                                              ctypes.c_char_p, D]
         _ref = lib
     return _ref
 
@@ -200,3 +203,66 @@ def jacobian_t(shape, sigmas, orders, bc, w, truncation=3.0):
         out[j] = float((gauss_fir(e, sigmas, orders, bc, truncation) * w).sum())
         ef[j] = 0.0
     return out.reshape(shape)
+
+
+DERIVATIVE_FUNCTIONS = ("gradient", "gradmag", "hessian", "laplace")
+
+
+def ref_derivative(which, x, sigmas, bc=("mirror",), method="gaussfir", truncation=3.0):
+    """The reference's dip::Gradient / GradientMagnitude / Hessian / Laplace; returns the
+    tensor elements stacked on a new first axis (element order as dip stores them)."""
+    x = np.ascontiguousarray(x)
+    odt = _flex(x.dtype)
+    cap = 64
+    out = np.zeros((cap,) + x.shape, odt)
+    sizes = _sizes(x)
+    sig = np.ascontiguousarray(list(sigmas), np.float64)
+    bcl = list(bc) if not isinstance(bc, str) else [bc]
+    arr = (ctypes.c_char_p * max(1, len(bcl)))(*[b.encode() for b in bcl])
+    n = ctypes.c_int(0)
+    st = ref().dipref_derivative(which.encode(), _ptr(x), DT[x.dtype], _ptr(out), DT[odt], cap, ctypes.byref(n),
+                                 x.ndim, _ptr(sizes), _ptr(sig), len(sig), arr, len(bcl), method.encode(), truncation)
+    if st != 0:
+        raise RuntimeError(ref().dipref_last_error().decode())
+    return out[:n.value].copy()
+
+
+def derivative(which, x, sigmas, bc=("mirror",), process=None, truncation=3.0):
+    """The oracle's restatement of the derivative family (derivative.cpp:120-418): Gauss
+    derivatives from the oracle, combined in the reference's order and type (numpy
+    float32 / float64 arithmetic rounds each operation as the reference does)."""
+    nd = x.ndim
+    sig = list(sigmas) if len(sigmas) == nd else list(sigmas) * nd
+    proc = [True] * nd if process is None else list(process)
+    sizes = list(reversed(x.shape))
+    dims = [d for d in range(nd) if proc[d] and sizes[d] > 1 and sig[d] > 0]
+    def d_of(order):
+        return gauss_fir(x, sig, order, bc, truncation)
+    def unit(pairs):
+        o = [0] * nd
+        for d, v in pairs:
+            o[d] += v
+        return o
+    if which == "gradient":
+        return np.stack([d_of(unit([(d, 1)])) for d in dims])
+    if which == "hessian":
+        el = [d_of(unit([(d, 2)])) for d in dims]
+        for j in range(1, len(dims)):
+            for i in range(j):
+                el.append(d_of(unit([(dims[i], 1), (dims[j], 1)])))
+        return np.stack(el)
+    if which == "laplace":
+        out = d_of(unit([(dims[0], 2)]))
+        for d in dims[1:]:
+            out = out + d_of(unit([(d, 2)]))
+        return out[None]
+    if which == "gradmag":
+        out = d_of(unit([(dims[0], 1)]))
+        if len(dims) == 1:
+            return np.abs(out)[None]
+        out = out * out
+        for d in dims[1:]:
+            t = d_of(unit([(d, 1)]))
+            out = out + t * t
+        return np.sqrt(out)[None]
+    raise ValueError(which)

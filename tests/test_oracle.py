@@ -22,6 +22,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TAPS = np.load(os.path.join(HERE, "golden", "taps.npz"))
 GOLD = np.load(os.path.join(HERE, "golden", "small_cases.npz"))
 EXTRAP = np.load(os.path.join(HERE, "golden", "extrap_cases.npz"))
+DERIV = np.load(os.path.join(HERE, "golden", "deriv_cases.npz"))
 
 
 @pytest.mark.parametrize("sigma", cases.TAP_SIGMAS)
@@ -253,3 +254,34 @@ def test_antisym_reflect_bounces():
     x = np.array([1.0, 3.0], np.float32)  # slope 2: the reflection continues it
     ext = extension(x, 4, "antisym reflect")
     assert np.array_equal(ext, [-7, -5, -3, -1, 1, 3, 5, 7, 9, 11])
+
+
+@pytest.mark.parametrize("case", cases.DERIV_CASES, ids=[c[0] for c in cases.DERIV_CASES])
+def test_oracle_derivative_family_matches_reference(case):
+    """dip::Gradient / GradientMagnitude / Hessian / Laplace: the oracle's composition
+    of its Gauss derivatives against the reference's own results, bit for bit."""
+    name, fn, shape, dt, sig, bc = case
+    x = DERIV[name + "__in"]
+    exp = DERIV[name + "__out"]
+    got = oc.derivative(fn, x, sig, bc)
+    assert got.shape == exp.shape and got.dtype == exp.dtype
+    assert np.array_equal(got.view(np.uint8), exp.view(np.uint8))
+
+
+def test_derivative_family_identities():
+    """What the mathematics fixes: on a quadratic ramp the Gaussian gradient is the
+    exact derivative away from the edges, the Laplace is the constant trace, and the
+    gradient magnitude is the norm of the gradient."""
+    yy, xx = np.mgrid[0:40, 0:48].astype(np.float64)
+    img = 0.5 * xx ** 2 + 2.0 * yy ** 2 + 3.0 * xx * yy
+    g = oc.derivative("gradient", img, [2.0], ["mirror"])
+    h = oc.derivative("hessian", img, [2.0], ["mirror"])
+    lap = oc.derivative("laplace", img, [2.0], ["mirror"])[0]
+    mag = oc.derivative("gradmag", img, [2.0], ["mirror"])[0]
+    inner = (slice(12, -12), slice(12, -12))
+    assert np.allclose(g[0][inner], (xx + 3 * yy)[inner], rtol=1e-9, atol=1e-6)
+    assert np.allclose(g[1][inner], (4 * yy + 3 * xx)[inner], rtol=1e-9, atol=1e-6)
+    assert np.allclose(h[0][inner], 1.0, atol=1e-6) and np.allclose(h[1][inner], 4.0, atol=1e-6)
+    assert np.allclose(h[2][inner], 3.0, atol=1e-6)
+    assert np.allclose(lap[inner], 5.0, atol=1e-6)
+    assert np.allclose(mag, np.sqrt(g[0] ** 2 + g[1] ** 2), rtol=1e-12)
