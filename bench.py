@@ -229,6 +229,7 @@ def main():
         extra.update(bench_volume(dip, torch, cases, dev))
         extra.update(bench_batched(dip, torch, np, dev, L))
         extra.update(bench_backward(dip, torch, dev))
+        extra.update(bench_derivatives(dip, torch, x))
         extra.update(bench_fp32_vs_fp64(dip, torch, np, cases, dev))
     del outs
     cpu = None
@@ -425,6 +426,37 @@ def bench_batched(dip, torch, np, dev, L):
         loop.close()
     except Exception as e:
         out["batched_uneven_512"] = {"error": repr(e)}
+    return out
+
+
+def bench_derivatives(dip, torch, x):
+    """The derivative family on the 8192^2 uint8 input (FIR): time per call and input
+    pixels per second; each call is n Gaussian derivative passes (plus one element-wise
+    pass for GradientMagnitude and Laplace)."""
+    out = {}
+    try:
+        res = {}
+        n = x.numel()
+        for s in (1.0, 4.0):
+            for name, fn in (("Gradient", dip.Gradient), ("Hessian", dip.Hessian), ("Laplace", dip.Laplace),
+                             ("GradientMagnitude", dip.GradientMagnitude)):
+                for _ in range(2):
+                    y = fn(x, [s], "FIR", ["mirror"])
+                torch.cuda.synchronize()
+                del y
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(3):
+                    y = fn(x, [s], "FIR", ["mirror"])
+                e1.record()
+                torch.cuda.synchronize()
+                del y
+                ms = e0.elapsed_time(e1) / 3
+                res[f"{name}_s{int(s)}"] = {"ms": round(ms, 4), "GPix_s": round(n / (ms * 1e-3) / 1e9, 2)}
+        out["derivative_family_8192sq_u8"] = res
+        log(f"derivative family: {res}")
+    except Exception as e:  # report, do not hide
+        out["derivative_family_8192sq_u8"] = {"error": repr(e)}
     return out
 
 
