@@ -298,8 +298,11 @@ static int64_t pick_chunk(int64_t len, int64_t base, int granule, int border, in
     int64_t want = std::max<int64_t>(1, (target_blocks + base - 1) / std::max<int64_t>(base, 1));
     int64_t chunk = (len + want - 1) / want;
     chunk = ((chunk + granule - 1) / granule) * granule;
-    // the halo (2*border rows) is re-read by each chunk: keep it under ~1/8 of the chunk
+    // the halo (2*border rows) is re-read by each chunk: keep it under ~1/8 of the chunk,
+    // unless that leaves fewer than 4 workgroups per CU (small images: then up to 1/2)
     int64_t minc = ((16 * static_cast<int64_t>(border) + granule - 1) / granule) * granule;
+    if (base * ((len + minc - 1) / minc) < 1024)
+        minc = ((4 * static_cast<int64_t>(border) + granule - 1) / granule) * granule;
     chunk = std::max<int64_t>(chunk, std::max<int64_t>(granule, minc));
     int64_t full = ((len + granule - 1) / granule) * granule;
     return std::min(chunk, full);

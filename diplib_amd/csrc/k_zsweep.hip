@@ -1,4 +1,4 @@
-// K2R: sweep along a strided axis for short symmetric filters (radius R <= 8, a
+// K2R: sweep along a strided axis for short symmetric filters (radius R <= 12, a
 // compile-time constant).  One thread per position of the lane axis, as K2, but
 // the window is a register array that slides down the chunk: each step keeps the
 // 2R samples it shares with the next step, so every input sample is loaded once
@@ -126,6 +126,10 @@ hipError_t zsweep_r(int r, int sym, bool fast, const PassArgs<TB> &a, int64_t e,
         case 6: return zsweep_sym<TI, TB, 6>(sym, fast, a, e, nc, no, st);
         case 7: return zsweep_sym<TI, TB, 7>(sym, fast, a, e, nc, no, st);
         case 8: return zsweep_sym<TI, TB, 8>(sym, fast, a, e, nc, no, st);
+        case 9: return zsweep_sym<TI, TB, 9>(sym, fast, a, e, nc, no, st);
+        case 10: return zsweep_sym<TI, TB, 10>(sym, fast, a, e, nc, no, st);
+        case 11: return zsweep_sym<TI, TB, 11>(sym, fast, a, e, nc, no, st);
+        case 12: return zsweep_sym<TI, TB, 12>(sym, fast, a, e, nc, no, st);
     }
     return hipErrorInvalidValue;
 }

@@ -8,6 +8,6 @@ constexpr int kSweepP = 8;  // outputs per thread and step, sweep kernel
 constexpr int kColPF = 8;   // outputs per thread and step, column-tile sweep (float)
 constexpr int kColPD = 4;   // outputs per thread and step, column-tile sweep (double)
 constexpr int kColsweepMinBorder = 16;  // shorter filters on strided axes use the global sweep
-constexpr int kZSweepMaxR = 8;          // the register-window sweep takes symmetric radii 1..8
+constexpr int kZSweepMaxR = 12;         // the register-window sweep takes symmetric radii 1..12
 
 }  // namespace dipamd
