@@ -99,8 +99,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--case", nargs=2, metavar=("SIGMA", "ORDERS"),
-                    help="profiling mode: run one case of the sweep (e.g. 16 00) --steps times, then a "
-                         "256 MiB device copy as the calibration kernel for the traffic counters")
+                    help="profiling mode: run one case of the sweep (e.g. 16 00; or 'vol mirror' for the "
+                         "512x512x256 sfloat volume, sigma (2,2,1)) --steps times, then a 256 MiB device copy "
+                         "as the calibration kernel for the traffic counters")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; the multi-GPU (sharded) path is a design only "
@@ -118,6 +119,18 @@ def main():
     dev = torch.device("cuda:0")
     log(f"device {torch.cuda.get_device_name(0)}; library {L.dipamd_version().decode()}")
 
+    if a.case and a.case[0] == "vol":
+        xv = torch.from_numpy(cases.make_input((256, 512, 512), "float32", 3, 0.01)).to(dev)
+        yv = torch.empty_like(xv)
+        for _ in range(a.steps):
+            dip.GaussFIR(xv, [2.0, 2.0, 1.0], [0], [a.case[1]], 3.0, out=yv)
+        src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(a.steps):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"case volume {a.case[1]}: {a.steps} calls, then {a.steps} copies of 256 MiB")
+        return
     x_np = cases.make_input((SIZE, SIZE), "uint8", seed=2)
     x = torch.from_numpy(x_np).to(dev)
     if a.case:
