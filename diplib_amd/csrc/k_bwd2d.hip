@@ -41,20 +41,33 @@ namespace {
 // (n-1-r <= c <= n-2), sign -1 for the antisymmetric mirror; zero order: every
 // l < 0 onto c = 0 and every l > n-1 onto c = n-1.  At l = -t only samples k - t
 // (k >= t) are inside the line, at l = n-1+t only n-1+t-k (negated for odd filters).
-template <int SYM, typename F>
+// The mirror sums run over k = 1..RMAX (the compile-time radius class) with the terms
+// outside c..r selected away, so the reads are independent of each other and of the
+// sums (all in flight at once) instead of a chain of dependent loads.
+template <int SYM, int RMAX, typename F>
 __device__ __forceinline__ float fold_sum(const float *ct, int r, int bc, int n, int c, F src) {
     float acc = 0.f;
     if (bc == DIPAMD_BC_SYMMETRIC_MIRROR || bc == DIPAMD_BC_ASYMMETRIC_MIRROR) {
         const float sg = bc == DIPAMD_BC_ASYMMETRIC_MIRROR ? -1.f : 1.f;
         if (c >= 1 && c <= r) {
             float t = 0.f;
-            for (int k = c; k <= r; ++k) t = __builtin_fmaf(ct[k], src(k - c), t);
+#pragma unroll
+            for (int k = 1; k <= RMAX; ++k) {
+                const bool in = k >= c && k <= r;
+                const float v = src(in ? k - c : 0);
+                t = __builtin_fmaf(ct[k], in ? v : 0.f, t);
+            }
             acc = __builtin_fmaf(sg, t, acc);
         }
         if (c >= n - 1 - r && c <= n - 2) {
             const int tt = n - 1 - c;
             float t = 0.f;
-            for (int k = tt; k <= r; ++k) t = __builtin_fmaf(ct[k], src(n - 1 + tt - k), t);
+#pragma unroll
+            for (int k = 1; k <= RMAX; ++k) {
+                const bool in = k >= tt && k <= r;
+                const float v = src(in ? n - 1 + tt - k : n - 1);
+                t = __builtin_fmaf(ct[k], in ? v : 0.f, t);
+            }
             acc = __builtin_fmaf(SYM == SYM_EVEN ? sg : -sg, t, acc);
         }
     } else if (bc == DIPAMD_BC_ZERO_ORDER_EXTRAPOLATE) {
@@ -288,8 +301,8 @@ __global__ __launch_bounds__(kBlock) void k_bwd2d(const BwdArgs a) {
                 const int t = i0 + row;
                 if (t >= ny || (t > ry && t < ny - 1 - ry)) continue;
                 const int ip = row * ROWF + ipos_lp<LP>(col);
-                hbuf[ip] += fold_sum<SY>(a.tya, ry, a.bcy, ny, t, gsrc);
-                h2buf[ip] += fold_sum<SY>(a.tdya, ry, a.bcy, ny, t, gsrc);
+                hbuf[ip] += fold_sum<SY, R>(a.tya, ry, a.bcy, ny, t, gsrc);
+                h2buf[ip] += fold_sum<SY, R>(a.tdya, ry, a.bcy, ny, t, gsrc);
             }
             lds_barrier();  // folded h rows ready
         }
@@ -305,7 +318,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd2d(const BwdArgs a) {
                 if (t >= c0 && t < c0 + TXC && t < nx) {
                     const float *hrow0 = hbuf + row * ROWF;
                     auto hsrc = [&](int q) { return hrow0[ipos_lp<LP>(q - cs0)]; };
-                    v = fold_sum<SX>(a.txa, rx, a.bcx, nx, t, hsrc);
+                    v = fold_sum<SX, R>(a.txa, rx, a.bcx, nx, t, hsrc);
                 }
                 foldx[row * Gm::FOLDW + k] = v;
             }
