@@ -7,10 +7,11 @@ device tensors.  Importing the package does not touch the GPU.
 from ._lib import DipError, lib  # noqa: F401
 from .autograd import gauss_fir_autograd, gauss_fir_backward  # noqa: F401
 from .batch import GaussBatch  # noqa: F401
-from .derivative import Derivative, Gradient, GradientMagnitude, Hessian, Laplace  # noqa: F401
+from .derivative import (Derivative, Dgg, Gradient, GradientMagnitude, Hessian, Laplace,  # noqa: F401
+                         LaplaceMinusDgg, LaplacePlusDgg, StructureTensor)
 from .gauss import (Gauss, GaussFIR, MakeHalfGaussian, OneDimensionalFilter,  # noqa: F401
                     SeparableConvolution, best_gauss_method, gauss_radius, image_struct)
 
 __all__ = ["Gauss", "GaussFIR", "SeparableConvolution", "OneDimensionalFilter", "MakeHalfGaussian", "DipError",
            "gauss_radius", "best_gauss_method", "lib", "Derivative", "Gradient", "GradientMagnitude", "Hessian",
-           "Laplace"]
+           "Laplace", "Dgg", "LaplacePlusDgg", "LaplaceMinusDgg", "StructureTensor"]

@@ -1,8 +1,9 @@
-// The derivative family on the Gaussian FIR path (derivative.cpp:120-418 of the
-// reference): dip::Gradient, dip::GradientMagnitude, dip::Hessian and dip::Laplace
-// with the FIR method.  Each Gaussian derivative is the plain drop-in call
-// (dipamd_gauss_fir); GradientMagnitude and Laplace combine them in one element-wise
-// pass (k_elementwise.hip) with the reference's operation order and type.
+// The derivative family on the Gaussian FIR path (derivative.cpp:120-521 of the
+// reference) and the structure tensor (structure_tensor.cpp:32-57): dip::Gradient,
+// GradientMagnitude, Hessian, Laplace, Dgg, LaplacePlusDgg, LaplaceMinusDgg and
+// StructureTensor with the FIR method.  Each Gaussian derivative is the plain drop-in
+// call (dipamd_gauss_fir); the combined forms add one element-wise pass
+// (k_elementwise.hip) with the reference's operation order and type.
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -13,7 +14,7 @@
 #include "sep_kernels.hpp"
 
 namespace dipamd {
-hipError_t launch_combine(bool is_double, int op, const void *const *d, int nd, void *out, int64_t n,
+hipError_t launch_combine(bool is_double, int op, const void *const *d, int nd, void *const *out, int64_t n,
                           hipStream_t stream);
 }  // namespace dipamd
 
@@ -39,6 +40,125 @@ bool contiguous(const dipamd_image *im) {
     return true;
 }
 
+int array_param(const double *v, int32_t nv, int nd, double dflt, double *out) {
+    if (nv != 0 && nv != 1 && nv != nd)
+        return set_error(DIPAMD_E_ARRAY_LENGTH, "Array parameter has the wrong number of elements");
+    for (int d = 0; d < nd; ++d) out[d] = nv == 0 ? dflt : (nv == 1 ? v[0] : v[d]);
+    return DIPAMD_OK;
+}
+
+int check_outputs(int function, const dipamd_image *in, const dipamd_image *outs, int32_t nouts, int want) {
+    int st;
+    if (nouts != want) return set_error(DIPAMD_E_ARRAY_LENGTH, "wrong number of output images for this function");
+    for (int k = 0; k < nouts; ++k) {
+        if ((st = check_image(&outs[k], "out")) != DIPAMD_OK) return st;
+        if (outs[k].ndim != in->ndim) return set_error(DIPAMD_E_SIZES, "Sizes don't match");
+        for (int d = 0; d < in->ndim; ++d)
+            if (outs[k].sizes[d] != in->sizes[d]) return set_error(DIPAMD_E_SIZES, "Sizes don't match");
+        // the reference strips an output that aliases the input (derivative.cpp:156-158);
+        // here the caller owns the memory, so an aliasing output is an error
+        if ((function == DIPAMD_FN_GRADIENT || function == DIPAMD_FN_HESSIAN ||
+             function == DIPAMD_FN_STRUCTURE_TENSOR) &&
+            views_overlap(to_view(in), to_view(&outs[k]), in->ndim))
+            return set_error(DIPAMD_E_PARAMETER, "an output image overlaps the input");
+    }
+    return DIPAMD_OK;
+}
+
+// Scratch images of the buffer type, contiguous, released together.
+struct Scratch {
+    hipStream_t stream;
+    std::vector<void *> bufs;
+    ~Scratch() {
+        for (void *p : bufs) scratch_free(p, stream);
+    }
+    int make(const dipamd_image *in, int dt, dipamd_image *t) {
+        int64_t npx = 1;
+        for (int d = 0; d < in->ndim; ++d) npx *= in->sizes[d];
+        void *p = nullptr;
+        const hipError_t e = scratch_alloc(&p, static_cast<size_t>(npx) * (dt == DIPAMD_DT_DFLOAT ? 8 : 4), stream);
+        if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("scratch allocation: ") + hipGetErrorString(e));
+        bufs.push_back(p);
+        t->origin = p;
+        t->dtype = dt;
+        t->ndim = in->ndim;
+        int64_t stride = 1;
+        for (int d = 0; d < DIPAMD_MAX_DIMS; ++d) {
+            t->sizes[d] = d < in->ndim ? in->sizes[d] : 1;
+            t->strides[d] = d < in->ndim ? stride : 0;
+            if (d < in->ndim) stride *= in->sizes[d];
+        }
+        return DIPAMD_OK;
+    }
+};
+
+// The Gaussian derivatives of `in` with the given per-dim orders (one list of nd
+// orders per image), each into `dst[k]`.
+int derivatives(const dipamd_image *in, const std::vector<std::vector<int32_t>> &orders, const dipamd_image *dst,
+                const double *sig, const int32_t *bc, int32_t nbc, double truncation, uint32_t flags, void *stream) {
+    for (size_t k = 0; k < orders.size(); ++k) {
+        const int st = dipamd_gauss_fir(in, &dst[k], sig, in->ndim, orders[k].data(), in->ndim, bc, nbc, truncation,
+                                        flags, stream);
+        if (st != DIPAMD_OK) return st;
+    }
+    return DIPAMD_OK;
+}
+
+// Orders of the gradient (first derivative along each of `dims`) and of the Hessian
+// (second derivatives on the diagonal, then the mixed ones above the diagonal column
+// by column: derivative.cpp:348-369).
+std::vector<std::vector<int32_t>> gradient_orders(int nd, const std::vector<int> &dims) {
+    std::vector<std::vector<int32_t>> o;
+    for (int d : dims) {
+        o.emplace_back(nd, 0);
+        o.back()[d] = 1;
+    }
+    return o;
+}
+std::vector<std::vector<int32_t>> hessian_orders(int nd, const std::vector<int> &dims) {
+    std::vector<std::vector<int32_t>> o;
+    for (int d : dims) {
+        o.emplace_back(nd, 0);
+        o.back()[d] = 2;
+    }
+    for (size_t j = 1; j < dims.size(); ++j)
+        for (size_t i = 0; i < j; ++i) {
+            o.emplace_back(nd, 0);
+            o.back()[dims[i]] = 1;
+            o.back()[dims[j]] = 1;
+        }
+    return o;
+}
+
+// Combine `terms` (contiguous scratch images) into the outputs with element-wise
+// operation `op`; outputs that are not contiguous are written through a scratch copy.
+int combine(int op, bool is_double, const std::vector<dipamd_image> &terms, int n, const dipamd_image *outs,
+            int nouts, Scratch &scr, const dipamd_image *in) {
+    std::vector<const void *> d;
+    for (const auto &t : terms) d.push_back(t.origin);
+    std::vector<void *> o(static_cast<size_t>(nouts));
+    std::vector<dipamd_image> staged(static_cast<size_t>(nouts));
+    int st;
+    for (int k = 0; k < nouts; ++k) {
+        if (contiguous(&outs[k])) {
+            o[k] = outs[k].origin;
+        } else {
+            if ((st = scr.make(in, outs[k].dtype, &staged[k])) != DIPAMD_OK) return st;
+            o[k] = staged[k].origin;
+        }
+    }
+    int64_t npx = 1;
+    for (int dd = 0; dd < in->ndim; ++dd) npx *= in->sizes[dd];
+    const hipError_t e = launch_combine(is_double, op, d.data(), n, o.data(), npx, scr.stream);
+    if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    for (int k = 0; k < nouts; ++k) {
+        if (o[k] == outs[k].origin) continue;
+        st = cast_from_buffer(to_view(&staged[k]), to_view(&outs[k]), in->ndim, scr.stream);
+        if (st != DIPAMD_OK) return st;
+    }
+    return DIPAMD_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -47,9 +167,13 @@ int dipamd_derivative_outputs(int32_t function, int32_t ndiff) {
     if (ndiff < 1 || ndiff > DIPAMD_MAX_DIMS) return set_error(DIPAMD_E_PARAMETER, "Dimensionality not supported");
     switch (function) {
         case DIPAMD_FN_GRADIENT: return ndiff;
-        case DIPAMD_FN_HESSIAN: return ndiff * (ndiff + 1) / 2;
+        case DIPAMD_FN_HESSIAN:
+        case DIPAMD_FN_STRUCTURE_TENSOR: return ndiff * (ndiff + 1) / 2;
         case DIPAMD_FN_GRADIENT_MAGNITUDE:
-        case DIPAMD_FN_LAPLACE: return 1;
+        case DIPAMD_FN_LAPLACE:
+        case DIPAMD_FN_DGG:
+        case DIPAMD_FN_LAPLACE_PLUS_DGG:
+        case DIPAMD_FN_LAPLACE_MINUS_DGG: return 1;
     }
     return set_error(DIPAMD_E_PARAMETER, "unknown derivative function");
 }
@@ -59,125 +183,89 @@ int dipamd_derivative(int32_t function, const dipamd_image *in, const dipamd_ima
                       int32_t nprocess, double truncation, uint32_t flags, void *stream) {
     int st;
     if ((st = check_image(in, "in")) != DIPAMD_OK) return st;
+    if (function == DIPAMD_FN_STRUCTURE_TENSOR)
+        return set_error(DIPAMD_E_PARAMETER, "the structure tensor takes dipamd_structure_tensor");
     const int nd = in->ndim;
     // ArrayUseParameter: sigmas default 1, process default all (derivative.cpp:127-129)
     double sig[DIPAMD_MAX_DIMS];
     uint8_t proc[DIPAMD_MAX_DIMS];
-    if (nsigmas != 0 && nsigmas != 1 && nsigmas != nd)
-        return set_error(DIPAMD_E_ARRAY_LENGTH, "Array parameter has the wrong number of elements");
+    if ((st = array_param(sigmas, nsigmas, nd, 1.0, sig)) != DIPAMD_OK) return st;
     if (nprocess != 0 && nprocess != 1 && nprocess != nd)
         return set_error(DIPAMD_E_ARRAY_LENGTH, "Array parameter has the wrong number of elements");
-    for (int d = 0; d < nd; ++d) {
-        sig[d] = nsigmas == 0 ? 1.0 : (nsigmas == 1 ? sigmas[0] : sigmas[d]);
-        proc[d] = nprocess == 0 ? 1 : (nprocess == 1 ? process[0] : process[d]);
-    }
+    for (int d = 0; d < nd; ++d) proc[d] = nprocess == 0 ? 1 : (nprocess == 1 ? process[0] : process[d]);
     const std::vector<int> dims = gradient_dims(in, sig, proc);
     const int n = static_cast<int>(dims.size());
     if (n < 1) return set_error(DIPAMD_E_PARAMETER, "Dimensionality not supported");
     const int want = dipamd_derivative_outputs(function, n);
     if (want < 0) return want;
-    if (nouts != want) return set_error(DIPAMD_E_ARRAY_LENGTH, "wrong number of output images for this function");
-    for (int k = 0; k < nouts; ++k) {
-        if ((st = check_image(&outs[k], "out")) != DIPAMD_OK) return st;
-        if (outs[k].ndim != nd) return set_error(DIPAMD_E_SIZES, "Sizes don't match");
-        for (int d = 0; d < nd; ++d)
-            if (outs[k].sizes[d] != in->sizes[d]) return set_error(DIPAMD_E_SIZES, "Sizes don't match");
-        // the reference strips an output that aliases the input (derivative.cpp:156-158);
-        // here the caller owns the memory, so an aliasing output is an error
-        if (function != DIPAMD_FN_GRADIENT_MAGNITUDE && function != DIPAMD_FN_LAPLACE &&
-            views_overlap(to_view(in), to_view(&outs[k]), nd))
-            return set_error(DIPAMD_E_PARAMETER, "an output image overlaps the input");
-    }
-    int32_t order[DIPAMD_MAX_DIMS] = {0};
-    auto derivative = [&](const dipamd_image *out) {
-        return dipamd_gauss_fir(in, out, sig, nd, order, nd, bc, nbc, truncation, flags, stream);
-    };
-    if (function == DIPAMD_FN_GRADIENT) {  // derivative.cpp:160-169
-        for (int i = 0; i < n; ++i) {
-            order[dims[i]] = 1;
-            if ((st = derivative(&outs[i])) != DIPAMD_OK) return st;
-            order[dims[i]] = 0;
-        }
-        return DIPAMD_OK;
-    }
-    if (function == DIPAMD_FN_HESSIAN) {  // derivative.cpp:348-369
-        int k = 0;
-        for (int i = 0; i < n; ++i) {  // the diagonal first
-            order[dims[i]] = 2;
-            if ((st = derivative(&outs[k++])) != DIPAMD_OK) return st;
-            order[dims[i]] = 0;
-        }
-        for (int j = 1; j < n; ++j) {  // then above the diagonal, column by column
-            for (int i = 0; i < j; ++i) {
-                order[dims[i]] = 1;
-                order[dims[j]] = 1;
-                if ((st = derivative(&outs[k++])) != DIPAMD_OK) return st;
-                order[dims[i]] = 0;
-                order[dims[j]] = 0;
-            }
-        }
-        return DIPAMD_OK;
-    }
-    // GradientMagnitude (derivative.cpp:174-210) and Laplace (:374-418): the terms in
-    // buffer-type scratch images, combined in one element-wise pass
+    if ((st = check_outputs(function, in, outs, nouts, want)) != DIPAMD_OK) return st;
+    if (function == DIPAMD_FN_GRADIENT)  // derivative.cpp:160-169
+        return derivatives(in, gradient_orders(nd, dims), outs, sig, bc, nbc, truncation, flags, stream);
+    if (function == DIPAMD_FN_HESSIAN)  // derivative.cpp:348-369
+        return derivatives(in, hessian_orders(nd, dims), outs, sig, bc, nbc, truncation, flags, stream);
+    // GradientMagnitude (derivative.cpp:174-210), Laplace (:374-418) and the Dgg family
+    // (:420-521): the terms in buffer-type scratch images, combined in one pass
     const int buf_dt = suggest_flex(in->dtype);
-    const bool is_double = buf_dt == DIPAMD_DT_DFLOAT;
     if (outs[0].dtype != buf_dt)
         return set_error(DIPAMD_E_UNSUPPORTED, "the output must have the floating-point type of the input");
-    const hipStream_t hs = static_cast<hipStream_t>(stream);
-    int64_t npx = 1;
-    for (int d = 0; d < nd; ++d) npx *= in->sizes[d];
-    const size_t bytes = static_cast<size_t>(npx) * (is_double ? 8 : 4);
-    std::vector<void *> bufs;
-    auto release = [&]() {
-        for (void *p : bufs) scratch_free(p, hs);
-    };
-    const void *terms[DIPAMD_MAX_DIMS];
-    for (int i = 0; i < n; ++i) {
-        void *p = nullptr;
-        hipError_t e = scratch_alloc(&p, bytes, hs);
-        if (e != hipSuccess) {
-            release();
-            return set_error(DIPAMD_E_HIP, std::string("scratch allocation: ") + hipGetErrorString(e));
+    std::vector<std::vector<int32_t>> orders;
+    int op = 0;
+    if (function == DIPAMD_FN_LAPLACE) {
+        for (int d : dims) {
+            orders.emplace_back(nd, 0);
+            orders.back()[d] = 2;
         }
-        bufs.push_back(p);
-        dipamd_image t;
-        t.origin = p;
-        t.dtype = buf_dt;
-        t.ndim = nd;
-        int64_t stride = 1;
-        for (int d = 0; d < DIPAMD_MAX_DIMS; ++d) {
-            t.sizes[d] = d < nd ? in->sizes[d] : 1;
-            t.strides[d] = d < nd ? stride : 0;
-            if (d < nd) stride *= in->sizes[d];
-        }
-        order[dims[i]] = function == DIPAMD_FN_LAPLACE ? 2 : 1;
-        st = derivative(&t);
-        order[dims[i]] = 0;
-        if (st != DIPAMD_OK) {
-            release();
-            return st;
-        }
-        terms[i] = p;
+        op = 0;
+    } else if (function == DIPAMD_FN_GRADIENT_MAGNITUDE) {
+        orders = gradient_orders(nd, dims);
+        op = 1;
+    } else {
+        orders = gradient_orders(nd, dims);
+        for (auto &o : hessian_orders(nd, dims)) orders.push_back(o);
+        op = function == DIPAMD_FN_DGG ? 2 : (function == DIPAMD_FN_LAPLACE_PLUS_DGG ? 3 : 4);
     }
-    const int op = function == DIPAMD_FN_LAPLACE ? 0 : 1;
-    hipError_t e;
-    if (contiguous(&outs[0])) {
-        e = launch_combine(is_double, op, terms, n, outs[0].origin, npx, hs);
-    } else {  // combine into the first term, then copy into the strided output
-        e = launch_combine(is_double, op, terms, n, bufs[0], npx, hs);
-        if (e == hipSuccess) {
-            int64_t sz[DIPAMD_MAX_DIMS];
-            for (int d = 0; d < nd; ++d) sz[d] = in->sizes[d];
-            st = cast_from_buffer(contiguous_view(bufs[0], buf_dt, nd, sz), to_view(&outs[0]), nd, hs);
-            if (st != DIPAMD_OK) {
-                release();
-                return st;
-            }
-        }
+    Scratch scr{static_cast<hipStream_t>(stream), {}};
+    std::vector<dipamd_image> terms(orders.size());
+    for (auto &t : terms)
+        if ((st = scr.make(in, buf_dt, &t)) != DIPAMD_OK) return st;
+    if ((st = derivatives(in, orders, terms.data(), sig, bc, nbc, truncation, flags, stream)) != DIPAMD_OK) return st;
+    return combine(op, buf_dt == DIPAMD_DT_DFLOAT, terms, n, outs, 1, scr, in);
+}
+
+int dipamd_structure_tensor(const dipamd_image *in, const dipamd_image *outs, int32_t nouts,
+                            const double *gradient_sigmas, int32_t ngs, const double *tensor_sigmas, int32_t nts,
+                            const int32_t *bc, int32_t nbc, double truncation, uint32_t flags, void *stream) {
+    int st;
+    if ((st = check_image(in, "in")) != DIPAMD_OK) return st;
+    const int nd = in->ndim;
+    double gsig[DIPAMD_MAX_DIMS], tsig[DIPAMD_MAX_DIMS];
+    if ((st = array_param(gradient_sigmas, ngs, nd, 1.0, gsig)) != DIPAMD_OK) return st;
+    if ((st = array_param(tensor_sigmas, nts, nd, 5.0, tsig)) != DIPAMD_OK) return st;
+    uint8_t proc[DIPAMD_MAX_DIMS];
+    for (int d = 0; d < nd; ++d) proc[d] = 1;
+    const std::vector<int> dims = gradient_dims(in, gsig, proc);
+    const int n = static_cast<int>(dims.size());
+    if (n < 1) return set_error(DIPAMD_E_PARAMETER, "Dimensionality not supported");
+    const int want = dipamd_derivative_outputs(DIPAMD_FN_STRUCTURE_TENSOR, n);
+    if ((st = check_outputs(DIPAMD_FN_STRUCTURE_TENSOR, in, outs, nouts, want)) != DIPAMD_OK) return st;
+    const int buf_dt = suggest_flex(in->dtype);
+    for (int k = 0; k < nouts; ++k)
+        if (outs[k].dtype != buf_dt)
+            return set_error(DIPAMD_E_UNSUPPORTED, "the output must have the floating-point type of the input");
+    // gradient, products g g^T, then each element smoothed with the tensor sigmas
+    // (structure_tensor.cpp:53-56)
+    Scratch scr{static_cast<hipStream_t>(stream), {}};
+    std::vector<dipamd_image> g(static_cast<size_t>(n));
+    for (auto &t : g)
+        if ((st = scr.make(in, buf_dt, &t)) != DIPAMD_OK) return st;
+    if ((st = derivatives(in, gradient_orders(nd, dims), g.data(), gsig, bc, nbc, truncation, flags, stream)) !=
+        DIPAMD_OK)
+        return st;
+    if ((st = combine(5, buf_dt == DIPAMD_DT_DFLOAT, g, n, outs, nouts, scr, in)) != DIPAMD_OK) return st;
+    for (int k = 0; k < nouts; ++k) {
+        st = dipamd_gauss_fir(&outs[k], &outs[k], tsig, nd, nullptr, 0, bc, nbc, truncation, flags, stream);
+        if (st != DIPAMD_OK) return st;
     }
-    release();
-    if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
     return DIPAMD_OK;
 }
 

@@ -1,5 +1,6 @@
 // Element-wise combinations of the derivative family (derivative.cpp:190-209,
-// 404-416), over contiguous buffer-type images, in the reference's operation order
+// 404-416, 424-484) and of the structure tensor (structure_tensor.cpp:55;
+// arithmetic.cpp:116-170), over contiguous buffer-type images, in the reference's order
 // and type: each product and sum rounded in the buffer type (no contraction), the
 // square root correctly rounded (computed in double, which rounds once more exactly
 // for a square root of a float), dip::Abs for a single dim.
@@ -9,12 +10,15 @@ namespace dipamd {
 
 namespace {
 
+constexpr int kEwMaxIn = 2 * DIPAMD_MAX_DIMS + DIPAMD_MAX_DIMS * (DIPAMD_MAX_DIMS + 1) / 2;
+
 struct EwArgs {
-    const void *d[DIPAMD_MAX_DIMS];  // the derivative images
-    void *out;
+    const void *d[kEwMaxIn];  // the derivative images (gradient, then Hessian for Dgg)
+    void *out[DIPAMD_MAX_DIMS * (DIPAMD_MAX_DIMS + 1) / 2];
     int64_t n;
     int32_t nd;
-    int32_t op;  // 0: Laplace (sum), 1: gradient magnitude
+    int32_t op;  // 0: Laplace (sum), 1: gradient magnitude, 2/3/4: Dgg, Laplace + Dgg,
+                 // Laplace - Dgg, 5: the symmetric products g g^T of the structure tensor
 };
 
 template <typename T>
@@ -29,9 +33,57 @@ __device__ __forceinline__ double ew_sqrt<double>(double v) {
 }
 
 template <typename T>
+__device__ __forceinline__ T ew_in(const EwArgs &a, int k, int64_t i) {
+    return static_cast<const T *>(a.d[k])[i];
+}
+
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_combine(const EwArgs a) {
     for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < a.n;
          i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int n = a.nd;
+        if (a.op == 5) {
+            // StructureTensor's g * g^T (arithmetic.cpp:116-170): a sum started at 0 for each
+            // element, the diagonal first, then above the diagonal column by column
+            int e = 0;
+            for (int c = 0; c < n; ++c) {
+                const T g = ew_in<T>(a, c, i);
+                static_cast<T *>(a.out[e++])[i] = T(0) + g * g;
+            }
+            for (int c = 1; c < n; ++c)
+                for (int r = 0; r < c; ++r)
+                    static_cast<T *>(a.out[e++])[i] = T(0) + ew_in<T>(a, r, i) * ew_in<T>(a, c, i);
+            continue;
+        }
+        if (a.op >= 2) {
+            // the Dgg family (derivative.cpp:424-484): g (n images), then H (diagonal, then
+            // above the diagonal column by column); each image operation rounded in T
+            const T g0 = ew_in<T>(a, 0, i);
+            T gs = g0 * g0;
+            T o = gs * ew_in<T>(a, n, i);
+            for (int k = 1; k < n; ++k) {
+                const T gk = ew_in<T>(a, k, i);
+                T t = gk * gk;
+                gs = gs + t;
+                t = t * ew_in<T>(a, n + k, i);
+                o = o + t;
+            }
+            for (int r = 1; r < n - 1; ++r) {  // the reference's loop starts at 1
+                for (int c = r + 1; c < n; ++c) {
+                    T t = ew_in<T>(a, r, i) * ew_in<T>(a, c, i);
+                    t = t * ew_in<T>(a, n + n + c * (c - 1) / 2 + r, i);
+                    o = o * T(1) + t * T(2);  // LinearCombination(out, tmp, 1, 2)
+                }
+            }
+            o = gs == T(0) ? T(0) : o / gs;  // SafeDivide
+            if (a.op != 2) {
+                T tr = T(0);  // Trace: std::accumulate from 0 over the diagonal
+                for (int k = 0; k < n; ++k) tr = tr + ew_in<T>(a, n + k, i);
+                o = a.op == 3 ? tr + o : tr - o;
+            }
+            static_cast<T *>(a.out[0])[i] = o;
+            continue;
+        }
         const T d0 = static_cast<const T *>(a.d[0])[i];
         T s;
         if (a.op == 0) {
@@ -48,17 +100,22 @@ __global__ __launch_bounds__(kBlock) void k_combine(const EwArgs a) {
             }
             s = ew_sqrt<T>(s);
         }
-        static_cast<T *>(a.out)[i] = s;
+        static_cast<T *>(a.out[0])[i] = s;
     }
 }
 
 }  // namespace
 
-hipError_t launch_combine(bool is_double, int op, const void *const *d, int nd, void *out, int64_t n,
+// d: the input images (op 0/1: nd derivatives; op 2-4: nd gradient images then the
+// nd(nd+1)/2 Hessian elements; op 5: nd gradient images); out: one image (op 5:
+// nd(nd+1)/2 images).
+hipError_t launch_combine(bool is_double, int op, const void *const *d, int nd, void *const *out, int64_t n,
                           hipStream_t stream) {
     EwArgs a;
-    for (int k = 0; k < DIPAMD_MAX_DIMS; ++k) a.d[k] = k < nd ? d[k] : nullptr;
-    a.out = out;
+    const int nin = op >= 2 && op <= 4 ? nd + nd * (nd + 1) / 2 : nd;
+    const int nout = op == 5 ? nd * (nd + 1) / 2 : 1;
+    for (int k = 0; k < kEwMaxIn; ++k) a.d[k] = k < nin ? d[k] : nullptr;
+    for (int k = 0; k < DIPAMD_MAX_DIMS * (DIPAMD_MAX_DIMS + 1) / 2; ++k) a.out[k] = k < nout ? out[k] : nullptr;
     a.n = n;
     a.nd = nd;
     a.op = op;

@@ -1,6 +1,7 @@
 """The derivative family on the Gaussian FIR path, with PyDIP's names and arguments:
-dip.Derivative, dip.Gradient, dip.GradientMagnitude, dip.Hessian, dip.Laplace
-(derivative.cpp:88-418; linear.h:666, 697, 862, 904).
+dip.Derivative, dip.Gradient, dip.GradientMagnitude, dip.Hessian, dip.Laplace, dip.Dgg,
+dip.LaplacePlusDgg, dip.LaplaceMinusDgg (derivative.cpp:88-521; linear.h:666, 697,
+862, 904) and dip.StructureTensor (structure_tensor.cpp:32-57; analysis.h:454).
 
 Tensor results (Gradient: the partial derivatives in dim order; Hessian: the
 symmetric matrix as dip stores it, diagonal first, then the upper triangle column by
@@ -105,3 +106,63 @@ def Laplace(in_, sigmas=(1.0,), method="best", boundaryCondition=(), process=(),
             fast=False):
     """dip::Laplace (derivative.cpp:374-418), Gaussian methods."""
     return _scalar_result(_lib.FN_LAPLACE, in_, sigmas, method, boundaryCondition, process, truncation, out, fast)
+
+
+def Dgg(in_, sigmas=(1.0,), method="best", boundaryCondition=(), process=(), truncation=3.0, out=None, fast=False):
+    """dip::Dgg (derivative.cpp:486-496): the second derivative in the gradient
+    direction, as the reference computes it (for two dims its sum over the off-diagonal
+    Hessian terms starts at index 1 and so leaves the mixed term out)."""
+    return _scalar_result(_lib.FN_DGG, in_, sigmas, method, boundaryCondition, process, truncation, out, fast)
+
+
+def LaplacePlusDgg(in_, sigmas=(1.0,), method="best", boundaryCondition=(), process=(), truncation=3.0, out=None,
+                   fast=False):
+    """dip::LaplacePlusDgg (derivative.cpp:498-508)."""
+    return _scalar_result(_lib.FN_LAPLACE_PLUS_DGG, in_, sigmas, method, boundaryCondition, process, truncation, out,
+                          fast)
+
+
+def LaplaceMinusDgg(in_, sigmas=(1.0,), method="best", boundaryCondition=(), process=(), truncation=3.0, out=None,
+                    fast=False):
+    """dip::LaplaceMinusDgg (derivative.cpp:510-520)."""
+    return _scalar_result(_lib.FN_LAPLACE_MINUS_DGG, in_, sigmas, method, boundaryCondition, process, truncation,
+                          out, fast)
+
+
+def StructureTensor(in_, mask=None, gradientSigmas=(1.0,), tensorSigmas=(5.0,), method="best",
+                    boundaryCondition=(), truncation=3.0, fast=False):
+    """dip::StructureTensor (structure_tensor.cpp:32-57) without a mask: the outer
+    product of the gradient, each element smoothed with `tensorSigmas`.  Elements on the
+    last axis, as dip stores a symmetric matrix (diagonal first, then the upper triangle
+    column by column)."""
+    if mask is not None:
+        raise NotImplementedError("StructureTensor with a mask (normalized differential convolution) is outside "
+                                  "this library")
+    x, was_np = _to_device(in_)
+    nd = x.dim()
+    gs = [float(s) for s in _as_list(gradientSigmas)] or [1.0]
+    ts = [float(s) for s in _as_list(tensorSigmas)] or [5.0]
+    gs = gs * nd if len(gs) == 1 else gs
+    ts = ts * nd if len(ts) == 1 else ts
+    if len(gs) != nd or len(ts) != nd:
+        raise _lib.DipError(_lib.E_ARRAY_LENGTH, "Array parameter has the wrong number of elements")
+    sizes = list(reversed(x.shape))
+    dims = [d for d in range(nd) if sizes[d] > 1 and gs[d] > 0.0]
+    if not dims:
+        raise _lib.DipError(_lib.E_PARAMETER, "Dimensionality not supported")
+    _fir_method(method, gs, [1] * nd)
+    _fir_method(method, ts, [0] * nd)
+    n = check(lib().dipamd_derivative_outputs(_lib.FN_STRUCTURE_TENSOR, len(dims)))
+    planes = torch.empty((n,) + tuple(x.shape), dtype=_flex_torch(x.dtype), device=x.device)
+    bcs = _bc_codes(boundaryCondition)
+    imgs = (_lib.DipamdImage * n)(*[image_struct(planes[i]) for i in range(n)])
+    g_arr = (ctypes.c_double * nd)(*gs)
+    t_arr = (ctypes.c_double * nd)(*ts)
+    b_arr = (ctypes.c_int32 * max(1, len(bcs)))(*bcs)
+    a_in = image_struct(x)
+    with torch.cuda.device(x.device):
+        check(lib().dipamd_structure_tensor(ctypes.byref(a_in), imgs, n, g_arr, nd, t_arr, nd, b_arr, len(bcs),
+                                            float(truncation), _lib.FAST if fast else _lib.EXACT,
+                                            _stream_handle(x.device)))
+    y = planes.movedim(0, -1)
+    return y.cpu().numpy() if was_np else y

@@ -120,27 +120,40 @@ int dipamd_gauss_fir(const dipamd_image *in, const dipamd_image *out, const doub
                      const int32_t *orders, int32_t norders, const int32_t *bc, int32_t nbc, double truncation,
                      uint32_t flags, void *stream);
 
-/* The derivative family on the Gaussian FIR path (src/linear/derivative.cpp:120-418;
+/* The derivative family on the Gaussian FIR path (src/linear/derivative.cpp:120-521;
  * include/diplib/linear.h:666, 697, 862, 904): dip::Gradient, dip::GradientMagnitude,
- * dip::Hessian and dip::Laplace with method "FIR" (or "best" resolving to FIR), each Gaussian derivative
+ * dip::Hessian, dip::Laplace and the Dgg family with method "FIR" (or "best" resolving to FIR), each Gaussian derivative
  * computed as dipamd_gauss_fir computes it and combined in the reference's order and
  * arithmetic type.  The differentiated dims are those with process != 0 (NULL: all),
  * size > 1 and sigma > 0; sigmas apply to every dim (a dim that is not differentiated
  * is still smoothed).  `outs` holds dipamd_derivative_outputs(function, n) images
  * of the sizes of `in`, n = the number of differentiated dims: Gradient the partial
  * derivatives in dim order; Hessian the symmetric matrix as dip stores it, the n
- * diagonal elements first, then the upper triangle column by column; GradientMagnitude
- * and Laplace one image, of type SuggestFlex(in). */
+ * diagonal elements first, then the upper triangle column by column; GradientMagnitude,
+ * Laplace and the Dgg family one image, of type SuggestFlex(in). */
 enum {
     DIPAMD_FN_GRADIENT = 0,
     DIPAMD_FN_GRADIENT_MAGNITUDE = 1,
     DIPAMD_FN_HESSIAN = 2,
-    DIPAMD_FN_LAPLACE = 3
+    DIPAMD_FN_LAPLACE = 3,
+    DIPAMD_FN_DGG = 4,               /* dip::Dgg (derivative.cpp:486-496) */
+    DIPAMD_FN_LAPLACE_PLUS_DGG = 5,  /* dip::LaplacePlusDgg (:498-508) */
+    DIPAMD_FN_LAPLACE_MINUS_DGG = 6, /* dip::LaplaceMinusDgg (:510-520) */
+    DIPAMD_FN_STRUCTURE_TENSOR = 7   /* only through dipamd_structure_tensor */
 };
 int dipamd_derivative_outputs(int32_t function, int32_t ndiff);
 int dipamd_derivative(int32_t function, const dipamd_image *in, const dipamd_image *outs, int32_t nouts,
                       const double *sigmas, int32_t nsigmas, const int32_t *bc, int32_t nbc, const uint8_t *process,
                       int32_t nprocess, double truncation, uint32_t flags, void *stream);
+
+/* Replaces dip::StructureTensor without a mask (src/analysis/structure_tensor.cpp:32-57;
+ * include/diplib/analysis.h:454): the gradient (gradient_sigmas, default 1), its outer
+ * product g g^T as dip stores a symmetric matrix (diagonal first, then the upper
+ * triangle column by column), each element smoothed by dip::Gauss with tensor_sigmas
+ * (default 5), the FIR method.  outs: n(n+1)/2 images of type SuggestFlex(in). */
+int dipamd_structure_tensor(const dipamd_image *in, const dipamd_image *outs, int32_t nouts,
+                            const double *gradient_sigmas, int32_t ngs, const double *tensor_sigmas, int32_t nts,
+                            const int32_t *bc, int32_t nbc, double truncation, uint32_t flags, void *stream);
 
 /* Replaces dip::SeparableConvolution (include/diplib/linear.h:114-130;
  * src/linear/convolution.cpp:319-421) for real filters.  nfilters: 1 or ndim.

@@ -479,6 +479,49 @@ inline Image Laplace(Image const &in, FloatArray sigmas = {1.0}, String const &m
     Laplace(in, out, std::move(sigmas), method, boundaryCondition, std::move(process), truncation);
     return out;
 }
+/* dip::Dgg, LaplacePlusDgg, LaplaceMinusDgg (derivative.cpp:486-520). */
+inline void Dgg(Image const &in, Image &out, FloatArray sigmas = {1.0}, String const &method = "best",
+                StringArray const &boundaryCondition = {}, BooleanArray process = {}, dfloat truncation = 3) {
+    detail::ScalarResult(DIPAMD_FN_DGG, in, out, sigmas, method, boundaryCondition, process, truncation);
+}
+inline void LaplacePlusDgg(Image const &in, Image &out, FloatArray sigmas = {1.0}, String const &method = "best",
+                           StringArray const &boundaryCondition = {}, BooleanArray process = {},
+                           dfloat truncation = 3) {
+    detail::ScalarResult(DIPAMD_FN_LAPLACE_PLUS_DGG, in, out, sigmas, method, boundaryCondition, process, truncation);
+}
+inline void LaplaceMinusDgg(Image const &in, Image &out, FloatArray sigmas = {1.0}, String const &method = "best",
+                            StringArray const &boundaryCondition = {}, BooleanArray process = {},
+                            dfloat truncation = 3) {
+    detail::ScalarResult(DIPAMD_FN_LAPLACE_MINUS_DGG, in, out, sigmas, method, boundaryCondition, process,
+                         truncation);
+}
+/* dip::StructureTensor (analysis.h:454; structure_tensor.cpp:32-57) without a mask: the
+ * symmetric-matrix elements as an array of images. */
+inline void StructureTensor(Image const &in, Image const &mask, ImageArray &out, FloatArray gradientSigmas = {1.0},
+                            FloatArray tensorSigmas = {5.0}, String const &method = "best",
+                            StringArray const &boundaryCondition = {}, dfloat truncation = 3) {
+    if (!in.IsForged()) throw ParameterError("Image is not forged");
+    if (mask.IsForged()) throw ParameterError("StructureTensor with a mask is not provided by this library");
+    detail::CheckFirMethod(method, gradientSigmas);
+    detail::CheckFirMethod(method, tensorSigmas);
+    const int n = dipamd_derivative_outputs(DIPAMD_FN_STRUCTURE_TENSOR,
+                                            detail::DifferentiatedDims(in, gradientSigmas, {}));
+    if (n < 0) detail::Check(n);
+    out.clear();
+    std::vector<dipamd_image> views;
+    for (int k = 0; k < n; ++k) {
+        out.emplace_back(in.Sizes(), 1, DataType::SuggestFlex(in.DataType()));
+        views.push_back(out.back().View());
+    }
+    std::vector<int32_t> bc = detail::BoundaryCodes(boundaryCondition);
+    dipamd_image a = in.View();
+    detail::Check(dipamd_structure_tensor(&a, views.data(), n, gradientSigmas.data(),
+                                          static_cast<int32_t>(gradientSigmas.size()), tensorSigmas.data(),
+                                          static_cast<int32_t>(tensorSigmas.size()), bc.data(),
+                                          static_cast<int32_t>(bc.size()), truncation, DIPAMD_EXACT,
+                                          dipamd::CurrentStream()));
+}
+
 /* dip::Derivative (derivative.cpp:88-115), Gaussian methods: dip::Gauss. */
 inline void Derivative(Image const &in, Image &out, UnsignedArray derivativeOrder, FloatArray sigmas = {1.0},
                        String const &method = "best", StringArray const &boundaryCondition = {},

@@ -12,6 +12,7 @@
 #include "diplib/linear.h"
 #include "diplib/generation.h"
 #include "diplib/multithreading.h"
+#include "diplib/analysis.h"
 
 namespace {
 
@@ -142,7 +143,8 @@ int dipref_separable( void const* in, int in_dtype, void* out, int out_dtype, in
 }
 
 // The derivative family on dip::Gauss (derivative.cpp:142-521): `which` is one of
-// "gradient", "gradmag", "hessian", "laplace", "dgg", "laplace+dgg", "laplace-dgg".
+// "gradient", "gradmag", "hessian", "laplace", "dgg", "laplace+dgg", "laplace-dgg", or
+// "structure:<tensor sigma>" (dip::StructureTensor, no mask, structure_tensor.cpp:32-57).
 // The result's tensor elements are copied, in the reference's element order, into
 // `out` as `cap` consecutive images of the input's sizes and of type `out_dtype`
 // (sfloat or dfloat); the element count is returned in *nelem.
@@ -164,6 +166,9 @@ int dipref_derivative( char const* which, void const* in, int in_dtype, void* ou
       else if( w == "dgg" ) { dip::Dgg( imgIn, res, s, method, bc, {}, truncation ); }
       else if( w == "laplace+dgg" ) { dip::LaplacePlusDgg( imgIn, res, s, method, bc, {}, truncation ); }
       else if( w == "laplace-dgg" ) { dip::LaplaceMinusDgg( imgIn, res, s, method, bc, {}, truncation ); }
+      else if( w.substr( 0, 10 ) == "structure:" ) {  // "structure:<tensor sigma>"
+         dip::StructureTensor( imgIn, {}, res, s, { std::stod( w.substr( 10 )) }, method, bc, truncation );
+      }
       else { throw std::runtime_error( "unknown function" ); }
       int n = static_cast< int >( res.TensorElements() );
       *nelem = n;

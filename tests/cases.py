@@ -125,6 +125,20 @@ for fn in ("gradient", "gradmag", "hessian", "laplace"):
         (f"{fn}_f32_50x40_s0x3", fn, (40, 50), "float32", [0.0, 3.0], ["asym mirror"]),
     ]
 
+# The Dgg family and the structure tensor ("structure:<tensor sigma>"), same layout.
+DERIV2_CASES = []
+for fn in ("dgg", "laplace+dgg", "laplace-dgg", "structure:3"):
+    t = fn.replace("+", "p").replace("-", "m").replace(":", "")
+    DERIV2_CASES += [
+        (f"{t}_f32_64x48_s2", fn, (48, 64), "float32", [2.0], ["mirror"]),
+        (f"{t}_u8_97x61_s15_periodic", fn, (61, 97), "uint8", [1.5], ["periodic"]),
+        (f"{t}_f64_33x20_s3", fn, (20, 33), "float64", [3.0], ["mirror"]),
+        (f"{t}_f32_37x29x11_s221", fn, (11, 29, 37), "float32", [2.0, 2.0, 1.0], ["mirror"]),
+        (f"{t}_f32_1x40_s2", fn, (40, 1), "float32", [2.0], ["mirror"]),
+    ]
+# a flat image: the gradient is zero everywhere and SafeDivide gives 0
+DERIV2_CASES.append(("dgg_flat_f32_30x20", "dgg", (20, 30), "flat", [2.0], ["mirror"]))
+
 TAP_SIGMAS = [0.5, 0.8, 1.0, 1.5, 2.0, 3.0, 4.0, 5.0, 8.0, 16.0, 31.7]
 
 # Full-size workloads: (name, numpy shape, dtype, seed, impulses, sigmas, orders, bc, out dtype)

@@ -9,6 +9,8 @@ Outputs (all small):
   deriv_cases.npz    input and result (tensor elements on the first axis) of
                      tests/cases.py:DERIV_CASES (dip::Gradient, GradientMagnitude,
                      Hessian, Laplace with the FIR method)
+  deriv2_cases.npz   the same for tests/cases.py:DERIV2_CASES (dip::Dgg, LaplacePlusDgg,
+                     LaplaceMinusDgg, StructureTensor)
   taps.npz           MakeHalfGaussian(sigma, order, 3, sfloat/dfloat) for TAP_SIGMAS x orders 0..3
   small_cases.npz    input and dip::Gauss output of every case in tests/cases.py:SMALL_CASES
   large_cases.json   SHA-256 of the exact output bytes + sampled pixels for LARGE_CASES
@@ -44,13 +46,13 @@ def write_cases(case_list, fname, seed0):
     np.savez_compressed(os.path.join(HERE, fname), **arrs)
 
 
-def write_derivative_cases():
+def write_derivative_cases(case_list=None, fname="deriv_cases.npz", seed0=4000):
     arrs = {}
-    for i, (name, fn, shape, dt, sig, bc) in enumerate(cases.DERIV_CASES):
-        x = cases.make_input(shape, dt, seed=4000 + i)
+    for i, (name, fn, shape, dt, sig, bc) in enumerate(case_list or cases.DERIV_CASES):
+        x = np.ones(shape, np.float32) if dt == "flat" else cases.make_input(shape, dt, seed=seed0 + i)
         arrs[name + "__in"] = x
         arrs[name + "__out"] = oc.ref_derivative(fn, x, sig, bc)
-    np.savez_compressed(os.path.join(HERE, "deriv_cases.npz"), **arrs)
+    np.savez_compressed(os.path.join(HERE, fname), **arrs)
 
 
 def main():
@@ -60,6 +62,7 @@ def main():
         write_cases(cases.EXTRAP_CASES, "extrap_cases.npz", 3000)
     if only in (None, "deriv"):
         write_derivative_cases()
+        write_derivative_cases(cases.DERIV2_CASES, "deriv2_cases.npz", 5000)
     if only is not None:
         return
     # taps

@@ -265,4 +265,43 @@ def derivative(which, x, sigmas, bc=("mirror",), process=None, truncation=3.0):
             t = d_of(unit([(d, 1)]))
             out = out + t * t
         return np.sqrt(out)[None]
+    if which in ("dgg", "laplace+dgg", "laplace-dgg"):
+        # derivative.cpp:424-484, image operation by image operation
+        g = derivative("gradient", x, sigmas, bc, process, truncation)
+        H = derivative("hessian", x, sigmas, bc, process, truncation)
+        n = g.shape[0]
+        one = g.dtype.type(1)
+        two = g.dtype.type(2)
+        zero = g.dtype.type(0)
+        gs = g[0] * g[0]
+        out = gs * H[0]
+        for k in range(1, n):
+            t = g[k] * g[k]
+            gs = gs + t
+            t = t * H[k]
+            out = out + t
+        for r in range(1, n - 1):  # the reference's loop starts at 1
+            for c in range(r + 1, n):
+                t = g[r] * g[c]
+                t = t * H[n + c * (c - 1) // 2 + r]
+                out = out * one + t * two
+        with np.errstate(divide="ignore", invalid="ignore"):
+            out = np.where(gs == 0, zero, out / np.where(gs == 0, one, gs))
+        if which == "dgg":
+            return out[None]
+        tr = np.zeros_like(out)
+        for k in range(n):
+            tr = tr + H[k]
+        return (tr + out if which == "laplace+dgg" else tr - out)[None]
+    if which.startswith("structure:"):
+        # structure_tensor.cpp:53-56: g g^T (a sum started at 0), then Gauss per element
+        tsig = float(which.split(":")[1])
+        g = derivative("gradient", x, sigmas, bc, None, truncation)
+        n = g.shape[0]
+        zero = g.dtype.type(0)
+        el = [zero + g[c] * g[c] for c in range(n)]
+        for c in range(1, n):
+            for r in range(c):
+                el.append(zero + g[r] * g[c])
+        return np.stack([gauss_fir(e, [tsig], [0], bc, truncation) for e in el])
     raise ValueError(which)

@@ -15,15 +15,23 @@ import oracle_ctypes as oc
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 DERIV = np.load(os.path.join(HERE, "golden", "deriv_cases.npz"))
-FN = {"gradient": dip.Gradient, "gradmag": dip.GradientMagnitude, "hessian": dip.Hessian, "laplace": dip.Laplace}
+DERIV2 = np.load(os.path.join(HERE, "golden", "deriv2_cases.npz"))
+FN = {"gradient": dip.Gradient, "gradmag": dip.GradientMagnitude, "hessian": dip.Hessian, "laplace": dip.Laplace,
+      "dgg": dip.Dgg, "laplace+dgg": dip.LaplacePlusDgg, "laplace-dgg": dip.LaplaceMinusDgg}
 
 
 def _elements(fn, y):
     """Tensor elements on the first axis, as the fixtures hold them."""
     y = y.cpu().numpy()
-    if fn in ("gradient", "hessian"):
+    if fn in ("gradient", "hessian") or fn.startswith("structure"):
         return np.moveaxis(y, -1, 0)
     return y[None]
+
+
+def _call(fn, x, sig, method, bc):
+    if fn.startswith("structure:"):
+        return dip.StructureTensor(x, None, sig, [float(fn.split(":")[1])], method, bc)
+    return FN[fn](x, sig, method, bc)
 
 
 @pytest.mark.parametrize("case", cases.DERIV_CASES, ids=[c[0] for c in cases.DERIV_CASES])
@@ -65,3 +73,26 @@ def test_derivative_family_process_and_errors(cuda):
     # Derivative is dip::Gauss
     d = dip.Derivative(x, [1, 0], [2.0], "FIR")
     assert torch.equal(d, dip.GaussFIR(x, [2.0], [1, 0]))
+
+
+@pytest.mark.parametrize("case", cases.DERIV2_CASES, ids=[c[0] for c in cases.DERIV2_CASES])
+def test_dgg_and_structure_tensor_bitexact(cuda, case):
+    name, fn, shape, dt, sig, bc = case
+    x = DERIV2[name + "__in"]
+    exp = DERIV2[name + "__out"]
+    y = _call(fn, torch.from_numpy(x).to(cuda), sig, "FIR", bc)
+    torch.cuda.synchronize()
+    got = _elements(fn, y)
+    assert got.shape == exp.shape and got.dtype == exp.dtype
+    assert np.array_equal(got.view(np.uint8), exp.view(np.uint8)), \
+        f"{name}: max abs diff {np.abs(got.astype(np.float64) - exp.astype(np.float64)).max()}"
+
+
+@pytest.mark.parametrize("fn", ["dgg", "laplace+dgg", "laplace-dgg", "structure:4"])
+def test_dgg_and_structure_tensor_larger_vs_oracle(cuda, fn):
+    x = cases.make_input((300, 257), "uint8", seed=53)
+    y = _call(fn, torch.from_numpy(x).to(cuda), [2.0], "best", ["mirror"])
+    assert np.array_equal(_elements(fn, y), oc.derivative(fn, x, [2.0], ["mirror"]))
+    v = cases.make_input((20, 33, 28), "float32", seed=54)
+    y = _call(fn, torch.from_numpy(v).to(cuda), [1.0, 2.0, 1.5], "gaussFIR", ["periodic"])
+    assert np.array_equal(_elements(fn, y), oc.derivative(fn, v, [1.0, 2.0, 1.5], ["periodic"]))

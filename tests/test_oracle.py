@@ -23,6 +23,7 @@ TAPS = np.load(os.path.join(HERE, "golden", "taps.npz"))
 GOLD = np.load(os.path.join(HERE, "golden", "small_cases.npz"))
 EXTRAP = np.load(os.path.join(HERE, "golden", "extrap_cases.npz"))
 DERIV = np.load(os.path.join(HERE, "golden", "deriv_cases.npz"))
+DERIV2 = np.load(os.path.join(HERE, "golden", "deriv2_cases.npz"))
 
 
 @pytest.mark.parametrize("sigma", cases.TAP_SIGMAS)
@@ -285,3 +286,40 @@ def test_derivative_family_identities():
     assert np.allclose(h[2][inner], 3.0, atol=1e-6)
     assert np.allclose(lap[inner], 5.0, atol=1e-6)
     assert np.allclose(mag, np.sqrt(g[0] ** 2 + g[1] ** 2), rtol=1e-12)
+
+
+@pytest.mark.parametrize("case", cases.DERIV2_CASES, ids=[c[0] for c in cases.DERIV2_CASES])
+def test_oracle_dgg_and_structure_tensor_match_reference(case):
+    """dip::Dgg / LaplacePlusDgg / LaplaceMinusDgg / StructureTensor: the oracle's
+    image-by-image restatement against the reference's own results, bit for bit."""
+    name, fn, shape, dt, sig, bc = case
+    x = DERIV2[name + "__in"]
+    exp = DERIV2[name + "__out"]
+    got = oc.derivative(fn, x, sig, bc)
+    assert got.shape == exp.shape and got.dtype == exp.dtype
+    assert np.array_equal(got.view(np.uint8), exp.view(np.uint8))
+
+
+def test_dgg_and_structure_tensor_identities():
+    """On f = (a x + b y)^2 / 2 (g = s (a, b), s = a x + b y; H = [[a^2, ab], [ab, b^2]]):
+    the reference's Dgg sums g_i g_j H_ij over the diagonal and over i < j starting at
+    i = 1 (derivative.cpp:463-469), which in 2-D leaves out the mixed term: Dgg is
+    (a^4 + b^4) / (a^2 + b^2) there, not the curvature a^2 + b^2.  Laplace -+ Dgg follow.
+    The structure tensor of a ramp is the constant g g^T."""
+    yy, xx = np.mgrid[0:40, 0:48].astype(np.float64)
+    a, b = 0.6, -0.8
+    inner = (slice(14, -14), slice(14, -14))
+    q = 0.5 * (a * xx + b * yy) ** 2
+    dgg = oc.derivative("dgg", q, [2.0], ["mirror"])[0]
+    lmd = oc.derivative("laplace-dgg", q, [2.0], ["mirror"])[0]
+    lpd = oc.derivative("laplace+dgg", q, [2.0], ["mirror"])[0]
+    ref2d = (a ** 4 + b ** 4) / (a * a + b * b)
+    away = np.abs(a * xx + b * yy)[inner] > 1.0  # the gradient vanishes on the line s = 0
+    assert np.allclose(dgg[inner][away], ref2d, atol=1e-6)
+    assert np.allclose(lmd[inner][away], (a * a + b * b) - ref2d, atol=1e-6)
+    assert np.allclose(lpd[inner][away], (a * a + b * b) + ref2d, atol=1e-6)
+    st = oc.derivative("structure:2", a * xx + b * yy, [1.0], ["mirror"])
+    inner2 = (slice(16, -16), slice(16, -16))
+    assert np.allclose(st[0][inner2], a * a, atol=1e-6)
+    assert np.allclose(st[1][inner2], b * b, atol=1e-6)
+    assert np.allclose(st[2][inner2], a * b, atol=1e-6)
