@@ -236,7 +236,7 @@ View contiguous_view(void *p, int dt, int ndim, const int64_t *sizes) {
     return v;
 }
 
-static void fill_outer(Outer &o, const View &src, const View &dst, int ndim, int skip0, int skip1) {
+void fill_outer(Outer &o, const View &src, const View &dst, int ndim, int skip0, int skip1) {
     // remaining dims, ascending |input stride| for locality
     int dims[DIPAMD_MAX_DIMS];
     int n = 0;

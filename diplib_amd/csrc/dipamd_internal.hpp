@@ -44,6 +44,10 @@ class TraceScope {
     int index_;
 };
 
+struct Outer;
+// Geometry of the dims other than skip0 and skip1 (size > 1), ascending |input stride|.
+void fill_outer(Outer &o, const View &src, const View &dst, int ndim, int skip0, int skip1);
+
 int set_error(int code, const std::string &msg);
 const char *last_error();
 size_t dt_size(int dt);

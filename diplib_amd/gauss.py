@@ -140,10 +140,45 @@ def GaussFIR(in_, sigmas=(1.0,), derivativeOrder=(0,), boundaryCondition=(), tru
     return y
 
 
+IIR_DESIGN_METHODS = {"forward backward": _lib.IIR_FORWARD_BACKWARD, "discrete time fit": _lib.IIR_DISCRETE_TIME_FIT}
+
+
+def GaussIIR(in_, sigmas=(1.0,), derivativeOrder=(0,), boundaryCondition=(), filterOrder=(),
+             designMethod="discrete time fit", truncation=3.0, out=None):
+    """dip::GaussIIR (linear.h:407-429; gaussiir.cpp:741-806): the recursive Gaussian
+    and its derivatives up to order 4, computed in double, bit-identical to the
+    reference.  An empty `filterOrder` means 3 + order, capped at 5."""
+    x, was_np = _to_device(in_)
+    sig = [float(s) for s in _as_list(sigmas)]
+    ords = [int(o) for o in _as_list(derivativeOrder)]
+    fos = [int(f) for f in _as_list(filterOrder)]
+    nd = x.dim()
+    if len(sig) not in (0, 1, nd) or len(ords) not in (0, 1, nd) or len(fos) not in (0, 1, nd):
+        raise _lib.DipError(_lib.E_ARRAY_LENGTH, "Array parameter has the wrong number of elements")
+    if designMethod not in IIR_DESIGN_METHODS:
+        raise _lib.DipError(_lib.E_PARAMETER, "Invalid flag: " + str(designMethod))
+    bcs = _bc_codes(boundaryCondition)
+    y = _prepare_out(x, out, _flex_torch(x.dtype))
+    L = lib()
+    a_in, a_out = image_struct(x), image_struct(y)
+    s_arr = (ctypes.c_double * max(1, len(sig)))(*sig)
+    o_arr = (ctypes.c_int32 * max(1, len(ords)))(*ords)
+    f_arr = (ctypes.c_int32 * max(1, len(fos)))(*fos)
+    b_arr = (ctypes.c_int32 * max(1, len(bcs)))(*bcs)
+    with torch.cuda.device(x.device):
+        check(L.dipamd_gauss_iir(ctypes.byref(a_in), ctypes.byref(a_out), s_arr, len(sig), o_arr, len(ords), b_arr,
+                                 len(bcs), f_arr, len(fos), IIR_DESIGN_METHODS[designMethod], float(truncation), 0,
+                                 _stream_handle(x.device)))
+    if was_np and out is None:
+        return y.cpu().numpy()
+    return y
+
+
 def Gauss(in_, sigmas=(1.0,), derivativeOrder=(0,), method="best", boundaryCondition=(), truncation=3.0, out=None,
           fast=False):
-    """dip::Gauss (linear.h:452-472; derivative.cpp:62-86).  Only the FIR method is
-    part of this library: "best" resolving to FT or IIR raises NotImplementedError."""
+    """dip::Gauss (linear.h:452-472; derivative.cpp:62-86).  The FIR and IIR methods
+    are part of this library; "FT" (and "best" resolving to it) raises
+    NotImplementedError."""
     m = method[5:] if method[:5] == "gauss" else method
     sig = [float(s) for s in _as_list(sigmas)]
     ords = [int(o) for o in _as_list(derivativeOrder)]
@@ -151,8 +186,10 @@ def Gauss(in_, sigmas=(1.0,), derivativeOrder=(0,), method="best", boundaryCondi
         m = best_gauss_method(sig, ords)
     if m in ("FIR", "fir"):
         return GaussFIR(in_, sig, ords, boundaryCondition, truncation, out=out, fast=fast)
-    if m in ("FT", "ft", "IIR", "iir"):
-        raise NotImplementedError(f"dip::Gauss method '{m}' is outside this library (FIR only); pass method='FIR'")
+    if m in ("IIR", "iir"):
+        return GaussIIR(in_, sig, ords, boundaryCondition, (), "discrete time fit", truncation, out=out)
+    if m in ("FT", "ft"):
+        raise NotImplementedError(f"dip::Gauss method '{m}' is outside this library (FIR and IIR only)")
     raise _lib.DipError(_lib.E_PARAMETER, "Invalid flag: " + method)
 
 

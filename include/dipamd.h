@@ -120,6 +120,21 @@ int dipamd_gauss_fir(const dipamd_image *in, const dipamd_image *out, const doub
                      const int32_t *orders, int32_t norders, const int32_t *bc, int32_t nbc, double truncation,
                      uint32_t flags, void *stream);
 
+/* dip::GaussIIR (src/linear/gaussiir.cpp:741-806; include/diplib/linear.h:407-416):
+ * the recursive Gaussian and its derivatives up to order 4, in double, bit-identical to
+ * the reference.  sigmas / orders / filter_orders broadcast from 0 (defaults: sigma 1,
+ * order 0, filter order 3 + order capped at 5), 1 or ndim elements; dims with sigma <= 0
+ * or size 1 are not filtered.  design_method is one of DIPAMD_IIR_*.  `out` has the
+ * sizes of `in` (normally dtype SuggestFlex(in)); it may be `in` itself. */
+enum {
+    DIPAMD_IIR_FORWARD_BACKWARD = 1, /* "forward backward" (Young and van Vliet 1995) */
+    DIPAMD_IIR_DISCRETE_TIME_FIT = 2 /* "discrete time fit" (van Vliet et al. 1998), the default */
+};
+int dipamd_gauss_iir(const dipamd_image *in, const dipamd_image *out, const double *sigmas, int32_t nsigmas,
+                     const int32_t *orders, int32_t norders, const int32_t *bc, int32_t nbc,
+                     const int32_t *filter_orders, int32_t nfilter_orders, int32_t design_method,
+                     double truncation, uint32_t flags, void *stream);
+
 /* The derivative family on the Gaussian FIR path (src/linear/derivative.cpp:120-521;
  * include/diplib/linear.h:666, 697, 862, 904): dip::Gradient, dip::GradientMagnitude,
  * dip::Hessian, dip::Laplace and the Dgg family with method "FIR" (or "best" resolving to FIR), each Gaussian derivative

@@ -669,3 +669,373 @@ int orc_num_threads(void) {
     return 1;
 #endif
 }
+
+/* ===========================================================================
+ * dip::GaussIIR (gaussiir.cpp:54-806): the recursive Gaussian, in double.
+ * ========================================================================= */
+typedef struct { double re, im; } orc_cx;
+static orc_cx cx_mul(orc_cx a, orc_cx b) { orc_cx r = {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; return r; }
+static orc_cx cx_add(orc_cx a, orc_cx b) { orc_cx r = {a.re + b.re, a.im + b.im}; return r; }
+
+typedef struct {
+    long border;
+    int nn;
+    double a1[6], a2[6], b1[6], b2[6], cc;
+    int ma[6], ar[6]; /* iir_order_num / iir_order_den */
+} orc_iir;
+
+enum { IIR_FB = 1, IIR_DTF = 2 };
+
+/* FillPoleCoefficients (gaussiir.cpp:82-205): pp[0] holds the pole count. */
+static void iir_poles(int nn, orc_cx *pp, int order, int *method) {
+    static const double dtf5[3][5][2] = {
+        {{0.85991, 1.45235}, {0.85991, -1.45235}, {1.60953, 0.83009}, {1.60953, -0.83009}, {1.87040, 0.0}},
+        {{0.70237, 1.38717}, {0.70237, -1.38717}, {1.43280, 0.77903}, {1.43280, -0.77903}, {1.70346, 0.0}},
+        {{0.70381, 1.38271}, {0.70381, -1.38271}, {1.42239, 0.77978}, {1.42239, -0.77978}, {1.69319, 0.0}}};
+    static const double dtf4[3][4][2] = {
+        {{1.13231, 1.28122}, {1.13231, -1.28122}, {1.78532, 0.46766}, {1.78532, -0.46766}},
+        {{1.04198, 1.25046}, {1.04198, -1.25046}, {1.69337, 0.45006}, {1.69337, -0.45006}},
+        {{0.94576, 1.21364}, {0.94576, -1.21364}, {1.59892, 0.42668}, {1.59892, -0.42668}}};
+    static const double dtf3[3][3][2] = {
+        {{1.41650, 1.00829}, {1.41650, -1.00829}, {1.86131, 0.0}},
+        {{1.32094, 0.97057}, {1.32094, -0.97057}, {1.77635, 0.0}},
+        {{1.21969, 0.91724}, {1.21969, -0.91724}, {1.69485, 0.0}}};
+    static const double fb3[3][2] = {{2.1078345, 1.4058574}, {2.1078345, -1.4058574}, {2.1668048, 0.0}};
+    static const double fb4[4][2] = {{2.06954, 1.90416}, {2.06954, -1.90416}, {2.18403, 0.593635}, {2.18403, -0.593635}};
+    static const double fb5[5][2] = {{2.19406, 1.90251}, {2.19406, -1.90251}, {2.31029, 0.598022},
+                                     {2.31029, -0.598022}, {7.64071, 0.0}};
+    const double (*src)[2] = NULL;
+    int o = (order == 1 || order == 2) ? order : 0;
+    int k = nn;
+    if (*method == IIR_DTF) {
+        switch (nn) {
+            case 5: src = dtf5[o]; break;
+            case 4: src = dtf4[o]; break;
+            case 3: src = dtf3[o]; break;
+            case 2: { static const double p2[2][2] = {{1.69580, 0.59955}, {1.69580, -0.59955}}; src = p2; break; }
+            case 1: { static const double p1[1][2] = {{2.00000, 0.00000}}; src = p1; break; }
+            default: *method = IIR_FB; src = fb3; k = 3; break;
+        }
+    } else {
+        *method = IIR_FB;
+        switch (nn) {
+            case 5: src = fb5; break;
+            case 4: src = fb4; break;
+            default: src = fb3; k = 3; break;
+        }
+    }
+    pp[0].re = (double)k;
+    pp[0].im = 0.0;
+    for (int m = 1; m <= k; m++) { pp[m].re = src[m - 1][0]; pp[m].im = src[m - 1][1]; }
+}
+
+/* q2sigma (gaussiir.cpp:54-80). */
+static double iir_q2sigma(int nn, const orc_cx *pp, double q) {
+    double var = 0.0;
+    for (int mm = 1; mm <= nn - (nn & 1); mm += 2) {
+        double modulus = sqrt(pp[mm].re * pp[mm].re + pp[mm].im * pp[mm].im);
+        double phase = atan(pp[mm].im / pp[mm].re);
+        modulus = exp(log(modulus) / q);
+        phase = phase / q;
+        double re = modulus * cos(phase), im = modulus * sin(phase);
+        double d = 1 - 2 * re + re * re + im * im;
+        d *= d;
+        var += (4 * (re + (re - 2) * (re * re + im * im))) / d;
+    }
+    if (nn & 1) {
+        double modulus = sqrt(pp[nn].re * pp[nn].re + pp[nn].im * pp[nn].im);
+        double phase = atan(pp[nn].im / pp[nn].re);
+        modulus = exp(log(modulus) / q);
+        phase = phase / q;
+        double re = modulus * cos(phase);
+        var += (2 * re) / ((re - 1) * (re - 1));
+    }
+    return sqrt(var);
+}
+
+/* FilterCoefficients (gaussiir.cpp:207-231): the sum of all products of `start`
+ * poles chosen with decreasing indices from 1..stop, accumulated into *acc. */
+static void iir_coef(int start, int stop, const orc_cx *pp, orc_cx tmp, orc_cx *acc) {
+    if (start > 1) {
+        for (int ii = start; ii <= stop; ii++) iir_coef(start - 1, ii - 1, pp, cx_mul(tmp, pp[ii]), acc);
+    } else if (start == 1) {
+        for (int ii = 1; ii <= stop; ii++) *acc = cx_add(cx_mul(tmp, pp[ii]), *acc);
+    }
+}
+
+/* FillGaussIIRParams (gaussiir.cpp:233-425). Returns -1 for order > 4. */
+static int iir_params(double sigma, int order, int filter_order, int method, double truncation, orc_iir *P) {
+    memset(P, 0, sizeof(*P));
+    if (truncation <= 0.0) truncation = 3;
+    if (truncation > max_trunc(0)) truncation = max_trunc(0);
+    double bv = sigma * truncation + 0.5;
+    P->border = (long)bv < 5 ? 5 : (long)bv;
+    orc_cx pp[10];
+    iir_poles(filter_order, pp, order, &method);
+    int nn = (int)pp[0].re;
+    double q = 0;
+    if (method == IIR_FB) {
+        double q0 = -sigma * sigma, q1 = 0.0, q2 = 0.0;
+        for (int mm = 1; mm <= nn - (nn & 1); mm += 2) {
+            double re = pp[mm].re, im = pp[mm].im;
+            double t1 = 4 * (-1 + 3 * re - 3 * re * re - im * im + re * (re * re + im * im));
+            double t2 = 4 * (1 - 2 * re + re * re - im * im);
+            double d = 1 - 2 * re + re * re + im * im;
+            d *= d;
+            q1 += t1 / d;
+            q2 += t2 / d;
+        }
+        if (nn & 1) {
+            double d = pp[nn].re - 1.0;
+            q1 += 2.0 / d;
+            q2 += 2.0 / (d * d);
+        }
+        double t = sqrt(q1 * q1 - 4.0 * q2 * q0);
+        double qa = (-q1 + t) / (2 * q2), qb = (-q1 - t) / (2 * q2);
+        q = qa > qb ? qa : qb;
+        for (int mm = 1; mm <= nn; mm++) {
+            orc_cx s = {1.0 + (pp[mm].re - 1.0) / q, pp[mm].im / q};
+            pp[mm] = s;
+        }
+    } else {
+        q = sigma / 2.0;
+        double q0 = q, qs = iir_q2sigma(nn, pp, q);
+        int it = 0;
+        while (fabs(sigma - qs) > 0.000001 && it < 100) {
+            q += q0 - qs / 2.0;
+            qs = iir_q2sigma(nn, pp, q);
+            ++it;
+        }
+        for (int mm = 1; mm <= nn; mm++) {
+            double modulus = sqrt(pp[mm].re * pp[mm].re + pp[mm].im * pp[mm].im);
+            double phase = atan(pp[mm].im / pp[mm].re);
+            modulus = exp(log(modulus) / q);
+            phase = phase / q;
+            orc_cx s = {modulus * cos(phase), modulus * sin(phase)};
+            pp[mm] = s;
+        }
+    }
+    orc_cx bb[10];
+    for (int mm = 0; mm <= nn; mm++) {
+        orc_cx zero = {0.0, 0.0}, one = {1.0, 0.0};
+        bb[nn - mm] = mm == 0 ? one : zero;
+        if (mm > 0) iir_coef(mm, nn, pp, one, &bb[nn - mm]);
+    }
+    for (int mm = 0; mm <= nn; mm++) {
+        double s = pow(-1.0, (double)(nn - mm)) / bb[0].re;
+        bb[nn - mm].re *= s;
+        bb[nn - mm].im *= s;
+        P->b1[nn - mm] = P->b2[nn - mm] = bb[nn - mm].re;
+    }
+    P->cc = 0.0;
+    for (int mm = 0; mm <= nn; mm++) P->cc += P->b1[mm];
+    P->cc = P->cc * P->cc;
+    P->nn = nn;
+    P->ar[0] = P->ar[2] = P->ar[3] = P->ar[5] = nn;
+    P->ar[1] = P->ar[4] = 1;
+    switch (order) {
+        case 0: P->a1[0] = 1.0; P->a2[0] = 1.0; break;
+        case 1: P->a1[0] = 0.5; P->a1[2] = -0.5; P->a2[1] = 1.0; break;
+        case 2: P->a1[0] = 1.0; P->a1[1] = -1.0; P->a2[1] = 1.0; P->a2[0] = -1.0; break;
+        case 3: P->a1[0] = 1.0; P->a1[1] = -2.0; P->a1[2] = 1.0; P->a2[2] = 0.5; P->a2[0] = -0.5; break;
+        case 4: P->a1[0] = 1.0; P->a1[1] = -2.0; P->a1[2] = 1.0; P->a2[2] = 1.0; P->a2[1] = -2.0; P->a2[0] = 1.0; break;
+        default: return -1;
+    }
+    for (int j = 0; j < 6; j++) P->ma[j] = 7;
+    for (int j = 0; j < 6; j++) {
+        if (P->a1[j] != 0.0 && P->ma[1] > 6) P->ma[1] = j;
+        if (P->a2[j] != 0.0 && P->ma[4] > 6) P->ma[4] = j;
+    }
+    for (int j = 5; j >= 0; j--) {
+        if (P->a1[j] != 0.0 && P->ma[2] > 6) P->ma[2] = j;
+        if (P->a2[j] != 0.0 && P->ma[5] > 6) P->ma[5] = j;
+    }
+    P->ma[0] = P->ma[2];
+    P->ma[3] = P->ma[5];
+    return 0;
+}
+
+/* GaussIIRLineFilter::Filter (gaussiir.cpp:468-738) on one extended line p0 of
+ * `length` samples; p2 receives the whole extended result.  The unrolled loops of
+ * the reference for 3..5 poles are the recursion with a history that starts at the
+ * initial value; they are written here as that recursion, stopping where the
+ * reference's unrolled loop stops, then continued by the general loop. */
+static void iir_line(const orc_iir *P, const double *p0, double *p1, double *p2, long length) {
+    const double *a1 = P->a1, *a2 = P->a2, *b1 = P->b1, *b2 = P->b2;
+    const int *ma = P->ma, *ar = P->ar;
+    double c = P->cc;
+    long order1 = ar[0] > ma[0] ? ar[0] : ma[0];
+    long order2 = ar[3] > ma[3] ? ar[3] : ma[3];
+    int copy_f = ma[0] == 0 && a1[0] == 1.0, copy_b = ma[3] == 0 && a2[0] == 1.0;
+    double sum1 = 1.0 + b1[1] + b1[2] + b1[3] + b1[4] + b1[5];
+    double sum2 = 1.0 + b2[1] + b2[2] + b2[3] + b2[4] + b2[5];
+    long ii = 0;
+    double h[6];
+    /* --- causal --- */
+    int unrolled = 0;
+    long K = order1, start = 0;
+    double r = 0;
+    if (order1 >= 3 && order1 <= 5 && copy_f) {
+        r = p0[0] / sum1; unrolled = 1; start = 0;
+    } else if (order1 == 4 && a1[0] == 0.5 && a1[1] == 0.0 && a1[2] == -0.5 && a1[3] == 0.0) {
+        r = (p0[1] - p0[0]) / sum1; unrolled = 2;
+        p1[0] = p1[1] = r; start = 2;
+    } else if (order1 == 5 && a1[0] == 1.0 && a1[1] == -1.0 && a1[2] == 0.0 && a1[3] == 0.0) {
+        r = (p0[1] - p0[0]) / sum1; unrolled = 3;
+        p1[0] = r; start = 1;
+    }
+    if (unrolled) {
+        for (int k = 1; k <= 5; k++) h[k] = r; /* h[k] = y[ii - k] */
+        /* whole blocks of K samples while the block start is below length - K */
+        for (ii = start; ii < length - K;) {
+            for (long t = 0; t < K; t++, ii++) {
+                double v;
+                if (unrolled == 1) v = p0[ii];
+                else if (unrolled == 2) v = 0.5 * (p0[ii] - p0[ii - 2]);
+                else v = p0[ii] - p0[ii - 1];
+                for (int k = 1; k <= K; k++) v = v - b1[k] * h[k];
+                p1[ii] = v;
+                for (int k = 5; k > 1; k--) h[k] = h[k - 1];
+                h[1] = v;
+            }
+        }
+    }
+    {
+        double val = 0.0;
+        for (int jj = ma[1]; jj <= ma[2]; jj++) val += a1[jj] * p0[ma[2] - jj];
+        double r1 = val / sum1;
+        for (; ii < order1; ii++) p1[ii] = r1;
+        for (; ii < length; ii++) {
+            if (!copy_f) {
+                val = 0.0;
+                for (int jj = ma[1]; jj <= ma[2]; jj++) val += a1[jj] * p0[ii - jj];
+            } else {
+                val = p0[ii];
+            }
+            for (int jj = ar[1]; jj <= ar[2]; jj++) val -= b1[jj] * p1[ii - jj];
+            p1[ii] = val;
+        }
+    }
+    /* --- anti-causal --- */
+    ii = length - 1;
+    unrolled = 0;
+    K = order2;
+    if (order2 >= 3 && order2 <= 5 && copy_b) {
+        r = c * p1[length - 1] / sum2; unrolled = 1;
+    } else if (order2 == 4 && a2[0] == 0.0 && a2[1] == 1.0 && a2[2] == 0.0 && a2[3] == 0.0) {
+        r = c * p1[length - 1] / sum2; unrolled = 2;
+        p2[ii] = r; ii -= 1;
+    } else if (order2 == 5 && a2[0] == -1.0 && a2[1] == 1.0 && a2[2] == 0.0 && a2[3] == 0.0) {
+        r = c * (-p1[length - 2] + p1[length - 1]) / sum2; unrolled = 3;
+        p2[ii] = r; ii -= 1;
+    }
+    if (unrolled) {
+        for (int k = 1; k <= 5; k++) h[k] = r; /* h[k] = z[ii + k] */
+        while (ii >= K) {
+            for (long t = 0; t < K; t++, ii--) {
+                double v;
+                if (unrolled == 1) v = c * p1[ii];
+                else if (unrolled == 2) v = c * p1[ii + 1];
+                else v = c * (-p1[ii] + p1[ii + 1]);
+                for (int k = 1; k <= K; k++) v = v - b2[k] * h[k];
+                p2[ii] = v;
+                for (int k = 5; k > 1; k--) h[k] = h[k - 1];
+                h[1] = v;
+            }
+        }
+    }
+    {
+        double val = 0.0;
+        for (int jj = ma[4]; jj <= ma[5]; jj++) val += a2[jj] * p1[length - 1 - ma[5] + jj];
+        double r1 = val / sum2;
+        for (; ii > length - 1 - order2; ii--) p2[ii] = c * r1;
+        ii++;
+        while (ii > 0) {
+            ii--;
+            if (!copy_b) {
+                val = 0.0;
+                for (int jj = ma[4]; jj <= ma[5]; jj++) val += a2[jj] * p1[ii + jj];
+                val = c * val;
+            } else {
+                val = c * p1[ii];
+            }
+            for (int jj = ar[4]; jj <= ar[5]; jj++) val -= b2[jj] * p2[ii + jj];
+            p2[ii] = val;
+        }
+    }
+}
+
+/* dip::GaussIIR (gaussiir.cpp:741-806) through Framework::Separable with a dfloat
+ * buffer, for images with normal strides.  filter_orders: NULL/0 elements = the
+ * default (3 + order, capped at 5).  method: 1 forward-backward, 2 discrete time fit. */
+int orc_gauss_iir(const void *in, int in_dt, void *out, int out_dt, int ndim, const int64_t *sizes,
+                  const double *sigmas, int nsig, const int *orders, int nord, const int *bc, int nbc,
+                  const int *filter_orders, int nfo, int method, double truncation, int nthreads) {
+    if (ndim < 1 || ndim > ORC_MAX_DIMS) return -1;
+    if ((nsig != 0 && nsig != 1 && nsig != ndim) || (nord != 0 && nord != 1 && nord != ndim) ||
+        (nbc != 0 && nbc != 1 && nbc != ndim) || (nfo != 0 && nfo != 1 && nfo != ndim))
+        return -1;
+    orc_iir P[ORC_MAX_DIMS];
+    int proc[ORC_MAX_DIMS], bcs[ORC_MAX_DIMS];
+    long npx = 1;
+    for (int d = 0; d < ndim; d++) npx *= (long)sizes[d];
+    for (int d = 0; d < ndim; d++) {
+        double s = nsig == 0 ? 1.0 : (nsig == 1 ? sigmas[0] : sigmas[d]);
+        int o = nord == 0 ? 0 : (nord == 1 ? orders[0] : orders[d]);
+        int fo = nfo == 0 ? (o > 2 ? 5 : o + 3) : (nfo == 1 ? filter_orders[0] : filter_orders[d]);
+        bcs[d] = nbc == 0 ? BC_MIRROR : (nbc == 1 ? bc[0] : bc[d]);
+        proc[d] = s > 0.0 && sizes[d] > 1;
+        if (proc[d] && iir_params(s, o, fo, method, truncation, &P[d]) != 0) return -1;
+    }
+    int order[ORC_MAX_DIMS], nord2 = 0;
+    for (int d = 0; d < ndim; d++)
+        if (proc[d]) order[nord2++] = d;
+    if (nord2 == 0) {
+        for (long i = 0; i < npx; i++) store_sample(out, out_dt, i, load_sample(in, in_dt, i), in_dt == DT_SF);
+        return 0;
+    }
+    int use_interm = out_dt != DT_DF;
+    double *interm = use_interm ? (double *)malloc((size_t)npx * sizeof(double)) : NULL;
+    if (use_interm && !interm) return -2;
+    long strides[ORC_MAX_DIMS];
+    strides[0] = 1;
+    for (int d = 1; d < ndim; d++) strides[d] = strides[d - 1] * (long)sizes[d - 1];
+    const void *src = in;
+    int src_dt = in_dt, status = 0;
+    for (int rep = 0; rep < nord2; rep++) {
+        int pd = order[rep], last = rep == nord2 - 1;
+        void *dst = last ? out : (use_interm ? (void *)interm : out);
+        int dst_dt = last ? out_dt : DT_DF;
+        long len = (long)sizes[pd], st = strides[pd], nlines = npx / len, border = P[pd].border;
+        long L = len + 2 * border;
+        int nt = nthreads > 0 ? nthreads : 1;
+#pragma omp parallel num_threads(nt)
+        {
+            double *b0 = (double *)malloc((size_t)L * sizeof(double));
+            double *b1 = (double *)malloc((size_t)L * sizeof(double));
+            double *b2 = (double *)malloc((size_t)L * sizeof(double));
+#pragma omp for schedule(static)
+            for (long line = 0; line < nlines; line++) {
+                long rem = line, base = 0;
+                for (int d = 0; d < ndim; d++) {
+                    if (d == pd) continue;
+                    long cc = rem % (long)sizes[d];
+                    rem /= (long)sizes[d];
+                    base += cc * strides[d];
+                }
+                double *x = b0 + border;
+                for (long i = 0; i < len; i++) x[i] = load_sample(src, src_dt, base + i * st);
+                if (expand_d(x, len, border, border, bcs[pd]) != 0) status = -3;
+                iir_line(&P[pd], b0, b1, b2, L);
+                for (long i = 0; i < len; i++) store_sample(dst, dst_dt, base + i * st, b2[border + i], 0);
+            }
+            free(b0);
+            free(b1);
+            free(b2);
+        }
+        src = dst;
+        src_dt = dst_dt;
+    }
+    free(interm);
+    return status;
+}

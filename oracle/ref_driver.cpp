@@ -76,6 +76,35 @@ int dipref_gauss( void const* in, int in_dtype, void* out, int out_dtype, int nd
    }
 }
 
+// dip::GaussIIR( in, out, sigmas, orders, bc, filterOrder, designMethod, truncation ):
+// as dipref_gauss; `filter_orders` holds `nfo` values (0 = the default per order).
+int dipref_gauss_iir( void const* in, int in_dtype, void* out, int out_dtype, int ndim, int64_t const* sizes,
+                      double const* sigmas, int nsigmas, int const* orders, int norders,
+                      char const* const* bcs, int nbc, int const* filter_orders, int nfo,
+                      char const* method, double truncation, int nthreads ) {
+   try {
+      dip::Image imgIn = Wrap( in, in_dtype, ndim, sizes );
+      dip::Image imgOut = Wrap( out, out_dtype, ndim, sizes );
+      imgOut.Protect();
+      dip::FloatArray s( static_cast< dip::uint >( nsigmas ));
+      for( int ii = 0; ii < nsigmas; ++ii ) { s[ static_cast< dip::uint >( ii ) ] = sigmas[ ii ]; }
+      dip::UnsignedArray o( static_cast< dip::uint >( norders ));
+      for( int ii = 0; ii < norders; ++ii ) { o[ static_cast< dip::uint >( ii ) ] = static_cast< dip::uint >( orders[ ii ] ); }
+      dip::UnsignedArray fo( static_cast< dip::uint >( nfo ));
+      for( int ii = 0; ii < nfo; ++ii ) { fo[ static_cast< dip::uint >( ii ) ] = static_cast< dip::uint >( filter_orders[ ii ] ); }
+      dip::StringArray bc;
+      for( int ii = 0; ii < nbc; ++ii ) { bc.emplace_back( bcs[ ii ] ); }
+      if( nthreads > 0 ) {
+         dip::SetNumberOfThreads( static_cast< dip::uint >( nthreads ));
+      }
+      dip::GaussIIR( imgIn, imgOut, s, o, bc, fo, method, truncation );
+      return 0;
+   } catch( std::exception const& e ) {
+      g_err = e.what();
+      return 1;
+   }
+}
+
 // In-place form: `buf` is both the input and the output (same dtype, which must be
 // sfloat or dfloat).
 int dipref_gauss_inplace( void* buf, int dtype, int ndim, int64_t const* sizes, double const* sigmas, int nsigmas,

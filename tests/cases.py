@@ -152,3 +152,35 @@ for bc in ("mirror", "periodic"):
                         [bc], None))
 LARGE_CASES.append(("f64_256x256x128_s221_mirror", (128, 256, 256), "float64", 4, 0.01, [2.0, 2.0, 1.0], [0],
                     ["mirror"], None))
+
+# dip::GaussIIR: (name, numpy shape, dtype, sigmas (dip order), orders, boundary list,
+# filter orders (() = default), design method, truncation, out dtype or None)
+DTF, FB = "discrete time fit", "forward backward"
+IIR_CASES = [
+    ("iir_f32_64x48_s12", (48, 64), "float32", [12.0], [0, 0], ["mirror"], (), DTF, 3.0, None),
+    ("iir_u8_97x61_s11_o10_periodic", (61, 97), "uint8", [11.0], [1, 0], ["periodic"], (), DTF, 3.0, None),
+    ("iir_f64_80x40_s15_o02", (40, 80), "float64", [15.0], [0, 2], ["mirror"], (), DTF, 3.0, None),
+    ("iir_f32_37x29x11_s11", (11, 29, 37), "float32", [11.0, 11.0, 12.0], [0], ["mirror"], (), DTF, 3.0, None),
+    ("iir_f32_50x40_o30", (40, 50), "float32", [4.0], [3, 0], ["mirror"], (), DTF, 3.0, None),
+    ("iir_f32_50x40_o04", (40, 50), "float32", [4.0], [0, 4], ["mirror"], (), DTF, 3.0, None),
+    ("iir_f32_50x40_s1", (40, 50), "float32", [1.0], [0, 1], ["mirror"], (), DTF, 3.0, None),
+    ("iir_u8_50x40_u8out", (40, 50), "uint8", [3.0], [0, 0], ["mirror"], (), DTF, 3.0, "uint8"),
+    ("iir_s16_45x33", (33, 45), "int16", [2.5], [0, 0], ["mirror"], (), DTF, 3.0, None),
+    ("iir_f32_1x40", (40, 1), "float32", [12.0], [0], ["mirror"], (), DTF, 3.0, None),
+    ("iir_f32_40x30_s0x5", (30, 40), "float32", [0.0, 5.0], [0, 1], ["mirror"], (), DTF, 3.0, None),
+    ("iir_f32_2x3_tiny", (3, 2), "float32", [2.0], [0, 1], ["mirror"], (), DTF, 3.0, None),
+    ("iir_f32_60x50_t2", (50, 60), "float32", [6.0], [1, 1], ["mirror"], (), DTF, 2.0, None),
+    ("iir_f32_60x50_t5", (50, 60), "float32", [6.0], [2, 0], ["mirror"], (), DTF, 5.0, None),
+]
+for o in range(5):
+    for fo in (1, 2, 3, 4, 5, 6):
+        IIR_CASES.append((f"iir_dtf_o{o}_fo{fo}", (20, 40), "float32", [5.0, 3.0], [o, 0], ["mirror"], (fo,), DTF,
+                          3.0, None))
+    for fo in (3, 4, 5, 2):
+        IIR_CASES.append((f"iir_fb_o{o}_fo{fo}", (20, 40), "float64", [5.0, 3.0], [0, o], ["mirror"], (fo,), FB,
+                          3.0, None))
+for bc in ("asym mirror", "periodic", "asym periodic", "add zeros", "zero order", "antisym reflect",
+           "first order", "second order", "third order"):
+    t = bc.replace(" ", "_")
+    IIR_CASES.append((f"iir_f32_40x20_{t}", (20, 40), "float32", [6.0, 4.0], [1, 2], [bc], (), DTF, 3.0, None))
+    IIR_CASES.append((f"iir_line_3_{t}", (1, 3), "float64", [2.0], [0, 0], [bc], (), DTF, 3.0, None))

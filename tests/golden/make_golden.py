@@ -1,7 +1,7 @@
 """Writes the golden fixtures under tests/golden/ from the REFERENCE library itself
 (oracle/_ref/libdipref.so, built from the reference sources by oracle/ref.mk).
 
-    make -f oracle/ref.mk -j8 && python tests/golden/make_golden.py [--only extrap|deriv]
+    make -f oracle/ref.mk -j8 && python tests/golden/make_golden.py [--only extrap|deriv|iir]
 
 Outputs (all small):
   extrap_cases.npz   input and dip::Gauss output of tests/cases.py:EXTRAP_CASES (the
@@ -11,6 +11,7 @@ Outputs (all small):
                      Hessian, Laplace with the FIR method)
   deriv2_cases.npz   the same for tests/cases.py:DERIV2_CASES (dip::Dgg, LaplacePlusDgg,
                      LaplaceMinusDgg, StructureTensor)
+  iir_cases.npz      input and dip::GaussIIR output of tests/cases.py:IIR_CASES
   taps.npz           MakeHalfGaussian(sigma, order, 3, sfloat/dfloat) for TAP_SIGMAS x orders 0..3
   small_cases.npz    input and dip::Gauss output of every case in tests/cases.py:SMALL_CASES
   large_cases.json   SHA-256 of the exact output bytes + sampled pixels for LARGE_CASES
@@ -55,11 +56,23 @@ def write_derivative_cases(case_list=None, fname="deriv_cases.npz", seed0=4000):
     np.savez_compressed(os.path.join(HERE, fname), **arrs)
 
 
+def write_iir_cases(fname="iir_cases.npz", seed0=6000):
+    arrs = {}
+    for i, (name, shape, dt, sig, ords, bc, fo, method, trunc, odt) in enumerate(cases.IIR_CASES):
+        x = cases.make_input(shape, dt, seed=seed0 + i)
+        arrs[name + "__in"] = x
+        arrs[name + "__out"] = oc.gauss_iir(x, sig, ords, bc, fo, method, trunc, out_dtype=odt, nthreads=1,
+                                            use_ref=True)
+    np.savez_compressed(os.path.join(HERE, fname), **arrs)
+
+
 def main():
     assert oc.ref_available(), "build oracle/_ref/libdipref.so first (make -f oracle/ref.mk)"
     only = sys.argv[sys.argv.index("--only") + 1] if "--only" in sys.argv else None
     if only in (None, "extrap"):
         write_cases(cases.EXTRAP_CASES, "extrap_cases.npz", 3000)
+    if only in (None, "iir"):
+        write_iir_cases()
     if only in (None, "deriv"):
         write_derivative_cases()
         write_derivative_cases(cases.DERIV2_CASES, "deriv2_cases.npz", 5000)

@@ -36,6 +36,7 @@ def orc():
         lib.orc_separable.argtypes = [P, I, P, I, I, P, P, P, P, P, I, P, I, P, I]
         lib.orc_bc_map.argtypes = [ctypes.c_long, ctypes.c_long, I, ctypes.POINTER(I)]
         lib.orc_bc_map.restype = ctypes.c_long
+        lib.orc_gauss_iir.argtypes = [P, I, P, I, I, P, P, I, P, I, P, I, P, I, I, D, I]
         _orc = lib
     return _orc
 
@@ -54,6 +55,8 @@ def ref():
         lib.dipref_half_gaussian.argtypes = [D, I, D, I, P, I]
         lib.dipref_separable.argtypes = [P, I, P, I, I, P, P, P, P, I, P, I, P]
         lib.dipref_last_error.restype = ctypes.c_char_p
+        if hasattr(lib, "dipref_gauss_iir"):
+            lib.dipref_gauss_iir.argtypes = [P, I, P, I, I, P, P, I, P, I, P, I, P, I, ctypes.c_char_p, D, I]
         if hasattr(lib, "dipref_derivative"):
             lib.dipref_derivative.argtypes = [ctypes.c_char_p, P, I, P, I, I, ctypes.POINTER(I), I, P, P, I, P, I,
                                               ctypes.c_char_p, D]
@@ -122,6 +125,41 @@ def gauss_fir(x, sigmas, orders=(0,), bc=("mirror",), truncation=3.0, out_dtype=
                                  truncation, nthreads)
         if st != 0:
             raise RuntimeError(f"oracle gauss_fir failed ({st})")
+    return out
+
+
+IIR_METHODS = {"forward backward": 1, "discrete time fit": 2}
+
+
+def gauss_iir(x, sigmas, orders=(0,), bc=("mirror",), filter_orders=(), method="discrete time fit",
+              truncation=3.0, out_dtype=None, nthreads=1, use_ref=False):
+    """dip::GaussIIR(x, sigmas, orders, bc, filter_orders, method, truncation) on a
+    C-contiguous numpy array (arguments in dip axis order)."""
+    x = np.ascontiguousarray(x)
+    out_dtype = np.dtype(out_dtype) if out_dtype is not None else _flex(x.dtype)
+    out = np.zeros(x.shape, out_dtype)
+    sizes = _sizes(x)
+    sig = np.ascontiguousarray(list(sigmas) if hasattr(sigmas, "__len__") else [sigmas], np.float64)
+    ords = np.ascontiguousarray(list(orders) if hasattr(orders, "__len__") else [orders], np.int32)
+    fo = np.ascontiguousarray(list(filter_orders) if hasattr(filter_orders, "__len__") else [filter_orders],
+                              np.int32)
+    bcl = list(bc) if not isinstance(bc, str) else [bc]
+    if use_ref:
+        arr = (ctypes.c_char_p * max(1, len(bcl)))(*[b.encode() for b in bcl])
+        fop = _ptr(fo) if len(fo) else None
+        st = ref().dipref_gauss_iir(_ptr(x), DT[x.dtype], _ptr(out), DT[out_dtype], x.ndim, _ptr(sizes),
+                                    _ptr(sig), len(sig), _ptr(ords), len(ords), arr, len(bcl), fop, len(fo),
+                                    method.encode(), truncation, nthreads)
+        if st != 0:
+            raise RuntimeError(ref().dipref_last_error().decode())
+    else:
+        bci = np.ascontiguousarray([BC[b] for b in bcl] or [0], np.int32)
+        fop = _ptr(fo) if len(fo) else None
+        st = orc().orc_gauss_iir(_ptr(x), DT[x.dtype], _ptr(out), DT[out_dtype], x.ndim, _ptr(sizes),
+                                 _ptr(sig), len(sig), _ptr(ords), len(ords), _ptr(bci), len(bcl), fop, len(fo),
+                                 IIR_METHODS[method], truncation, nthreads)
+        if st != 0:
+            raise RuntimeError(f"oracle gauss_iir failed ({st})")
     return out
 
 

@@ -323,3 +323,86 @@ def test_dgg_and_structure_tensor_identities():
     assert np.allclose(st[0][inner2], a * a, atol=1e-6)
     assert np.allclose(st[1][inner2], b * b, atol=1e-6)
     assert np.allclose(st[2][inner2], a * b, atol=1e-6)
+
+
+# ---------------------------------------------------------------------------
+# dip::GaussIIR
+# ---------------------------------------------------------------------------
+IIR = np.load(os.path.join(HERE, "golden", "iir_cases.npz"))
+
+
+@pytest.mark.parametrize("case", cases.IIR_CASES, ids=[c[0] for c in cases.IIR_CASES])
+def test_oracle_iir_matches_reference(case):
+    """The oracle's GaussIIR against the reference's own outputs, bit for bit
+    (tests/golden/iir_cases.npz: every pole count and design method, derivative orders
+    0-4, every boundary condition, lines down to 2 samples)."""
+    name, shape, dt, sig, ords, bc, fo, method, trunc, odt = case
+    x = IIR[name + "__in"]
+    y = oc.gauss_iir(x, sig, ords, bc, fo, method, trunc, out_dtype=odt)
+    assert np.array_equal(y, IIR[name + "__out"])
+
+
+def _iir1(x, sigma, order, fo, method):
+    return oc.gauss_iir(x, [sigma], [order], ["mirror"], [fo], method)
+
+
+def test_iir_gaussian_properties():
+    """What the mathematics fixes, as the reference's own test checks it
+    (gaussiir.cpp:819-876): the impulse response of the smoothing filter peaks at
+    1/(sqrt(2 pi) sigma) and sums to 1, the filter orders agree within their
+    accuracy, and the derivative filters give 1, 2 and 6 on x, x^2 and x^3."""
+    sigma = 5.0
+    imp = np.zeros(256)
+    imp[128] = 1.0
+    r1 = _iir1(imp, sigma, 0, 5, "discrete time fit")
+    assert abs(r1[128] - 1.0 / (np.sqrt(2 * np.pi) * sigma)) < 0.00015
+    assert r1.sum() == pytest.approx(1.0, rel=1e-5)
+    for fo, method, tol in ((4, "discrete time fit", 0.0003), (3, "discrete time fit", 0.002),
+                            (2, "discrete time fit", 0.004), (5, "forward backward", 0.003),
+                            (4, "forward backward", 0.002), (3, "forward backward", 0.009)):
+        r2 = _iir1(imp, sigma, 0, fo, method)
+        assert np.max(np.abs(r1 - r2)) <= tol, (fo, method)
+    x = np.arange(-128.0, 128.0)
+    for fo, method in ((5, "discrete time fit"), (4, "discrete time fit"), (3, "discrete time fit"),
+                       (5, "forward backward"), (4, "forward backward"), (3, "forward backward")):
+        assert _iir1(x, sigma, 1, fo, method)[128] == pytest.approx(1.0, rel=1e-5)
+        assert _iir1(x * x, sigma, 2, fo, method)[128] == pytest.approx(2.0, rel=1e-5)
+    for fo, method in ((5, "discrete time fit"), (4, "discrete time fit"), (5, "forward backward"),
+                       (4, "forward backward")):
+        assert _iir1(x * x * x, sigma, 3, fo, method)[128] == pytest.approx(6.0, rel=1e-5)
+
+
+def test_iir_separable_and_linear():
+    """Size-independent properties: a 2-D result equals the two 1-D passes applied in
+    turn (x first, in double), and the filter is linear up to rounding."""
+    rng = np.random.default_rng(11)
+    x = rng.random((40, 60))
+    y = oc.gauss_iir(x, [6.0, 4.0], [1, 0], ["mirror"])
+    tx = oc.gauss_iir(x, [6.0, 0.0], [1, 0], ["mirror"])
+    ty = oc.gauss_iir(tx, [0.0, 4.0], [0, 0], ["mirror"])
+    assert np.array_equal(y, ty)
+    a = oc.gauss_iir(x, [7.0], [0], ["periodic"])
+    b = oc.gauss_iir(2.0 * x, [7.0], [0], ["periodic"])
+    assert np.array_equal(2.0 * a, b)  # scaling by 2 is exact in every step
+
+
+@pytest.mark.skipif(not oc.ref_available(), reason="reference library not built")
+def test_oracle_iir_random_cases_against_reference():
+    """Seeded random sweep over shapes, types, sigmas, orders, filter orders, design
+    methods and boundary conditions, oracle against the reference library."""
+    rng = np.random.default_rng(2024)
+    bcs = ["mirror", "asym mirror", "periodic", "asym periodic", "add zeros", "zero order", "antisym reflect",
+           "first order", "second order", "third order"]
+    for it in range(60):
+        nd = int(rng.integers(1, 4))
+        shape = tuple(int(v) for v in rng.integers(2, 40 if nd < 3 else 12, size=nd))
+        dt = [np.uint8, np.float32, np.float64][int(rng.integers(0, 3))]
+        x = cases.make_input(shape, dt, seed=7000 + it)
+        sig = [float(v) for v in rng.uniform(0.5, 15.0, size=nd)]
+        ords = [int(v) for v in rng.integers(0, 5, size=nd)]
+        fo = [] if rng.random() < 0.5 else [int(rng.integers(1, 6))]
+        method = "discrete time fit" if rng.random() < 0.7 else "forward backward"
+        bc = [bcs[int(rng.integers(0, len(bcs)))]]
+        r = oc.gauss_iir(x, sig, ords, bc, fo, method, use_ref=True)
+        q = oc.gauss_iir(x, sig, ords, bc, fo, method)
+        assert np.array_equal(r, q), (shape, dt, sig, ords, fo, method, bc)
