@@ -243,6 +243,7 @@ def main():
         extra.update(bench_batched(dip, torch, np, dev, L))
         extra.update(bench_backward(dip, torch, dev))
         extra.update(bench_derivatives(dip, torch, x))
+        extra.update(bench_iir(dip, torch, np, x, x_np, not a.no_cpu))
         extra.update(bench_fp32_vs_fp64(dip, torch, np, cases, dev))
     del outs
     cpu = None
@@ -470,6 +471,51 @@ def bench_derivatives(dip, torch, x):
         log(f"derivative family: {res}")
     except Exception as e:  # report, do not hide
         out["derivative_family_8192sq_u8"] = {"error": repr(e)}
+    return out
+
+
+def bench_iir(dip, torch, np, x, x_np, with_cpu):
+    """dip::GaussIIR (what "best" picks for sigma > 10) on the 8192^2 uint8 input: time
+    per call on the GPU; with the CPU leg, the reference's own GaussIIR on the same
+    image with OpenMP on this host's cores, and the two outputs compared bit for bit."""
+    out = {}
+    try:
+        res = {}
+        n = x.numel()
+        y = torch.empty(x.shape, dtype=torch.float32, device=x.device)
+        for s in (16.0, 64.0):
+            for o in ((0, 0), (1, 0), (0, 2)):
+                dip.GaussIIR(x, [s], list(o), ["mirror"], out=y)
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(3):
+                    dip.GaussIIR(x, [s], list(o), ["mirror"], out=y)
+                e1.record()
+                torch.cuda.synchronize()
+                ms = e0.elapsed_time(e1) / 3
+                res[f"s{int(s)}_o{o[0]}{o[1]}"] = {"ms": round(ms, 4), "GPix_s": round(n / (ms * 1e-3) / 1e9, 2)}
+        out["gauss_iir_8192sq_u8"] = res
+        log(f"GaussIIR: {res}")
+        if with_cpu:
+            import oracle_ctypes as oc
+            if oc.ref_available():
+                try:
+                    cores = len(os.sched_getaffinity(0))
+                except AttributeError:
+                    cores = os.cpu_count() or 1
+                dip.GaussIIR(x, [16.0], [0, 0], ["mirror"], out=y)
+                gpu = y.cpu().numpy()
+                t0 = time.perf_counter()
+                ref = oc.gauss_iir(x_np, [16.0], [0, 0], ["mirror"], nthreads=cores, use_ref=True)
+                t = time.perf_counter() - t0
+                out["gauss_iir_cpu_reference"] = {
+                    "value": round(n / t / 1e9, 4), "unit": "GPix/s", "cores": cores, "kind": "reference",
+                    "sample": "one dip::GaussIIR call, sigma 16, order (0,0), on the full 8192x8192 uint8 image",
+                    "bitexact_vs_gpu": bool(np.array_equal(gpu, ref))}
+                log(f"GaussIIR CPU reference: {out['gauss_iir_cpu_reference']}")
+    except Exception as e:  # report, do not hide
+        out["gauss_iir_8192sq_u8"] = {"error": repr(e)}
     return out
 
 

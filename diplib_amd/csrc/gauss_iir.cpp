@@ -5,6 +5,7 @@
 // through a double intermediate image (framework_separable.cpp:190-206), and the last
 // pass casts its result to the output type.
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <limits>
 #include <vector>
@@ -120,11 +121,28 @@ int run_iir_pass(const View &src, const View &dst, int ndim, int d, const IirFil
     const int64_t px = npx(src, ndim);
     TraceScope ts("iir", px * static_cast<int64_t>(dt_size(src.dt) + dt_size(dst.dt)), stream,
                   a.nlines * a.L * static_cast<int64_t>(2 * (2 * f.nn + 3)));
+    // vector loads and stores along a contiguous filter axis when every line starts
+    // aligned: V input samples of dt_size bytes, W output samples per access
+    const int64_t vin = src.dt == DIPAMD_DT_DFLOAT ? 2 : 4;
+    const int64_t wout = dst.dt == DIPAMD_DT_DFLOAT ? 2 : 4;
+    bool vec = src.strides[d] == 1 && dst.strides[d] == 1 &&
+               reinterpret_cast<uintptr_t>(src.p) % static_cast<uintptr_t>(vin * dt_size(src.dt)) == 0 &&
+               reinterpret_cast<uintptr_t>(dst.p) % static_cast<uintptr_t>(wout * dt_size(dst.dt)) == 0;
+    for (int k = 0; k < ndim && vec; ++k) {
+        if (k == d || src.sizes[k] <= 1) continue;
+        if (src.strides[k] % vin != 0 || dst.strides[k] % wout != 0) vec = false;
+    }
     hipError_t e;
     switch (src.dt) {
-        case DIPAMD_DT_UINT8: e = launch_iir_u8(f.nn, order, a, stream); break;
-        case DIPAMD_DT_SFLOAT: e = launch_iir_f32(f.nn, order, a, stream); break;
-        case DIPAMD_DT_DFLOAT: e = launch_iir_f64(f.nn, order, a, stream); break;
+        case DIPAMD_DT_UINT8:
+            e = vec ? launch_iir_u8_vec(f.nn, order, a, stream) : launch_iir_u8(f.nn, order, a, stream);
+            break;
+        case DIPAMD_DT_SFLOAT:
+            e = vec ? launch_iir_f32_vec(f.nn, order, a, stream) : launch_iir_f32(f.nn, order, a, stream);
+            break;
+        case DIPAMD_DT_DFLOAT:
+            e = vec ? launch_iir_f64_vec(f.nn, order, a, stream) : launch_iir_f64(f.nn, order, a, stream);
+            break;
         default: return set_error(DIPAMD_E_PARAMETER, "Data type not supported");
     }
     if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
@@ -180,7 +198,7 @@ int gauss_iir_device(const View &in, const View &out, int ndim, const IirFilter 
     // longest pass
     size_t p1_bytes = 0;
     for (int d : order) {
-        const int64_t kept = in.sizes[d] + static_cast<int64_t>(filters[d].border);
+        const int64_t kept = (in.sizes[d] + static_cast<int64_t>(filters[d].border) + 1) / 2 * 2;  // in pairs
         p1_bytes = std::max(p1_bytes, static_cast<size_t>(px / in.sizes[d] * kept) * sizeof(double));
     }
     void *p1 = nullptr;

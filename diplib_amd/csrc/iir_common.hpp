@@ -25,8 +25,8 @@ struct IirFilter {
 int design_gauss_iir(double sigma, int order, int filter_order, int method, double truncation, IirFilter *out);
 
 // Kernel arguments of one pass: one thread per line; the causal result of each line
-// goes to a double scratch laid out [position][line], so that both recursions read
-// and write it coalesced whatever the filter axis.
+// goes to a double scratch laid out [position pair][line][2], so that both recursions
+// read and write it coalesced, 16 bytes per access, whatever the filter axis.
 struct IirArgs {
     const void *in;
     void *out;
@@ -55,9 +55,13 @@ struct IirArgs {
     int32_t pad_;
 };
 
-// Launch one pass (k_iir_u8.hip, k_iir_f32.hip, k_iir_f64.hip: by input type).
+// Launch one pass (k_iir_<type>.hip; the _vec forms need a contiguous filter axis,
+// stride 1 in input and output, with every line aligned to the vector widths).
 hipError_t launch_iir_u8(int nn, int order, const IirArgs &a, hipStream_t st);
 hipError_t launch_iir_f32(int nn, int order, const IirArgs &a, hipStream_t st);
 hipError_t launch_iir_f64(int nn, int order, const IirArgs &a, hipStream_t st);
+hipError_t launch_iir_u8_vec(int nn, int order, const IirArgs &a, hipStream_t st);
+hipError_t launch_iir_f32_vec(int nn, int order, const IirArgs &a, hipStream_t st);
+hipError_t launch_iir_f64_vec(int nn, int order, const IirArgs &a, hipStream_t st);
 
 }  // namespace dipamd

@@ -8,9 +8,12 @@
 // the interior positions to the destination.
 //
 // Throughput: each step is a chain of dependent double operations, so a pass costs
-// (n + 2 border) steps of that chain per line; lines run in parallel.  Loads of the
-// next P samples are issued before the current P steps, so they are in flight
-// while the chain runs.
+// about 2 (n + border) steps of that chain per line; lines run in parallel.  With
+// few lines (one wave per SIMD or less) the chain latency and the memory latency
+// bound a pass, so memory operations per step are kept low: the next P samples are
+// loaded before the current P steps run; the causal scratch is written and read in
+// pairs (16-byte accesses); on a contiguous filter axis with aligned lines, samples
+// are loaded and stored V at a time.
 #pragma once
 
 #include <type_traits>
@@ -54,13 +57,95 @@ struct IirMa {
     static constexpr int BM5 = ORDER == 0 ? 0 : (ORDER <= 2 ? 1 : 2);
 };
 
-template <typename TI, typename TO, int NN, int ORDER>
+// Vector access to V consecutive samples of a line (V = 1: one sample).  Loaded words
+// stay one per 32-bit register: a byte array would be packed, which forces the loads
+// to complete where they are issued.
+template <typename T, int V>
+struct IirVec;
+template <>
+struct IirVec<uint8_t, 1> {
+    using W = uint32_t;
+    static __device__ __forceinline__ W load(const uint8_t *p) { return *p; }
+    template <int K>
+    static __device__ __forceinline__ double get(W w) { return static_cast<double>(w); }
+};
+template <>
+struct IirVec<uint8_t, 4> {
+    using W = uint32_t;
+    static __device__ __forceinline__ W load(const uint8_t *p) { return *reinterpret_cast<const uint32_t *>(p); }
+    template <int K>
+    static __device__ __forceinline__ double get(W w) { return static_cast<double>((w >> (8 * K)) & 0xffu); }
+};
+template <>
+struct IirVec<float, 1> {
+    using W = float;
+    static __device__ __forceinline__ W load(const float *p) { return *p; }
+    template <int K>
+    static __device__ __forceinline__ double get(W w) { return static_cast<double>(w); }
+};
+template <>
+struct IirVec<float, 4> {
+    using W = float4;
+    static __device__ __forceinline__ W load(const float *p) { return *reinterpret_cast<const float4 *>(p); }
+    template <int K>
+    static __device__ __forceinline__ double get(W w) {
+        return static_cast<double>(K == 0 ? w.x : (K == 1 ? w.y : (K == 2 ? w.z : w.w)));
+    }
+};
+template <>
+struct IirVec<double, 1> {
+    using W = double;
+    static __device__ __forceinline__ W load(const double *p) { return *p; }
+    template <int K>
+    static __device__ __forceinline__ double get(W w) { return w; }
+};
+template <>
+struct IirVec<double, 2> {
+    using W = double2;
+    static __device__ __forceinline__ W load(const double *p) { return *reinterpret_cast<const double2 *>(p); }
+    template <int K>
+    static __device__ __forceinline__ double get(W w) { return K == 0 ? w.x : w.y; }
+};
+
+// Input vector width on a contiguous axis, and the output store width.
+template <typename TI>
+struct IirVin {
+    static constexpr int V = sizeof(TI) == 8 ? 2 : 4;
+};
+template <typename TO, int V>
+struct IirStore {
+    static constexpr int W = V == 1 ? 1 : (sizeof(TO) == 8 ? 2 : 4);
+    static __device__ __forceinline__ void store(TO *p, const double *v) {
+        if (W == 1) {
+            *p = static_cast<TO>(v[0]);
+        } else if (sizeof(TO) == 8) {
+            *reinterpret_cast<double2 *>(p) = make_double2(v[0], v[1]);
+        } else {
+            *reinterpret_cast<float4 *>(p) = make_float4(static_cast<float>(v[0]), static_cast<float>(v[1]),
+                                                         static_cast<float>(v[2]), static_cast<float>(v[3]));
+        }
+    }
+};
+
+template <int K, int P, typename F>
+__device__ __forceinline__ void iir_unroll(F &&f) {
+    if constexpr (K < P) {
+        f(std::integral_constant<int, K>());
+        iir_unroll<K + 1, P>(f);
+    }
+}
+
+// VEC: the filter axis is contiguous (stride 1) in input and output and every line
+// starts aligned to the vector widths.
+template <typename TI, typename TO, int NN, int ORDER, bool VEC>
 __global__ __launch_bounds__(kIirBlock) void k_iir(const IirArgs a) {
     using MA = IirMa<ORDER>;
-    // loaded samples stay one per 32-bit register (a byte array would be packed, which
-    // forces the loads to complete where they are issued)
-    using TL = typename std::conditional<(sizeof(TI) < 4), uint32_t, TI>::type;
+    constexpr int V = VEC ? IirVin<TI>::V : 1;
+    using IV = IirVec<TI, V>;
+    using WI = typename IV::W;
+    using OS = IirStore<TO, V>;
     constexpr int P = kIirP;
+    constexpr int NV = P / V;
     const int64_t line = static_cast<int64_t>(blockIdx.x) * kIirBlock + threadIdx.x;
     if (line >= a.nlines) return;
     const int64_t lane = line % a.e;
@@ -70,10 +155,14 @@ __global__ __launch_bounds__(kIirBlock) void k_iir(const IirArgs a) {
     TO *dst = static_cast<TO *>(a.out) + oout + lane * a.eout;
     const int64_t ps = a.nlines;
     const int n = static_cast<int>(a.n), B = a.border, L = static_cast<int>(a.L);
-    // the causal result is kept for positions B..L-1 only: the anti-causal recursion
-    // stops at the first interior sample, the positions before it feed nothing
-    double *p1 = a.p1 + line - static_cast<int64_t>(B) * ps;
-    const int64_t sin = a.sin, sout = a.sout;
+    // Causal result for positions B..L-1 only (the anti-causal recursion stops at the
+    // first interior sample; the positions before it feed nothing), q = position - B,
+    // stored in pairs: element q of this line at pq[(q >> 1) * 2 ps + (q & 1)].
+    double *pq = a.p1 + 2 * line;
+    const int64_t pstride = 2 * ps;
+    auto p1_at = [&](int q) -> double & { return pq[static_cast<int64_t>(q >> 1) * pstride + (q & 1)]; };
+    auto p1_pair = [&](int q) { return reinterpret_cast<double2 *>(pq + static_cast<int64_t>(q >> 1) * pstride); };
+    const int64_t sin = VEC ? 1 : a.sin, sout = VEC ? 1 : a.sout;
     const double S = a.S;
     double b[NN + 1];
 #pragma unroll
@@ -126,39 +215,43 @@ __global__ __launch_bounds__(kIirBlock) void k_iir(const IirArgs a) {
         return v;
     };
     for (int ii = a.fnset; ii < B; ++ii) fstep(ext(ii));
-    // interior: image samples 0..n-1 in blocks of P; the next block's loads are issued
-    // (unconditionally, clamped to the line) before this block's steps
+    const int nfull = n / P;
+    // interior: blocks of P samples from the first; the next block's loads (clamped to
+    // the line) are issued before this block's steps
     {
-        const int nfull = n / P;
-        TL cur[P], nxt[P];
+        WI cur[NV], nxt[NV];
         if (nfull > 0) {
 #pragma unroll
-            for (int p = 0; p < P; ++p) cur[p] = src[static_cast<int64_t>(p) * sin];
+            for (int v = 0; v < NV; ++v) cur[v] = IV::load(src + static_cast<int64_t>(v * V) * sin);
         }
         for (int blk = 0; blk < nfull; ++blk) {
             const int j0 = blk * P;
+            const int jn = blk + 1 < nfull ? j0 + P : j0;
 #pragma unroll
-            for (int p = 0; p < P; ++p) nxt[p] = src[static_cast<int64_t>(min(j0 + P + p, n - 1)) * sin];
+            for (int v = 0; v < NV; ++v) nxt[v] = IV::load(src + static_cast<int64_t>(jn + v * V) * sin);
             __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the steps
+            double y[P];
+            iir_unroll<0, P>([&](auto kc) {
+                constexpr int p = decltype(kc)::value;
+                y[p] = fstep(IV::template get<p % V>(cur[p / V]));
+            });
 #pragma unroll
-            for (int p = 0; p < P; ++p)
-                p1[static_cast<int64_t>(B + j0 + p) * ps] = fstep(to_buf<double>(static_cast<TI>(cur[p])));
+            for (int p = 0; p < P; p += 2) *p1_pair(j0 + p) = make_double2(y[p], y[p + 1]);
 #pragma unroll
-            for (int p = 0; p < P; ++p) cur[p] = nxt[p];
+            for (int v = 0; v < NV; ++v) cur[v] = nxt[v];
         }
-        for (int j = nfull * P; j < n; ++j)
-            p1[static_cast<int64_t>(B + j) * ps] = fstep(to_buf<double>(src[static_cast<int64_t>(j) * sin]));
+        for (int j = nfull * P; j < n; ++j) p1_at(j) = fstep(to_buf<double>(src[static_cast<int64_t>(j) * sin]));
     }
-    for (int ii = B + n; ii < L; ++ii) p1[static_cast<int64_t>(ii) * ps] = fstep(ext(ii));
+    for (int ii = B + n; ii < L; ++ii) p1_at(ii - B) = fstep(ext(ii));
 
     // ---- anti-causal recursion: z[i] = c MA(y)[i] - b1 z[i+1] - ... - bnn z[i+nn] ----
     const double c = a.c;
     double g[NN];  // g[k] = z[i + 1 + k]
     double y1 = 0.0, y2 = 0.0;  // y[i + 1], y[i + 2]
-    const int top = L - 1 - a.bnset;
+    const int qtop = L - 1 - a.bnset - B;
     {
-        const double yl = p1[static_cast<int64_t>(L - 1) * ps];
-        const double yl2 = p1[static_cast<int64_t>(L - 2) * ps];
+        const double yl = p1_at(L - 1 - B);
+        const double yl2 = p1_at(L - 2 - B);
         double init;
         if (a.binit == 2) {
             init = (c * (-yl2 + yl)) / S;
@@ -167,14 +260,13 @@ __global__ __launch_bounds__(kIirBlock) void k_iir(const IirArgs a) {
         } else {
             double v = 0.0;
 #pragma unroll
-            for (int j = MA::BM4; j <= MA::BM5; ++j)
-                v = v + ba[j] * p1[static_cast<int64_t>(L - 1 - MA::BM5 + j) * ps];
+            for (int j = MA::BM4; j <= MA::BM5; ++j) v = v + ba[j] * p1_at(L - 1 - B - MA::BM5 + j);
             init = c * (v / S);
         }
 #pragma unroll
         for (int k = 0; k < NN; ++k) g[k] = init;
-        if (MA::BM5 >= 1) y1 = p1[static_cast<int64_t>(top + 1) * ps];
-        if (MA::BM5 >= 2) y2 = p1[static_cast<int64_t>(top + 2) * ps];
+        if (MA::BM5 >= 1) y1 = p1_at(qtop + 1);
+        if (MA::BM5 >= 2) y2 = p1_at(qtop + 2);
     }
     auto bstep = [&](double y) {
         double v;
@@ -196,29 +288,31 @@ __global__ __launch_bounds__(kIirBlock) void k_iir(const IirArgs a) {
         y1 = y;
         return v;
     };
-    // trailing border: state only
-    for (int i = top; i >= B + n; --i) bstep(p1[static_cast<int64_t>(i) * ps]);
-    // interior, from the last sample down, in blocks of P with the next block in flight
+    // trailing border (state only), then the samples after the last full block
+    for (int q = qtop; q >= n; --q) bstep(p1_at(q));
+    for (int j = n - 1; j >= nfull * P; --j) dst[static_cast<int64_t>(j) * sout] = static_cast<TO>(bstep(p1_at(j)));
+    // full blocks, from the last down, the next (lower) block's pairs in flight
     {
-        const int nfull = n / P;
-        double cur[P], nxt[P];
+        constexpr int NP = P / 2;
+        double2 cur[NP], nxt[NP];
         if (nfull > 0) {
 #pragma unroll
-            for (int p = 0; p < P; ++p) cur[p] = p1[static_cast<int64_t>(B + n - 1 - p) * ps];
+            for (int k = 0; k < NP; ++k) cur[k] = *p1_pair((nfull - 1) * P + 2 * k);
         }
-        for (int blk = 0; blk < nfull; ++blk) {
-            const int j0 = n - 1 - blk * P;  // image index of the block's first (highest) sample
+        for (int blk = nfull - 1; blk >= 0; --blk) {
+            const int j0 = blk * P;
+            const int jn = blk > 0 ? j0 - P : j0;
 #pragma unroll
-            for (int p = 0; p < P; ++p) nxt[p] = p1[static_cast<int64_t>(B + max(j0 - P - p, 0)) * ps];
+            for (int k = 0; k < NP; ++k) nxt[k] = *p1_pair(jn + 2 * k);
             __builtin_amdgcn_sched_barrier(0);
+            double z[P];  // z[t]: sample j0 + t
 #pragma unroll
-            for (int p = 0; p < P; ++p)
-                dst[static_cast<int64_t>(j0 - p) * sout] = static_cast<TO>(bstep(cur[p]));
+            for (int t = P - 1; t >= 0; --t) z[t] = bstep((t & 1) ? cur[t >> 1].y : cur[t >> 1].x);
 #pragma unroll
-            for (int p = 0; p < P; ++p) cur[p] = nxt[p];
+            for (int t = 0; t < P; t += OS::W) OS::store(dst + static_cast<int64_t>(j0 + t) * sout, z + t);
+#pragma unroll
+            for (int k = 0; k < NP; ++k) cur[k] = nxt[k];
         }
-        for (int j = n - 1 - nfull * P; j >= 0; --j)
-            dst[static_cast<int64_t>(j) * sout] = static_cast<TO>(bstep(p1[static_cast<int64_t>(B + j) * ps]));
     }
 }
 

@@ -1,4 +1,4 @@
-// Instances of the recursive Gaussian line kernel for uint8_t input samples.
+// Instances of the recursive Gaussian line kernel: uint8_t input samples, strided or unaligned filter axis.
 #include "k_iir.hpp"
 
 namespace dipamd {
@@ -9,11 +9,11 @@ template <typename TO, int NN>
 hipError_t iir_nn(int order, const IirArgs &a, hipStream_t st) {
     const unsigned nblk = static_cast<unsigned>((a.nlines + kIirBlock - 1) / kIirBlock);
     switch (order) {
-        case 0: hipLaunchKernelGGL((k_iir<uint8_t, TO, NN, 0>), dim3(nblk), dim3(kIirBlock), 0, st, a); break;
-        case 1: hipLaunchKernelGGL((k_iir<uint8_t, TO, NN, 1>), dim3(nblk), dim3(kIirBlock), 0, st, a); break;
-        case 2: hipLaunchKernelGGL((k_iir<uint8_t, TO, NN, 2>), dim3(nblk), dim3(kIirBlock), 0, st, a); break;
-        case 3: hipLaunchKernelGGL((k_iir<uint8_t, TO, NN, 3>), dim3(nblk), dim3(kIirBlock), 0, st, a); break;
-        case 4: hipLaunchKernelGGL((k_iir<uint8_t, TO, NN, 4>), dim3(nblk), dim3(kIirBlock), 0, st, a); break;
+        case 0: hipLaunchKernelGGL((k_iir<uint8_t, TO, NN, 0, false>), dim3(nblk), dim3(kIirBlock), 0, st, a); break;
+        case 1: hipLaunchKernelGGL((k_iir<uint8_t, TO, NN, 1, false>), dim3(nblk), dim3(kIirBlock), 0, st, a); break;
+        case 2: hipLaunchKernelGGL((k_iir<uint8_t, TO, NN, 2, false>), dim3(nblk), dim3(kIirBlock), 0, st, a); break;
+        case 3: hipLaunchKernelGGL((k_iir<uint8_t, TO, NN, 3, false>), dim3(nblk), dim3(kIirBlock), 0, st, a); break;
+        case 4: hipLaunchKernelGGL((k_iir<uint8_t, TO, NN, 4, false>), dim3(nblk), dim3(kIirBlock), 0, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -349,8 +349,41 @@ inline Image GaussFIR(Image const &in, FloatArray sigmas = {1.0}, UnsignedArray 
     return out;
 }
 
-/* dip::Gauss (linear.h:452-472; dispatch derivative.cpp:35-86).  Only the FIR
- * method is in this library: "best" resolving to FT or IIR throws. */
+/* dip::GaussIIR (linear.h:407-429). */
+inline void GaussIIR(Image const &in, Image &out, FloatArray sigmas = {1.0}, UnsignedArray derivativeOrder = {0},
+                     StringArray const &boundaryCondition = {}, UnsignedArray filterOrder = {},
+                     String const &designMethod = "discrete time fit", dfloat truncation = 3) {
+    if (!in.IsForged()) throw ParameterError("Image is not forged");
+    int32_t method;
+    if (designMethod == "discrete time fit") {
+        method = DIPAMD_IIR_DISCRETE_TIME_FIT;
+    } else if (designMethod == "forward backward") {
+        method = DIPAMD_IIR_FORWARD_BACKWARD;
+    } else {
+        throw ParameterError("Invalid flag: " + designMethod);
+    }
+    Image src = in;
+    detail::PrepareOutput(in, out);
+    std::vector<int32_t> ord(derivativeOrder.begin(), derivativeOrder.end());
+    std::vector<int32_t> fo(filterOrder.begin(), filterOrder.end());
+    std::vector<int32_t> bc = detail::BoundaryCodes(boundaryCondition);
+    dipamd_image a = src.View(), b = out.View();
+    detail::Check(dipamd_gauss_iir(&a, &b, sigmas.data(), static_cast<int32_t>(sigmas.size()), ord.data(),
+                                   static_cast<int32_t>(ord.size()), bc.data(), static_cast<int32_t>(bc.size()),
+                                   fo.data(), static_cast<int32_t>(fo.size()), method, truncation, 0,
+                                   dipamd::CurrentStream()));
+}
+inline Image GaussIIR(Image const &in, FloatArray sigmas = {1.0}, UnsignedArray derivativeOrder = {0},
+                      StringArray const &boundaryCondition = {}, UnsignedArray filterOrder = {},
+                      String const &designMethod = "discrete time fit", dfloat truncation = 3) {
+    Image out;
+    GaussIIR(in, out, std::move(sigmas), std::move(derivativeOrder), boundaryCondition, std::move(filterOrder),
+             designMethod, truncation);
+    return out;
+}
+
+/* dip::Gauss (linear.h:452-472; dispatch derivative.cpp:35-86).  The FIR and IIR
+ * methods are in this library: "FT" (and "best" resolving to it) throws. */
 inline void Gauss(Image const &in, Image &out, FloatArray sigmas = {1.0}, UnsignedArray derivativeOrder = {0},
                   String const &c_method = "best", StringArray const &boundaryCondition = {}, dfloat truncation = 3) {
     String method = c_method.substr(0, 5) == "gauss" ? c_method.substr(5) : c_method;
@@ -368,8 +401,13 @@ inline void Gauss(Image const &in, Image &out, FloatArray sigmas = {1.0}, Unsign
         GaussFIR(in, out, std::move(sigmas), std::move(derivativeOrder), boundaryCondition, truncation);
         return;
     }
-    if (method == "FT" || method == "ft" || method == "IIR" || method == "iir")
-        throw ParameterError("dip::Gauss method \"" + method + "\" is not provided by this library (FIR only)");
+    if (method == "IIR" || method == "iir") {
+        GaussIIR(in, out, std::move(sigmas), std::move(derivativeOrder), boundaryCondition, {}, "discrete time fit",
+                 truncation);
+        return;
+    }
+    if (method == "FT" || method == "ft")
+        throw ParameterError("dip::Gauss method \"" + method + "\" is not provided by this library (FIR and IIR only)");
     throw ParameterError("Invalid flag: " + c_method);
 }
 inline Image Gauss(Image const &in, FloatArray sigmas = {1.0}, UnsignedArray derivativeOrder = {0},

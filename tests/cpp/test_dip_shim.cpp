@@ -111,6 +111,21 @@ int main(int argc, char **argv) {
             CHECK(lap_ok, "dip::Laplace = dxx + dyy in float");
             CHECK(mag_ok, "dip::GradientMagnitude = sqrt(dx*dx + dy*dy) in float");
         }
+        // the IIR method: "best" picks it for sigma > 10, and "IIR" is dip::GaussIIR with
+        // the discrete-time-fit design
+        {
+            auto host = [&](dip::Image const &im) {
+                std::vector<float> v(W * H);
+                (void)hipDeviceSynchronize();
+                im.CopyToHost(v.data());
+                return v;
+            };
+            std::vector<float> iir = host(dip::GaussIIR(img, {12.0}, {1, 0}, {"mirror"}));
+            CHECK(host(dip::Gauss(img, {12.0}, {1, 0}, "best", {"mirror"})) == iir, "dip::Gauss best -> GaussIIR");
+            CHECK(host(dip::Gauss(img, {12.0}, {1, 0}, "IIR", {"mirror"})) == iir, "dip::Gauss IIR -> GaussIIR");
+            CHECK(host(dip::GaussIIR(img, {12.0}, {1, 0}, {"mirror"}, {4, 3}, "discrete time fit")) == iir,
+                  "default filter orders: 3 + derivative order");
+        }
         // protected output of another type keeps its type (clamp_cast)
         dip::Image prot({W, H}, 1, dip::DT_DFLOAT);
         prot.Protect();
