@@ -2,7 +2,8 @@
 // reference) and the structure tensor (structure_tensor.cpp:32-57): dip::Gradient,
 // GradientMagnitude, Hessian, Laplace, Dgg, LaplacePlusDgg, LaplaceMinusDgg and
 // StructureTensor with the FIR method.  Each Gaussian derivative is the plain drop-in
-// call (dipamd_gauss_fir); the combined forms add one element-wise pass
+// call (dipamd_gauss_fir, or dipamd_gauss_iir with DIPAMD_GAUSS_IIR); the combined forms
+// add one element-wise pass
 // (k_elementwise.hip) with the reference's operation order and type.
 #include <hip/hip_runtime.h>
 
@@ -94,11 +95,20 @@ struct Scratch {
 
 // The Gaussian derivatives of `in` with the given per-dim orders (one list of nd
 // orders per image), each into `dst[k]`.
+int gauss(bool iir, const dipamd_image *in, const dipamd_image *out, const double *sig, const int32_t *orders,
+          int32_t norders, const int32_t *bc, int32_t nbc, double truncation, uint32_t flags, void *stream) {
+    if (iir)
+        return dipamd_gauss_iir(in, out, sig, in->ndim, orders, norders, bc, nbc, nullptr, 0,
+                                DIPAMD_IIR_DISCRETE_TIME_FIT, truncation, 0, stream);
+    return dipamd_gauss_fir(in, out, sig, in->ndim, orders, norders, bc, nbc, truncation,
+                            flags & ~static_cast<uint32_t>(DIPAMD_GAUSS_IIR | DIPAMD_TENSOR_IIR), stream);
+}
+
 int derivatives(const dipamd_image *in, const std::vector<std::vector<int32_t>> &orders, const dipamd_image *dst,
                 const double *sig, const int32_t *bc, int32_t nbc, double truncation, uint32_t flags, void *stream) {
+    const bool iir = (flags & DIPAMD_GAUSS_IIR) != 0;
     for (size_t k = 0; k < orders.size(); ++k) {
-        const int st = dipamd_gauss_fir(in, &dst[k], sig, in->ndim, orders[k].data(), in->ndim, bc, nbc, truncation,
-                                        flags, stream);
+        const int st = gauss(iir, in, &dst[k], sig, orders[k].data(), in->ndim, bc, nbc, truncation, flags, stream);
         if (st != DIPAMD_OK) return st;
     }
     return DIPAMD_OK;
@@ -263,7 +273,8 @@ int dipamd_structure_tensor(const dipamd_image *in, const dipamd_image *outs, in
         return st;
     if ((st = combine(5, buf_dt == DIPAMD_DT_DFLOAT, g, n, outs, nouts, scr, in)) != DIPAMD_OK) return st;
     for (int k = 0; k < nouts; ++k) {
-        st = dipamd_gauss_fir(&outs[k], &outs[k], tsig, nd, nullptr, 0, bc, nbc, truncation, flags, stream);
+        st = gauss((flags & DIPAMD_TENSOR_IIR) != 0, &outs[k], &outs[k], tsig, nullptr, 0, bc, nbc, truncation, flags,
+                   stream);
         if (st != DIPAMD_OK) return st;
     }
     return DIPAMD_OK;

@@ -75,7 +75,13 @@ enum {
     DIPAMD_EXACT = 0,          /* default: the reference's arithmetic, bit-identical output */
     DIPAMD_FAST = 1,           /* fused multiply-add in the tap loop (not bit-identical) */
     DIPAMD_NO_FUSION = 2,      /* one pass per axis (no fused 2-D kernel) */
-    DIPAMD_FORCE_FUSION = 4    /* fused 2-D kernel wherever it applies (ignores the tuner) */
+    DIPAMD_FORCE_FUSION = 4,   /* fused 2-D kernel wherever it applies (ignores the tuner) */
+    /* Derivative family and structure tensor: the Gaussian derivatives by dip::GaussIIR
+     * (discrete time fit, default filter orders: what method "IIR", or "best" with a sigma
+     * above 10, runs -- derivative.cpp:35-58, 111-114); DIPAMD_TENSOR_IIR: the structure
+     * tensor's smoothing too (its method resolves with the tensor sigmas). */
+    DIPAMD_GAUSS_IIR = 8,
+    DIPAMD_TENSOR_IIR = 16
 };
 
 /* Error codes. */

@@ -417,14 +417,17 @@ inline Image Gauss(Image const &in, FloatArray sigmas = {1.0}, UnsignedArray der
     return out;
 }
 
-/* The derivative family (derivative.cpp:120-418; linear.h:666, 697, 862, 904) with the
- * FIR method ("best" must resolve to it).  This shim has no tensor images: Gradient and
+/* The derivative family (derivative.cpp:120-521; linear.h:666, 697, 862, 904) with the
+ * Gaussian FIR or IIR method ("best" resolves to one of them, or throws for FT).  This
+ * shim has no tensor images: Gradient and
  * Hessian return their tensor elements as an array of images, in dip's element order
  * (Hessian: the diagonal first, then the upper triangle column by column). */
 using ImageArray = std::vector<Image>;
 namespace detail {
-inline void CheckFirMethod(String const &c_method, FloatArray const &sigmas) {
-    String method = c_method.substr(0, 5) == "gauss" ? c_method.substr(5) : c_method;
+/* The Gaussian method the derivatives run (dip::Derivative passes `method` on to
+ * dip::Gauss, derivative.cpp:111-114): DIPAMD_GAUSS_IIR for IIR, 0 for FIR. */
+inline uint32_t GaussMethodFlag(String const &c_method, FloatArray const &sigmas) {
+    String method = c_method == "gauss" ? "best" : (c_method.substr(0, 5) == "gauss" ? c_method.substr(5) : c_method);
     if (method == "best") {  // derivative orders <= 2 here: sigma decides (derivative.cpp:35-58)
         method = "fir";
         for (auto s : sigmas)
@@ -433,20 +436,22 @@ inline void CheckFirMethod(String const &c_method, FloatArray const &sigmas) {
             for (auto s : sigmas)
                 if (s > 10) method = "iir";
     }
-    if (method == "FIR" || method == "fir") return;
-    if (method == "FT" || method == "ft" || method == "IIR" || method == "iir" || method == "finitediff")
-        throw ParameterError("method \"" + method + "\" is not provided by this library (Gaussian FIR only)");
+    if (method == "FIR" || method == "fir") return 0;
+    if (method == "IIR" || method == "iir") return DIPAMD_GAUSS_IIR;
+    if (method == "FT" || method == "ft" || method == "finitediff")
+        throw ParameterError("method \"" + method + "\" is not provided by this library (Gaussian FIR and IIR only)");
     throw ParameterError("Invalid flag: " + c_method);
 }
 inline void RunDerivative(int32_t fn, Image const &in, std::vector<dipamd_image> &outs, FloatArray const &sigmas,
-                          StringArray const &boundaryCondition, BooleanArray const &process, dfloat truncation) {
+                          StringArray const &boundaryCondition, BooleanArray const &process, dfloat truncation,
+                          uint32_t method_flag) {
     std::vector<int32_t> bc = BoundaryCodes(boundaryCondition);
     std::vector<uint8_t> proc(process.begin(), process.end());
     dipamd_image a = in.View();
     Check(dipamd_derivative(fn, &a, outs.data(), static_cast<int32_t>(outs.size()), sigmas.data(),
                             static_cast<int32_t>(sigmas.size()), bc.data(), static_cast<int32_t>(bc.size()),
                             proc.empty() ? nullptr : proc.data(), static_cast<int32_t>(proc.size()), truncation,
-                            DIPAMD_EXACT, dipamd::CurrentStream()));
+                            DIPAMD_EXACT | method_flag, dipamd::CurrentStream()));
 }
 inline int DifferentiatedDims(Image const &in, FloatArray const &sigmas, BooleanArray const &process) {
     const uint nd = in.Dimensionality();
@@ -463,7 +468,7 @@ inline void TensorResult(int32_t fn, Image const &in, ImageArray &out, FloatArra
                          String const &method, StringArray const &bc, BooleanArray const &process,
                          dfloat truncation) {
     if (!in.IsForged()) throw ParameterError("Image is not forged");
-    CheckFirMethod(method, sigmas);
+    const uint32_t mf = GaussMethodFlag(method, sigmas);
     const int n = dipamd_derivative_outputs(fn, DifferentiatedDims(in, sigmas, process));
     if (n < 0) Check(n);
     out.clear();
@@ -472,17 +477,17 @@ inline void TensorResult(int32_t fn, Image const &in, ImageArray &out, FloatArra
         out.emplace_back(in.Sizes(), 1, DataType::SuggestFlex(in.DataType()));
         views.push_back(out.back().View());
     }
-    RunDerivative(fn, in, views, sigmas, bc, process, truncation);
+    RunDerivative(fn, in, views, sigmas, bc, process, truncation, mf);
 }
 inline void ScalarResult(int32_t fn, Image const &in, Image &out, FloatArray const &sigmas, String const &method,
                          StringArray const &bc, BooleanArray const &process, dfloat truncation) {
     if (!in.IsForged()) throw ParameterError("Image is not forged");
-    CheckFirMethod(method, sigmas);
+    const uint32_t mf = GaussMethodFlag(method, sigmas);
     DifferentiatedDims(in, sigmas, process);
     Image src = in;
     PrepareOutput(in, out);
     std::vector<dipamd_image> views{out.View()};
-    RunDerivative(fn, src, views, sigmas, bc, process, truncation);
+    RunDerivative(fn, src, views, sigmas, bc, process, truncation, mf);
 }
 }  // namespace detail
 
@@ -540,8 +545,8 @@ inline void StructureTensor(Image const &in, Image const &mask, ImageArray &out,
                             StringArray const &boundaryCondition = {}, dfloat truncation = 3) {
     if (!in.IsForged()) throw ParameterError("Image is not forged");
     if (mask.IsForged()) throw ParameterError("StructureTensor with a mask is not provided by this library");
-    detail::CheckFirMethod(method, gradientSigmas);
-    detail::CheckFirMethod(method, tensorSigmas);
+    const uint32_t flags = detail::GaussMethodFlag(method, gradientSigmas) |
+                           (detail::GaussMethodFlag(method, tensorSigmas) ? DIPAMD_TENSOR_IIR : 0u);
     const int n = dipamd_derivative_outputs(DIPAMD_FN_STRUCTURE_TENSOR,
                                             detail::DifferentiatedDims(in, gradientSigmas, {}));
     if (n < 0) detail::Check(n);
@@ -556,7 +561,7 @@ inline void StructureTensor(Image const &in, Image const &mask, ImageArray &out,
     detail::Check(dipamd_structure_tensor(&a, views.data(), n, gradientSigmas.data(),
                                           static_cast<int32_t>(gradientSigmas.size()), tensorSigmas.data(),
                                           static_cast<int32_t>(tensorSigmas.size()), bc.data(),
-                                          static_cast<int32_t>(bc.size()), truncation, DIPAMD_EXACT,
+                                          static_cast<int32_t>(bc.size()), truncation, DIPAMD_EXACT | flags,
                                           dipamd::CurrentStream()));
 }
 

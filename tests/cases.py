@@ -184,3 +184,26 @@ for bc in ("asym mirror", "periodic", "asym periodic", "add zeros", "zero order"
     t = bc.replace(" ", "_")
     IIR_CASES.append((f"iir_f32_40x20_{t}", (20, 40), "float32", [6.0, 4.0], [1, 2], [bc], (), DTF, 3.0, None))
     IIR_CASES.append((f"iir_line_3_{t}", (1, 3), "float64", [2.0], [0, 0], [bc], (), DTF, 3.0, None))
+
+# The derivative family and the structure tensor with the IIR method ("best" with a sigma
+# above 10, or "gaussiir"): (name, function, numpy shape, dtype, sigmas, boundary list, method)
+DERIV_IIR_CASES = []
+for fn in ("gradient", "gradmag", "hessian", "laplace", "dgg", "laplace-dgg", "structure:12", "structure:3"):
+    t = fn.replace("+", "p").replace("-", "m").replace(":", "")
+    DERIV_IIR_CASES += [
+        (f"{t}_f32_64x48_s12_best", fn, (48, 64), "float32", [12.0], ["mirror"], "best"),
+        (f"{t}_u8_61x37_s3_iir", fn, (37, 61), "uint8", [3.0], ["periodic"], "gaussiir"),
+        (f"{t}_f64_40x30_s11_best", fn, (30, 40), "float64", [11.0], ["mirror"], "best"),
+    ]
+
+
+def iir_flags(fn, sigmas, method):
+    """(derivatives by IIR, structure-tensor smoothing by IIR) as dip::Gauss resolves
+    `method` (derivative.cpp:35-58)."""
+    def one(sig):
+        m = method[5:] if method.startswith("gauss") and method != "gauss" else method
+        if m == "best":
+            return any(s > 10 for s in sig)
+        return m == "iir"
+    tsig = [float(fn.split(":")[1])] if fn.startswith("structure:") else sigmas
+    return one(sigmas), one(tsig)

@@ -125,6 +125,9 @@ int main(int argc, char **argv) {
             CHECK(host(dip::Gauss(img, {12.0}, {1, 0}, "IIR", {"mirror"})) == iir, "dip::Gauss IIR -> GaussIIR");
             CHECK(host(dip::GaussIIR(img, {12.0}, {1, 0}, {"mirror"}, {4, 3}, "discrete time fit")) == iir,
                   "default filter orders: 3 + derivative order");
+            dip::ImageArray gi;
+            dip::Gradient(img, gi, {12.0}, "best", {"mirror"});
+            CHECK(gi.size() == 2 && host(gi[0]) == iir, "dip::Gradient best -> the IIR derivatives");
         }
         // protected output of another type keeps its type (clamp_cast)
         dip::Image prot({W, H}, 1, dip::DT_DFLOAT);

@@ -12,6 +12,8 @@ Outputs (all small):
   deriv2_cases.npz   the same for tests/cases.py:DERIV2_CASES (dip::Dgg, LaplacePlusDgg,
                      LaplaceMinusDgg, StructureTensor)
   iir_cases.npz      input and dip::GaussIIR output of tests/cases.py:IIR_CASES
+  deriv_iir_cases.npz  the derivative family and structure tensor with the IIR method
+                     (tests/cases.py:DERIV_IIR_CASES)
   taps.npz           MakeHalfGaussian(sigma, order, 3, sfloat/dfloat) for TAP_SIGMAS x orders 0..3
   small_cases.npz    input and dip::Gauss output of every case in tests/cases.py:SMALL_CASES
   large_cases.json   SHA-256 of the exact output bytes + sampled pixels for LARGE_CASES
@@ -66,6 +68,15 @@ def write_iir_cases(fname="iir_cases.npz", seed0=6000):
     np.savez_compressed(os.path.join(HERE, fname), **arrs)
 
 
+def write_derivative_iir_cases(fname="deriv_iir_cases.npz", seed0=8000):
+    arrs = {}
+    for i, (name, fn, shape, dt, sig, bc, method) in enumerate(cases.DERIV_IIR_CASES):
+        x = cases.make_input(shape, dt, seed=seed0 + i)
+        arrs[name + "__in"] = x
+        arrs[name + "__out"] = oc.ref_derivative(fn, x, sig, bc, method)
+    np.savez_compressed(os.path.join(HERE, fname), **arrs)
+
+
 def main():
     assert oc.ref_available(), "build oracle/_ref/libdipref.so first (make -f oracle/ref.mk)"
     only = sys.argv[sys.argv.index("--only") + 1] if "--only" in sys.argv else None
@@ -73,6 +84,7 @@ def main():
         write_cases(cases.EXTRAP_CASES, "extrap_cases.npz", 3000)
     if only in (None, "iir"):
         write_iir_cases()
+        write_derivative_iir_cases()
     if only in (None, "deriv"):
         write_derivative_cases()
         write_derivative_cases(cases.DERIV2_CASES, "deriv2_cases.npz", 5000)

@@ -265,17 +265,19 @@ def ref_derivative(which, x, sigmas, bc=("mirror",), method="gaussfir", truncati
     return out[:n.value].copy()
 
 
-def derivative(which, x, sigmas, bc=("mirror",), process=None, truncation=3.0):
+def derivative(which, x, sigmas, bc=("mirror",), process=None, truncation=3.0, iir=False, tensor_iir=False):
     """The oracle's restatement of the derivative family (derivative.cpp:120-418): Gauss
-    derivatives from the oracle, combined in the reference's order and type (numpy
-    float32 / float64 arithmetic rounds each operation as the reference does)."""
+    derivatives from the oracle (FIR, or the IIR Gaussian with `iir`; the structure
+    tensor's smoothing with `tensor_iir`), combined in the reference's order and type
+    (numpy float32 / float64 arithmetic rounds each operation as the reference does)."""
     nd = x.ndim
     sig = list(sigmas) if len(sigmas) == nd else list(sigmas) * nd
     proc = [True] * nd if process is None else list(process)
     sizes = list(reversed(x.shape))
     dims = [d for d in range(nd) if proc[d] and sizes[d] > 1 and sig[d] > 0]
     def d_of(order):
-        return gauss_fir(x, sig, order, bc, truncation)
+        return gauss_iir(x, sig, order, bc, (), "discrete time fit", truncation) if iir else \
+            gauss_fir(x, sig, order, bc, truncation)
     def unit(pairs):
         o = [0] * nd
         for d, v in pairs:
@@ -305,8 +307,8 @@ def derivative(which, x, sigmas, bc=("mirror",), process=None, truncation=3.0):
         return np.sqrt(out)[None]
     if which in ("dgg", "laplace+dgg", "laplace-dgg"):
         # derivative.cpp:424-484, image operation by image operation
-        g = derivative("gradient", x, sigmas, bc, process, truncation)
-        H = derivative("hessian", x, sigmas, bc, process, truncation)
+        g = derivative("gradient", x, sigmas, bc, process, truncation, iir)
+        H = derivative("hessian", x, sigmas, bc, process, truncation, iir)
         n = g.shape[0]
         one = g.dtype.type(1)
         two = g.dtype.type(2)
@@ -334,12 +336,14 @@ def derivative(which, x, sigmas, bc=("mirror",), process=None, truncation=3.0):
     if which.startswith("structure:"):
         # structure_tensor.cpp:53-56: g g^T (a sum started at 0), then Gauss per element
         tsig = float(which.split(":")[1])
-        g = derivative("gradient", x, sigmas, bc, None, truncation)
+        g = derivative("gradient", x, sigmas, bc, None, truncation, iir)
         n = g.shape[0]
         zero = g.dtype.type(0)
         el = [zero + g[c] * g[c] for c in range(n)]
         for c in range(1, n):
             for r in range(c):
                 el.append(zero + g[r] * g[c])
+        if tensor_iir:
+            return np.stack([gauss_iir(e, [tsig], [0], bc, (), "discrete time fit", truncation) for e in el])
         return np.stack([gauss_fir(e, [tsig], [0], bc, truncation) for e in el])
     raise ValueError(which)

@@ -16,6 +16,7 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 DERIV = np.load(os.path.join(HERE, "golden", "deriv_cases.npz"))
 DERIV2 = np.load(os.path.join(HERE, "golden", "deriv2_cases.npz"))
+DERIV_IIR = np.load(os.path.join(HERE, "golden", "deriv_iir_cases.npz"))
 FN = {"gradient": dip.Gradient, "gradmag": dip.GradientMagnitude, "hessian": dip.Hessian, "laplace": dip.Laplace,
       "dgg": dip.Dgg, "laplace+dgg": dip.LaplacePlusDgg, "laplace-dgg": dip.LaplaceMinusDgg}
 
@@ -96,3 +97,18 @@ def test_dgg_and_structure_tensor_larger_vs_oracle(cuda, fn):
     v = cases.make_input((20, 33, 28), "float32", seed=54)
     y = _call(fn, torch.from_numpy(v).to(cuda), [1.0, 2.0, 1.5], "gaussFIR", ["periodic"])
     assert np.array_equal(_elements(fn, y), oc.derivative(fn, v, [1.0, 2.0, 1.5], ["periodic"]))
+
+
+@pytest.mark.parametrize("case", cases.DERIV_IIR_CASES, ids=[c[0] for c in cases.DERIV_IIR_CASES])
+def test_derivative_iir_bitexact(cuda, case):
+    """The derivative family and the structure tensor through the IIR Gaussian ("best"
+    with sigma > 10, "gaussiir") against the reference's own results, bit for bit."""
+    name, fn, shape, dt, sig, bc, method = case
+    x = DERIV_IIR[name + "__in"]
+    exp = DERIV_IIR[name + "__out"]
+    y = _call(fn, torch.from_numpy(x).to(cuda), sig, method, bc)
+    torch.cuda.synchronize()
+    got = _elements(fn, y)
+    assert got.shape == exp.shape and got.dtype == exp.dtype
+    assert np.array_equal(got.view(np.uint8), exp.view(np.uint8)), \
+        f"{name}: max abs diff {np.abs(got.astype(np.float64) - exp.astype(np.float64)).max()}"

@@ -406,3 +406,20 @@ def test_oracle_iir_random_cases_against_reference():
         r = oc.gauss_iir(x, sig, ords, bc, fo, method, use_ref=True)
         q = oc.gauss_iir(x, sig, ords, bc, fo, method)
         assert np.array_equal(r, q), (shape, dt, sig, ords, fo, method, bc)
+
+
+DERIV_IIR = np.load(os.path.join(HERE, "golden", "deriv_iir_cases.npz"))
+
+
+@pytest.mark.parametrize("case", cases.DERIV_IIR_CASES, ids=[c[0] for c in cases.DERIV_IIR_CASES])
+def test_oracle_derivative_iir_matches_reference(case):
+    """The derivative family and the structure tensor with the IIR method ("best" with
+    sigma > 10, "gaussiir"): the oracle against the reference's own results, bit for bit
+    (the structure tensor's smoothing resolves "best" with its own sigmas)."""
+    name, fn, shape, dt, sig, bc, method = case
+    x = DERIV_IIR[name + "__in"]
+    exp = DERIV_IIR[name + "__out"]
+    iir, tensor_iir = cases.iir_flags(fn, sig, method)
+    got = oc.derivative(fn, x, sig, bc, None, 3.0, iir, tensor_iir)
+    assert got.shape == exp.shape and got.dtype == exp.dtype
+    assert np.array_equal(got.view(np.uint8), exp.view(np.uint8))
