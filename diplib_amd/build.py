@@ -81,7 +81,7 @@ def build(changed_only=False, jobs=None, verbose=True):
             print(f"--- {src}\n{err}", file=sys.stderr)
         raise RuntimeError("hipcc failed for: " + ", ".join(os.path.basename(s) for s, _ in failed))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + [_obj(s) for s in srcs] + ["-ldl"]
+    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + [_obj(s) for s in srcs] + ["-ldl", "-lrocfft"]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError("link failed:\n" + p.stderr)

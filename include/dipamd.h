@@ -141,6 +141,18 @@ int dipamd_gauss_iir(const dipamd_image *in, const dipamd_image *out, const doub
                      const int32_t *filter_orders, int32_t nfilter_orders, int32_t design_method,
                      double truncation, uint32_t flags, void *stream);
 
+/* dip::GaussFT (src/linear/gauss.cpp:397-487; include/diplib/linear.h:355-364) for real
+ * images, spatial domain in and out: the Gaussian and its derivatives of any order as a
+ * product in the frequency domain (rocFFT; single precision for 8-bit and sfloat images,
+ * double for dfloat, as the reference's DFT).  nbc == 0 transforms the image as it is
+ * (periodic); otherwise the image is extended by the boundary condition (the ones that
+ * copy samples) to a 2-3-5-smooth size holding its Gaussian border, and cropped back.
+ * dip::Gauss with method "FT" (and "best" for sigma < 0.8 or order > 3) passes "mirror".
+ * At most three dims may be filtered.  Results match the reference to FFT rounding. */
+int dipamd_gauss_ft(const dipamd_image *in, const dipamd_image *out, const double *sigmas, int32_t nsigmas,
+                    const int32_t *orders, int32_t norders, const int32_t *bc, int32_t nbc, double truncation,
+                    uint32_t flags, void *stream);
+
 /* The derivative family on the Gaussian FIR path (src/linear/derivative.cpp:120-521;
  * include/diplib/linear.h:666, 697, 862, 904): dip::Gradient, dip::GradientMagnitude,
  * dip::Hessian, dip::Laplace and the Dgg family with method "FIR" (or "best" resolving to FIR), each Gaussian derivative

@@ -105,6 +105,29 @@ int dipref_gauss_iir( void const* in, int in_dtype, void* out, int out_dtype, in
    }
 }
 
+// dip::GaussFT( in, out, sigmas, orders, truncation, "spatial", "spatial", bc ): as
+// dipref_gauss; nbc == 0 passes an empty boundary condition array (no extension).
+int dipref_gauss_ft( void const* in, int in_dtype, void* out, int out_dtype, int ndim, int64_t const* sizes,
+                     double const* sigmas, int nsigmas, int const* orders, int norders,
+                     char const* const* bcs, int nbc, double truncation ) {
+   try {
+      dip::Image imgIn = Wrap( in, in_dtype, ndim, sizes );
+      dip::Image imgOut = Wrap( out, out_dtype, ndim, sizes );
+      imgOut.Protect();
+      dip::FloatArray s( static_cast< dip::uint >( nsigmas ));
+      for( int ii = 0; ii < nsigmas; ++ii ) { s[ static_cast< dip::uint >( ii ) ] = sigmas[ ii ]; }
+      dip::UnsignedArray o( static_cast< dip::uint >( norders ));
+      for( int ii = 0; ii < norders; ++ii ) { o[ static_cast< dip::uint >( ii ) ] = static_cast< dip::uint >( orders[ ii ] ); }
+      dip::StringArray bc;
+      for( int ii = 0; ii < nbc; ++ii ) { bc.emplace_back( bcs[ ii ] ); }
+      dip::GaussFT( imgIn, imgOut, s, o, truncation, "spatial", "spatial", bc );
+      return 0;
+   } catch( std::exception const& e ) {
+      g_err = e.what();
+      return 1;
+   }
+}
+
 // In-place form: `buf` is both the input and the output (same dtype, which must be
 // sfloat or dfloat).
 int dipref_gauss_inplace( void* buf, int dtype, int ndim, int64_t const* sizes, double const* sigmas, int nsigmas,

@@ -207,3 +207,33 @@ def iir_flags(fn, sigmas, method):
         return m == "iir"
     tsig = [float(fn.split(":")[1])] if fn.startswith("structure:") else sigmas
     return one(sigmas), one(tsig)
+
+# dip::GaussFT (spatial in and out): (name, numpy shape, dtype, sigmas, orders, boundary list
+# (empty: no extension), truncation).  Compared with a tolerance: the FFTs round differently.
+FT_CASES = [
+    ("ft_f32_64x48_s05", (48, 64), "float32", [0.5], [0, 0], ["mirror"], 3.0),
+    ("ft_u8_61x37_s06_o10_periodic", (37, 61), "uint8", [0.6], [1, 0], ["periodic"], 3.0),
+    ("ft_f64_40x30_s2_o40", (30, 40), "float64", [2.0], [4, 0], ["mirror"], 3.0),
+    ("ft_f32_40x29x11_s07", (11, 29, 40), "float32", [0.7, 0.7, 0.5], [0], ["mirror"], 3.0),
+    ("ft_f32_50x40_s0_o11", (40, 50), "float32", [0.0], [1, 1], ["mirror"], 3.0),
+    ("ft_f32_64x48_s3_noext", (48, 64), "float32", [3.0], [0, 0], [], 3.0),
+    ("ft_f32_97x61_s15_o21", (61, 97), "float32", [1.5], [2, 1], ["mirror"], 3.0),
+    ("ft_f32_46x30_add_zeros", (30, 46), "float32", [0.5], [0, 1], ["add zeros"], 3.0),
+    ("ft_f32_40x30_asym_mirror", (30, 40), "float32", [0.7], [1, 0], ["asym mirror"], 3.0),
+    ("ft_f32_40x30_zero_order", (30, 40), "float32", [2.5], [0, 2], ["zero order"], 3.0),
+    ("ft_f32_40x30_asym_periodic", (30, 40), "float32", [1.2], [3, 0], ["asym periodic"], 3.0),
+    ("ft_f32_1x42", (42, 1), "float32", [0.5], [0], ["mirror"], 3.0),
+    ("ft_f64_33x20_s04_o50", (20, 33), "float64", [0.4], [5, 0], ["mirror"], 3.0),
+    ("ft_f32_45x33_mixed_bc", (33, 45), "float32", [0.6, 1.0], [0, 1], ["periodic", "add zeros"], 3.0),
+    ("ft_f32_64x64_t2", (64, 64), "float32", [1.0], [1, 0], ["mirror"], 2.0),
+    ("ft_s16_45x33", (33, 45), "int16", [0.5], [0, 0], ["mirror"], 3.0),
+]
+# Odd padded length along the real-to-complex dim: the reference's output is not shift
+# invariant there (its odd-length real DFT path), so these are checked against the
+# oracle's exact convolution and the shift invariance of the result, not the reference.
+FT_ODD_CASES = [
+    ("ftodd_f32_37x29x11", (11, 29, 37), "float32", [0.7, 0.7, 0.5], [0], ["mirror"], 3.0),
+    ("ftodd_f32_40x30_add_zeros", (30, 40), "float32", [0.5], [0, 1], ["add zeros"], 3.0),
+    ("ftodd_f32_41_noext", (41,), "float32", [0.5], [0], [], 3.0),
+    ("ftodd_f32_45x5_noext_o1", (5, 45), "float32", [0.8], [1, 0], [], 3.0),
+]

@@ -1,7 +1,7 @@
 """Writes the golden fixtures under tests/golden/ from the REFERENCE library itself
 (oracle/_ref/libdipref.so, built from the reference sources by oracle/ref.mk).
 
-    make -f oracle/ref.mk -j8 && python tests/golden/make_golden.py [--only extrap|deriv|iir]
+    make -f oracle/ref.mk -j8 && python tests/golden/make_golden.py [--only extrap|deriv|iir|ft]
 
 Outputs (all small):
   extrap_cases.npz   input and dip::Gauss output of tests/cases.py:EXTRAP_CASES (the
@@ -14,6 +14,7 @@ Outputs (all small):
   iir_cases.npz      input and dip::GaussIIR output of tests/cases.py:IIR_CASES
   deriv_iir_cases.npz  the derivative family and structure tensor with the IIR method
                      (tests/cases.py:DERIV_IIR_CASES)
+  ft_cases.npz       input and dip::GaussFT output of tests/cases.py:FT_CASES
   taps.npz           MakeHalfGaussian(sigma, order, 3, sfloat/dfloat) for TAP_SIGMAS x orders 0..3
   small_cases.npz    input and dip::Gauss output of every case in tests/cases.py:SMALL_CASES
   large_cases.json   SHA-256 of the exact output bytes + sampled pixels for LARGE_CASES
@@ -77,11 +78,22 @@ def write_derivative_iir_cases(fname="deriv_iir_cases.npz", seed0=8000):
     np.savez_compressed(os.path.join(HERE, fname), **arrs)
 
 
+def write_ft_cases(fname="ft_cases.npz", seed0=9500):
+    arrs = {}
+    for i, (name, shape, dt, sig, ords, bc, trunc) in enumerate(cases.FT_CASES):
+        x = cases.make_input(shape, dt, seed=seed0 + i)
+        arrs[name + "__in"] = x
+        arrs[name + "__out"] = oc.ref_gauss_ft(x, sig, ords, bc, trunc)
+    np.savez_compressed(os.path.join(HERE, fname), **arrs)
+
+
 def main():
     assert oc.ref_available(), "build oracle/_ref/libdipref.so first (make -f oracle/ref.mk)"
     only = sys.argv[sys.argv.index("--only") + 1] if "--only" in sys.argv else None
     if only in (None, "extrap"):
         write_cases(cases.EXTRAP_CASES, "extrap_cases.npz", 3000)
+    if only in (None, "ft"):
+        write_ft_cases()
     if only in (None, "iir"):
         write_iir_cases()
         write_derivative_iir_cases()

@@ -55,6 +55,8 @@ def ref():
         lib.dipref_half_gaussian.argtypes = [D, I, D, I, P, I]
         lib.dipref_separable.argtypes = [P, I, P, I, I, P, P, P, P, I, P, I, P]
         lib.dipref_last_error.restype = ctypes.c_char_p
+        if hasattr(lib, "dipref_gauss_ft"):
+            lib.dipref_gauss_ft.argtypes = [P, I, P, I, I, P, P, I, P, I, P, I, D]
         if hasattr(lib, "dipref_gauss_iir"):
             lib.dipref_gauss_iir.argtypes = [P, I, P, I, I, P, P, I, P, I, P, I, P, I, ctypes.c_char_p, D, I]
         if hasattr(lib, "dipref_derivative"):
@@ -160,6 +162,24 @@ def gauss_iir(x, sigmas, orders=(0,), bc=("mirror",), filter_orders=(), method="
                                  IIR_METHODS[method], truncation, nthreads)
         if st != 0:
             raise RuntimeError(f"oracle gauss_iir failed ({st})")
+    return out
+
+
+def ref_gauss_ft(x, sigmas, orders=(0,), bc=(), truncation=3.0, out_dtype=None):
+    """The reference's dip::GaussFT(x, sigmas, orders, truncation, "spatial", "spatial",
+    bc) on a C-contiguous numpy array (arguments in dip axis order)."""
+    x = np.ascontiguousarray(x)
+    out_dtype = np.dtype(out_dtype) if out_dtype is not None else _flex(x.dtype)
+    out = np.zeros(x.shape, out_dtype)
+    sizes = _sizes(x)
+    sig = np.ascontiguousarray(list(sigmas), np.float64)
+    ords = np.ascontiguousarray(list(orders), np.int32)
+    bcl = list(bc) if not isinstance(bc, str) else [bc]
+    arr = (ctypes.c_char_p * max(1, len(bcl)))(*[b.encode() for b in bcl])
+    st = ref().dipref_gauss_ft(_ptr(x), DT[x.dtype], _ptr(out), DT[out_dtype], x.ndim, _ptr(sizes), _ptr(sig),
+                               len(sig), _ptr(ords), len(ords), arr, len(bcl), truncation)
+    if st != 0:
+        raise RuntimeError(ref().dipref_last_error().decode())
     return out
 
 

@@ -164,8 +164,8 @@ def test_errors(cuda):
         dip.GaussBatch([x], [torch.empty_like(x)], [(1.0, 1.0)], boundary="first order")
     with pytest.raises(dip.DipError):
         dip.gauss_fir_backward(x, torch.ones_like(x), [1.0], [0], ["third order"])
-    with pytest.raises(NotImplementedError):
-        dip.Gauss(x, [0.5])
+    # "best" with sigma < 0.8 runs the FT method with the default boundary condition
+    assert torch.equal(dip.Gauss(x, [0.5]), dip.GaussFT(x, [0.5], [0], 3.0, "spatial", "spatial", ["default"]))
 
 
 @pytest.mark.parametrize("case", cases.EXTRAP_CASES, ids=[c[0] for c in cases.EXTRAP_CASES])

@@ -11,6 +11,7 @@
 import hashlib
 import json
 import os
+import sys
 
 import numpy as np
 import pytest
@@ -423,3 +424,69 @@ def test_oracle_derivative_iir_matches_reference(case):
     got = oc.derivative(fn, x, sig, bc, None, 3.0, iir, tensor_iir)
     assert got.shape == exp.shape and got.dtype == exp.dtype
     assert np.array_equal(got.view(np.uint8), exp.view(np.uint8))
+
+
+# ---------------------------------------------------------------------------
+# dip::GaussFT (numpy restatement in oracle/gauss_ft_oracle.py)
+# ---------------------------------------------------------------------------
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "oracle"))
+import gauss_ft_oracle as fto  # noqa: E402
+
+FT = np.load(os.path.join(HERE, "golden", "ft_cases.npz"))
+
+
+def ft_tolerance(dtype):
+    """Relative to the largest output magnitude: FFT rounding in single precision for
+    8-bit / sfloat images, double for dfloat."""
+    return 1e-5 if np.dtype(dtype) != np.float64 else 1e-12
+
+
+@pytest.mark.parametrize("case", cases.FT_CASES, ids=[c[0] for c in cases.FT_CASES])
+def test_ft_oracle_matches_reference(case):
+    """The FT oracle against the reference's own dip::GaussFT outputs (tolerance: the
+    FFTs round differently), including odd derivative orders on even sizes, where the
+    reference's inverse takes the non-positive half of the spectrum."""
+    name, shape, dt, sig, ords, bc, trunc = case
+    x = FT[name + "__in"]
+    exp = FT[name + "__out"]
+    got = fto.gauss_ft(x, sig, ords, bc, trunc)
+    assert got.dtype == exp.dtype and got.shape == exp.shape
+    assert np.max(np.abs(got.astype(np.float64) - exp)) <= ft_tolerance(exp.dtype) * np.max(np.abs(exp))
+
+
+def test_ft_oracle_properties():
+    """What the mathematics fixes: the response to a constant image is that constant
+    (order 0, unit DC gain), filtering is shift invariant (periodic, any length),
+    sigma 0 with order 0 is the identity, and for sigma 2 the FT Gaussian equals the
+    sampled FIR Gaussian to within the FIR's own truncation and sampling error."""
+    c = np.full((20, 30), 7.0)
+    assert np.allclose(fto.gauss_ft(c, [1.3], [0], ["mirror"]), 7.0, rtol=1e-12)
+    for n in (9, 10, 41):
+        base = np.zeros(n)
+        base[0] = 1.0
+        r0 = fto.gauss_ft(base, [0.6], [1], [])
+        for s in (1, n // 2, n - 1):
+            r = fto.gauss_ft(np.roll(base, s), [0.6], [1], [])
+            assert np.allclose(r, np.roll(r0, s), atol=1e-12)
+    x = np.random.default_rng(3).random((12, 17))
+    assert np.array_equal(fto.gauss_ft(x, [0.0], [0], ["mirror"]), x)
+    y_ft = fto.gauss_ft(x.astype(np.float64), [2.0], [0], ["mirror"])
+    y_fir = oc.gauss_fir(x.astype(np.float64), [2.0], [0], ["mirror"])
+    assert np.max(np.abs(y_ft - y_fir)) < 2e-3
+
+
+@pytest.mark.skipif(not oc.ref_available(), reason="reference library not built")
+def test_ft_reference_odd_length_defect():
+    """Recorded behaviour of the reference: along an odd-length real-to-complex dim its
+    output is not shift invariant (an impulse at the edge of a periodic line comes out
+    distorted, one at the centre does not), so odd lengths are compared with the
+    oracle's exact convolution instead (FT_ODD_CASES)."""
+    n = 9
+    a = np.zeros(n, np.float32)
+    a[0] = 1.0
+    b = np.zeros(n, np.float32)
+    b[n // 2] = 1.0
+    ra = oc.ref_gauss_ft(a, [0.5], [0], [])
+    rb = oc.ref_gauss_ft(b, [0.5], [0], [])
+    assert np.max(np.abs(np.roll(rb, -(n // 2)) - ra)) > 0.05  # not a shifted copy
+    assert np.allclose(rb, fto.gauss_ft(b, [0.5], [0], []), atol=1e-6)  # the centre one is right
