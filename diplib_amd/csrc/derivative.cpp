@@ -95,20 +95,35 @@ struct Scratch {
 
 // The Gaussian derivatives of `in` with the given per-dim orders (one list of nd
 // orders per image), each into `dst[k]`.
-int gauss(bool iir, const dipamd_image *in, const dipamd_image *out, const double *sig, const int32_t *orders,
+// One dip::Gauss call with the method the flags select: 0 FIR, 1 IIR, 2 FT.
+int gauss(int method, const dipamd_image *in, const dipamd_image *out, const double *sig, const int32_t *orders,
           int32_t norders, const int32_t *bc, int32_t nbc, double truncation, uint32_t flags, void *stream) {
-    if (iir)
+    if (method == 1)
         return dipamd_gauss_iir(in, out, sig, in->ndim, orders, norders, bc, nbc, nullptr, 0,
                                 DIPAMD_IIR_DISCRETE_TIME_FIT, truncation, 0, stream);
+    if (method == 2) {
+        const int32_t mirror = DIPAMD_BC_SYMMETRIC_MIRROR;  // dip::Gauss's FT default (derivative.cpp:74-77)
+        return nbc == 0 ? dipamd_gauss_ft(in, out, sig, in->ndim, orders, norders, &mirror, 1, truncation, 0, stream)
+                        : dipamd_gauss_ft(in, out, sig, in->ndim, orders, norders, bc, nbc, truncation, 0, stream);
+    }
     return dipamd_gauss_fir(in, out, sig, in->ndim, orders, norders, bc, nbc, truncation,
-                            flags & ~static_cast<uint32_t>(DIPAMD_GAUSS_IIR | DIPAMD_TENSOR_IIR), stream);
+                            flags & static_cast<uint32_t>(DIPAMD_FAST | DIPAMD_NO_FUSION | DIPAMD_FORCE_FUSION),
+                            stream);
+}
+
+int derivative_method(uint32_t flags) {
+    return (flags & DIPAMD_GAUSS_FT) ? 2 : ((flags & DIPAMD_GAUSS_IIR) ? 1 : 0);
+}
+int tensor_method(uint32_t flags) {
+    return (flags & DIPAMD_TENSOR_FT) ? 2 : ((flags & DIPAMD_TENSOR_IIR) ? 1 : 0);
 }
 
 int derivatives(const dipamd_image *in, const std::vector<std::vector<int32_t>> &orders, const dipamd_image *dst,
                 const double *sig, const int32_t *bc, int32_t nbc, double truncation, uint32_t flags, void *stream) {
-    const bool iir = (flags & DIPAMD_GAUSS_IIR) != 0;
+    const int method = derivative_method(flags);
     for (size_t k = 0; k < orders.size(); ++k) {
-        const int st = gauss(iir, in, &dst[k], sig, orders[k].data(), in->ndim, bc, nbc, truncation, flags, stream);
+        const int st = gauss(method, in, &dst[k], sig, orders[k].data(), in->ndim, bc, nbc, truncation, flags,
+                             stream);
         if (st != DIPAMD_OK) return st;
     }
     return DIPAMD_OK;
@@ -273,8 +288,7 @@ int dipamd_structure_tensor(const dipamd_image *in, const dipamd_image *outs, in
         return st;
     if ((st = combine(5, buf_dt == DIPAMD_DT_DFLOAT, g, n, outs, nouts, scr, in)) != DIPAMD_OK) return st;
     for (int k = 0; k < nouts; ++k) {
-        st = gauss((flags & DIPAMD_TENSOR_IIR) != 0, &outs[k], &outs[k], tsig, nullptr, 0, bc, nbc, truncation, flags,
-                   stream);
+        st = gauss(tensor_method(flags), &outs[k], &outs[k], tsig, nullptr, 0, bc, nbc, truncation, flags, stream);
         if (st != DIPAMD_OK) return st;
     }
     return DIPAMD_OK;

@@ -18,18 +18,21 @@ from .gauss import (Gauss, _as_list, _bc_codes, _flex_torch, _stream_handle, _to
                     image_struct)
 
 
-def _gauss_method(method, sigmas, orders):
+def _gauss_method(method, sigmas, orders, tensor=False):
     """The Gaussian method the derivatives run (dip::Derivative passes `method` on to
-    dip::Gauss, derivative.cpp:111-114): True for IIR, False for FIR."""
+    dip::Gauss, derivative.cpp:111-114), as C ABI flags: 0 for FIR, the IIR or FT flag
+    otherwise (the structure tensor's smoothing has flags of its own)."""
     m = "best" if method == "gauss" else (method[5:] if method[:5] == "gauss" else method)
     if m == "best":
         m = best_gauss_method(sigmas, orders)
     if m in ("FIR", "fir"):
-        return False
+        return 0
     if m in ("IIR", "iir"):
-        return True
-    if m in ("FT", "ft", "finitediff"):
-        raise NotImplementedError(f"method '{m}' is outside this library (Gaussian FIR and IIR only)")
+        return _lib.TENSOR_IIR if tensor else _lib.GAUSS_IIR
+    if m in ("FT", "ft"):
+        return _lib.TENSOR_FT if tensor else _lib.GAUSS_FT
+    if m == "finitediff":
+        raise NotImplementedError("method 'finitediff' is outside this library (Gaussian methods only)")
     raise _lib.DipError(_lib.E_PARAMETER, "Invalid flag: " + method)
 
 
@@ -45,7 +48,7 @@ def _diff_dims(x, sigmas, process):
     return sig, proc, [d for d in range(nd) if proc[d] and sizes[d] > 1 and sig[d] > 0.0]
 
 
-def _run(function, x, outs, sig, proc, bc, truncation, fast, iir=False):
+def _run(function, x, outs, sig, proc, bc, truncation, fast, method_flags=0):
     nd = x.dim()
     bcs = _bc_codes(bc)
     imgs = (_lib.DipamdImage * len(outs))(*[image_struct(o) for o in outs])
@@ -56,7 +59,7 @@ def _run(function, x, outs, sig, proc, bc, truncation, fast, iir=False):
     with torch.cuda.device(x.device):
         check(lib().dipamd_derivative(function, ctypes.byref(a_in), imgs, len(outs), s_arr, nd, b_arr, len(bcs),
                                       p_arr, nd, float(truncation),
-                                      (_lib.FAST if fast else _lib.EXACT) | (_lib.GAUSS_IIR if iir else 0),
+                                      (_lib.FAST if fast else _lib.EXACT) | method_flags,
                                       _stream_handle(x.device)))
 
 
@@ -65,10 +68,10 @@ def _tensor_result(function, in_, sigmas, method, boundaryCondition, process, tr
     sig, proc, dims = _diff_dims(x, sigmas, process)
     if not dims:
         raise _lib.DipError(_lib.E_PARAMETER, "Dimensionality not supported")
-    iir = _gauss_method(method, sig, [2] * x.dim() if function == _lib.FN_HESSIAN else [1] * x.dim())
+    mflags = _gauss_method(method, sig, [2] * x.dim() if function == _lib.FN_HESSIAN else [1] * x.dim())
     n = check(lib().dipamd_derivative_outputs(function, len(dims)))
     planes = torch.empty((n,) + tuple(x.shape), dtype=_flex_torch(x.dtype), device=x.device)
-    _run(function, x, [planes[i] for i in range(n)], sig, proc, boundaryCondition, truncation, fast, iir)
+    _run(function, x, [planes[i] for i in range(n)], sig, proc, boundaryCondition, truncation, fast, mflags)
     y = planes.movedim(0, -1)
     return y.cpu().numpy() if was_np else y
 
@@ -78,9 +81,9 @@ def _scalar_result(function, in_, sigmas, method, boundaryCondition, process, tr
     sig, proc, dims = _diff_dims(x, sigmas, process)
     if not dims:
         raise _lib.DipError(_lib.E_PARAMETER, "Dimensionality not supported")
-    iir = _gauss_method(method, sig, [2] * x.dim())
+    mflags = _gauss_method(method, sig, [2] * x.dim())
     y = torch.empty(x.shape, dtype=_flex_torch(x.dtype), device=x.device) if out is None else out
-    _run(function, x, [y], sig, proc, boundaryCondition, truncation, fast, iir)
+    _run(function, x, [y], sig, proc, boundaryCondition, truncation, fast, mflags)
     return y.cpu().numpy() if (was_np and out is None) else y
 
 
@@ -155,8 +158,8 @@ def StructureTensor(in_, mask=None, gradientSigmas=(1.0,), tensorSigmas=(5.0,), 
     dims = [d for d in range(nd) if sizes[d] > 1 and gs[d] > 0.0]
     if not dims:
         raise _lib.DipError(_lib.E_PARAMETER, "Dimensionality not supported")
-    flags = (_lib.FAST if fast else _lib.EXACT) | (_lib.GAUSS_IIR if _gauss_method(method, gs, [1] * nd) else 0) | \
-        (_lib.TENSOR_IIR if _gauss_method(method, ts, [0] * nd) else 0)
+    flags = (_lib.FAST if fast else _lib.EXACT) | _gauss_method(method, gs, [1] * nd) | \
+        _gauss_method(method, ts, [0] * nd, tensor=True)
     n = check(lib().dipamd_derivative_outputs(_lib.FN_STRUCTURE_TENSOR, len(dims)))
     planes = torch.empty((n,) + tuple(x.shape), dtype=_flex_torch(x.dtype), device=x.device)
     bcs = _bc_codes(boundaryCondition)

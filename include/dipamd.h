@@ -81,7 +81,11 @@ enum {
      * above 10, runs -- derivative.cpp:35-58, 111-114); DIPAMD_TENSOR_IIR: the structure
      * tensor's smoothing too (its method resolves with the tensor sigmas). */
     DIPAMD_GAUSS_IIR = 8,
-    DIPAMD_TENSOR_IIR = 16
+    DIPAMD_TENSOR_IIR = 16,
+    /* The same through dip::GaussFT ("FT", or "best" with a sigma below 0.8; an empty
+     * boundary condition array means "mirror" there, as dip::Gauss passes it). */
+    DIPAMD_GAUSS_FT = 32,
+    DIPAMD_TENSOR_FT = 64
 };
 
 /* Error codes. */

@@ -382,8 +382,33 @@ inline Image GaussIIR(Image const &in, FloatArray sigmas = {1.0}, UnsignedArray 
     return out;
 }
 
-/* dip::Gauss (linear.h:452-472; dispatch derivative.cpp:35-86).  The FIR and IIR
- * methods are in this library: "FT" (and "best" resolving to it) throws. */
+/* dip::GaussFT (linear.h:355-364) for real images, spatial domain in and out. */
+inline void GaussFT(Image const &in, Image &out, FloatArray sigmas = {1.0}, UnsignedArray derivativeOrder = {0},
+                    dfloat truncation = 3, String const &inRepresentation = "spatial",
+                    String const &outRepresentation = "spatial", StringArray const &boundaryCondition = {}) {
+    if (!in.IsForged()) throw ParameterError("Image is not forged");
+    if (inRepresentation != "spatial" || outRepresentation != "spatial")
+        throw ParameterError("GaussFT in or out of the frequency domain is not provided by this library");
+    Image src = in;
+    detail::PrepareOutput(in, out);
+    std::vector<int32_t> ord(derivativeOrder.begin(), derivativeOrder.end());
+    std::vector<int32_t> bc = detail::BoundaryCodes(boundaryCondition);
+    dipamd_image a = src.View(), b = out.View();
+    detail::Check(dipamd_gauss_ft(&a, &b, sigmas.data(), static_cast<int32_t>(sigmas.size()), ord.data(),
+                                  static_cast<int32_t>(ord.size()), bc.data(), static_cast<int32_t>(bc.size()),
+                                  truncation, 0, dipamd::CurrentStream()));
+}
+inline Image GaussFT(Image const &in, FloatArray sigmas = {1.0}, UnsignedArray derivativeOrder = {0},
+                     dfloat truncation = 3, String const &inRepresentation = "spatial",
+                     String const &outRepresentation = "spatial", StringArray const &boundaryCondition = {}) {
+    Image out;
+    GaussFT(in, out, std::move(sigmas), std::move(derivativeOrder), truncation, inRepresentation, outRepresentation,
+            boundaryCondition);
+    return out;
+}
+
+/* dip::Gauss (linear.h:452-472; dispatch derivative.cpp:35-86): "FIR", "IIR", "FT" and
+ * "best". */
 inline void Gauss(Image const &in, Image &out, FloatArray sigmas = {1.0}, UnsignedArray derivativeOrder = {0},
                   String const &c_method = "best", StringArray const &boundaryCondition = {}, dfloat truncation = 3) {
     String method = c_method.substr(0, 5) == "gauss" ? c_method.substr(5) : c_method;
@@ -406,8 +431,12 @@ inline void Gauss(Image const &in, Image &out, FloatArray sigmas = {1.0}, Unsign
                  truncation);
         return;
     }
-    if (method == "FT" || method == "ft")
-        throw ParameterError("dip::Gauss method \"" + method + "\" is not provided by this library (FIR and IIR only)");
+    if (method == "FT" || method == "ft") {
+        // the empty condition is not GaussFT's default (derivative.cpp:74-77)
+        GaussFT(in, out, std::move(sigmas), std::move(derivativeOrder), truncation, "spatial", "spatial",
+                boundaryCondition.empty() ? StringArray{"default"} : boundaryCondition);
+        return;
+    }
     throw ParameterError("Invalid flag: " + c_method);
 }
 inline Image Gauss(Image const &in, FloatArray sigmas = {1.0}, UnsignedArray derivativeOrder = {0},
@@ -418,8 +447,8 @@ inline Image Gauss(Image const &in, FloatArray sigmas = {1.0}, UnsignedArray der
 }
 
 /* The derivative family (derivative.cpp:120-521; linear.h:666, 697, 862, 904) with the
- * Gaussian FIR or IIR method ("best" resolves to one of them, or throws for FT).  This
- * shim has no tensor images: Gradient and
+ * Gaussian FIR, IIR or FT method ("best" resolves to one of them).  This shim has no
+ * tensor images: Gradient and
  * Hessian return their tensor elements as an array of images, in dip's element order
  * (Hessian: the diagonal first, then the upper triangle column by column). */
 using ImageArray = std::vector<Image>;
@@ -438,8 +467,9 @@ inline uint32_t GaussMethodFlag(String const &c_method, FloatArray const &sigmas
     }
     if (method == "FIR" || method == "fir") return 0;
     if (method == "IIR" || method == "iir") return DIPAMD_GAUSS_IIR;
-    if (method == "FT" || method == "ft" || method == "finitediff")
-        throw ParameterError("method \"" + method + "\" is not provided by this library (Gaussian FIR and IIR only)");
+    if (method == "FT" || method == "ft") return DIPAMD_GAUSS_FT;
+    if (method == "finitediff")
+        throw ParameterError("method \"finitediff\" is not provided by this library (Gaussian methods only)");
     throw ParameterError("Invalid flag: " + c_method);
 }
 inline void RunDerivative(int32_t fn, Image const &in, std::vector<dipamd_image> &outs, FloatArray const &sigmas,
@@ -545,8 +575,9 @@ inline void StructureTensor(Image const &in, Image const &mask, ImageArray &out,
                             StringArray const &boundaryCondition = {}, dfloat truncation = 3) {
     if (!in.IsForged()) throw ParameterError("Image is not forged");
     if (mask.IsForged()) throw ParameterError("StructureTensor with a mask is not provided by this library");
+    const uint32_t tf = detail::GaussMethodFlag(method, tensorSigmas);
     const uint32_t flags = detail::GaussMethodFlag(method, gradientSigmas) |
-                           (detail::GaussMethodFlag(method, tensorSigmas) ? DIPAMD_TENSOR_IIR : 0u);
+                           (tf == DIPAMD_GAUSS_IIR ? DIPAMD_TENSOR_IIR : (tf == DIPAMD_GAUSS_FT ? DIPAMD_TENSOR_FT : 0u));
     const int n = dipamd_derivative_outputs(DIPAMD_FN_STRUCTURE_TENSOR,
                                             detail::DifferentiatedDims(in, gradientSigmas, {}));
     if (n < 0) detail::Check(n);

@@ -237,3 +237,29 @@ FT_ODD_CASES = [
     ("ftodd_f32_41_noext", (41,), "float32", [0.5], [0], [], 3.0),
     ("ftodd_f32_45x5_noext_o1", (5, 45), "float32", [0.8], [1, 0], [], 3.0),
 ]
+
+# The derivative family and the structure tensor through the FT method ("best" with a sigma
+# below 0.8, or "gaussft"); compared with a tolerance (FFT rounding).  Dgg is left out: its
+# division by the squared gradient magnitude amplifies rounding where the gradient vanishes.
+DERIV_FT_CASES = []
+for fn in ("gradient", "gradmag", "hessian", "laplace", "structure:0.6", "structure:3"):
+    t = fn.replace(":", "").replace(".", "")
+    DERIV_FT_CASES += [
+        (f"{t}_f32_64x48_s05_best", fn, (48, 64), "float32", [0.5], ["mirror"], "best"),
+        (f"{t}_u8_62x38_s2_ft", fn, (38, 62), "uint8", [2.0], ["periodic"], "gaussft"),
+        (f"{t}_f64_42x30_s07_best", fn, (30, 42), "float64", [0.7], ["mirror"], "best"),
+    ]
+
+
+def method_flags(fn, sigmas, method):
+    """Which Gaussian the derivatives (first) and the structure tensor's smoothing
+    (second) run, "fir" / "iir" / "ft", as dip::Gauss resolves `method`."""
+    def one(sig):
+        m = "best" if method == "gauss" else (method[5:] if method.startswith("gauss") else method)
+        if m == "best":
+            if any(0 < s < 0.8 for s in sig):
+                return "ft"
+            return "iir" if any(s > 10 for s in sig) else "fir"
+        return m
+    tsig = [float(fn.split(":")[1])] if fn.startswith("structure:") else sigmas
+    return one(sigmas), one(tsig)

@@ -15,6 +15,8 @@ Outputs (all small):
   deriv_iir_cases.npz  the derivative family and structure tensor with the IIR method
                      (tests/cases.py:DERIV_IIR_CASES)
   ft_cases.npz       input and dip::GaussFT output of tests/cases.py:FT_CASES
+  deriv_ft_cases.npz the derivative family and structure tensor with the FT method
+                     (tests/cases.py:DERIV_FT_CASES)
   taps.npz           MakeHalfGaussian(sigma, order, 3, sfloat/dfloat) for TAP_SIGMAS x orders 0..3
   small_cases.npz    input and dip::Gauss output of every case in tests/cases.py:SMALL_CASES
   large_cases.json   SHA-256 of the exact output bytes + sampled pixels for LARGE_CASES
@@ -87,6 +89,15 @@ def write_ft_cases(fname="ft_cases.npz", seed0=9500):
     np.savez_compressed(os.path.join(HERE, fname), **arrs)
 
 
+def write_derivative_ft_cases(fname="deriv_ft_cases.npz", seed0=9900):
+    arrs = {}
+    for i, (name, fn, shape, dt, sig, bc, method) in enumerate(cases.DERIV_FT_CASES):
+        x = cases.make_input(shape, dt, seed=seed0 + i)
+        arrs[name + "__in"] = x
+        arrs[name + "__out"] = oc.ref_derivative(fn, x, sig, bc, method)
+    np.savez_compressed(os.path.join(HERE, fname), **arrs)
+
+
 def main():
     assert oc.ref_available(), "build oracle/_ref/libdipref.so first (make -f oracle/ref.mk)"
     only = sys.argv[sys.argv.index("--only") + 1] if "--only" in sys.argv else None
@@ -94,6 +105,7 @@ def main():
         write_cases(cases.EXTRAP_CASES, "extrap_cases.npz", 3000)
     if only in (None, "ft"):
         write_ft_cases()
+        write_derivative_ft_cases()
     if only in (None, "iir"):
         write_iir_cases()
         write_derivative_iir_cases()

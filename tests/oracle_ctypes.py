@@ -4,6 +4,7 @@ imported by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg only
 """
 import ctypes
 import os
+import sys
 
 import numpy as np
 
@@ -285,7 +286,14 @@ def ref_derivative(which, x, sigmas, bc=("mirror",), method="gaussfir", truncati
     return out[:n.value].copy()
 
 
-def derivative(which, x, sigmas, bc=("mirror",), process=None, truncation=3.0, iir=False, tensor_iir=False):
+def _ft_oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import gauss_ft_oracle
+    return gauss_ft_oracle
+
+
+def derivative(which, x, sigmas, bc=("mirror",), process=None, truncation=3.0, iir=False, tensor_iir=False,
+               ft=False, tensor_ft=False):
     """The oracle's restatement of the derivative family (derivative.cpp:120-418): Gauss
     derivatives from the oracle (FIR, or the IIR Gaussian with `iir`; the structure
     tensor's smoothing with `tensor_iir`), combined in the reference's order and type
@@ -296,6 +304,8 @@ def derivative(which, x, sigmas, bc=("mirror",), process=None, truncation=3.0, i
     sizes = list(reversed(x.shape))
     dims = [d for d in range(nd) if proc[d] and sizes[d] > 1 and sig[d] > 0]
     def d_of(order):
+        if ft:  # dip::Gauss's FT branch: an empty condition means "default"
+            return _ft_oracle().gauss_ft(x, sig, order, list(bc) or ["default"], truncation)
         return gauss_iir(x, sig, order, bc, (), "discrete time fit", truncation) if iir else \
             gauss_fir(x, sig, order, bc, truncation)
     def unit(pairs):
@@ -327,8 +337,8 @@ def derivative(which, x, sigmas, bc=("mirror",), process=None, truncation=3.0, i
         return np.sqrt(out)[None]
     if which in ("dgg", "laplace+dgg", "laplace-dgg"):
         # derivative.cpp:424-484, image operation by image operation
-        g = derivative("gradient", x, sigmas, bc, process, truncation, iir)
-        H = derivative("hessian", x, sigmas, bc, process, truncation, iir)
+        g = derivative("gradient", x, sigmas, bc, process, truncation, iir, ft=ft)
+        H = derivative("hessian", x, sigmas, bc, process, truncation, iir, ft=ft)
         n = g.shape[0]
         one = g.dtype.type(1)
         two = g.dtype.type(2)
@@ -356,13 +366,15 @@ def derivative(which, x, sigmas, bc=("mirror",), process=None, truncation=3.0, i
     if which.startswith("structure:"):
         # structure_tensor.cpp:53-56: g g^T (a sum started at 0), then Gauss per element
         tsig = float(which.split(":")[1])
-        g = derivative("gradient", x, sigmas, bc, None, truncation, iir)
+        g = derivative("gradient", x, sigmas, bc, None, truncation, iir, ft=ft)
         n = g.shape[0]
         zero = g.dtype.type(0)
         el = [zero + g[c] * g[c] for c in range(n)]
         for c in range(1, n):
             for r in range(c):
                 el.append(zero + g[r] * g[c])
+        if tensor_ft:
+            return np.stack([_ft_oracle().gauss_ft(e, [tsig], [0], list(bc) or ["default"], truncation) for e in el])
         if tensor_iir:
             return np.stack([gauss_iir(e, [tsig], [0], bc, (), "discrete time fit", truncation) for e in el])
         return np.stack([gauss_fir(e, [tsig], [0], bc, truncation) for e in el])

@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 DERIV = np.load(os.path.join(HERE, "golden", "deriv_cases.npz"))
 DERIV2 = np.load(os.path.join(HERE, "golden", "deriv2_cases.npz"))
 DERIV_IIR = np.load(os.path.join(HERE, "golden", "deriv_iir_cases.npz"))
+DERIV_FT = np.load(os.path.join(HERE, "golden", "deriv_ft_cases.npz"))
 FN = {"gradient": dip.Gradient, "gradmag": dip.GradientMagnitude, "hessian": dip.Hessian, "laplace": dip.Laplace,
       "dgg": dip.Dgg, "laplace+dgg": dip.LaplacePlusDgg, "laplace-dgg": dip.LaplaceMinusDgg}
 
@@ -66,8 +67,10 @@ def test_derivative_family_process_and_errors(cuda):
     assert np.array_equal(np.moveaxis(g.cpu().numpy(), -1, 0), exp)
     with pytest.raises(dip.DipError):
         dip.Gradient(x, [0.0])  # no dim to differentiate
+    # "best" with sigma < 0.8 resolves to FT: the same as the FT method named
+    assert torch.equal(dip.Laplace(x, [0.5]), dip.Laplace(x, [0.5], "FT"))
     with pytest.raises(NotImplementedError):
-        dip.Laplace(x, [0.5])  # "best" resolves to FT
+        dip.Laplace(x, [2.0], "finitediff")
     # a protected output of another type is refused for the combined forms
     with pytest.raises(dip.DipError):
         dip.Laplace(x, [2.0], "FIR", out=torch.empty((30, 40), dtype=torch.float64, device=cuda))
@@ -112,3 +115,20 @@ def test_derivative_iir_bitexact(cuda, case):
     assert got.shape == exp.shape and got.dtype == exp.dtype
     assert np.array_equal(got.view(np.uint8), exp.view(np.uint8)), \
         f"{name}: max abs diff {np.abs(got.astype(np.float64) - exp.astype(np.float64)).max()}"
+
+
+@pytest.mark.parametrize("case", cases.DERIV_FT_CASES, ids=[c[0] for c in cases.DERIV_FT_CASES])
+def test_derivative_ft_vs_reference(cuda, case):
+    """The derivative family and the structure tensor through the FT Gaussian ("best"
+    with sigma < 0.8, "gaussft") against the reference: max |d| <= 1e-5 max |ref|
+    (single-precision transforms), 1e-12 for dfloat."""
+    name, fn, shape, dt, sig, bc, method = case
+    x = DERIV_FT[name + "__in"]
+    exp = DERIV_FT[name + "__out"]
+    y = _call(fn, torch.from_numpy(x).to(cuda), sig, method, bc)
+    torch.cuda.synchronize()
+    got = _elements(fn, y)
+    assert got.shape == exp.shape and got.dtype == exp.dtype
+    tol = 1e-5 if exp.dtype != np.float64 else 1e-12
+    err = np.max(np.abs(got.astype(np.float64) - exp)) / np.max(np.abs(exp))
+    assert err <= tol, f"{name}: {err:.3e}"

@@ -490,3 +490,19 @@ def test_ft_reference_odd_length_defect():
     rb = oc.ref_gauss_ft(b, [0.5], [0], [])
     assert np.max(np.abs(np.roll(rb, -(n // 2)) - ra)) > 0.05  # not a shifted copy
     assert np.allclose(rb, fto.gauss_ft(b, [0.5], [0], []), atol=1e-6)  # the centre one is right
+
+
+DERIV_FT = np.load(os.path.join(HERE, "golden", "deriv_ft_cases.npz"))
+
+
+@pytest.mark.parametrize("case", cases.DERIV_FT_CASES, ids=[c[0] for c in cases.DERIV_FT_CASES])
+def test_oracle_derivative_ft_matches_reference(case):
+    """The derivative family and the structure tensor through the FT method ("best"
+    with sigma < 0.8, "gaussft") against the reference, to FFT rounding."""
+    name, fn, shape, dt, sig, bc, method = case
+    x = DERIV_FT[name + "__in"]
+    exp = DERIV_FT[name + "__out"]
+    m, tm = cases.method_flags(fn, sig, method)
+    got = oc.derivative(fn, x, sig, bc, None, 3.0, m == "iir", tm == "iir", m == "ft", tm == "ft")
+    assert got.shape == exp.shape and got.dtype == exp.dtype
+    assert np.max(np.abs(got.astype(np.float64) - exp)) <= ft_tolerance(exp.dtype) * np.max(np.abs(exp))
