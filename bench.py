@@ -100,8 +100,9 @@ def main():
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--case", nargs=2, metavar=("SIGMA", "ORDERS"),
                     help="profiling mode: run one case of the sweep (e.g. 16 00; or 'vol mirror' for the "
-                         "512x512x256 sfloat volume, sigma (2,2,1)) --steps times, then a 256 MiB device copy "
-                         "as the calibration kernel for the traffic counters")
+                         "512x512x256 sfloat volume, sigma (2,2,1); or 'iir 00' for dip::GaussIIR sigma 16, "
+                         "'ft 00' for dip::GaussFT sigma 0.5, on the 8192^2 input) --steps times, then a "
+                         "256 MiB device copy as the calibration kernel for the traffic counters")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; the multi-GPU (sharded) path is a design only "
@@ -134,16 +135,23 @@ def main():
     x_np = cases.make_input((SIZE, SIZE), "uint8", seed=2)
     x = torch.from_numpy(x_np).to(dev)
     if a.case:
-        s, o = float(a.case[0]), [int(c) for c in a.case[1]]
+        kind = a.case[0] if a.case[0] in ("iir", "ft") else "fir"
+        s = {"iir": 16.0, "ft": 0.5}.get(kind) or float(a.case[0])
+        o = [int(c) for c in a.case[1]]
         y = torch.empty((SIZE, SIZE), dtype=torch.float32, device=dev)
         for _ in range(a.steps):
-            dip.GaussFIR(x, [s], o, ["mirror"], 3.0, out=y)
+            if kind == "iir":
+                dip.GaussIIR(x, [s], o, ["mirror"], out=y)
+            elif kind == "ft":
+                dip.GaussFT(x, [s], o, 3.0, "spatial", "spatial", ["mirror"], out=y)
+            else:
+                dip.GaussFIR(x, [s], o, ["mirror"], 3.0, out=y)
         src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev)  # 256 MiB read + 256 MiB written
         dst = torch.empty_like(src)
         for _ in range(a.steps):
             dst.copy_(src)
         torch.cuda.synchronize()
-        log(f"case sigma={s} orders={o}: {a.steps} calls, then {a.steps} copies of 256 MiB")
+        log(f"case {kind} sigma={s} orders={o}: {a.steps} calls, then {a.steps} copies of 256 MiB")
         return
     outs = [torch.empty((SIZE, SIZE), dtype=torch.float32, device=dev) for _ in range(len(SIGMAS) * len(ORDERS))]
     combos = [(s, o) for s in SIGMAS for o in ORDERS]
