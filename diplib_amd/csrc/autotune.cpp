@@ -65,10 +65,13 @@ bool fits(int cfg, bool is_double, int rx, int ry) {
 
 }  // namespace
 
+// Largest radius the fused 2-D kernel takes: beyond 16 the two single passes (K1 line +
+// the strided sweeps) are as fast or faster on 8192^2 u8 (sigma 6: 0.306 vs 0.317 ms,
+// sigma 8 order 0: 0.352 vs 0.381 ms; sigma 4, radius 12-16: fused 0.20-0.21 vs 0.27-0.30).
 int fuse_max_radius() {
     static int r = [] {
         const char *v = std::getenv("DIPAMD_FUSE_MAX_RADIUS");
-        return v ? std::atoi(v) : 24;
+        return v ? std::atoi(v) : 16;
     }();
     return r;
 }
