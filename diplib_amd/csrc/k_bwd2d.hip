@@ -117,9 +117,17 @@ struct BwdGeom {
     static constexpr int Hp = GH | 1;         // column pitch (column-major g)
     static constexpr int NW = P + 2 * R;      // window length
     static constexpr int LP = ((W + 2 * R + P) / P) | 1;  // interleave line length
-    static constexpr int ROWF = P * LP;       // floats per interleaved row
     static constexpr int GROWS = (GH + 3) / 4;
     static constexpr int FOLDW = 2 * (R + 1);  // x fold terms per row: targets 0..R, nx-1-R..nx-1
+    // Row pitch of the interleaved rows: the x stage's wave reads four rows at once,
+    // sixteen consecutive words from each; a pitch of 16 (mod 32) words puts the four
+    // runs on disjoint banks (32 or 64 banks).  Unpadded when the padding would not fit.
+    static constexpr int ROWF0 = P * LP;
+    static constexpr int ROWFP = ((ROWF0 - 16 + 31) / 32) * 32 + 16;  // smallest >= ROWF0, = 16 mod 32
+    static constexpr size_t LDS_PADDED =
+        (static_cast<size_t>(W) * Hp + 3 * static_cast<size_t>(S) * ROWFP + static_cast<size_t>(S) * FOLDW) *
+        sizeof(float);
+    static constexpr int ROWF = LDS_PADDED <= 65536 ? ROWFP : ROWF0;  // floats per interleaved row
     static constexpr size_t lds_floats =
         static_cast<size_t>(W) * Hp + 3 * static_cast<size_t>(S) * ROWF + static_cast<size_t>(S) * FOLDW;
 };

@@ -55,6 +55,8 @@ def test_ft_odd_lengths_vs_oracle(cuda, case):
     ((64, 96, 80), np.float32, [0.7], [0, 0, 1], ["mirror"]),
     ((128, 160), np.float64, [1.5], [4, 0], ["mirror"]),
     ((256, 256), np.float32, [0.0], [2, 1], ["mirror"]),
+    ((30, 32), np.float32, [20.0], [0, 0], ["mirror"]),  # padding far larger than the image
+    ((30, 32), np.float32, [8.0], [2, 0], ["periodic"]),
 ])
 def test_ft_larger_vs_oracle(cuda, shape, dt, sig, ords, bc):
     x = cases.make_input(shape, dt, seed=9800 + len(shape))

@@ -41,6 +41,10 @@ def test_iir_bitexact(cuda, case):
     ((32, 64, 64), np.float32, [12.0, 12.0, 11.0], [0], "mirror"),
     ((64, 48, 40), np.float64, [11.0], [1, 0, 1], "add zeros"),
     ((300, 257), np.float32, [35.0], [3, 0], "third order"),
+    # borders far longer than the image: boundary walks that bounce many times
+    ((20, 30), np.float32, [100.0], [0, 0], "mirror"),
+    ((20, 30), np.float32, [60.0], [1, 0], "third order"),
+    ((20, 30), np.float64, [100.0], [4, 0], "add zeros"),
 ])
 def test_iir_larger_vs_oracle(cuda, shape, dt, sig, ords, bc):
     x = cases.make_input(shape, dt, seed=9100 + len(shape))
