@@ -147,12 +147,23 @@ int fused_backward_2d(const View &x, const View &g, const View &dx, int nd, cons
         if (x.sizes[d] < 2 * f.border + 1 || x.sizes[d] >= (int64_t(1) << 30)) return -100;
     }
     if (nd - 2 > kMaxOuter) return -100;
+    // the kernel addresses a plane through 32-bit byte offsets from its first sample
+    auto plane_bytes = [&](const View &v) -> int64_t {
+        if (v.strides[1] <= 0) return -1;
+        return ((x.sizes[1] - 1) * v.strides[1] + x.sizes[0]) * static_cast<int64_t>(sizeof(float));
+    };
+    const int64_t kMaxPlane = (int64_t(1) << 31) - 256;
+    const int64_t xb = plane_bytes(x), gb = plane_bytes(g), db = want_dx ? plane_bytes(dx) : 0;
+    if (xb < 0 || gb < 0 || db < 0 || xb > kMaxPlane || gb > kMaxPlane || db > kMaxPlane) return -100;
     BwdArgs a;
     std::memset(&a, 0, sizeof(a));
     a.x = static_cast<const float *>(x.p);
     a.g = static_cast<const float *>(g.p);
     a.dx = want_dx ? static_cast<float *>(dx.p) : nullptr;
     a.want_dx = want_dx ? 1 : 0;
+    a.xbytes = static_cast<int>(xb);
+    a.gbytes = static_cast<int>(gb);
+    a.dxbytes = static_cast<int>(db);
     a.nx = static_cast<int>(x.sizes[0]);
     a.ny = static_cast<int>(x.sizes[1]);
     a.syx = x.strides[1];
@@ -192,11 +203,23 @@ int fused_backward_2d(const View &x, const View &g, const View &dx, int nd, cons
     const int txc = bwd2d_tile_columns(rc);
     a.ntiles = (a.nx + txc - 1) / txc;
     const int64_t base = static_cast<int64_t>(a.ntiles) * nplanes;
+    // With a folding y condition the rows that receive folded terms get chunks of their
+    // own (whole steps from each edge), so that every other chunk runs without folds.
+    a.ytop = 0;
+    a.ybot = a.ny;
+    if (a.modey == kBwdFold) {
+        const int e = kBwdStep * ((a.ry + kBwdStep) / kBwdStep);  // S ceil((ry + 1) / S)
+        if (2 * e < a.ny) {
+            a.ytop = e;
+            a.ybot = a.ny - e;
+        }
+    }
+    const int64_t mid = a.ybot - a.ytop;
     int64_t want = std::max<int64_t>(1, (2048 + base - 1) / base);
-    int64_t chunk = (a.ny + want - 1) / want;
+    int64_t chunk = (mid + want - 1) / want;
     chunk = std::max<int64_t>(kBwdStep, ((chunk + kBwdStep - 1) / kBwdStep) * kBwdStep);
     a.chunk = static_cast<int>(chunk);
-    a.nchunks = static_cast<int>((a.ny + chunk - 1) / chunk);
+    a.nchunks = static_cast<int>((mid + chunk - 1) / chunk) + (a.ytop > 0 ? 2 : 0);
     const int64_t nblocks = base * a.nchunks;
     if (nblocks >= (int64_t(1) << 31)) return -100;
     if (bwd2d_lds_bytes(rc) > lds_limit_bytes()) return -100;

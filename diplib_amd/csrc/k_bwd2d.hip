@@ -27,6 +27,7 @@
 // consecutive words (no bank conflicts) at offsets that depend on the tap only.
 // Arithmetic is fused multiply-add: gradients are checked against FP64 references.
 #include <cstddef>
+#include <type_traits>
 
 #include "sep_kernels_impl.hpp"
 
@@ -105,6 +106,19 @@ __device__ __forceinline__ ctap *taps_here(size_t offset) {
     return q;
 }
 
+// Raw buffer resource over `bytes` bytes from `base` (no stride, no swizzle; the last
+// word is the data format CDNA expects for 32-bit raw accesses).  Every access made
+// through it lies inside the plane it describes.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const float *base, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base), static_cast<short>(0), bytes, 0x00020000);
+}
+__device__ __forceinline__ float buffer_ld(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ void buffer_st(float v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
+}
+
 }  // namespace
 
 template <int R>
@@ -121,13 +135,17 @@ struct BwdGeom {
     static constexpr int FOLDW = 2 * (R + 1);  // x fold terms per row: targets 0..R, nx-1-R..nx-1
     // Row pitch of the interleaved rows: the x stage's wave reads four rows at once,
     // sixteen consecutive words from each; a pitch of 16 (mod 32) words puts the four
-    // runs on disjoint banks (32 or 64 banks).  Unpadded when the padding would not fit.
+    // runs on disjoint banks (32 or 64 banks).  Unpadded when the padding would exceed
+    // 64 KB or leave fewer workgroups resident on a CU (160 KB of LDS) than without it:
+    // occupancy matters more than the conflicts.
     static constexpr int ROWF0 = P * LP;
     static constexpr int ROWFP = ((ROWF0 - 16 + 31) / 32) * 32 + 16;  // smallest >= ROWF0, = 16 mod 32
-    static constexpr size_t LDS_PADDED =
-        (static_cast<size_t>(W) * Hp + 3 * static_cast<size_t>(S) * ROWFP + static_cast<size_t>(S) * FOLDW) *
-        sizeof(float);
-    static constexpr int ROWF = LDS_PADDED <= 65536 ? ROWFP : ROWF0;  // floats per interleaved row
+    static constexpr size_t FIXED = static_cast<size_t>(W) * Hp + static_cast<size_t>(S) * FOLDW;
+    static constexpr size_t LDS_P = (FIXED + 3 * static_cast<size_t>(S) * ROWFP) * sizeof(float);
+    static constexpr size_t LDS_U = (FIXED + 3 * static_cast<size_t>(S) * ROWF0) * sizeof(float);
+    static constexpr size_t kCuLds = 160 * 1024;
+    static constexpr bool PAD = LDS_P <= 65536 && kCuLds / LDS_P >= kCuLds / LDS_U;
+    static constexpr int ROWF = PAD ? ROWFP : ROWF0;  // floats per interleaved row
     static constexpr size_t lds_floats =
         static_cast<size_t>(W) * Hp + 3 * static_cast<size_t>(S) * ROWF + static_cast<size_t>(S) * FOLDW;
 };
@@ -172,19 +190,43 @@ __global__ __launch_bounds__(kBlock) void k_bwd2d(const BwdArgs a) {
     float *DX = a.dx + odx;
     const int c0 = tile * TXC;        // image column of output column 0
     const int cs0 = c0 - R;           // image column of staged column 0
-    const int z0 = chunk * a.chunk;
-    const int zend = min(z0 + a.chunk, ny);
+    int z0, zend;
+    if (a.ytop > 0) {
+        if (chunk == 0) {
+            z0 = 0;
+            zend = a.ytop;
+        } else if (chunk == a.nchunks - 1) {
+            z0 = a.ybot;
+            zend = ny;
+        } else {
+            z0 = a.ytop + (chunk - 1) * a.chunk;
+            zend = min(z0 + a.chunk, a.ybot);
+        }
+    } else {
+        z0 = chunk * a.chunk;
+        zend = min(z0 + a.chunk, ny);
+    }
     const int tid = threadIdx.x, lane = tid & 63, wave = wave_id();
 
-    // staged column slots of this lane (lane, lane + 64): source column and factor of
-    // g (the adjoint's extension) and of x (the forward extension; 2 = the constant)
-    int gcol[2], xcol[2];
+    // Global memory is read and written through raw buffer resources over the plane:
+    // a lane's column is a loop-invariant byte offset and a row's offset a scalar, so
+    // no per-access 64-bit address arithmetic.
+    const __amdgpu_buffer_rsrc_t gres = plane_rsrc(G, a.gbytes);
+    const __amdgpu_buffer_rsrc_t xres = plane_rsrc(X, a.xbytes);
+    const int syg4 = static_cast<int>(a.syg) * 4, syx4 = static_cast<int>(a.syx) * 4;
+
+    // staged column slots of this lane (lane, lane + 64): source column (byte offset)
+    // and factor of g (the adjoint's extension) and of x (the forward extension; a
+    // constant column takes the constant's bits instead of the sample's)
+    int gvo[2], xvo[2];
     float gmul[2], xmul[2];
+    uint32_t xkeep[2], xcbits[2];
 #pragma unroll
     for (int sl = 0; sl < 2; ++sl) {
         const int j = cs0 + lane + 64 * sl;
         int gj = j, xj = j, m;
         float gm = 1.f, xm = 1.f;
+        uint32_t keep = 0xffffffffu, cb = 0u;
         if (j < 0 || j >= nx) {
             if (a.modex == kBwdPeriodic) {
                 gj = bc_fold(j, nx, a.bcx, &m);
@@ -196,72 +238,116 @@ __global__ __launch_bounds__(kBlock) void k_bwd2d(const BwdArgs a) {
             xj = bc_fold(j, nx, a.bcx, &m);
             if (xj < 0) {
                 xj = 0;
-                xm = 2.f;
+                keep = 0u;
+                cb = __builtin_bit_cast(uint32_t, a.cvalx);
             } else {
                 xm = static_cast<float>(m);
             }
         }
-        gcol[sl] = gj;
-        xcol[sl] = xj;
+        gvo[sl] = gj * 4;
+        xvo[sl] = xj * 4;
         gmul[sl] = gm;
         xmul[sl] = xm;
+        xkeep[sl] = keep;
+        xcbits[sl] = cb;
     }
+    // no staged column outside the line: the tile's g and x go to LDS unchanged
+    const bool tile_plain = cs0 >= 0 && cs0 + W <= nx;
+    // x stage: this thread's row of the step, its first output column, how many of its
+    // P outputs lie in the tile and the line, and dL/dx's byte offset of output 0
+    const int xrow = tid >> 4, xl = tid & 15;
+    const int cb = c0 + 8 * xl;
+    const int nv = max(0, min(P, min(TXC, nx - c0) - 8 * xl));
+    const __amdgpu_buffer_rsrc_t dres = plane_rsrc(DX, a.dxbytes);
+    const int sydx4 = static_cast<int>(a.sydx) * 4;
+    const int dvo = xrow * sydx4 + cb * 4;
 
-    // register prefetch of one step (raw values; factors applied at the LDS store)
+    // register prefetch of one step (raw values; factors applied at the LDS store).
+    // Rows are wave-uniform: their folds and offsets are scalar.
     float pg[GROWS][2], px[4][2];
     float pgm[GROWS];
     auto load_step = [&](int i0) {
 #pragma unroll
         for (int rr = 0; rr < GROWS; ++rr) {
             const int hrow = wave + 4 * rr;
-            int j = i0 - R + hrow;
-            float rm = 1.f;
-            if (j < 0 || j >= ny) {
-                if (a.modey == kBwdPeriodic) {
-                    int m;
-                    j = bc_fold(j, ny, a.bcy, &m);
-                    rm = static_cast<float>(m);
-                } else {
-                    j = 0;
-                    rm = 0.f;
+            if (rr < GROWS - 1 || hrow < GH) {
+                int j = i0 - R + hrow;
+                float rm = 1.f;
+                if (j < 0 || j >= ny) {
+                    if (a.modey == kBwdPeriodic) {
+                        int m;
+                        j = __builtin_amdgcn_readfirstlane(bc_fold(j, ny, a.bcy, &m));
+                        rm = static_cast<float>(__builtin_amdgcn_readfirstlane(m));
+                    } else {
+                        j = 0;
+                        rm = 0.f;
+                    }
                 }
-            }
-            pgm[rr] = rm;
-            const float *rp = G + static_cast<int64_t>(j) * a.syg;
+                pgm[rr] = rm;
+                const int so = j * syg4;
 #pragma unroll
-            for (int sl = 0; sl < 2; ++sl) pg[rr][sl] = rp[gcol[sl]];
+                for (int sl = 0; sl < 2; ++sl) pg[rr][sl] = buffer_ld(gres, gvo[sl], so);
+            }
         }
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
             int j = i0 + wave + 4 * rr;
             j = j < ny ? j : ny - 1;
-            const float *rp = X + static_cast<int64_t>(j) * a.syx;
+            const int so = j * syx4;
 #pragma unroll
-            for (int sl = 0; sl < 2; ++sl) px[rr][sl] = rp[xcol[sl]];
+            for (int sl = 0; sl < 2; ++sl) px[rr][sl] = buffer_ld(xres, xvo[sl], so);
         }
     };
 
     double dsx = 0.0, dsy = 0.0;
     const int nsteps = (zend - z0 + S - 1) / S;
+    // Blocks with no folded halo terms (neither an edge tile nor a chunk holding rows
+    // within ry of the top or bottom, or conditions without folds) run a copy of the
+    // step loop without the fold code: its branches, registers and barriers stay out of
+    // the common path.  Rows past zend are computed but never stored.
+    const bool fold_x = a.modex == kBwdFold && (c0 <= rx || c0 + TXC - 1 >= nx - 1 - rx);
+    const bool fold_y = a.modey == kBwdFold && (z0 <= ry || zend - 1 >= ny - 1 - ry);
+    auto steps = [&](auto fold_tag) {
+    constexpr bool FOLD = decltype(fold_tag)::value;
     load_step(z0);
     for (int st = 0; st < nsteps; ++st) {
         const int i0 = z0 + st * S;
         // --- stage g (column-major) and x (interleaved rows) ---
+        const bool plain = tile_plain && i0 - R >= 0 && i0 + S + R <= ny;
+        if (plain) {
 #pragma unroll
-        for (int rr = 0; rr < GROWS; ++rr) {
-            const int hrow = wave + 4 * rr;
-            if (hrow < GH) {
+            for (int rr = 0; rr < GROWS; ++rr) {
+                const int hrow = wave + 4 * rr;
+                if (rr < GROWS - 1 || hrow < GH) {
 #pragma unroll
-                for (int sl = 0; sl < 2; ++sl)
-                    gbuf[(lane + 64 * sl) * Hp + hrow] = pg[rr][sl] * gmul[sl] * pgm[rr];
+                    for (int sl = 0; sl < 2; ++sl) gbuf[(lane + 64 * sl) * Hp + hrow] = pg[rr][sl];
+                }
             }
-        }
 #pragma unroll
-        for (int rr = 0; rr < 4; ++rr) {
-            float *xr = xbuf + (wave + 4 * rr) * ROWF;
+            for (int rr = 0; rr < 4; ++rr) {
+                float *xr = xbuf + (wave + 4 * rr) * ROWF;
 #pragma unroll
-            for (int sl = 0; sl < 2; ++sl)
-                xr[ipos_lp<LP>(lane + 64 * sl)] = xmul[sl] == 2.f ? a.cvalx : px[rr][sl] * xmul[sl];
+                for (int sl = 0; sl < 2; ++sl) xr[ipos_lp<LP>(lane + 64 * sl)] = px[rr][sl];
+            }
+        } else {
+#pragma unroll
+            for (int rr = 0; rr < GROWS; ++rr) {
+                const int hrow = wave + 4 * rr;
+                if (rr < GROWS - 1 || hrow < GH) {
+#pragma unroll
+                    for (int sl = 0; sl < 2; ++sl)
+                        gbuf[(lane + 64 * sl) * Hp + hrow] = pg[rr][sl] * gmul[sl] * pgm[rr];
+                }
+            }
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+                float *xr = xbuf + (wave + 4 * rr) * ROWF;
+#pragma unroll
+                for (int sl = 0; sl < 2; ++sl) {
+                    const uint32_t b = __builtin_bit_cast(uint32_t, px[rr][sl] * xmul[sl]);
+                    xr[ipos_lp<LP>(lane + 64 * sl)] = __builtin_bit_cast(float, (b & xkeep[sl]) | xcbits[sl]);
+                }
+            }
         }
         lds_barrier();  // staged; the previous step's x stage is done with hbuf / h2buf
         if (st + 1 < nsteps) load_step(i0 + S);
@@ -300,7 +386,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd2d(const BwdArgs a) {
         // --- folded halo terms (edge steps and edge tiles only; uniform branches) ---
         // y: halo rows fold onto rows within ry of the top / bottom edge; the rows they
         // read are in this step's staging.  Thread: column tid % 128, rows rg, rg+2, ...
-        const bool yfold = a.modey == kBwdFold && (i0 <= ry || i0 + S - 1 >= ny - 1 - ry);
+        const bool yfold = FOLD && a.modey == kBwdFold && (i0 <= ry || i0 + S - 1 >= ny - 1 - ry);
         if (yfold) {
             const int col = tid & (W - 1);
             const float *gcol0 = gbuf + col * Hp;
@@ -316,7 +402,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd2d(const BwdArgs a) {
         }
         // x: per row, the folded terms of the target columns 0..rx and nx-1-rx..nx-1
         // that fall in this tile, from the (y-folded) h row
-        const bool xfold = a.modex == kBwdFold && (c0 <= rx || c0 + TXC - 1 >= nx - 1 - rx);
+        const bool xfold = FOLD && fold_x;
         if (xfold) {
             const int FW = 2 * (rx + 1);
             for (int w = tid; w < S * FW; w += kBlock) {
@@ -335,7 +421,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd2d(const BwdArgs a) {
 
         // --- x stage: row tid / 16, staged columns R + 8 * (tid % 16) .. +7 ---
         {
-            const int row = tid >> 4, l = tid & 15;
+            const int row = xrow, l = xl;
             const int i = i0 + row;
             const float *hr = hbuf + row * ROWF + l;
             const float *xr = xbuf + row * ROWF + l;
@@ -363,7 +449,6 @@ __global__ __launch_bounds__(kBlock) void k_bwd2d(const BwdArgs a) {
                 u[p] = su;
                 u2[p] = su2;
             }
-            const int cb = c0 + 8 * l;  // image column of output p = 0
             if (xfold) {
 #pragma unroll
                 for (int p = 0; p < P; ++p) {
@@ -372,24 +457,45 @@ __global__ __launch_bounds__(kBlock) void k_bwd2d(const BwdArgs a) {
                     else if (t >= nx - 1 - rx && t < nx) d[p] += foldx[row * Gm::FOLDW + rx + 1 + (t - (nx - 1 - rx))];
                 }
             }
-            if (i < zend && 8 * l < TXC) {
+            if (i < zend) {
+                // outputs p < nv of this lane are in the tile and the line; the count is
+                // made opaque so its comparisons stay here instead of being hoisted out
+                // of the step loop as 64-bit lane masks
+                int nvv = nv;
+                asm volatile("" : "+v"(nvv));
                 const float *h2r = h2buf + row * ROWF + l;
                 float fx = 0.f, fy = 0.f;
 #pragma unroll
                 for (int p = 0; p < P; ++p) {
-                    const int c = cb + p;
-                    if (8 * l + p < TXC && c < nx) {
-                        const float h2v = h2r[((p + R) & 7) * LP + ((p + R) >> 3)];
-                        fx = __builtin_fmaf(hv[p + R], u2[p], fx);
-                        fy = __builtin_fmaf(h2v, u[p], fy);
-                        if (a.want_dx) DX[static_cast<int64_t>(i) * a.sydx + c] = d[p];
-                    }
+                    const bool ok = p < nvv;
+                    const float h2v = h2r[((p + R) & 7) * LP + ((p + R) >> 3)];
+                    // a select, not a zero factor: unused window words may hold any bits
+                    const float fxn = __builtin_fmaf(hv[p + R], u2[p], fx);
+                    const float fyn = __builtin_fmaf(h2v, u[p], fy);
+                    fx = ok ? fxn : fx;
+                    fy = ok ? fyn : fy;
                 }
                 dsx += static_cast<double>(fx);
                 dsy += static_cast<double>(fy);
+                if (a.want_dx) {
+                    const int so = i0 * sydx4;
+                    if (nvv == P) {
+#pragma unroll
+                        for (int p = 0; p < P; ++p) buffer_st(d[p], dres, dvo + 4 * p, so);
+                    } else {
+#pragma unroll
+                        for (int p = 0; p < P; ++p)
+                            if (p < nvv) buffer_st(d[p], dres, dvo + 4 * p, so);
+                    }
+                }
             }
         }
     }
+    };
+    if (fold_x || fold_y)
+        steps(std::true_type{});
+    else
+        steps(std::false_type{});
 
     // block sums in a fixed order: waves by shuffles, then the four waves
     __shared__ double red[2][4];

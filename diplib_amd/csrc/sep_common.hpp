@@ -251,7 +251,9 @@ struct BwdArgs {
     int modex, modey;  // kBwdZero, kBwdPeriodic, kBwdFold
     float cvalx;       // the forward x extension's constant
     int chunk, nchunks, ntiles;
+    int ytop, ybot;  // > 0: chunk 0 is rows [0, ytop), the last [ybot, ny), the others split the rest
     int want_dx;
+    int xbytes, gbytes, dxbytes;  // byte extent of one plane of x, g, dx (each < 2^31)
     Outer outer;
     int64_t gplane[kMaxOuter];
     float tx[kFusedCap], tdx[kFusedCap];    // forward x filter and its sigma derivative

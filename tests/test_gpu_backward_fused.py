@@ -19,7 +19,11 @@ CASES = [((1, 64, 64), [2.0, 2.0], [0, 0]),
          ((1, 300, 70), [8.0, 8.0], [0, 0]),
          ((1, 300, 70), [3.0, 5.0], [0, 2]),
          ((3, 53, 97), [1.0, 1.5], [1, 1]),
-         ((1, 129, 200), [3.0, 3.0], [0, 1])]
+         ((1, 129, 200), [3.0, 3.0], [0, 1]),
+         # fold rows in chunks of their own: y radius above the 16-row step, a middle
+         # part that is not a whole number of steps; and a plane too short for them
+         ((1, 100, 90), [2.0, 6.0], [0, 0]),
+         ((2, 30, 90), [3.0, 3.0], [0, 1])]
 
 
 def _run(x, w, sig, ords, bc):
