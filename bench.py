@@ -533,10 +533,25 @@ def bench_backward(dip, torch, dev):
     1024x1024 images (sigma shared by x and y) and a 128^3 sfloat volume (sigma shared
     by all three axes), derivative orders 0 and 1 along x.  x.grad is reset before
     each backward so no gradient accumulation is timed; sigma's host value is passed
-    (sigma_host) so the device sigma is not read back (a stream wait) every call."""
+    (sigma_host) so the device sigma is not read back (a stream wait) every call.
+    The wall-clock ratio includes the Python and autograd overhead of both loops (the
+    forward-only loop is partly bound by it); kernel_ratio is the same ratio over the
+    library's kernel time alone (HIP events around every launch, in separate loops)."""
     out = {}
     if not hasattr(dip, "gauss_fir_autograd"):
         return out
+    from diplib_amd import _lib
+    L = _lib.lib()
+
+    def kernel_ms(fn, n=5):
+        torch.cuda.synchronize()
+        L.dipamd_trace_enable(1)
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        tr = read_trace(L)
+        L.dipamd_trace_enable(0)
+        return sum(ms for _, ms, _, _ in tr) / n
 
     def run(shape, sdims, orders, seed):
         g = torch.Generator(device=dev)
@@ -572,9 +587,18 @@ def bench_backward(dip, torch, dev):
         e1.record()
         torch.cuda.synchronize()
         ms_fb = e0.elapsed_time(e1) / 5
+
+        def fwd_nograd():
+            with torch.no_grad():
+                fwd()
+
+        k_f = kernel_ms(fwd_nograd)
+        k_fb = kernel_ms(fwdbwd)
         return {"fwd_ms": round(ms_f, 4), "fwd_bwd_ms": round(ms_fb, 4), "ratio": round(ms_fb / ms_f, 3),
                 "fwd_GPix_s": round(x.numel() / (ms_f * 1e-3) / 1e9, 2),
-                "fwd_bwd_GPix_s": round(x.numel() / (ms_fb * 1e-3) / 1e9, 2)}
+                "fwd_bwd_GPix_s": round(x.numel() / (ms_fb * 1e-3) / 1e9, 2),
+                "fwd_kernel_ms": round(k_f, 4), "fwd_bwd_kernel_ms": round(k_fb, 4),
+                "kernel_ratio": round(k_fb / k_f, 3)}
 
     for key, shape, sdims, nd in (("fwd_bwd_32x1024x1024_s3", (32, 1024, 1024), (0, 1), 3),
                                   ("fwd_bwd_128x128x128_s3", (128, 128, 128), (0, 1, 2), 3)):

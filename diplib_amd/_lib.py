@@ -114,4 +114,5 @@ EXPORTED_SYMBOLS = [
 FN_GRADIENT, FN_GRADIENT_MAGNITUDE, FN_HESSIAN, FN_LAPLACE = 0, 1, 2, 3
 IIR_FORWARD_BACKWARD, IIR_DISCRETE_TIME_FIT = 1, 2
 GAUSS_IIR, TENSOR_IIR, GAUSS_FT, TENSOR_FT = 8, 16, 32, 64
+GRAD_SIGMA_TOTAL = 128  # dipamd_gauss_fir_backward: one dL/dsigma, summed over the dims
 FN_DGG, FN_LAPLACE_PLUS_DGG, FN_LAPLACE_MINUS_DGG, FN_STRUCTURE_TENSOR = 4, 5, 6, 7

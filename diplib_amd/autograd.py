@@ -59,7 +59,11 @@ class _GaussFIRFunction(torch.autograd.Function):
         want_ds = ctx.needs_input_grad[1]
         g = g.to(_flex_torch(x.dtype))
         dx = torch.empty(x.shape, dtype=_flex_torch(x.dtype), device=x.device) if want_dx else None
-        ds = torch.zeros(nd, dtype=torch.float64, device=x.device) if want_ds else None
+        # a scalar sigma: the library sums dL/dsigma over the dims itself (the dims it
+        # does not cover have sigma 0, are not filtered and contribute nothing)
+        total = want_ds and sigma.numel() == 1
+        ds = torch.empty(1 if total else nd, dtype=torch.float64, device=x.device) if want_ds else None
+        flags = (_lib.FAST if fast else _lib.EXACT) | (_lib.GRAD_SIGMA_TOTAL if total else 0)
         a_x, a_g = image_struct(x), image_struct(g)
         a_dx = image_struct(dx) if want_dx else _lib.DipamdImage()
         s_arr = (ctypes.c_double * nd)(*sig)
@@ -70,18 +74,8 @@ class _GaussFIRFunction(torch.autograd.Function):
             check(lib().dipamd_gauss_fir_backward(
                 ctypes.byref(a_x), ctypes.byref(a_g), ctypes.byref(a_dx),
                 ctypes.c_void_p(ds.data_ptr()) if want_ds else None, s_arr, nd, o_arr, len(orders), b_arr, len(bcs),
-                truncation, _lib.FAST if fast else _lib.EXACT, _stream_handle(x.device)))
-        dsigma = None
-        if want_ds:
-            if sigma.numel() == 1:
-                dims = sorted(range(nd) if sigma_dims is None else sigma_dims)
-                if dims == list(range(dims[0], dims[-1] + 1)):
-                    sel = ds[dims[0]:dims[-1] + 1]  # a view: no index tensor to copy to the device
-                else:
-                    sel = ds[torch.tensor(dims, device=ds.device)]
-                dsigma = sel.sum().to(sigma.dtype).reshape(sigma.shape)
-            else:
-                dsigma = ds.to(sigma.dtype).reshape(sigma.shape)
+                truncation, flags, _stream_handle(x.device)))
+        dsigma = ds.to(sigma.dtype).reshape(sigma.shape) if want_ds else None
         if want_dx:
             dx = dx.to(x.dtype)
         return dx, dsigma, None, None, None, None, None, None
@@ -103,8 +97,9 @@ def gauss_fir_autograd(x, sigma, orders=(0,), bc=("mirror",), truncation=3.0, si
 
 
 def gauss_fir_backward(x, grad_out, sigmas, orders=(0,), bc=("mirror",), truncation=3.0, fast=False,
-                       want_dx=True, want_dsigma=True):
-    """Direct call of the backward kernels: returns (dL/dx, dL/dsigma per dip dim)."""
+                       want_dx=True, want_dsigma=True, sigma_total=False):
+    """Direct call of the backward kernels: returns (dL/dx, dL/dsigma per dip dim), or
+    with sigma_total dL/dsigma as one value summed over the dims."""
     nd = x.dim()
     sig = [float(s) for s in (sigmas if hasattr(sigmas, "__len__") else [sigmas])]
     if len(sig) == 1:
@@ -112,7 +107,8 @@ def gauss_fir_backward(x, grad_out, sigmas, orders=(0,), bc=("mirror",), truncat
     orders = [int(o) for o in (orders if hasattr(orders, "__len__") else [orders])]
     bcs = _bc_codes(bc)
     dx = torch.empty(x.shape, dtype=_flex_torch(x.dtype), device=x.device) if want_dx else None
-    ds = torch.zeros(nd, dtype=torch.float64, device=x.device) if want_dsigma else None
+    ds = torch.empty(1 if sigma_total else nd, dtype=torch.float64, device=x.device) if want_dsigma else None
+    flags = (_lib.FAST if fast else _lib.EXACT) | (_lib.GRAD_SIGMA_TOTAL if sigma_total else 0)
     a_x, a_g = image_struct(x), image_struct(grad_out)
     a_dx = image_struct(dx) if want_dx else _lib.DipamdImage()
     s_arr = (ctypes.c_double * nd)(*sig)
@@ -122,5 +118,5 @@ def gauss_fir_backward(x, grad_out, sigmas, orders=(0,), bc=("mirror",), truncat
         check(lib().dipamd_gauss_fir_backward(
             ctypes.byref(a_x), ctypes.byref(a_g), ctypes.byref(a_dx),
             ctypes.c_void_p(ds.data_ptr()) if want_dsigma else None, s_arr, nd, o_arr, len(orders), b_arr, len(bcs),
-            float(truncation), _lib.FAST if fast else _lib.EXACT, _stream_handle(x.device)))
+            float(truncation), flags, _stream_handle(x.device)))
     return dx, ds

@@ -33,8 +33,9 @@ hipError_t launch_adjoint_fold(bool is_double, const void *args, int64_t nlines,
 hipError_t launch_dot(bool is_double, const void *a, const void *geo, const void *b, double *partials, double *out,
                       hipStream_t stream);
 hipError_t launch_fill_doubles(double *p, int n, double v, hipStream_t stream);
-hipError_t launch_bwd2d(int symx, int symy, int rc, const BwdArgs &a, int64_t nblocks, double *gx, double *gy,
-                        hipStream_t stream);
+hipError_t launch_bwd2d(int symx, int symy, int rc, const BwdArgs &a, int64_t nblocks, double *gs, int gs_mode,
+                        int gs_n, hipStream_t stream);
+hipError_t launch_sum_doubles(const double *in, int n, double *out, hipStream_t stream);
 size_t bwd2d_lds_bytes(int rc);
 int bwd2d_class(int rx, int ry);
 int bwd2d_tile_columns(int rc);
@@ -129,11 +130,13 @@ int bwd_mode(int bc) {
 // x (the forward input), g (dL/d(output of those two passes)), dx (dL/dx, or p == null
 // for none), in float with unit x strides, symmetric filters of radius <= 20, no line
 // shorter than its filter and a condition it supports; dims >= 2 are stacked planes.
-// dL/dsigma along dims 0 and 1 is ADDED to gs0 / gs1 (device doubles, nullable).
+// dL/dsigma along dims 0 and 1 goes to gs (device doubles, nullable) by gs_mode:
+// kBwdSumAdd gs[0..1] +=, kBwdSumWrite gs[0..1] = and gs[2..gs_n-1] = 0, kBwdSumTotal
+// gs[0] = their sum.
 // Returns -100 when it does not apply.
 int fused_backward_2d(const View &x, const View &g, const View &dx, int nd, const InternFilter &F0,
                       const InternFilter &F1, const InternFilter &dF0, const InternFilter &dF1, int bc0, int bc1,
-                      double *gs0, double *gs1, hipStream_t stream) {
+                      double *gs, int gs_mode, int gs_n, hipStream_t stream) {
     if (!bwd_fused_enabled()) return -100;
     if (x.dt != DIPAMD_DT_SFLOAT || g.dt != DIPAMD_DT_SFLOAT) return -100;
     const bool want_dx = dx.p != nullptr;
@@ -230,7 +233,7 @@ int fused_backward_2d(const View &x, const View &g, const View &dx, int nd, cons
     a.partials = partials;
     {
         TraceScope ts("bwd2d", static_cast<int64_t>(a.nx) * a.ny * nplanes * (want_dx ? 12 : 8), stream);
-        e = launch_bwd2d(F0.sym, F1.sym, rc, a, nblocks, gs0, gs1, stream);
+        e = launch_bwd2d(F0.sym, F1.sym, rc, a, nblocks, gs, gs_mode, gs_n, stream);
     }
     scratch_free(partials, stream);
     if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("backward launch: ") + hipGetErrorString(e));
@@ -272,6 +275,7 @@ extern "C" int dipamd_gauss_fir_backward(const dipamd_image *x, const dipamd_ima
     int bcs[DIPAMD_MAX_DIMS];
     if ((st = broadcast_bc(bc, nbc, nd, bcs)) != DIPAMD_OK) return st;
     const bool fast = (flags & DIPAMD_FAST) != 0;
+    const bool total = (flags & DIPAMD_GRAD_SIGMA_TOTAL) != 0;
     const bool compute_double = x->dtype == DIPAMD_DT_DFLOAT;
 
     // filters (GaussFIR, gauss.cpp:238-286) and their sigma derivatives
@@ -308,125 +312,137 @@ extern "C" int dipamd_gauss_fir_backward(const dipamd_image *x, const dipamd_ima
     // when only they are filtered, else after the adjoint passes of the other dims
     const bool fuse01 = m >= 2 && order[0] == 0 && order[1] == 1;
     if (fuse01 && m == 2) {
-        if (want_ds) {
-            hipError_t e = launch_fill_doubles(grad_sigma, nd, 0.0, stream);
-            if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("fill: ") + hipGetErrorString(e));
-        }
+        // the partial sums' kernel writes every output entry: no separate fill
         View dxv;
         std::memset(&dxv, 0, sizeof(dxv));
         if (want_dx) dxv = to_view(grad_in);
         st = fused_backward_2d(to_view(x), to_view(grad_out), dxv, nd, F[0], F[1], dF[0], dF[1], bcs[0], bcs[1],
-                               want_ds ? grad_sigma : nullptr, want_ds ? grad_sigma + 1 : nullptr, stream);
+                               want_ds ? grad_sigma : nullptr, total ? kBwdSumTotal : kBwdSumWrite, nd, stream);
         if (st != -100) return st;
     }
 
-    std::vector<void *> scratch;
-    auto alloc = [&](View *v) -> int {
-        int64_t n = 1;
-        for (int d = 0; d < nd; ++d) n *= x->sizes[d];
-        void *p = nullptr;
-        hipError_t e = scratch_alloc(&p, static_cast<size_t>(n) * dt_size(buf_dt), stream);
-        if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("scratch allocation: ") + hipGetErrorString(e));
-        scratch.push_back(p);
-        *v = contiguous_view(p, buf_dt, nd, xv.sizes);
-        return DIPAMD_OK;
-    };
-    auto finish = [&](int status) {
-        for (void *p : scratch) scratch_free(p, stream);
-        return status;
-    };
+    // the general path: dL/dsigma per dim into gsig (grad_sigma, or a scratch array
+    // whose entries are summed into grad_sigma[0] afterwards)
+    auto general = [&](double *gsig) -> int {
+        std::vector<void *> scratch;
+        auto alloc = [&](View *v) -> int {
+            int64_t n = 1;
+            for (int d = 0; d < nd; ++d) n *= x->sizes[d];
+            void *p = nullptr;
+            hipError_t e = scratch_alloc(&p, static_cast<size_t>(n) * dt_size(buf_dt), stream);
+            if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("scratch allocation: ") + hipGetErrorString(e));
+            scratch.push_back(p);
+            *v = contiguous_view(p, buf_dt, nd, xv.sizes);
+            return DIPAMD_OK;
+        };
+        auto finish = [&](int status) {
+            for (void *p : scratch) scratch_free(p, stream);
+            return status;
+        };
 
-    if (want_ds) {
-        hipError_t e = launch_fill_doubles(grad_sigma, nd, 0.0, stream);
-        if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("fill: ") + hipGetErrorString(e));
-    }
-    const View gv = to_view(grad_out);
-    if (m == 0) {
-        // identity forward: dL/dx = g
-        if (want_dx) return cast_from_buffer(gv, to_view(grad_in), nd, stream);
-        return DIPAMD_OK;
-    }
-
-    // the input as a kernel source (uint8 or the buffer type); others are converted
-    View x0 = xv;
-    if (!(x0.dt == DIPAMD_DT_UINT8 && !is_double) && x0.dt != buf_dt) {
-        View t;
-        if ((st = alloc(&t)) != DIPAMD_OK) return finish(st);
-        if ((st = cast_to_buffer(x0, t, nd, is_double, stream)) != DIPAMD_OK) return finish(st);
-        x0 = t;
-    }
-
-    // forward intermediates v_1..v_{m-1} (only needed for the sigma derivatives)
-    std::vector<View> v(static_cast<size_t>(m));
-    v[0] = x0;
-    if (want_ds) {
-        for (int i = 1; i < m; ++i) {
-            if ((st = alloc(&v[static_cast<size_t>(i)])) != DIPAMD_OK) return finish(st);
-            const int d = order[static_cast<size_t>(i - 1)];
-            if ((st = run_pass(v[static_cast<size_t>(i - 1)], v[static_cast<size_t>(i)], nd, d, F[d], bcs[d],
-                               is_double, fast, stream)) != DIPAMD_OK)
-                return finish(st);
-        }
-    }
-
-    // adjoint chain u_m = g, u_{i-1} = K_i^T u_i
-    std::vector<View> u(static_cast<size_t>(m + 1));
-    u[static_cast<size_t>(m)] = gv;
-    View w;
-    bool have_w = false;
-    double *partials = nullptr;
-    if (want_ds) {
-        hipError_t e = scratch_alloc(reinterpret_cast<void **>(&partials), 1024 * sizeof(double), stream);
-        if (e != hipSuccess) return finish(set_error(DIPAMD_E_HIP, "scratch allocation failed"));
-        scratch.push_back(partials);
-    }
-    for (int i = m; i >= 1; --i) {
-        const int d = order[static_cast<size_t>(i - 1)];
-        if (i == 2 && fuse01 && m > 2) {
-            // the remaining passes (dims 0 and 1) and their sigma derivatives in one kernel
-            View dxv;
-            std::memset(&dxv, 0, sizeof(dxv));
-            if (want_dx) dxv = to_view(grad_in);
-            st = fused_backward_2d(xv, u[2], dxv, nd, F[0], F[1], dF[0], dF[1], bcs[0], bcs[1],
-                                   want_ds ? grad_sigma : nullptr, want_ds ? grad_sigma + 1 : nullptr, stream);
-            if (st != -100) return finish(st);
-        }
         if (want_ds) {
-            // dL/dsigma_d = < u_i , K'_i v_{i-1} >
-            if (!have_w) {
-                if ((st = alloc(&w)) != DIPAMD_OK) return finish(st);
-                have_w = true;
+            hipError_t e = launch_fill_doubles(gsig, nd, 0.0, stream);
+            if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("fill: ") + hipGetErrorString(e));
+        }
+        const View gv = to_view(grad_out);
+        if (m == 0) {
+            // identity forward: dL/dx = g
+            if (want_dx) return cast_from_buffer(gv, to_view(grad_in), nd, stream);
+            return DIPAMD_OK;
+        }
+
+        // the input as a kernel source (uint8 or the buffer type); others are converted
+        View x0 = xv;
+        if (!(x0.dt == DIPAMD_DT_UINT8 && !is_double) && x0.dt != buf_dt) {
+            View t;
+            if ((st = alloc(&t)) != DIPAMD_OK) return finish(st);
+            if ((st = cast_to_buffer(x0, t, nd, is_double, stream)) != DIPAMD_OK) return finish(st);
+            x0 = t;
+        }
+
+        // forward intermediates v_1..v_{m-1} (only needed for the sigma derivatives)
+        std::vector<View> v(static_cast<size_t>(m));
+        v[0] = x0;
+        if (want_ds) {
+            for (int i = 1; i < m; ++i) {
+                if ((st = alloc(&v[static_cast<size_t>(i)])) != DIPAMD_OK) return finish(st);
+                const int d = order[static_cast<size_t>(i - 1)];
+                if ((st = run_pass(v[static_cast<size_t>(i - 1)], v[static_cast<size_t>(i)], nd, d, F[d], bcs[d],
+                                   is_double, fast, stream)) != DIPAMD_OK)
+                    return finish(st);
             }
-            if ((st = run_pass(v[static_cast<size_t>(i - 1)], w, nd, d, dF[d], bcs[d], is_double, fast, stream)) !=
+        }
+
+        // adjoint chain u_m = g, u_{i-1} = K_i^T u_i
+        std::vector<View> u(static_cast<size_t>(m + 1));
+        u[static_cast<size_t>(m)] = gv;
+        View w;
+        bool have_w = false;
+        double *partials = nullptr;
+        if (want_ds) {
+            hipError_t e = scratch_alloc(reinterpret_cast<void **>(&partials), 1024 * sizeof(double), stream);
+            if (e != hipSuccess) return finish(set_error(DIPAMD_E_HIP, "scratch allocation failed"));
+            scratch.push_back(partials);
+        }
+        for (int i = m; i >= 1; --i) {
+            const int d = order[static_cast<size_t>(i - 1)];
+            if (i == 2 && fuse01 && m > 2) {
+                // the remaining passes (dims 0 and 1) and their sigma derivatives in one kernel
+                View dxv;
+                std::memset(&dxv, 0, sizeof(dxv));
+                if (want_dx) dxv = to_view(grad_in);
+                st = fused_backward_2d(xv, u[2], dxv, nd, F[0], F[1], dF[0], dF[1], bcs[0], bcs[1],
+                                       want_ds ? gsig : nullptr, kBwdSumAdd, nd, stream);
+                if (st != -100) return finish(st);
+            }
+            if (want_ds) {
+                // dL/dsigma_d = < u_i , K'_i v_{i-1} >
+                if (!have_w) {
+                    if ((st = alloc(&w)) != DIPAMD_OK) return finish(st);
+                    have_w = true;
+                }
+                if ((st = run_pass(v[static_cast<size_t>(i - 1)], w, nd, d, dF[d], bcs[d], is_double, fast, stream)) !=
+                    DIPAMD_OK)
+                    return finish(st);
+                const View &ui = u[static_cast<size_t>(i)];
+                CastGeo geo;
+                std::memset(&geo, 0, sizeof(geo));
+                geo.total = 1;
+                for (int k = 0; k < nd; ++k) geo.total *= ui.sizes[k];
+                if (!contiguous(ui, nd)) {
+                    geo.ndim = nd;
+                    for (int k = 0; k < nd; ++k) {
+                        geo.size[k] = ui.sizes[k];
+                        geo.sin[k] = ui.strides[k];
+                    }
+                }
+                TraceScope ts("dot", geo.total * static_cast<int64_t>(2 * dt_size(buf_dt)), stream);
+                hipError_t e = launch_dot(is_double, ui.p, &geo, w.p, partials, gsig + d, stream);
+                if (e != hipSuccess) return finish(set_error(DIPAMD_E_HIP, std::string("dot: ") + hipGetErrorString(e)));
+            }
+            if (i == 1 && !want_dx) break;
+            View dst;
+            if (i == 1) {
+                dst = to_view(grad_in);
+            } else if ((st = alloc(&dst)) != DIPAMD_OK) {
+                return finish(st);
+            }
+            if ((st = adjoint_pass(u[static_cast<size_t>(i)], dst, nd, d, F[d], bcs[d], is_double, fast, stream)) !=
                 DIPAMD_OK)
                 return finish(st);
-            const View &ui = u[static_cast<size_t>(i)];
-            CastGeo geo;
-            std::memset(&geo, 0, sizeof(geo));
-            geo.total = 1;
-            for (int k = 0; k < nd; ++k) geo.total *= ui.sizes[k];
-            if (!contiguous(ui, nd)) {
-                geo.ndim = nd;
-                for (int k = 0; k < nd; ++k) {
-                    geo.size[k] = ui.sizes[k];
-                    geo.sin[k] = ui.strides[k];
-                }
-            }
-            TraceScope ts("dot", geo.total * static_cast<int64_t>(2 * dt_size(buf_dt)), stream);
-            hipError_t e = launch_dot(is_double, ui.p, &geo, w.p, partials, grad_sigma + d, stream);
-            if (e != hipSuccess) return finish(set_error(DIPAMD_E_HIP, std::string("dot: ") + hipGetErrorString(e)));
+            u[static_cast<size_t>(i - 1)] = dst;
         }
-        if (i == 1 && !want_dx) break;
-        View dst;
-        if (i == 1) {
-            dst = to_view(grad_in);
-        } else if ((st = alloc(&dst)) != DIPAMD_OK) {
-            return finish(st);
-        }
-        if ((st = adjoint_pass(u[static_cast<size_t>(i)], dst, nd, d, F[d], bcs[d], is_double, fast, stream)) !=
-            DIPAMD_OK)
-            return finish(st);
-        u[static_cast<size_t>(i - 1)] = dst;
+        return finish(DIPAMD_OK);
+    };
+    if (!(total && want_ds)) return general(grad_sigma);
+    double *gsd = nullptr;
+    hipError_t e = scratch_alloc(reinterpret_cast<void **>(&gsd), static_cast<size_t>(nd) * sizeof(double), stream);
+    if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("scratch allocation: ") + hipGetErrorString(e));
+    st = general(gsd);
+    if (st == DIPAMD_OK) {
+        e = launch_sum_doubles(gsd, nd, grad_sigma, stream);
+        if (e != hipSuccess) st = set_error(DIPAMD_E_HIP, std::string("sum: ") + hipGetErrorString(e));
     }
-    return finish(DIPAMD_OK);
+    scratch_free(gsd, stream);
+    return st;
 }

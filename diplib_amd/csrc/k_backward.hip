@@ -129,6 +129,18 @@ hipError_t launch_dot(bool is_double, const void *a, const void *geo, const void
     return hipGetLastError();
 }
 
+// out[0] = in[0] + in[1] + ... + in[n-1], in that order.
+__global__ void k_sum_doubles(const double *in, int n, double *out) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += in[i];
+    *out = s;
+}
+
+hipError_t launch_sum_doubles(const double *in, int n, double *out, hipStream_t stream) {
+    hipLaunchKernelGGL(k_sum_doubles, dim3(1), dim3(1), 0, stream, in, n, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_doubles(double *p, int n, double v, hipStream_t stream) {
     hipLaunchKernelGGL(k_fill_doubles, dim3((n + 63) / 64), dim3(64), 0, stream, p, n, v);
     return hipGetLastError();

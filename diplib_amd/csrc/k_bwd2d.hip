@@ -515,11 +515,14 @@ __global__ __launch_bounds__(kBlock) void k_bwd2d(const BwdArgs a) {
     }
 }
 
-// Sum of the per-block partials in a fixed order: out_x += even entries, out_y += odd.
-__global__ __launch_bounds__(kBlock) void k_bwd2d_sum(const double *partials, int64_t nblocks, double *out_x,
-                                                      double *out_y) {
+// Sum of the per-block partials in a fixed order (sx: even entries, sy: odd), then by
+// mode: kBwdSumAdd out[0] += sx, out[1] += sy; kBwdSumWrite out[0] = sx, out[1] = sy and
+// out[2 .. nout-1] = 0; kBwdSumTotal out[0] = sx + sy.
+__global__ __launch_bounds__(kBlock) void k_bwd2d_sum(const double *partials, int64_t nblocks, double *out, int mode,
+                                                      int nout) {
     __shared__ double red[2][kBlock];
     double sx = 0.0, sy = 0.0;
+#pragma unroll 8
     for (int64_t b = threadIdx.x; b < nblocks; b += kBlock) {
         sx += partials[2 * b];
         sy += partials[2 * b + 1];
@@ -535,9 +538,18 @@ __global__ __launch_bounds__(kBlock) void k_bwd2d_sum(const double *partials, in
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        if (out_x) *out_x += red[0][0];
-        if (out_y) *out_y += red[1][0];
+        if (mode == kBwdSumAdd) {
+            out[0] += red[0][0];
+            out[1] += red[1][0];
+        } else if (mode == kBwdSumWrite) {
+            out[0] = red[0][0];
+            out[1] = red[1][0];
+        } else {
+            out[0] = red[0][0] + red[1][0];
+        }
     }
+    if (mode == kBwdSumWrite)
+        for (int i = 2 + static_cast<int>(threadIdx.x); i < nout; i += kBlock) out[i] = 0.0;
 }
 
 namespace {
@@ -596,8 +608,8 @@ size_t bwd2d_lds_bytes(int rc) {
     }
 }
 
-hipError_t launch_bwd2d(int symx, int symy, int rc, const BwdArgs &a, int64_t nblocks, double *gx, double *gy,
-                        hipStream_t stream) {
+hipError_t launch_bwd2d(int symx, int symy, int rc, const BwdArgs &a, int64_t nblocks, double *gs, int gs_mode,
+                        int gs_n, hipStream_t stream) {
     bool ok;
     if (symx == SYM_EVEN && symy == SYM_EVEN)
         ok = launch_sym<SYM_EVEN, SYM_EVEN>(rc, a, nblocks, stream);
@@ -612,7 +624,8 @@ hipError_t launch_bwd2d(int symx, int symy, int rc, const BwdArgs &a, int64_t nb
     if (!ok) return hipErrorInvalidValue;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bwd2d_sum, dim3(1), dim3(kBlock), 0, stream, a.partials, nblocks, gx, gy);
+    if (!gs) return hipSuccess;
+    hipLaunchKernelGGL(k_bwd2d_sum, dim3(1), dim3(kBlock), 0, stream, a.partials, nblocks, gs, gs_mode, gs_n);
     return hipGetLastError();
 }
 

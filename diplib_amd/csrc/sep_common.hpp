@@ -266,5 +266,6 @@ struct BwdArgs {
 // forward extension copied them from (mirror conditions, zero order).
 constexpr int kBwdZero = 0, kBwdPeriodic = 1, kBwdFold = 2;
 constexpr int kBwdStep = 16;   // rows per step
+constexpr int kBwdSumAdd = 0, kBwdSumWrite = 1, kBwdSumTotal = 2;  // how dL/dsigma reaches its output
 
 }  // namespace dipamd

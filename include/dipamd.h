@@ -85,7 +85,10 @@ enum {
     /* The same through dip::GaussFT ("FT", or "best" with a sigma below 0.8; an empty
      * boundary condition array means "mirror" there, as dip::Gauss passes it). */
     DIPAMD_GAUSS_FT = 32,
-    DIPAMD_TENSOR_FT = 64
+    DIPAMD_TENSOR_FT = 64,
+    /* dipamd_gauss_fir_backward: grad_sigma is ONE device double, the sum over the dims
+     * of dL/dsigma_d -- the derivative for one sigma shared by every filtered dim. */
+    DIPAMD_GRAD_SIGMA_TOTAL = 128
 };
 
 /* Error codes. */
@@ -242,7 +245,8 @@ int dipamd_batch_info(const dipamd_batch_plan *plan, int64_t *ntiles, int32_t *c
  * condition); grad_sigma[d] (DEVICE doubles, ndim of them, overwritten) =
  * <grad_out, d out / d sigma_d>, differentiating the normalised discrete taps at a
  * fixed radius.  x and grad_out are sfloat or dfloat, grad_in has the dtype of
- * grad_out; either output may be NULL (origin NULL) to skip it. */
+ * grad_out; either output may be NULL (origin NULL) to skip it.  With
+ * DIPAMD_GRAD_SIGMA_TOTAL in flags, grad_sigma is one double: the sum over the dims. */
 int dipamd_gauss_fir_backward(const dipamd_image *x, const dipamd_image *grad_out, const dipamd_image *grad_in,
                               double *grad_sigma, const double *sigmas, int32_t nsigmas, const int32_t *orders,
                               int32_t norders, const int32_t *bc, int32_t nbc, double truncation, uint32_t flags,

@@ -111,3 +111,28 @@ def test_fused_backward_3d_vs_fp64(cuda, bc):
         dyds = (oc.gauss_fixed_radius(xn, sp, ords, [bc], radii) - oc.gauss_fixed_radius(xn, sm, ords, [bc], radii)) / 2e-4
         bound = np.linalg.norm(wn) * np.linalg.norm(dyds)
         assert abs(ds32[d] - ds64[d]) <= 1e-6 * bound, (d, ds32[d], ds64[d], bound)
+
+
+@pytest.mark.parametrize("shape,sig,ords,dt", [
+    ((1, 90, 120), [3.0, 3.0, 0.0], [0, 1, 0], torch.float32),  # fused 2-D kernel
+    ((12, 40, 50), [2.0, 2.0, 1.5], [1, 0, 0], torch.float32),  # adjoint pass along z, then fused
+    ((30, 40), [2.0, 2.5], [0, 0], torch.float64),             # per-axis passes
+])
+def test_backward_sigma_total(cuda, shape, sig, ords, dt):
+    """DIPAMD_GRAD_SIGMA_TOTAL: one dL/dsigma, the per-dim values summed in dim order;
+    dL/dx unchanged."""
+    g = torch.Generator(device=cuda)
+    g.manual_seed(11)
+    x = torch.rand(shape, generator=g, device=cuda, dtype=dt)
+    w = torch.randn(shape, generator=g, device=cuda, dtype=dt)
+    dx1, ds = dip.gauss_fir_backward(x, w, sig, ords, ["mirror"])
+    dx2, tot = dip.gauss_fir_backward(x, w, sig, ords, ["mirror"], sigma_total=True)
+    assert tot.shape == (1,)
+    vals = ds.cpu().tolist()
+    acc = 0.0
+    for v in vals:
+        acc += v
+    assert tot.item() == acc
+    assert torch.equal(dx1, dx2)
+    _, tot_only = dip.gauss_fir_backward(x, w, sig, ords, ["mirror"], want_dx=False, sigma_total=True)
+    assert tot_only.item() == acc
