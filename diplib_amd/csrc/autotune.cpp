@@ -65,15 +65,19 @@ bool fits(int cfg, bool is_double, int rx, int ry) {
 
 }  // namespace
 
-// Largest radius the fused 2-D kernel takes: beyond 16 the two single passes (K1 line +
-// the strided sweeps) are as fast or faster on 8192^2 u8 (sigma 6: 0.306 vs 0.317 ms,
-// sigma 8 order 0: 0.352 vs 0.381 ms; sigma 4, radius 12-16: fused 0.20-0.21 vs 0.27-0.30).
-int fuse_max_radius() {
+// Largest radius the fused 2-D kernel takes for a plane of px pixels.  On large planes
+// beyond radius 16 the two single passes (K1 line + the strided sweeps) are as fast or
+// faster (8192^2 u8, sigma 6: 0.306 vs 0.317 ms, sigma 8 order 0: 0.352 vs 0.381 ms;
+// sigma 4, radius 12-16: fused 0.20-0.21 vs 0.27-0.30); on planes up to 2048^2 the
+// single launch without an intermediate image wins up to radius 24 (512 images of
+// 64..2048 squared, sigma 0.5..8, one call each: 9.4 ms with 24, 15.5 ms with 16).
+int fuse_max_radius(int64_t px) {
     static int r = [] {
         const char *v = std::getenv("DIPAMD_FUSE_MAX_RADIUS");
-        return v ? std::atoi(v) : 16;
+        return v ? std::atoi(v) : -1;
     }();
-    return r;
+    if (r >= 0) return r;
+    return px <= int64_t(2048) * 2048 ? 24 : 16;
 }
 
 bool stream2d_enabled() {
@@ -140,7 +144,6 @@ size_t lds_limit_bytes() {
 
 int choose_fused_config(bool is_double, int rx, int ry, int64_t nx, int64_t ny, uint32_t flags) {
     std::call_once(g_tune_once, load_table);
-    (void)ny;
     const int r = rx > ry ? rx : ry;
     if (g_forced != -2) return fits(g_forced, is_double, rx, ry) ? g_forced : -1;
     int cfg = -2;
@@ -155,7 +158,7 @@ int choose_fused_config(bool is_double, int rx, int ry, int64_t nx, int64_t ny, 
         // round trip); long filters are VALU-bound and run as a line pass plus a
         // column-tile sweep, which recompute nothing.  The 128-column tile while its
         // ring fits comfortably, else 64 columns.
-        if (r > fuse_max_radius() && !(flags & DIPAMD_FORCE_FUSION)) return -1;
+        if (r > fuse_max_radius(nx * ny) && !(flags & DIPAMD_FORCE_FUSION)) return -1;
         cfg = (!is_double && r <= 16 && nx >= 256) ? 1 : 0;
     }
     if (cfg < 0) {
