@@ -101,8 +101,10 @@ def main():
     ap.add_argument("--case", nargs=2, metavar=("SIGMA", "ORDERS"),
                     help="profiling mode: run one case of the sweep (e.g. 16 00; or 'vol mirror' for the "
                          "512x512x256 sfloat volume, sigma (2,2,1); or 'iir 00' for dip::GaussIIR sigma 16, "
-                         "'ft 00' for dip::GaussFT sigma 0.5, on the 8192^2 input) --steps times, then a "
-                         "256 MiB device copy as the calibration kernel for the traffic counters")
+                         "'ft 00' for dip::GaussFT sigma 0.5, on the 8192^2 input; or 'bwd mirror' for "
+                         "forward + backward through the autograd function, 32 sfloat 1024^2 planes, sigma 3 "
+                         "on x and y, order 0) --steps times, then a 256 MiB device copy as the calibration "
+                         "kernel for the traffic counters")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; the multi-GPU (sharded) path is a design only "
@@ -120,6 +122,23 @@ def main():
     dev = torch.device("cuda:0")
     log(f"device {torch.cuda.get_device_name(0)}; library {L.dipamd_version().decode()}")
 
+    if a.case and a.case[0] == "bwd":
+        g = torch.Generator(device=dev)
+        g.manual_seed(5)
+        xb = torch.rand((32, 1024, 1024), generator=g, device=dev, dtype=torch.float32).requires_grad_(True)
+        wb = torch.randn((32, 1024, 1024), generator=g, device=dev, dtype=torch.float32)
+        sb = torch.tensor(3.0, device=dev, dtype=torch.float64, requires_grad=True)
+        for _ in range(a.steps):
+            xb.grad = None
+            sb.grad = None
+            dip.gauss_fir_autograd(xb, sb, [0, 0, 0], [a.case[1]], 3.0, sigma_dims=(0, 1), sigma_host=3.0).backward(wb)
+        src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(a.steps):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"case backward {a.case[1]}: {a.steps} forward + backward calls, then {a.steps} copies of 256 MiB")
+        return
     if a.case and a.case[0] == "vol":
         xv = torch.from_numpy(cases.make_input((256, 512, 512), "float32", 3, 0.01)).to(dev)
         yv = torch.empty_like(xv)
