@@ -128,7 +128,7 @@ int bwd_mode(int bc) {
 
 // The fused 2-D backward (k_bwd2d.hip) for the passes along dims 0 and 1, on views:
 // x (the forward input), g (dL/d(output of those two passes)), dx (dL/dx, or p == null
-// for none), in float with unit x strides, symmetric filters of radius <= 20, no line
+// for none), in float with unit x strides, symmetric filters of radius <= 16, no line
 // shorter than its filter and a condition it supports; dims >= 2 are stacked planes.
 // dL/dsigma along dims 0 and 1 goes to gs (device doubles, nullable) by gs_mode:
 // kBwdSumAdd gs[0..1] +=, kBwdSumWrite gs[0..1] = and gs[2..gs_n-1] = 0, kBwdSumTotal
@@ -145,7 +145,7 @@ int fused_backward_2d(const View &x, const View &g, const View &dx, int nd, cons
     const int bcs[2] = {bc0, bc1};
     for (int d = 0; d < 2; ++d) {
         const InternFilter &f = *Fs[d];
-        if ((f.sym != DIPAMD_SYM_EVEN && f.sym != DIPAMD_SYM_ODD) || f.shift != 0 || f.border > 20) return -100;
+        if ((f.sym != DIPAMD_SYM_EVEN && f.sym != DIPAMD_SYM_ODD) || f.shift != 0 || f.border > 16) return -100;
         if (bwd_mode(bcs[d]) < 0) return -100;
         if (x.sizes[d] < 2 * f.border + 1 || x.sizes[d] >= (int64_t(1) << 30)) return -100;
     }

@@ -554,7 +554,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd2d_sum(const double *partials, in
 
 namespace {
 
-constexpr int kBwdClasses[] = {3, 5, 7, 9, 11, 13, 16, 20};  // 24 would exceed 64 KB of LDS
+constexpr int kBwdClasses[] = {3, 5, 7, 9, 11, 13, 16};  // 20 would exceed 64 KB of LDS (67200 B)
 
 template <int R>
 size_t lds_of() {
@@ -577,7 +577,6 @@ bool launch_sym(int rc, const BwdArgs &a, int64_t nblocks, hipStream_t stream) {
         case 11: launch_r<SX, SY, 11>(a, nblocks, stream); return true;
         case 13: launch_r<SX, SY, 13>(a, nblocks, stream); return true;
         case 16: launch_r<SX, SY, 16>(a, nblocks, stream); return true;
-        case 20: launch_r<SX, SY, 20>(a, nblocks, stream); return true;
         default: return false;
     }
 }
@@ -603,7 +602,6 @@ size_t bwd2d_lds_bytes(int rc) {
         case 11: return lds_of<11>();
         case 13: return lds_of<13>();
         case 16: return lds_of<16>();
-        case 20: return lds_of<20>();
         default: return 0;
     }
 }
