@@ -201,7 +201,7 @@ int fused_backward_2d(const View &x, const View &g, const View &dx, int nd, cons
     put(a.txa, F0, true);
     put(a.tya, F1, true);
     put(a.tdya, dF1, true);
-    const int rc = bwd2d_class(a.rx, a.ry);  // compile-time window; taps stay zero beyond r
+    const int rc = bwd2d_class(a.rx, a.ry);  // compile-time windows; taps stay zero beyond r
     if (rc == 0) return -100;
     const int txc = bwd2d_tile_columns(rc);
     a.ntiles = (a.nx + txc - 1) / txc;

@@ -23,7 +23,10 @@ CASES = [((1, 64, 64), [2.0, 2.0], [0, 0]),
          # fold rows in chunks of their own: y radius above the 16-row step, a middle
          # part that is not a whole number of steps; and a plane too short for them
          ((1, 100, 90), [2.0, 6.0], [0, 0]),
-         ((2, 30, 90), [3.0, 3.0], [0, 1])]
+         ((2, 30, 90), [3.0, 3.0], [0, 1]),
+         # unequal x and y radius classes: 11 and 9, 16 and 13
+         ((2, 70, 130), [3.0, 3.0], [1, 0]),
+         ((1, 80, 160), [4.0, 4.0], [1, 0])]
 
 
 def _run(x, w, sig, ords, bc):
