@@ -407,9 +407,13 @@ __global__ __launch_bounds__(kBlock) void k_bwd2d(const BwdArgs a) {
         // that fall in this tile, from the (y-folded) h row
         const bool xfold = FOLD && fold_x;
         if (xfold) {
-            const int FW = 2 * (rx + 1);
+            // only the side(s) whose targets lie in this tile: entries 0..rx (left
+            // targets) and rx+1..2rx+1 (right); the x stage reads no others
+            const int klo = c0 <= rx ? 0 : rx + 1;
+            const int khi = c0 + TXC - 1 >= nx - 1 - rx ? 2 * (rx + 1) : rx + 1;
+            const int FW = khi - klo;
             for (int w = tid; w < S * FW; w += kBlock) {
-                const int row = w / FW, k = w % FW;
+                const int row = w / FW, k = klo + w % FW;
                 const int t = k <= rx ? k : nx - 1 - rx + (k - rx - 1);  // target column
                 float v = 0.f;
                 if (t >= c0 && t < c0 + TXC && t < nx) {
