@@ -11,7 +11,7 @@ so the two agree to FFT rounding (checked with a tolerance against the reference
 outputs in tests/test_oracle.py) -- except where the real-to-complex dim has an odd
 length: there the reference's result is not shift invariant (an impulse at the edge of
 an odd line comes out distorted, at the centre it does not), and this restatement
-keeps the exact convolution.  Only the tests use this module, as the checker.
+keeps the exact convolution.  Only the tests use this module, to compare against.
 """
 import math
 

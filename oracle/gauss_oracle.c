@@ -4,7 +4,7 @@
  * A plain-C restatement of dip::Gauss(method "FIR") -> dip::GaussFIR ->
  * dip::SeparableConvolution -> dip::Framework::Separable, line by line, with the
  * reference's arithmetic order.  Only the test suite, the smoke check and the
- * CPU-baseline leg of the benchmark load it, as the checker; the product (the HIP
+ * CPU-baseline leg of the benchmark load it, to compare against; the product (the HIP
  * library under diplib_amd/) never calls it.
  *
  * Build: oracle/Makefile  (gcc -O2 -ffp-contract=off, no -march: plain SSE2

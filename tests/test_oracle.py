@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/gauss_oracle.c) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/gauss_oracle.c) before the GPU results are compared with it.
 
 * taps: bit-identical to the reference's own MakeHalfGaussian (golden fixtures),
   plus the moment invariants of gauss.cpp:496-531 that fix the formulas;
