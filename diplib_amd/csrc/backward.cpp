@@ -363,7 +363,31 @@ extern "C" int dipamd_gauss_fir_backward(const dipamd_image *x, const dipamd_ima
         // forward intermediates v_1..v_{m-1} (only needed for the sigma derivatives)
         std::vector<View> v(static_cast<size_t>(m));
         v[0] = x0;
-        if (want_ds) {
+        // With dims 0 and 1 first and fused below, only v_2 (after both) is needed: one
+        // fused forward over them gives it; v_1 is made only if the fused backward
+        // turns out not to apply.
+        bool v1_pending = false;
+        auto make_v = [&](int i) -> int {
+            int s2 = alloc(&v[static_cast<size_t>(i)]);
+            if (s2 != DIPAMD_OK) return s2;
+            const int d = order[static_cast<size_t>(i - 1)];
+            return run_pass(v[static_cast<size_t>(i - 1)], v[static_cast<size_t>(i)], nd, d, F[d], bcs[d], is_double,
+                            fast, stream);
+        };
+        if (want_ds && fuse01 && m > 2) {
+            if ((st = alloc(&v[2])) != DIPAMD_OK) return finish(st);
+            const InternFilter *fp[DIPAMD_MAX_DIMS];
+            bool proc[DIPAMD_MAX_DIMS];
+            for (int d = 0; d < nd; ++d) {
+                fp[d] = &F[d];
+                proc[d] = d == 0 || d == 1;
+            }
+            if ((st = separable_device(x0, v[2], nd, fp, bcs, proc, fast ? DIPAMD_FAST : 0u, stream)) != DIPAMD_OK)
+                return finish(st);
+            v1_pending = true;
+            for (int i = 3; i < m; ++i)
+                if ((st = make_v(i)) != DIPAMD_OK) return finish(st);
+        } else if (want_ds) {
             for (int i = 1; i < m; ++i) {
                 if ((st = alloc(&v[static_cast<size_t>(i)])) != DIPAMD_OK) return finish(st);
                 const int d = order[static_cast<size_t>(i - 1)];
@@ -394,6 +418,8 @@ extern "C" int dipamd_gauss_fir_backward(const dipamd_image *x, const dipamd_ima
                 st = fused_backward_2d(xv, u[2], dxv, nd, F[0], F[1], dF[0], dF[1], bcs[0], bcs[1],
                                        want_ds ? gsig : nullptr, kBwdSumAdd, nd, stream);
                 if (st != -100) return finish(st);
+                if (v1_pending && (st = make_v(1)) != DIPAMD_OK) return finish(st);
+                v1_pending = false;
             }
             if (want_ds) {
                 // dL/dsigma_d = < u_i , K'_i v_{i-1} >

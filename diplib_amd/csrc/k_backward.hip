@@ -12,6 +12,8 @@
 
 namespace dipamd {
 
+constexpr int kFoldFast = 16;  // borders up to this take the register path
+
 template <typename TB>
 __global__ __launch_bounds__(kBlock) void k_adjoint_fold(const PassArgs<TB> a) {
     const int64_t line = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -24,6 +26,43 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_fold(const PassArgs<TB> a) {
     const int B = a.border;
     const bool odd = a.shift != 0;  // the host passes 1 for an odd filter
     // adjoint correlation taps: cT_k = c_{-k}; taps[] holds |k| magnitudes of cT
+    if (B <= kFoldFast) {
+        // the B + 1 samples at each end of the line are loaded once, all at a time; the
+        // sums run over them in the same order as below (k ascending)
+        for (int side = 0; side < 2; ++side) {
+            TB e[kFoldFast + 1];
+#pragma unroll
+            for (int q = 0; q <= kFoldFast; ++q) {
+                const int qq = q <= B && q < n ? q : 0;
+                e[q] = g[static_cast<int64_t>(side == 0 ? qq : n - 1 - qq) * a.sin];
+            }
+#pragma unroll
+            for (int t = 1; t <= kFoldFast; ++t) {
+                if (t > B) break;
+                const int j = side == 0 ? -t : n - 1 + t;
+                int mul;
+                const int m = bc_fold(j, n, a.bc, &mul);
+                if (m < 0) continue;
+                TB h = TB(0);
+                if (side == 0) {  // k = t..B reads sample k - t
+#pragma unroll
+                    for (int k = 1; k <= kFoldFast; ++k)
+                        if (k >= t && k <= B && k - t < n) h += a.taps[k] * e[k - t];
+                } else {  // k = -B..-t reads sample n - 1 - q, q = -t - k
+#pragma unroll
+                    for (int q = kFoldFast - 1; q >= 0; --q) {
+                        if (q <= B - t && q < n) {
+                            const TB c = a.taps[t + q];
+                            h += (odd ? -c : c) * e[q];
+                        }
+                    }
+                }
+                TB *o = out + static_cast<int64_t>(m) * a.sout;
+                *o = mul < 0 ? *o - h : *o + h;
+            }
+        }
+        return;
+    }
     for (int side = 0; side < 2; ++side) {
         for (int t = 1; t <= B; ++t) {
             const int j = side == 0 ? -t : n - 1 + t;
