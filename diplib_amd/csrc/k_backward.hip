@@ -17,6 +17,18 @@ constexpr int kFoldFast = 16;  // borders up to this take the register path
 template <typename TB>
 __global__ __launch_bounds__(kBlock) void k_adjoint_fold(const PassArgs<TB> a) {
     const int64_t line = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    // the boundary map of the 2B halo positions is the same for every line: computed
+    // once per workgroup (packed as in bc_fold_packed: source << 1 | negate, -1 constant)
+    __shared__ int fmap[2][kFoldFast + 1];
+    if (a.border <= kFoldFast) {
+        const int i = threadIdx.x;
+        if (i < 2 * a.border) {
+            const int side = i / a.border, t = i % a.border + 1;
+            const int n0 = static_cast<int>(a.n);
+            fmap[side][t] = bc_fold_packed(side == 0 ? -t : n0 - 1 + t, n0, a.bc);
+        }
+        __syncthreads();
+    }
     if (line >= a.nlines) return;
     int64_t oin, oout;
     outer_offsets(a.outer, line, &oin, &oout);
@@ -36,13 +48,9 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_fold(const PassArgs<TB> a) {
                 const int qq = q <= B && q < n ? q : 0;
                 e[q] = g[static_cast<int64_t>(side == 0 ? qq : n - 1 - qq) * a.sin];
             }
+            TB hs[kFoldFast + 1];
 #pragma unroll
             for (int t = 1; t <= kFoldFast; ++t) {
-                if (t > B) break;
-                const int j = side == 0 ? -t : n - 1 + t;
-                int mul;
-                const int m = bc_fold(j, n, a.bc, &mul);
-                if (m < 0) continue;
                 TB h = TB(0);
                 if (side == 0) {  // k = t..B reads sample k - t
 #pragma unroll
@@ -57,8 +65,16 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_fold(const PassArgs<TB> a) {
                         }
                     }
                 }
+                hs[t] = h;
+            }
+#pragma unroll
+            for (int t = 1; t <= kFoldFast; ++t) {
+                if (t > B) break;
+                const int pk = fmap[side][t];
+                if (pk < 0) continue;  // constant condition: no dependence on the input
+                const int m = pk >> 1, mul = (pk > 0 && (pk & 1)) ? -1 : 1;
                 TB *o = out + static_cast<int64_t>(m) * a.sout;
-                *o = mul < 0 ? *o - h : *o + h;
+                *o = mul < 0 ? *o - hs[t] : *o + hs[t];
             }
         }
         return;
