@@ -63,6 +63,7 @@ int fold_pass(const View &g, const View &out, int ndim, int d, const InternFilte
     a.border = FT.border;
     a.shift = FT.sym == DIPAMD_SYM_ODD ? 1 : 0;
     a.bc = bc;
+    a.zlo = INT32_MIN;
     for (size_t k = 0; k < FT.data.size() && k < static_cast<size_t>(kLineCap); ++k) a.taps[k] = static_cast<TB>(FT.data[k]);
     // lines: every other dim
     int dims[DIPAMD_MAX_DIMS];

@@ -155,6 +155,15 @@ int intern_filter(const double *taps, int n, int sym, int origin_in, InternFilte
     F.origin = F.size - origin - 1;
     F.border = std::max(F.origin, F.size - F.origin - 1);
     F.shift = (sym == DIPAMD_SYM_GENERAL) ? F.origin : F.origin - (n - 1);
+    F.zpad = 0;
+    if ((sym == DIPAMD_SYM_D_EVEN || sym == DIPAMD_SYM_D_ODD) && F.origin >= n) {
+        // The d-* line filter (convolution.cpp:256-283) reads x[i - origin - 1] for its
+        // last tap, one sample past the border of :355-368 when the origin lies left of
+        // the centre.  The reference's line buffer has no sample there (uninitialised
+        // memory, 0 in every observed run); here it is one more border sample, read as 0.
+        F.border = F.origin + 1;
+        F.zpad = 1;
+    }
     return DIPAMD_OK;
 }
 
@@ -328,6 +337,7 @@ static int run_single(const View &src, const View &dst, int ndim, int d, const I
     a.ntaps = static_cast<int32_t>(F.data.size());
     a.shift = F.shift;
     a.bc = bc;
+    a.zlo = F.zpad ? -(F.border - F.zpad) : INT32_MIN;
     a.cval = const_value<TB>(bc);
     for (size_t k = 0; k < F.data.size(); ++k) a.taps[k] = static_cast<TB>(F.data[k]);
     const InType ti = (src.dt == DIPAMD_DT_UINT8) ? IN_U8 : IN_BUF;
@@ -450,7 +460,7 @@ static int run_extended_pass(const View &src, const View &dst, int ndim, int d, 
         TraceScope ts("expand", total * static_cast<int64_t>(sizeof(TB)) +
                                     npx_of(src, ndim) * static_cast<int64_t>(dt_size(src.dt)), stream);
         e = launch_expand(src.dt == DIPAMD_DT_UINT8 ? IN_U8 : IN_BUF, sizeof(TB) == 8, src.p, t1.p, &g, d,
-                          static_cast<int>(src.sizes[d]), B, bc, stream);
+                          static_cast<int>(src.sizes[d]), B - F.zpad, F.zpad, bc, stream);
         if (e != hipSuccess) st = set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
     }
     if (st == DIPAMD_OK) st = run_single<TB>(t1, t2, ndim, d, F, DIPAMD_BC_ZERO_ORDER_EXTRAPOLATE, fast, stream);

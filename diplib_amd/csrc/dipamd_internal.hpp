@@ -29,6 +29,10 @@ struct InternFilter {
     int border = 0;
     int shift = 0;
     int sym = DIPAMD_SYM_GENERAL;
+    // 1 when the filter reads one sample past the reference's border on the left
+    // (d-even / d-odd with an origin left of the centre): `border` then includes that
+    // sample, which reads as 0.
+    int zpad = 0;
 };
 
 // Records HIP events around a kernel launch while dipamd_trace_enable(1) is on.

@@ -10,10 +10,11 @@ namespace dipamd {
 
 namespace {
 
-// One thread per sample of the extended buffer (contiguous, dim `d` of size n+2B).
+// One thread per sample of the extended buffer (contiguous, dim `d` of size
+// n + 2 (border + zpad)); the zpad outermost samples of each side are 0.
 template <typename TI, typename TB>
 __global__ __launch_bounds__(kBlock) void k_expand(const TI *in, TB *out, const CastGeo g, int d, int n,
-                                                   int border, int bc) {
+                                                   int border, int zpad, int bc) {
     for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < g.total;
          t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         int64_t r = t, oline = 0, oo = 0;
@@ -30,10 +31,12 @@ __global__ __launch_bounds__(kBlock) void k_expand(const TI *in, TB *out, const 
         }
         const TI *line = in + oline;
         const int64_t st = g.sin[d];
-        const int i = p - border;
+        const int i = p - border - zpad;
         TB v;
         if (i >= 0 && i < n) {
             v = to_buf<TB>(line[static_cast<int64_t>(i) * st]);
+        } else if (i < -border || i > n - 1 + border) {
+            v = TB(0);
         } else if (i < 0) {
             v = extension_value<TI, TB>(line, st, n, 0, -i, border, bc);
         } else {
@@ -44,23 +47,24 @@ __global__ __launch_bounds__(kBlock) void k_expand(const TI *in, TB *out, const 
 }
 
 template <typename TI, typename TB>
-hipError_t expand_t(const void *in, void *out, const CastGeo &g, int d, int n, int border, int bc, hipStream_t st) {
+hipError_t expand_t(const void *in, void *out, const CastGeo &g, int d, int n, int border, int zpad, int bc,
+                    hipStream_t st) {
     int64_t nblk = (g.total + kBlock - 1) / kBlock;
     if (nblk > 65536) nblk = 65536;
     if (nblk < 1) nblk = 1;
     hipLaunchKernelGGL((k_expand<TI, TB>), dim3(static_cast<unsigned>(nblk)), dim3(kBlock), 0, st,
-                       static_cast<const TI *>(in), static_cast<TB *>(out), g, d, n, border, bc);
+                       static_cast<const TI *>(in), static_cast<TB *>(out), g, d, n, border, zpad, bc);
     return hipGetLastError();
 }
 
 }  // namespace
 
 hipError_t launch_expand(InType ti, bool is_double, const void *in, void *out, const void *geo, int d, int n,
-                         int border, int bc, hipStream_t stream) {
+                         int border, int zpad, int bc, hipStream_t stream) {
     const CastGeo &g = *static_cast<const CastGeo *>(geo);
-    if (is_double) return expand_t<double, double>(in, out, g, d, n, border, bc, stream);
-    return ti == IN_U8 ? expand_t<uint8_t, float>(in, out, g, d, n, border, bc, stream)
-                       : expand_t<float, float>(in, out, g, d, n, border, bc, stream);
+    if (is_double) return expand_t<double, double>(in, out, g, d, n, border, zpad, bc, stream);
+    return ti == IN_U8 ? expand_t<uint8_t, float>(in, out, g, d, n, border, zpad, bc, stream)
+                       : expand_t<float, float>(in, out, g, d, n, border, zpad, bc, stream);
 }
 
 }  // namespace dipamd

@@ -44,7 +44,7 @@ __global__ __launch_bounds__(kBlock) void k_zsweep(const PassArgs<TB> a, int64_t
 
     // sample j of the line, boundary-extended where the block touches an end
     auto load = [&](int j, bool inside) -> TB {
-        return inside ? to_buf<TB>(src[static_cast<int64_t>(j) * sin]) : fetch_ext<TI, TB>(src, sin, j, n, a.bc, a.cval);
+        return inside ? to_buf<TB>(src[static_cast<int64_t>(j) * sin]) : fetch_ext<TI, TB>(src, sin, j, n, a.bc, a.cval, a.zlo);
     };
     TB w[NW];
     {

@@ -153,6 +153,7 @@ struct PassArgs {
     int32_t ntaps;    // stored taps (data size)
     int32_t shift;    // GENERAL: first tap reads x[i - shift]
     int32_t bc;
+    int32_t zlo;      // extended-line coordinates below zlo read as 0 (INT32_MIN: none)
     TB cval;          // constant boundary value
     TB taps[kLineCap];
 };

@@ -42,9 +42,10 @@ hipError_t launch_colsweep(InType ti, bool is_double, int sym, bool fast, const 
 hipError_t launch_fused(InType ti, bool is_double, int cfg, int symx, int symy, bool fast, const void *args,
                         int64_t nx, int64_t nchunks, int64_t nouter, int rx, int ry, hipStream_t stream);
 
-// Extended lines for the boundary conditions computed from edge values (k_expand.hip).
+// Extended lines for the boundary conditions computed from edge values (k_expand.hip):
+// `border` samples computed on each side, then `zpad` samples of 0 beyond them.
 hipError_t launch_expand(InType ti, bool is_double, const void *in, void *out, const void *geo, int d, int n,
-                         int border, int bc, hipStream_t stream);
+                         int border, int zpad, int bc, hipStream_t stream);
 
 hipError_t launch_cast_in(int dt_in, bool to_double, const void *in, void *out, const void *geo, int64_t total,
                           hipStream_t stream);

@@ -76,10 +76,13 @@ __device__ __forceinline__ int bc_fold(int j, int n, int bc, int *mul) {
     return v < 0 ? -1 : (v >> 1);
 }
 
-// Sample j of a line of n samples at `base` with stride `st`, boundary-extended.
+// Sample j of a line of n samples at `base` with stride `st`, boundary-extended;
+// coordinates below `zlo` (past the reference's border) read as 0.
 template <typename TI, typename TB>
-__device__ __forceinline__ TB fetch_ext(const TI *base, int64_t st, int j, int n, int bc, TB cval) {
+__device__ __forceinline__ TB fetch_ext(const TI *base, int64_t st, int j, int n, int bc, TB cval,
+                                        int zlo = INT32_MIN) {
     if (static_cast<unsigned>(j) < static_cast<unsigned>(n)) return to_buf<TB>(base[static_cast<int64_t>(j) * st]);
+    if (j < zlo) return TB(0);
     int mul;
     const int s = bc_fold(j, n, bc, &mul);
     if (s < 0) return cval;
@@ -154,11 +157,12 @@ struct GColExtSrc {  // a column in global memory near its ends: boundary-extend
     int z, n, bc;
     TB cval;
     int ri, li;
-    __device__ TB at(int q) const { return fetch_ext<TI, TB>(line, st, z + q, n, bc, cval); }
+    int zlo;
+    __device__ TB at(int q) const { return fetch_ext<TI, TB>(line, st, z + q, n, bc, cval, zlo); }
     __device__ void rbegin(int q) { ri = z + q; }
-    __device__ TB rnext() { return fetch_ext<TI, TB>(line, st, ri++, n, bc, cval); }
+    __device__ TB rnext() { return fetch_ext<TI, TB>(line, st, ri++, n, bc, cval, zlo); }
     __device__ void lbegin(int q) { li = z + q; }
-    __device__ TB lnext() { return fetch_ext<TI, TB>(line, st, li--, n, bc, cval); }
+    __device__ TB lnext() { return fetch_ext<TI, TB>(line, st, li--, n, bc, cval, zlo); }
 };
 
 // P outputs from a window source.  `shift` moves the window: sample q is read as
@@ -315,7 +319,7 @@ __global__ __launch_bounds__(kBlock) void k_line(const PassArgs<TB> a, int64_t n
             const TI *s0 = src + (x0 - B);
             for (int i = lane; i < W; i += 64) row[lpad(i)] = to_buf<TB>(s0[i]);
         } else {
-            for (int i = lane; i < W; i += 64) row[lpad(i)] = fetch_ext<TI, TB>(src, a.sin, x0 - B + i, n, a.bc, a.cval);
+            for (int i = lane; i < W; i += 64) row[lpad(i)] = fetch_ext<TI, TB>(src, a.sin, x0 - B + i, n, a.bc, a.cval, a.zlo);
         }
     }
     lds_barrier();
@@ -373,7 +377,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const PassArgs<TB> a, int64_t 
             GColSrc<TI, TB> s{src + zb * sin, sin, nullptr, nullptr};
             fir_window<TB, P, SYM, FAST>(a.taps, a.ntaps, a.shift, s, acc);
         } else {
-            GColExtSrc<TI, TB> s{src, sin, zb, n, a.bc, a.cval, 0, 0};
+            GColExtSrc<TI, TB> s{src, sin, zb, n, a.bc, a.cval, 0, 0, a.zlo};
             fir_window<TB, P, SYM, FAST>(a.taps, a.ntaps, a.shift, s, acc);
         }
         TB *d = dst + static_cast<int64_t>(zb) * a.sout;
@@ -441,7 +445,7 @@ __global__ __launch_bounds__(64 * NW) void k_colsweep(const PassArgs<TB> a, int6
             if (h < H) {
                 const int j = z0 - B + h;
                 v[i] = interior ? to_buf<TB>(src[static_cast<int64_t>(j) * a.sin])
-                                : fetch_ext<TI, TB>(src, a.sin, j, n, a.bc, a.cval);
+                                : fetch_ext<TI, TB>(src, a.sin, j, n, a.bc, a.cval, a.zlo);
             }
         }
 #pragma unroll

@@ -232,8 +232,10 @@ class OneDimensionalFilter:
         self.symmetry = symmetry
 
 
-def SeparableConvolution(in_, filter, boundaryCondition=(), process=(), out=None, fast=False):
-    """dip::SeparableConvolution (linear.h:114-130) with real 1-D filters."""
+def SeparableConvolution(in_, filter, boundaryCondition=(), process=(), out=None, fast=False, fusion=None):
+    """dip::SeparableConvolution (linear.h:114-130) with real 1-D filters.  `fusion`
+    False runs one pass per axis, True the fused 2-D kernel wherever it applies, None
+    lets the planner choose."""
     x, was_np = _to_device(in_)
     filters = filter if isinstance(filter, (list, tuple)) else [filter]
     nd = x.dim()
@@ -258,10 +260,14 @@ def SeparableConvolution(in_, filter, boundaryCondition=(), process=(), out=None
     b_arr = (ctypes.c_int32 * max(1, len(bcs)))(*bcs)
     y = _prepare_out(x, out, _flex_torch(x.dtype))
     a_in, a_out = image_struct(x), image_struct(y)
+    flags = _lib.FAST if fast else _lib.EXACT
+    if fusion is False:
+        flags |= _lib.NO_FUSION
+    elif fusion is True:
+        flags |= _lib.FORCE_FUSION
     with torch.cuda.device(x.device):
         check(lib().dipamd_separable_convolution(ctypes.byref(a_in), ctypes.byref(a_out), farr, len(filters), b_arr,
-                                                 len(bcs), p_arr, _lib.FAST if fast else _lib.EXACT,
-                                                 _stream_handle(x.device)))
+                                                 len(bcs), p_arr, flags, _stream_handle(x.device)))
     if was_np and out is None:
         return y.cpu().numpy()
     return y

@@ -491,7 +491,12 @@ static int separable_core(const void *in, int in_dt, void *out, int out_dt, int 
         int nt = nthreads > 0 ? nthreads : 1;
 #pragma omp parallel num_threads(nt)
         {
-            void *bufmem = malloc((size_t)(len + 2 * border) * bsz);
+            /* one spare sample before the buffer, held at 0: d-even / d-odd filters with an
+             * origin left of the centre read one sample beyond the border (convolution.cpp:
+             * 256-283 against the border of :355-368); the reference's line buffer
+             * (AlignedBuffer, types.h:874-930) has no sample there and reads uninitialised
+             * memory, which is 0 in every observed run */
+            void *bufmem = calloc((size_t)(len + 2 * border + 1), bsz);
             float *bufo_f = use_double ? NULL : (float *)malloc((size_t)len * sizeof(float));
             double *bufo_d = use_double ? (double *)malloc((size_t)len * sizeof(double)) : NULL;
 #pragma omp for schedule(static)
@@ -505,7 +510,7 @@ static int separable_core(const void *in, int in_dt, void *out, int out_dt, int 
                     base += c * strides[d];
                 }
                 if (use_double) {
-                    double *b = (double *)bufmem + border;
+                    double *b = (double *)bufmem + border + 1;
                     if (src_dt == DT_DF) {
                         const double *s = (const double *)src;
                         for (long i = 0; i < len; i++) b[i] = s[base + i * st];
@@ -521,7 +526,7 @@ static int separable_core(const void *in, int in_dt, void *out, int out_dt, int 
                         for (long i = 0; i < len; i++) store_sample(dst, dst_dt, base + i * st, bufo_d[i], 0);
                     }
                 } else {
-                    float *b = (float *)bufmem + border;
+                    float *b = (float *)bufmem + border + 1;
                     if (src_dt == DT_SF) {
                         const float *s = (const float *)src;
                         for (long i = 0; i < len; i++) b[i] = s[base + i * st];

@@ -29,6 +29,8 @@ dip::DataType ToDip( int code ) {
       case 6: return dip::DT_SFLOAT;
       case 7: return dip::DT_DFLOAT;
       case 8: return dip::DT_BIN;
+      case 9: return dip::DT_SCOMPLEX;
+      case 10: return dip::DT_DCOMPLEX;
       default: throw std::runtime_error( "unknown dtype code" );
    }
 }
@@ -164,11 +166,14 @@ int dipref_half_gaussian( double sigma, int order, double truncation, int is_dou
 }
 
 // dip::SeparableConvolution with one 1-D filter per dimension (or one for all), given as
-// full tap arrays with a symmetry string each ("general", "even", "odd", "d-even", "d-odd")
-// and origin -1 (centre).
+// full tap arrays with a symmetry string each ("general", "even", "odd", "conj", "d-even",
+// "d-odd", "d-conj"), an origin each (`origins` NULL: -1, the centre) and a complex flag
+// each (`is_complex` NULL: all real; a complex filter's array holds re, im pairs and
+// `ntaps` counts doubles, as OneDimensionalFilter::filter does).
 int dipref_separable( void const* in, int in_dtype, void* out, int out_dtype, int ndim, int64_t const* sizes,
                       double const* const* taps, int const* ntaps, char const* const* symmetry, int nfilters,
-                      char const* const* bcs, int nbc, unsigned char const* process ) {
+                      char const* const* bcs, int nbc, unsigned char const* process, int const* origins,
+                      int const* is_complex ) {
    try {
       dip::Image imgIn = Wrap( in, in_dtype, ndim, sizes );
       dip::Image imgOut = Wrap( out, out_dtype, ndim, sizes );
@@ -178,6 +183,8 @@ int dipref_separable( void const* in, int in_dtype, void* out, int out_dtype, in
          auto& f = fa[ static_cast< dip::uint >( ii ) ];
          f.filter.assign( taps[ ii ], taps[ ii ] + ntaps[ ii ] );
          f.symmetry = symmetry[ ii ];
+         if( origins ) { f.origin = origins[ ii ]; }
+         if( is_complex ) { f.isComplex = is_complex[ ii ] != 0; }
       }
       dip::StringArray bc;
       for( int ii = 0; ii < nbc; ++ii ) { bc.emplace_back( bcs[ ii ] ); }

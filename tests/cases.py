@@ -263,3 +263,85 @@ def method_flags(fn, sigmas, method):
         return m
     tsig = [float(fn.split(":")[1])] if fn.startswith("structure:") else sigmas
     return one(sigmas), one(tsig)
+
+
+# dip::SeparableConvolution with user filters (convolution.cpp:67-162, 202-297):
+# (name, numpy shape, dtype, filters, boundary list, process or None, out dtype or None),
+# each filter a (taps in the user's order, symmetry string, origin) triple; one filter is
+# used for every dim, else one per dim (dip order).  Taps are seeded N(0,1) values;
+# "neg" filters have only negative taps, so uint8 zeros give -0.0 products (the general
+# and d-* forms start their sums from +0, the others from the centre product).
+def sep_taps(n, seed, neg=False):
+    t = np.random.default_rng(seed).standard_normal(n)
+    return list(-np.abs(t)) if neg else list(t)
+
+
+SEP_SYMS = ("general", "even", "odd", "conj", "d-even", "d-odd", "d-conj")
+SEP_CASES = []
+# every symmetry, odd and even stored lengths, centre origin, both passes
+for i, sym in enumerate(SEP_SYMS):
+    t = sym.replace("-", "")
+    for n in (3, 4):
+        SEP_CASES.append((f"sep_{t}_n{n}_f32_45x33", (33, 45), "float32",
+                          [(sep_taps(n, 100 + 10 * i + n), sym, -1)], ["mirror"], None, None))
+    SEP_CASES.append((f"sep_{t}_n1_f32_30x20", (20, 30), "float32", [(sep_taps(1, 200 + i), sym, -1)],
+                      ["mirror"], None, None))
+# origins: 0, the last index, off-centre; per-dim filters of different symmetries
+for i, sym in enumerate(SEP_SYMS):
+    t = sym.replace("-", "")
+    n = 4
+    size = n if sym == "general" else (2 * n - 1 if sym in ("even", "odd", "conj") else 2 * n)
+    for org in (0, size - 1, 1):
+        SEP_CASES.append((f"sep_{t}_origin{org}_f32_40x31", (31, 40), "float32",
+                          [(sep_taps(n, 300 + 10 * i + org), sym, org), (sep_taps(3, 400 + i), "even", org % 5)],
+                          ["mirror", "periodic"], None, None))
+# every index-map boundary condition with lines shorter than the filter (bounce, wrap)
+for j, bc in enumerate(("mirror", "asym mirror", "periodic", "asym periodic", "add zeros", "add max", "add min",
+                        "zero order")):
+    t = bc.replace(" ", "_")
+    for i, sym in enumerate(("general", "d-odd", "even")):
+        SEP_CASES.append((f"sep_{sym.replace('-', '')}_{t}_short_5x3", (3, 5), "float32",
+                          [(sep_taps(7, 500 + 10 * j + i), sym, 2 if sym == "general" else -1)], [bc], None, None))
+    SEP_CASES.append((f"sep_general_{t}_f32_37x29", (29, 37), "float32",
+                      [(sep_taps(5, 600 + j), "general", 1), (sep_taps(4, 620 + j), "d-even", 0)], [bc], None, None))
+# conditions computed from the edge values
+for j, bc in enumerate(("antisym reflect", "first order", "second order", "third order")):
+    t = bc.replace(" ", "_")
+    SEP_CASES.append((f"sep_general_{t}_f32_37x29", (29, 37), "float32",
+                      [(sep_taps(6, 700 + j), "general", 4), (sep_taps(3, 720 + j), "odd", -1)], [bc], None, None))
+    SEP_CASES.append((f"sep_deven_{t}_short_4x3", (3, 4), "float32",
+                      [(sep_taps(5, 740 + j), "d-even", 1)], [bc], None, None))
+# process masks, meaningless and empty filters
+SEP_CASES += [
+    ("sep_process_x_f32_40x30", (30, 40), "float32", [(sep_taps(5, 800), "general", -1)], ["mirror"], [1, 0], None),
+    ("sep_process_y_f32_40x30", (30, 40), "float32", [(sep_taps(3, 801), "odd", -1)], ["mirror"], [0, 1], None),
+    ("sep_identity_x_f32_40x30", (30, 40), "float32", [([1.0], "general", -1), (sep_taps(3, 802), "even", -1)],
+     ["mirror"], None, None),
+    ("sep_empty_y_f32_40x30", (30, 40), "float32", [(sep_taps(4, 803), "d-odd", -1), ([], "general", -1)],
+     ["mirror"], None, None),
+    ("sep_process_none_u8_40x30", (30, 40), "uint8", [(sep_taps(3, 804), "even", -1)], ["mirror"], [0, 0], None),
+]
+# signed zeros: uint8 zeros times negative taps
+for i, sym in enumerate(("general", "d-even", "d-odd", "even", "odd")):
+    SEP_CASES.append((f"sep_{sym.replace('-', '')}_neg_u8_48x40", (40, 48), "uint8",
+                      [(sep_taps(3, 850 + i, neg=True), sym, -1)], ["add zeros"], None, None))
+# other types, dims and lengths: uint8 and dfloat input, 3-D, 1-D, long filters (past
+# the fused kernels' reach), narrow output types
+SEP_CASES += [
+    ("sep_even_u8_64x48", (48, 64), "uint8", [(sep_taps(5, 900), "even", -1)], ["mirror"], None, None),
+    ("sep_odd_f64_45x33", (33, 45), "float64", [(sep_taps(4, 901), "odd", -1)], ["periodic"], None, None),
+    ("sep_general_f64_45x33_origin0", (33, 45), "float64", [(sep_taps(6, 902), "general", 0)], ["mirror"], None,
+     None),
+    ("sep_deven_s16_45x33", (33, 45), "int16", [(sep_taps(3, 903), "d-even", -1)], ["mirror"], None, None),
+    ("sep_mixed_f32_21x17x9", (9, 17, 21), "float32",
+     [(sep_taps(3, 904), "even", -1), (sep_taps(4, 905), "general", 1), (sep_taps(2, 906), "d-odd", 0)],
+     ["mirror", "periodic", "add zeros"], None, None),
+    ("sep_general_1d_f32_200", (200,), "float32", [(sep_taps(9, 907), "general", 6)], ["mirror"], None, None),
+    ("sep_even_r40_f32_130x90", (90, 130), "float32", [(sep_taps(41, 908), "even", -1)], ["mirror"], None, None),
+    ("sep_general_n61_f32_150x80", (80, 150), "float32", [(sep_taps(61, 909), "general", 20)], ["periodic"], None,
+     None),
+    ("sep_dodd_n30_f32_120x70", (70, 120), "float32", [(sep_taps(30, 910), "d-odd", -1)], ["mirror"], None, None),
+    ("sep_even_f32_out_u8_64x48", (48, 64), "float32", [([0.25, 0.5], "even", -1)], ["mirror"], None, "uint8"),
+    ("sep_odd_u8_out_s16_64x48", (48, 64), "uint8", [([-1.0, 0.0], "odd", -1)], ["mirror"], None, "int16"),
+    ("sep_even_u8_r4_256x64", (64, 256), "uint8", [(sep_taps(5, 911), "even", -1)], ["periodic"], None, None),
+]

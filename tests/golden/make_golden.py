@@ -17,6 +17,9 @@ Outputs (all small):
   ft_cases.npz       input and dip::GaussFT output of tests/cases.py:FT_CASES
   deriv_ft_cases.npz the derivative family and structure tensor with the FT method
                      (tests/cases.py:DERIV_FT_CASES)
+  sep_cases.npz      input and dip::SeparableConvolution output of tests/cases.py:SEP_CASES
+                     (user filters of every symmetry, origins, process masks, every
+                     boundary condition)
   taps.npz           MakeHalfGaussian(sigma, order, 3, sfloat/dfloat) for TAP_SIGMAS x orders 0..3
   small_cases.npz    input and dip::Gauss output of every case in tests/cases.py:SMALL_CASES
   large_cases.json   SHA-256 of the exact output bytes + sampled pixels for LARGE_CASES
@@ -98,9 +101,22 @@ def write_derivative_ft_cases(fname="deriv_ft_cases.npz", seed0=9900):
     np.savez_compressed(os.path.join(HERE, fname), **arrs)
 
 
+def write_sep_cases(fname="sep_cases.npz", seed0=11000):
+    arrs = {}
+    for i, (name, shape, dt, filters, bc, proc, odt) in enumerate(cases.SEP_CASES):
+        x = cases.make_input(shape, dt, seed=seed0 + i)
+        taps = [f[0] for f in filters]
+        arrs[name + "__in"] = x
+        arrs[name + "__out"] = oc.separable(x, taps, [f[1] for f in filters], bc, proc, odt,
+                                            origins=[f[2] for f in filters], use_ref=True)
+    np.savez_compressed(os.path.join(HERE, fname), **arrs)
+
+
 def main():
     assert oc.ref_available(), "build oracle/_ref/libdipref.so first (make -f oracle/ref.mk)"
     only = sys.argv[sys.argv.index("--only") + 1] if "--only" in sys.argv else None
+    if only in (None, "sep"):
+        write_sep_cases()
     if only in (None, "extrap"):
         write_cases(cases.EXTRAP_CASES, "extrap_cases.npz", 3000)
     if only in (None, "ft"):

@@ -15,7 +15,8 @@ REF_PATH = os.path.join(ROOT, "oracle", "_ref", "libdipref.so")
 # data-type codes shared by the oracle, the reference wrapper and include/dipamd.h
 DT = {np.dtype(np.uint8): 0, np.dtype(np.uint16): 1, np.dtype(np.uint32): 2,
       np.dtype(np.int8): 3, np.dtype(np.int16): 4, np.dtype(np.int32): 5,
-      np.dtype(np.float32): 6, np.dtype(np.float64): 7, np.dtype(np.bool_): 8}
+      np.dtype(np.float32): 6, np.dtype(np.float64): 7, np.dtype(np.bool_): 8,
+      np.dtype(np.complex64): 9, np.dtype(np.complex128): 10}
 BC = {"": 0, "default": 0, "mirror": 0, "asym mirror": 1, "antisym reflect": 2, "periodic": 3,
       "asym periodic": 4, "add zeros": 5, "add max": 6, "add min": 7, "zero order": 8,
       "first order": 9, "second order": 10, "third order": 11}
@@ -54,7 +55,7 @@ def ref():
         lib.dipref_gauss.argtypes = [P, I, P, I, I, P, P, I, P, I, P, I, D, ctypes.c_char_p, I]
         lib.dipref_gauss_inplace.argtypes = [P, I, I, P, P, I, P, I, ctypes.c_char_p, D]
         lib.dipref_half_gaussian.argtypes = [D, I, D, I, P, I]
-        lib.dipref_separable.argtypes = [P, I, P, I, I, P, P, P, P, I, P, I, P]
+        lib.dipref_separable.argtypes = [P, I, P, I, I, P, P, P, P, I, P, I, P, P, P]
         lib.dipref_last_error.restype = ctypes.c_char_p
         if hasattr(lib, "dipref_gauss_ft"):
             lib.dipref_gauss_ft.argtypes = [P, I, P, I, I, P, P, I, P, I, P, I, D]
@@ -185,29 +186,40 @@ def ref_gauss_ft(x, sigmas, orders=(0,), bc=(), truncation=3.0, out_dtype=None):
 
 
 def separable(x, filters, symmetries, bc=("mirror",), process=None, out_dtype=None, origins=None,
-              nthreads=1, use_ref=False):
-    """dip::SeparableConvolution with real 1-D filters given as tap lists (user order)."""
+              nthreads=1, use_ref=False, is_complex=None):
+    """dip::SeparableConvolution with 1-D filters given as tap lists (user order).  A
+    complex filter (`is_complex[i]`, reference only) is a complex tap list."""
     x = np.ascontiguousarray(x)
-    out_dtype = np.dtype(out_dtype) if out_dtype is not None else _flex(x.dtype)
+    cplx = [bool(c) for c in is_complex] if is_complex is not None else [False] * len(filters)
+    if out_dtype is None:
+        out_dtype = _flex(x.dtype)
+        if any(cplx) or x.dtype.kind == "c":
+            out_dtype = np.dtype(np.complex128) if out_dtype == np.float64 else np.dtype(np.complex64)
+    out_dtype = np.dtype(out_dtype)
     out = np.zeros(x.shape, out_dtype)
     sizes = _sizes(x)
     nd = x.ndim
-    taps = [np.ascontiguousarray(f, np.float64) for f in filters]
+    taps = [np.ascontiguousarray(np.asarray(f, np.complex128).view(np.float64) if c else np.asarray(f, np.float64))
+            for f, c in zip(filters, cplx)]
     tptrs = (ctypes.c_void_p * len(taps))(*[t.ctypes.data for t in taps])
     ntaps = np.ascontiguousarray([len(t) for t in taps], np.int32)
     bcl = list(bc) if not isinstance(bc, str) else [bc]
     proc = None if process is None else np.ascontiguousarray(process, np.uint8)
+    org = None if origins is None else np.ascontiguousarray(origins, np.int32)
     if use_ref:
         syms = (ctypes.c_char_p * len(taps))(*[s.encode() for s in symmetries])
         arr = (ctypes.c_char_p * max(1, len(bcl)))(*[b.encode() for b in bcl])
+        cx = np.ascontiguousarray([int(c) for c in cplx], np.int32)
         st = ref().dipref_separable(_ptr(x), DT[x.dtype], _ptr(out), DT[out_dtype], nd, _ptr(sizes),
                                     tptrs, _ptr(ntaps), syms, len(taps), arr, len(bcl),
-                                    None if proc is None else _ptr(proc))
+                                    None if proc is None else _ptr(proc), None if org is None else _ptr(org),
+                                    _ptr(cx))
         if st != 0:
             raise RuntimeError(ref().dipref_last_error().decode())
     else:
+        if any(cplx) or x.dtype.kind == "c" or out_dtype.kind == "c":
+            raise ValueError("the C oracle takes real filters and images only")
         syms = np.ascontiguousarray([SYM[s] for s in symmetries], np.int32)
-        org = None if origins is None else np.ascontiguousarray(origins, np.int32)
         bci = np.ascontiguousarray([BC[b] for b in bcl], np.int32)
         st = orc().orc_separable(_ptr(x), DT[x.dtype], _ptr(out), DT[out_dtype], nd, _ptr(sizes),
                                  tptrs, _ptr(ntaps), _ptr(syms), None if org is None else _ptr(org),

@@ -25,6 +25,7 @@ GOLD = np.load(os.path.join(HERE, "golden", "small_cases.npz"))
 EXTRAP = np.load(os.path.join(HERE, "golden", "extrap_cases.npz"))
 DERIV = np.load(os.path.join(HERE, "golden", "deriv_cases.npz"))
 DERIV2 = np.load(os.path.join(HERE, "golden", "deriv2_cases.npz"))
+SEP = np.load(os.path.join(HERE, "golden", "sep_cases.npz"))
 
 
 @pytest.mark.parametrize("sigma", cases.TAP_SIGMAS)
@@ -143,6 +144,77 @@ def test_oracle_matches_reference_outputs(case):
     got = oc.gauss_fir(x, sig, ords, bc, 3.0, out_dtype=odt)
     assert got.dtype == exp.dtype
     assert np.array_equal(got.view(np.uint8), exp.view(np.uint8))
+
+
+@pytest.mark.parametrize("case", cases.SEP_CASES, ids=[c[0] for c in cases.SEP_CASES])
+def test_oracle_separable_matches_reference(case):
+    """dip::SeparableConvolution with user filters of every symmetry, origins, process
+    masks and boundary conditions: the oracle against the reference's outputs, bit for bit
+    (signed zeros included)."""
+    name, shape, dt, filters, bc, proc, odt = case
+    x = SEP[name + "__in"]
+    exp = SEP[name + "__out"]
+    got = oc.separable(x, [f[0] for f in filters], [f[1] for f in filters], bc, proc, odt,
+                       origins=[f[2] for f in filters])
+    assert got.dtype == exp.dtype
+    assert np.array_equal(got.view(np.uint8), exp.view(np.uint8))
+
+
+def _full_filter(taps, sym):
+    """The full convolution kernel a symmetric filter description stands for
+    (linear.h:70-80): even [a,b,c,b,a], odd [a,b,c,-b,-a], d-even [a,b,c,c,b,a],
+    d-odd [a,b,c,-c,-b,-a]; conj and d-conj are even and d-even for real taps."""
+    t = list(taps)
+    sym = {"conj": "even", "d-conj": "d-even"}.get(sym, sym)
+    if sym == "general":
+        return t
+    if sym == "even":
+        return t + t[-2::-1]
+    if sym == "odd":
+        return t + [-v for v in t[-2::-1]]
+    if sym == "d-even":
+        return t + t[::-1]
+    return t + [-v for v in t[::-1]]
+
+
+@pytest.mark.parametrize("sym", ["general", "even", "odd", "conj", "d-even", "d-odd", "d-conj"])
+@pytest.mark.parametrize("n,origin", [(3, -1), (4, -1), (4, 0), (4, 2), (5, 1)])
+@pytest.mark.parametrize("bc", ["mirror", "periodic", "add zeros"])
+def test_separable_dense_convolution(sym, n, origin, bc):
+    """The oracle's line filter against the textbook definition: out[i] = sum_j h[j]
+    x[i + o - j] with the full kernel h of the symmetry table and the user origin o
+    (centre = size // 2), samples outside the line taken from the boundary map.  A
+    transposed kernel, a wrong origin or a wrong d-* pairing fails this.
+
+    Two behaviours of the reference are part of the definition: the d-even / d-odd /
+    d-conj forms act one sample left of the documented origin (convolution.cpp:256-283
+    pair x[i] with x[i-1] for the centre taps), and for an origin left of the centre
+    they reach one sample past the border the framework extends (:355-368), which
+    reads as 0 (the reference's fixtures, test_oracle_separable_matches_reference)."""
+    taps = list(np.random.default_rng(n * 7 + max(origin, 0)).standard_normal(n))
+    h = _full_filter(taps, sym)
+    size = len(h)
+    if origin >= size:
+        pytest.skip("origin outside the filter")
+    o = size // 2 if origin < 0 else origin
+    oi = size - o - 1
+    border = max(oi, size - oi - 1)
+    if sym.startswith("d-"):
+        o -= 1
+    W = 11
+    x = np.random.default_rng(5).standard_normal((1, W))
+    exp = np.zeros(W)
+    for i in range(W):
+        for j in range(size):
+            q = i + o - j
+            if q < -border:
+                assert sym.startswith("d-") and q == -border - 1 and i == 0
+                continue
+            idx, sgn = oc.bc_map(q, W, bc)
+            if idx >= 0:
+                exp[i] += h[j] * sgn * x[0, idx]
+    got = oc.separable(x, [taps, [1.0]], [sym, "general"], [bc], process=[1, 0], origins=[origin, -1])
+    assert np.allclose(got[0], exp, rtol=1e-12, atol=1e-12)
 
 
 def test_thread_independence():
