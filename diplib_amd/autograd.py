@@ -75,7 +75,8 @@ class _GaussFIRFunction(torch.autograd.Function):
                 ctypes.byref(a_x), ctypes.byref(a_g), ctypes.byref(a_dx),
                 ctypes.c_void_p(ds.data_ptr()) if want_ds else None, s_arr, nd, o_arr, len(orders), b_arr, len(bcs),
                 truncation, flags, _stream_handle(x.device)))
-        dsigma = ds.to(sigma.dtype).reshape(sigma.shape) if want_ds else None
+        # on sigma's own device: a CPU sigma (e.g. a CPU nn.Parameter) with a GPU image
+        dsigma = ds.to(device=sigma.device, dtype=sigma.dtype).reshape(sigma.shape) if want_ds else None
         if want_dx:
             dx = dx.to(x.dtype)
         return dx, dsigma, None, None, None, None, None, None

@@ -309,9 +309,10 @@ static int64_t pick_chunk(int64_t len, int64_t base, int granule, int border, in
     chunk = ((chunk + granule - 1) / granule) * granule;
     // the halo (2*border rows) is re-read by each chunk: keep it under ~1/8 of the chunk,
     // unless that leaves fewer than 4 workgroups per CU (small images: then up to 1/2)
-    int64_t minc = ((16 * static_cast<int64_t>(border) + granule - 1) / granule) * granule;
+    // (a filter without border, e.g. a single tap, needs no halo: minc is one granule)
+    int64_t minc = std::max<int64_t>(granule, ((16 * static_cast<int64_t>(border) + granule - 1) / granule) * granule);
     if (base * ((len + minc - 1) / minc) < 1024)
-        minc = ((4 * static_cast<int64_t>(border) + granule - 1) / granule) * granule;
+        minc = std::max<int64_t>(granule, ((4 * static_cast<int64_t>(border) + granule - 1) / granule) * granule);
     chunk = std::max<int64_t>(chunk, std::max<int64_t>(granule, minc));
     int64_t full = ((len + granule - 1) / granule) * granule;
     return std::min(chunk, full);

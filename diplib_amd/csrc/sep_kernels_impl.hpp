@@ -174,8 +174,10 @@ __device__ __forceinline__ void fir_window(const TB *taps, int ntaps, int shift,
         const int r = ntaps - 1;
         TB rw[P], lw[P];
         TB w[P + 2];
+        // the outer two samples feed the k = 1 pairs: a single-tap filter (r = 0, border
+        // 0) must not read them, they lie outside its window
 #pragma unroll
-        for (int q = 0; q < P + 2; ++q) w[q] = s.at(q - 1 - shift);
+        for (int q = 0; q < P + 2; ++q) w[q] = (q > 0 && q < P + 1) || r > 0 ? s.at(q - 1 - shift) : TB(0);
         const TB f0 = taps[0];
 #pragma unroll
         for (int p = 0; p < P; ++p) {

@@ -120,3 +120,18 @@ def test_autograd_sigma_host(cuda):
         sigma.add_(1.0)
     y2 = dip.gauss_fir_autograd(x, sigma, [0], ["mirror"], 3.0)
     assert torch.equal(y2, dip.GaussFIR(x, [3.0], [0], ["mirror"], 3.0))
+
+
+def test_autograd_cpu_sigma_per_dim(cuda):
+    """A per-dim sigma held on the CPU (e.g. a CPU nn.Parameter) with a GPU image: the
+    gradient comes back on the CPU, equal to the one for the same sigma on the GPU."""
+    x = torch.from_numpy(np.random.default_rng(21).random((40, 52), dtype=np.float32)).to(cuda)
+    w = torch.from_numpy(np.random.default_rng(22).standard_normal((40, 52)).astype(np.float32)).to(cuda)
+    grads = []
+    for dev in ("cpu", cuda):
+        s = torch.tensor([2.2, 3.1], dtype=torch.float64, device=dev, requires_grad=True)
+        y = dip.gauss_fir_autograd(x, s, [0], ["mirror"], 3.0)
+        (y * w).sum().backward()
+        assert s.grad.device == s.device
+        grads.append(s.grad.detach().cpu())
+    assert torch.equal(grads[0], grads[1])
