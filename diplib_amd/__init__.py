@@ -6,7 +6,7 @@ device tensors.  Importing the package does not touch the GPU.
 """
 from ._lib import DipError, lib  # noqa: F401
 from .autograd import gauss_fir_autograd, gauss_fir_backward  # noqa: F401
-from .batch import GaussBatch  # noqa: F401
+from .batch import GaussBatch, gauss_batch_noalloc  # noqa: F401
 from .derivative import (Derivative, Dgg, Gradient, GradientMagnitude, Hessian, Laplace,  # noqa: F401
                          LaplaceMinusDgg, LaplacePlusDgg, StructureTensor)
 from .gauss import (Gauss, GaussFIR, GaussFT, GaussIIR, MakeHalfGaussian, OneDimensionalFilter,  # noqa: F401

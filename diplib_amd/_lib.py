@@ -85,6 +85,9 @@ def lib():
         L.dipamd_batch_run_per_image.argtypes = [P, P]
         L.dipamd_batch_destroy.argtypes = [P]
         L.dipamd_batch_info.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(I32)]
+    if hasattr(L, "dipamd_batch_gauss_noalloc"):
+        L.dipamd_batch_gauss_noalloc.argtypes = [ctypes.POINTER(DipamdBatchImage), I32, I32, I32, P, I32, D, U32, I32,
+                                                 P]
     L.dipamd_derivative.argtypes = [I32, IMG, IMG, I32, P, I32, P, I32, P, I32, D, U32, P]
     L.dipamd_gauss_iir.argtypes = [IMG, IMG, P, I32, P, I32, P, I32, P, I32, I32, D, U32, P]
     L.dipamd_gauss_ft.argtypes = [IMG, IMG, P, I32, P, I32, P, I32, D, U32, P]
@@ -108,7 +111,7 @@ EXPORTED_SYMBOLS = [
     "dipamd_batch_run", "dipamd_batch_run_graph", "dipamd_batch_run_per_image", "dipamd_batch_destroy", "dipamd_batch_num_configs",
     "dipamd_batch_info", "dipamd_gauss_fir_backward", "dipamd_gauss_radius", "dipamd_last_error", "dipamd_version",
     "dipamd_trace_enable", "dipamd_trace_read", "dipamd_derivative", "dipamd_derivative_outputs",
-    "dipamd_structure_tensor", "dipamd_gauss_iir", "dipamd_gauss_ft",
+    "dipamd_structure_tensor", "dipamd_gauss_iir", "dipamd_gauss_ft", "dipamd_batch_gauss_noalloc",
 ]
 
 FN_GRADIENT, FN_GRADIENT_MAGNITUDE, FN_HESSIAN, FN_LAPLACE = 0, 1, 2, 3

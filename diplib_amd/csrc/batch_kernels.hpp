@@ -10,6 +10,7 @@ namespace dipamd {
 hipError_t launch_batched(InType ti, bool is_double, int cfg, bool fast, const void *args, int grid, size_t lds,
                           hipStream_t stream);
 int batched_occupancy(InType ti, bool is_double, int cfg, bool fast, size_t lds);
-hipError_t launch_reset_counter(int32_t *counter, hipStream_t stream);
+// Clears `n` consecutive work-queue counters with a kernel (no memset node under capture).
+hipError_t launch_reset_counter(int32_t *counter, int n, hipStream_t stream);
 
 }  // namespace dipamd

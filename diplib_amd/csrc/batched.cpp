@@ -42,14 +42,21 @@ std::pair<int, int> batch_chunk_rule() {
 }
 }  // namespace
 
+// One persistent launch of a plan: a tile config, its dynamic LDS bytes, a slice of the
+// work-item table and the grid.
+struct BatchGroup {
+    int cfg;
+    size_t lds;
+    int64_t item0, nitems;
+    int grid;
+};
+
 struct dipamd_batch_plan {
     int n = 0;
     int in_dtype = 0, out_dtype = 0;
     bool is_double = false;
     bool fast = false;
     int cfg = 1;
-    int grid = 0;
-    size_t lds = 0;
     int64_t nitems = 0;
     int bc[2] = {0, 0};
     double truncation = 3;
@@ -59,7 +66,8 @@ struct dipamd_batch_plan {
     void *d_taps = nullptr;
     int32_t *d_counter = nullptr;
     std::vector<dipamd_batch_image> host_images;
-    std::vector<uint8_t> args;  // BatchArgs<TB> bytes
+    std::vector<BatchGroup> groups;
+    std::vector<std::vector<uint8_t>> args;  // BatchArgs<TB> bytes, one per group
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t cap_stream = nullptr;
@@ -70,13 +78,29 @@ namespace {
 std::mutex g_tune_mutex;
 std::map<std::tuple<int, int, int, int>, int> g_batch_tuned;  // (in dtype, fast, rmax, bc) -> cfg
 
+// Counters of the groups sit 64 bytes apart (one per cache line).
+constexpr int kCounterStride = 16;
+constexpr int kMaxGroups = 16;
+
+// The host side of a plan: per-image descriptors, work items (longest-processing-time
+// order), taps, and the launch groups.  No device memory is touched.
 template <typename TB>
-int build_plan(dipamd_batch_plan *P, const dipamd_batch_image *images, int n) {
-    const bool compute_double = P->in_dtype == DIPAMD_DT_DFLOAT;
-    std::vector<BatchImage<TB>> ims(static_cast<size_t>(n));
+struct BatchTables {
+    std::vector<BatchImage<TB>> ims;
+    std::vector<BatchItem> items;
     std::vector<TB> taps;
+    std::vector<BatchGroup> groups;
+};
+
+template <typename TB>
+int plan_tables(const dipamd_batch_image *images, int n, int in_dtype, double truncation, bool fast, int cfg,
+                BatchTables<TB> &T) {
+    const bool compute_double = in_dtype == DIPAMD_DT_DFLOAT;
+    T.ims.assign(static_cast<size_t>(n), BatchImage<TB>{});
+    T.items.clear();
+    T.taps.clear();
+    T.groups.clear();
     int rmax_x = 0, rmax_y = 0;
-    std::vector<std::vector<double>> cache;
     for (int i = 0; i < n; ++i) {
         const dipamd_batch_image &b = images[i];
         if (!b.in || !b.out) return set_error(DIPAMD_E_NOT_FORGED, "Image is not forged");
@@ -89,7 +113,7 @@ int build_plan(dipamd_batch_plan *P, const dipamd_batch_image *images, int n) {
                 *r = 0;
                 return {1.0};
             }
-            std::vector<double> h = half_gaussian(s, 0, P->truncation, compute_double);
+            std::vector<double> h = half_gaussian(s, 0, truncation, compute_double);
             *r = static_cast<int>(h.size()) - 1;
             std::reverse(h.begin(), h.end());  // correlation order: centre first
             return h;
@@ -99,7 +123,7 @@ int build_plan(dipamd_batch_plan *P, const dipamd_batch_image *images, int n) {
         std::vector<double> hy = axis(b.sigma_y, b.height, &ry);
         if (rx > kFusedCap - 1 || ry > kFusedCap - 1)
             return set_error(DIPAMD_E_UNSUPPORTED, "batched form: radius above 64 (sigma too large)");
-        BatchImage<TB> &im = ims[static_cast<size_t>(i)];
+        BatchImage<TB> &im = T.ims[static_cast<size_t>(i)];
         im.in = b.in;
         im.out = static_cast<TB *>(b.out);
         im.nx = b.width;
@@ -108,17 +132,17 @@ int build_plan(dipamd_batch_plan *P, const dipamd_batch_image *images, int n) {
         im.syout = b.out_stride_y;
         im.rx = rx;
         im.ry = ry;
-        im.tap_offset = static_cast<int32_t>(taps.size());
+        im.tap_offset = static_cast<int32_t>(T.taps.size());
         im.pad_ = 0;
-        for (double v : hx) taps.push_back(static_cast<TB>(v));
-        for (double v : hy) taps.push_back(static_cast<TB>(v));
+        for (double v : hx) T.taps.push_back(static_cast<TB>(v));
+        for (double v : hy) T.taps.push_back(static_cast<TB>(v));
         rmax_x = std::max(rmax_x, rx);
         rmax_y = std::max(rmax_y, ry);
     }
     // a tile config whose LDS (staging + ring for the largest radii) fits; the
     // requested one when it does
     const size_t limit = lds_limit_bytes();
-    if (fused_lds_bytes(P->cfg, rmax_x, rmax_y, sizeof(TB) == 8) > limit) {
+    if (fused_lds_bytes(cfg, rmax_x, rmax_y, sizeof(TB) == 8) > limit) {
         int fit = -1;
         for (int c = kNumFusedConfigs - 1; c >= 0; --c)
             if (!(sizeof(TB) == 8 && c == 2) && fused_lds_bytes(c, rmax_x, rmax_y, sizeof(TB) == 8) <= limit) {
@@ -126,67 +150,97 @@ int build_plan(dipamd_batch_plan *P, const dipamd_batch_image *images, int n) {
                 break;
             }
         if (fit < 0) return set_error(DIPAMD_E_UNSUPPORTED, "batched form: tile does not fit in LDS");
-        P->cfg = fit;
+        cfg = fit;
     }
     // work items, costliest images first (longest-processing-time order)
-    const int txc = kFusedConfigs[P->cfg].txc;
+    const int txc = kFusedConfigs[cfg].txc;
     std::vector<int> order(static_cast<size_t>(n));
     for (int i = 0; i < n; ++i) order[static_cast<size_t>(i)] = i;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-        const auto &A = ims[static_cast<size_t>(a)];
-        const auto &B = ims[static_cast<size_t>(b)];
+        const auto &A = T.ims[static_cast<size_t>(a)];
+        const auto &B = T.ims[static_cast<size_t>(b)];
         return (A.rx + A.ry + 2) > (B.rx + B.ry + 2);
     });
-    std::vector<BatchItem> items;
     for (int i : order) {
-        const auto &im = ims[static_cast<size_t>(i)];
+        const auto &im = T.ims[static_cast<size_t>(i)];
         const int ntiles = static_cast<int>((im.nx + txc - 1) / txc);
         // row chunks: long enough that the y halo stays small, short enough to balance
         int ch = std::max(batch_chunk_rule().first, ((batch_chunk_rule().second * im.ry + 15) / 16) * 16);
         ch = std::min<int>(ch, static_cast<int>(((im.ny + 15) / 16) * 16));
         for (int64_t z = 0; z < im.ny; z += ch)
             for (int t = 0; t < ntiles; ++t)
-                items.push_back({i, t, static_cast<int32_t>(z), static_cast<int32_t>(std::min<int64_t>(z + ch, im.ny))});
+                T.items.push_back({i, t, static_cast<int32_t>(z), static_cast<int32_t>(std::min<int64_t>(z + ch, im.ny))});
     }
-    P->nitems = static_cast<int64_t>(items.size());
-    P->lds = fused_lds_bytes(P->cfg, rmax_x, rmax_y, sizeof(TB) == 8);
-    // device tables
-    hipError_t e;
-    if ((e = hipMalloc(&P->d_images, ims.size() * sizeof(BatchImage<TB>))) != hipSuccess ||
-        (e = hipMalloc(&P->d_items, std::max<size_t>(1, items.size()) * sizeof(BatchItem))) != hipSuccess ||
-        (e = hipMalloc(&P->d_taps, taps.size() * sizeof(TB))) != hipSuccess ||
-        (e = hipMalloc(reinterpret_cast<void **>(&P->d_counter), 256)) != hipSuccess)
-        return set_error(DIPAMD_E_HIP, std::string("batch plan allocation: ") + hipGetErrorString(e));
-    if ((e = hipMemcpy(P->d_images, ims.data(), ims.size() * sizeof(BatchImage<TB>), hipMemcpyHostToDevice)) !=
-            hipSuccess ||
-        (e = hipMemcpy(P->d_items, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice)) !=
-            hipSuccess ||
-        (e = hipMemcpy(P->d_taps, taps.data(), taps.size() * sizeof(TB), hipMemcpyHostToDevice)) != hipSuccess)
-        return set_error(DIPAMD_E_HIP, std::string("batch plan upload: ") + hipGetErrorString(e));
-    BatchArgs<TB> a;
-    std::memset(&a, 0, sizeof(a));
-    a.images = static_cast<const BatchImage<TB> *>(P->d_images);
-    a.items = static_cast<const BatchItem *>(P->d_items);
-    a.taps = static_cast<const TB *>(P->d_taps);
-    a.counter = P->d_counter;
-    a.nitems = static_cast<int32_t>(items.size());
-    a.bcx = P->bc[0];
-    a.bcy = P->bc[1];
-    auto cv = [](int bc) -> TB {
-        if (bc == DIPAMD_BC_ADD_MAX_VALUE) return std::numeric_limits<TB>::max();
-        if (bc == DIPAMD_BC_ADD_MIN_VALUE) return std::numeric_limits<TB>::lowest();
-        return TB(0);
-    };
-    a.cvalx = cv(P->bc[0]);
-    a.cvaly = cv(P->bc[1]);
-    P->args.resize(sizeof(a));
-    std::memcpy(P->args.data(), &a, sizeof(a));
+    if (T.items.size() > static_cast<size_t>(INT32_MAX))
+        return set_error(DIPAMD_E_UNSUPPORTED, "batched form: too many work items");
+    BatchGroup g;
+    g.cfg = cfg;
+    g.lds = fused_lds_bytes(cfg, rmax_x, rmax_y, sizeof(TB) == 8);
+    g.item0 = 0;
+    g.nitems = static_cast<int64_t>(T.items.size());
     int dev = 0, ncu = 256;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    const InType ti = P->in_dtype == DIPAMD_DT_UINT8 ? IN_U8 : IN_BUF;
-    const int occ = batched_occupancy(ti, P->is_double, P->cfg, P->fast, P->lds);
-    P->grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(static_cast<int64_t>(ncu) * occ, P->nitems)));
+    const InType ti = in_dtype == DIPAMD_DT_UINT8 ? IN_U8 : IN_BUF;
+    const int occ = batched_occupancy(ti, sizeof(TB) == 8, g.cfg, fast, g.lds);
+    g.grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(static_cast<int64_t>(ncu) * occ, g.nitems)));
+    T.groups.push_back(g);
+    return DIPAMD_OK;
+}
+
+// Kernel arguments of group `gi` over device tables at the given addresses.
+template <typename TB>
+std::vector<uint8_t> group_args(const BatchTables<TB> &T, size_t gi, const void *d_images, const void *d_items,
+                                const void *d_taps, int32_t *d_counter, const int bc[2]) {
+    const BatchGroup &g = T.groups[gi];
+    BatchArgs<TB> a;
+    std::memset(&a, 0, sizeof(a));
+    a.images = static_cast<const BatchImage<TB> *>(d_images);
+    a.items = static_cast<const BatchItem *>(d_items) + g.item0;
+    a.taps = static_cast<const TB *>(d_taps);
+    a.counter = d_counter + gi * kCounterStride;
+    a.nitems = static_cast<int32_t>(g.nitems);
+    a.bcx = bc[0];
+    a.bcy = bc[1];
+    auto cv = [](int b) -> TB {
+        if (b == DIPAMD_BC_ADD_MAX_VALUE) return std::numeric_limits<TB>::max();
+        if (b == DIPAMD_BC_ADD_MIN_VALUE) return std::numeric_limits<TB>::lowest();
+        return TB(0);
+    };
+    a.cvalx = cv(bc[0]);
+    a.cvaly = cv(bc[1]);
+    std::vector<uint8_t> bytes(sizeof(a));
+    std::memcpy(bytes.data(), &a, sizeof(a));
+    return bytes;
+}
+
+template <typename TB>
+int build_plan(dipamd_batch_plan *P, const dipamd_batch_image *images, int n) {
+    BatchTables<TB> T;
+    int st = plan_tables<TB>(images, n, P->in_dtype, P->truncation, P->fast, P->cfg, T);
+    if (st != DIPAMD_OK) return st;
+    if (T.groups.size() > static_cast<size_t>(kMaxGroups))
+        return set_error(DIPAMD_E_UNSUPPORTED, "batched form: too many launch groups");
+    P->cfg = T.groups[0].cfg;
+    P->nitems = static_cast<int64_t>(T.items.size());
+    P->groups = T.groups;
+    // device tables
+    hipError_t e;
+    if ((e = hipMalloc(&P->d_images, T.ims.size() * sizeof(BatchImage<TB>))) != hipSuccess ||
+        (e = hipMalloc(&P->d_items, std::max<size_t>(1, T.items.size()) * sizeof(BatchItem))) != hipSuccess ||
+        (e = hipMalloc(&P->d_taps, T.taps.size() * sizeof(TB))) != hipSuccess ||
+        (e = hipMalloc(reinterpret_cast<void **>(&P->d_counter), kMaxGroups * kCounterStride * sizeof(int32_t))) !=
+            hipSuccess)
+        return set_error(DIPAMD_E_HIP, std::string("batch plan allocation: ") + hipGetErrorString(e));
+    if ((e = hipMemcpy(P->d_images, T.ims.data(), T.ims.size() * sizeof(BatchImage<TB>), hipMemcpyHostToDevice)) !=
+            hipSuccess ||
+        (e = hipMemcpy(P->d_items, T.items.data(), T.items.size() * sizeof(BatchItem), hipMemcpyHostToDevice)) !=
+            hipSuccess ||
+        (e = hipMemcpy(P->d_taps, T.taps.data(), T.taps.size() * sizeof(TB), hipMemcpyHostToDevice)) != hipSuccess)
+        return set_error(DIPAMD_E_HIP, std::string("batch plan upload: ") + hipGetErrorString(e));
+    P->args.clear();
+    for (size_t gi = 0; gi < T.groups.size(); ++gi)
+        P->args.push_back(group_args<TB>(T, gi, P->d_images, P->d_items, P->d_taps, P->d_counter, P->bc));
     return DIPAMD_OK;
 }
 
@@ -211,13 +265,125 @@ int make_plan(dipamd_batch_plan *P, int cfg) {
                         : build_plan<float>(P, P->host_images.data(), P->n);
 }
 
-int run_plan(dipamd_batch_plan *P, hipStream_t stream) {
-    hipError_t e = launch_reset_counter(P->d_counter, stream);
-    if (e == hipSuccess) {
-        const InType ti = P->in_dtype == DIPAMD_DT_UINT8 ? IN_U8 : IN_BUF;
-        e = launch_batched(ti, P->is_double, P->cfg, P->fast, P->args.data(), P->grid, P->lds, stream);
-    }
+// The groups' launches on `stream`: counters cleared by a kernel (no memset node
+// under graph capture), then one persistent launch per group.
+int launch_groups(const std::vector<BatchGroup> &groups, const std::vector<std::vector<uint8_t>> &args,
+                  int32_t *d_counter, int in_dtype, bool is_double, bool fast, hipStream_t stream) {
+    hipError_t e = launch_reset_counter(d_counter, static_cast<int>(groups.size()) * kCounterStride, stream);
+    const InType ti = in_dtype == DIPAMD_DT_UINT8 ? IN_U8 : IN_BUF;
+    for (size_t gi = 0; e == hipSuccess && gi < groups.size(); ++gi)
+        e = launch_batched(ti, is_double, groups[gi].cfg, fast, args[gi].data(), groups[gi].grid, groups[gi].lds,
+                           stream);
     if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("batched launch: ") + hipGetErrorString(e));
+    return DIPAMD_OK;
+}
+
+int run_plan(dipamd_batch_plan *P, hipStream_t stream) {
+    return launch_groups(P->groups, P->args, P->d_counter, P->in_dtype, P->is_double, P->fast, stream);
+}
+
+// ---------------------------------------------------------------------------
+// The entry point that allocates nothing: the plan's tables and counters go to a
+// file-scope device array (one copy per GPU), whose device address is looked up
+// once per device.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = size_t(8) << 20;  // DIPAMD_BATCH_NOALLOC_SCRATCH_BYTES
+static_assert(kScratchBytes == DIPAMD_BATCH_NOALLOC_SCRATCH_BYTES, "scratch size and header disagree");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::once_flag g_scratch_once[kMaxDevices];
+void *g_scratch_ptr[kMaxDevices];
+hipError_t g_scratch_err[kMaxDevices];
+
+int scratch_address(void **ptr) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices) return set_error(DIPAMD_E_UNSUPPORTED, "device index out of range");
+    std::call_once(g_scratch_once[dev], [dev]() {
+        void *p = nullptr;
+        g_scratch_err[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_scratch_ptr[dev] = g_scratch_err[dev] == hipSuccess ? p : nullptr;
+    });
+    if (g_scratch_err[dev] != hipSuccess || !g_scratch_ptr[dev])
+        return set_error(DIPAMD_E_HIP, std::string("scratch symbol address: ") + hipGetErrorString(g_scratch_err[dev]));
+    *ptr = g_scratch_ptr[dev];
+    return DIPAMD_OK;
+}
+
+size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+template <typename TB>
+int run_noalloc(const dipamd_batch_image *images, int n, int in_dtype, const int bc[2], double truncation, bool fast,
+                int cfg, hipStream_t stream) {
+    BatchTables<TB> T;
+    int st = plan_tables<TB>(images, n, in_dtype, truncation, fast, cfg, T);
+    if (st != DIPAMD_OK) return st;
+    if (T.groups.size() > static_cast<size_t>(kMaxGroups))
+        return set_error(DIPAMD_E_UNSUPPORTED, "batched form: too many launch groups");
+    // layout: counters, image descriptors, work items, taps (each 256-byte aligned)
+    const size_t off_counter = 0;
+    const size_t off_images = align256(kMaxGroups * kCounterStride * sizeof(int32_t));
+    const size_t off_items = align256(off_images + T.ims.size() * sizeof(BatchImage<TB>));
+    const size_t off_taps = align256(off_items + T.items.size() * sizeof(BatchItem));
+    const size_t total = off_taps + T.taps.size() * sizeof(TB);
+    if (total > kScratchBytes)
+        return set_error(DIPAMD_E_UNSUPPORTED,
+                         "batched form without allocation: the batch needs " + std::to_string(total) +
+                             " bytes of tables, above the fixed scratch of " + std::to_string(kScratchBytes) +
+                             " bytes (use dipamd_batch_create)");
+    void *base = nullptr;
+    if ((st = scratch_address(&base)) != DIPAMD_OK) return st;
+    unsigned char *b = static_cast<unsigned char *>(base);
+    // host tables are pageable memory: these copies complete before the calls return
+    hipError_t e;
+    if ((e = hipMemcpyAsync(b + off_images, T.ims.data(), T.ims.size() * sizeof(BatchImage<TB>),
+                            hipMemcpyHostToDevice, stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(b + off_items, T.items.data(), T.items.size() * sizeof(BatchItem), hipMemcpyHostToDevice,
+                            stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(b + off_taps, T.taps.data(), T.taps.size() * sizeof(TB), hipMemcpyHostToDevice, stream)) !=
+            hipSuccess)
+        return set_error(DIPAMD_E_HIP, std::string("batch table upload: ") + hipGetErrorString(e));
+    int32_t *counters = reinterpret_cast<int32_t *>(b + off_counter);
+    std::vector<std::vector<uint8_t>> args;
+    for (size_t gi = 0; gi < T.groups.size(); ++gi)
+        args.push_back(group_args<TB>(T, gi, b + off_images, b + off_items, b + off_taps, counters, bc));
+    return launch_groups(T.groups, args, counters, in_dtype, sizeof(TB) == 8, fast, stream);
+}
+
+// Arguments shared by dipamd_batch_create and dipamd_batch_gauss_noalloc.
+int check_batch(const dipamd_batch_image *images, int32_t n, int32_t in_dtype, int32_t out_dtype, const int32_t *bc,
+                int32_t nbc, int *bcs) {
+    if (n < 1 || !images) return set_error(DIPAMD_E_PARAMETER, "empty batch");
+    if (in_dtype != DIPAMD_DT_UINT8 && in_dtype != DIPAMD_DT_SFLOAT && in_dtype != DIPAMD_DT_DFLOAT)
+        return set_error(DIPAMD_E_UNSUPPORTED, "batched form: input must be uint8, sfloat or dfloat");
+    if (out_dtype != suggest_flex(in_dtype))
+        return set_error(DIPAMD_E_UNSUPPORTED, "batched form: output type must be SuggestFlex(input)");
+    int st = broadcast_bc(bc, nbc, 2, bcs);
+    if (st != DIPAMD_OK) return st;
+    for (int k = 0; k < 2; ++k)
+        if (!bc_supported(bcs[k])) return set_error(DIPAMD_E_UNSUPPORTED, "boundary condition not supported");
+    // The tiles of one image run concurrently and read halo rows that other tiles
+    // write: an image's output must not share memory with its input (the per-image
+    // loop would handle it through a scratch copy; the persistent kernel cannot).
+    for (int i = 0; i < n; ++i) {
+        const dipamd_batch_image &b = images[i];
+        if (!b.in || !b.out || b.width < 1 || b.height < 1) continue;  // reported by the planner
+        View vi, vo;
+        vi.p = const_cast<void *>(b.in);
+        vi.dt = in_dtype;
+        vo.p = b.out;
+        vo.dt = out_dtype;
+        vi.sizes[0] = vo.sizes[0] = b.width;
+        vi.sizes[1] = vo.sizes[1] = b.height;
+        vi.strides[0] = vo.strides[0] = 1;
+        vi.strides[1] = b.in_stride_y;
+        vo.strides[1] = b.out_stride_y;
+        if (views_overlap(vi, vo, 2))
+            return set_error(DIPAMD_E_PARAMETER,
+                             "batched form: image " + std::to_string(i) + " has an output that overlaps its input");
+    }
     return DIPAMD_OK;
 }
 
@@ -232,16 +398,9 @@ int dipamd_batch_create(dipamd_batch_plan **plan, const dipamd_batch_image *imag
                         int32_t tile_config, void *stream) {
     if (!plan) return set_error(DIPAMD_E_PARAMETER, "null plan pointer");
     *plan = nullptr;
-    if (n < 1 || !images) return set_error(DIPAMD_E_PARAMETER, "empty batch");
-    if (in_dtype != DIPAMD_DT_UINT8 && in_dtype != DIPAMD_DT_SFLOAT && in_dtype != DIPAMD_DT_DFLOAT)
-        return set_error(DIPAMD_E_UNSUPPORTED, "batched form: input must be uint8, sfloat or dfloat");
-    if (out_dtype != suggest_flex(in_dtype))
-        return set_error(DIPAMD_E_UNSUPPORTED, "batched form: output type must be SuggestFlex(input)");
     int bcs[2];
-    int st = broadcast_bc(bc, nbc, 2, bcs);
+    int st = check_batch(images, n, in_dtype, out_dtype, bc, nbc, bcs);
     if (st != DIPAMD_OK) return st;
-    for (int b : bcs)
-        if (!bc_supported(b)) return set_error(DIPAMD_E_UNSUPPORTED, "boundary condition not supported");
     if (tile_config >= kNumFusedConfigs) return set_error(DIPAMD_E_PARAMETER, "tile config out of range");
     auto *P = new dipamd_batch_plan();
     P->n = n;
@@ -364,6 +523,33 @@ int dipamd_batch_run_per_image(dipamd_batch_plan *plan, void *stream) {
         if (st != DIPAMD_OK) return st;
     }
     return DIPAMD_OK;
+}
+
+int dipamd_batch_gauss_noalloc(const dipamd_batch_image *images, int32_t n, int32_t in_dtype, int32_t out_dtype,
+                               const int32_t *bc, int32_t nbc, double truncation, uint32_t flags,
+                               int32_t tile_config, void *stream) {
+    int bcs[2];
+    int st = check_batch(images, n, in_dtype, out_dtype, bc, nbc, bcs);
+    if (st != DIPAMD_OK) return st;
+    if (tile_config >= kNumFusedConfigs) return set_error(DIPAMD_E_PARAMETER, "tile config out of range");
+    const bool is_double = out_dtype == DIPAMD_DT_DFLOAT;
+    const bool fast = (flags & DIPAMD_FAST) != 0;
+    int cfg = tile_config;
+    if (cfg < 0) {
+        cfg = 1;
+        int rmax = 0;
+        for (int i = 0; i < n; ++i)
+            rmax = std::max(rmax, gauss_radius(std::max(images[i].sigma_x, images[i].sigma_y), 0, truncation,
+                                               in_dtype == DIPAMD_DT_DFLOAT));
+        const auto key = std::make_tuple(static_cast<int>(in_dtype), fast ? 1 : 0, rmax, bcs[0] * 16 + bcs[1]);
+        std::lock_guard<std::mutex> lk(g_tune_mutex);
+        auto it = g_batch_tuned.find(key);
+        if (it != g_batch_tuned.end()) cfg = it->second;
+    }
+    if (is_double && cfg == 2) cfg = 1;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    return is_double ? run_noalloc<double>(images, n, in_dtype, bcs, truncation, fast, cfg, s)
+                     : run_noalloc<float>(images, n, in_dtype, bcs, truncation, fast, cfg, s);
 }
 
 int dipamd_batch_destroy(dipamd_batch_plan *plan) {

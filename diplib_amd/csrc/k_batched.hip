@@ -4,9 +4,9 @@
 
 namespace dipamd {
 
-// Clears the work-queue counter with a kernel (graph-safe: no memset node).
-__global__ void k_reset_counter(int32_t *counter) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+// Clears the work-queue counters with a kernel (graph-safe: no memset node).
+__global__ void k_reset_counter(int32_t *counter, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) counter[i] = 0;
 }
 
 namespace {
@@ -68,8 +68,8 @@ int batched_occupancy(InType ti, bool is_double, int cfg, bool fast, size_t lds)
     return fast ? occupancy_cfg<float, float, true>(cfg, lds) : occupancy_cfg<float, float, false>(cfg, lds);
 }
 
-hipError_t launch_reset_counter(int32_t *counter, hipStream_t stream) {
-    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, stream, counter);
+hipError_t launch_reset_counter(int32_t *counter, int n, hipStream_t stream) {
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(256), 0, stream, counter, n);
     return hipGetLastError();
 }
 

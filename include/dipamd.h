@@ -224,7 +224,9 @@ typedef struct dipamd_batch_image {
 
 typedef struct dipamd_batch_plan dipamd_batch_plan;
 
-/* tile_config: -1 = autotuned choice, else an index < dipamd_batch_num_configs(). */
+/* tile_config: -1 = autotuned choice, else an index < dipamd_batch_num_configs().
+ * An image's output must not overlap its own input (DIPAMD_E_PARAMETER otherwise):
+ * the tiles of one image run concurrently and read halo rows other tiles write. */
 int dipamd_batch_create(dipamd_batch_plan **plan, const dipamd_batch_image *images, int32_t n, int32_t in_dtype,
                         int32_t out_dtype, const int32_t *bc, int32_t nbc, double truncation, uint32_t flags,
                         int32_t tile_config, void *stream);
@@ -235,6 +237,23 @@ int dipamd_batch_run_graph(dipamd_batch_plan *plan, void *stream);
  * the baseline the batched kernel is measured against, and its reference result. */
 int dipamd_batch_run_per_image(dipamd_batch_plan *plan, void *stream);
 int dipamd_batch_destroy(dipamd_batch_plan *plan);
+
+/* No counterpart in the reference.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * the batched form in one call, with no plan object and no allocation -- the work
+ * queue, its counters, the per-image descriptors and the taps are written into one
+ * fixed device array of the library (DIPAMD_BATCH_NOALLOC_SCRATCH_BYTES per GPU, held
+ * until the process ends), the counters cleared by a kernel, then the persistent
+ * kernel runs.  Same arguments and results as dipamd_batch_create + dipamd_batch_run
+ * (tile_config -1 takes the tuned choice of an earlier plan with the same key, else
+ * the default tile).  A batch whose tables exceed the array is rejected with
+ * DIPAMD_E_UNSUPPORTED (roughly 100 000 work items, or 60 000 images).  Every call on
+ * a GPU shares that GPU's array: calls that overlap on one GPU (different streams, or
+ * host threads) race.  The descriptor upload reads host memory before the call returns. */
+#define DIPAMD_BATCH_NOALLOC_SCRATCH_BYTES (8u << 20)
+int dipamd_batch_gauss_noalloc(const dipamd_batch_image *images, int32_t n, int32_t in_dtype, int32_t out_dtype,
+                               const int32_t *bc, int32_t nbc, double truncation, uint32_t flags,
+                               int32_t tile_config, void *stream);
 int dipamd_batch_num_configs(void);
 /* Number of work-queue tiles and the config index of a plan. */
 int dipamd_batch_info(const dipamd_batch_plan *plan, int64_t *ntiles, int32_t *config);
