@@ -79,19 +79,27 @@ def test_separable_larger_vs_oracle(cuda, case, fusion):
 
 
 def test_separable_strided_views(cuda):
-    """Transposed and sub-sampled views of the input and output (strides in samples,
-    negative included), against the oracle on the same pixels."""
+    """Sub-sampled, cropped, reversed and transposed views of the input, written into a
+    transposed output view (strides in samples, negative included), against the oracle
+    on the same pixels.  The passes run in order of increasing input stride
+    (framework_separable.cpp:146-163), so a transposed input filters its dim 1 first:
+    its oracle is the contiguous image with the filters swapped, transposed back."""
     base = cases.make_input((96, 140), "float32", seed=78)
     filters = [(cases.sep_taps(5, 11), "d-even", 0), (cases.sep_taps(4, 12), "general", 1)]
     xt = _t(base, cuda)
-    for view in (lambda t: t.t(), lambda t: t[::2, ::3], lambda t: t.flip(0), lambda t: t[5:80, 7:129]):
+    taps, syms, orgs = [f[0] for f in filters], [f[1] for f in filters], [f[2] for f in filters]
+    views = [(lambda t: t[::2, ::3], False), (lambda t: t.flip(1), False), (lambda t: t[5:80, 7:129], False),
+             (lambda t: t.t(), True)]
+    for view, transposed in views:
         v = view(xt)
-        ref = oc.separable(np.ascontiguousarray(v.cpu().numpy()), [f[0] for f in filters], [f[1] for f in filters],
-                           ["mirror"], origins=[f[2] for f in filters])
+        if transposed:
+            ref = oc.separable(base, taps[::-1], syms[::-1], ["mirror"], origins=orgs[::-1]).T
+        else:
+            ref = oc.separable(np.ascontiguousarray(v.cpu().numpy()), taps, syms, ["mirror"], origins=orgs)
         out = torch.full((v.shape[1], v.shape[0]), -1.0, device=cuda).t()  # a transposed output view
         y = dip.SeparableConvolution(v, _filters(filters), ["mirror"], out=out)
         torch.cuda.synchronize()
-        assert np.array_equal(y.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+        assert np.array_equal(y.cpu().numpy().view(np.uint32), np.ascontiguousarray(ref).view(np.uint32))
 
 
 def test_separable_in_place(cuda):
