@@ -80,6 +80,14 @@ int fuse_max_radius(int64_t px) {
     return px <= int64_t(2048) * 2048 ? 24 : 16;
 }
 
+bool fused_rc_enabled() {
+    static bool on = [] {
+        const char *v = std::getenv("DIPAMD_FUSED_RC");
+        return !(v && v[0] == '0');
+    }();
+    return on;
+}
+
 bool stream2d_enabled() {
     static bool on = [] {
         const char *v = std::getenv("DIPAMD_STREAM2D");

@@ -27,6 +27,7 @@
 #include "sep_common.hpp"
 #include "sep_config.hpp"
 #include "sep_kernels.hpp"
+#include "fused_rc.hpp"
 
 namespace dipamd {
 
@@ -507,8 +508,24 @@ static int run_fused(const View &src, const View &dst, int ndim, int dx, int dy,
     TraceScope ts(Fx.sym == SYM_NONE ? "ringsweep" : "fused2d",
                   npx_of(src, ndim) * static_cast<int64_t>(dt_size(src.dt) + sizeof(TB)), stream,
                   npx_of(src, ndim) * (filter_ops(Fx, fast) + filter_ops(Fy, fast)));
-    hipError_t err = launch_fused(ti, sizeof(TB) == 8, cfg, Fx.sym, Fy.sym, fast, &a, a.nx, nchunks, nouter,
-                                  Fx.border, Fy.border, stream);
+    // the radius-class kernel (x window in registers) where the plane's rows allow
+    // vector loads; the runtime-radius kernel otherwise
+    const int rxc = Fx.sym == SYM_NONE ? 0 : std::max(4, rc_class(Fx.border));
+    hipError_t err = hipErrorNotSupported;
+    if (fused_rc_enabled() && src.strides[dx] == 1 && rxc <= kFusedRcMaxClass &&
+        fused_rc_lds_bytes(cfg, rxc, Fy.border, sizeof(TB) == 8) <= lds_limit_bytes()) {
+        const int64_t es = static_cast<int64_t>(dt_size(src.dt));
+        const int64_t vb = src.dt == DIPAMD_DT_UINT8 ? 4 : 16;  // bytes per vector load
+        bool aligned = reinterpret_cast<uintptr_t>(src.p) % static_cast<uintptr_t>(vb) == 0;
+        for (int d = 0; d < ndim; ++d)
+            if (d != dx && src.sizes[d] > 1 && (src.strides[d] * es) % vb != 0) aligned = false;
+        if (aligned)
+            err = launch_fused_rc(ti, sizeof(TB) == 8, cfg, Fx.sym, Fy.sym, fast, &a, a.nx, nchunks, nouter, rxc,
+                                  Fy.border, stream);
+    }
+    if (err == hipErrorNotSupported)
+        err = launch_fused(ti, sizeof(TB) == 8, cfg, Fx.sym, Fy.sym, fast, &a, a.nx, nchunks, nouter, Fx.border,
+                           Fy.border, stream);
     if (err != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(err));
     return DIPAMD_OK;
 }

@@ -90,6 +90,8 @@ int choose_fused_config(bool is_double, int rx, int ry, int64_t nx, int64_t ny, 
 bool stream2d_enabled();
 // DIPAMD_ZSWEEP=0 turns the register-window sweep off (experiments).
 bool zsweep_enabled();
+// DIPAMD_FUSED_RC=0 turns the radius-class fused kernel off (experiments).
+bool fused_rc_enabled();
 // Workgroups a strided sweep aims for when it splits the filter axis into chunks:
 // `dflt`, or DIPAMD_SWEEP_BLOCKS when set (experiments).
 int sweep_target_blocks(int dflt);

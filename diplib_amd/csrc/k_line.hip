@@ -1,6 +1,7 @@
 // Launch tables (see sep_kernels_impl.hpp).
 #include "sep_kernels_impl.hpp"
 
+#include "fused_rc.hpp"
 #include "sep_config.hpp"
 
 namespace dipamd {
@@ -17,6 +18,11 @@ size_t fused_lds_bytes(int cfg, int rx, int ry, bool is_double) {
     const size_t words = static_cast<size_t>(S) * fused_pitch(c.txc + 2 * rx, c.p) +
                          static_cast<size_t>(RC) * fused_pitch(c.txc, c.p);
     return words * (is_double ? 8 : 4);
+}
+
+size_t fused_rc_lds_bytes(int cfg, int rxc, int ry, bool is_double) {
+    const int txc = kFusedConfigs[cfg].txc;
+    return is_double ? rc_lds_bytes<double>(txc, rxc, ry) : rc_lds_bytes<float>(txc, rxc, ry);
 }
 
 namespace {

@@ -42,6 +42,15 @@ hipError_t launch_colsweep(InType ti, bool is_double, int sym, bool fast, const 
 hipError_t launch_fused(InType ti, bool is_double, int cfg, int symx, int symy, bool fast, const void *args,
                         int64_t nx, int64_t nchunks, int64_t nouter, int rx, int ry, hipStream_t stream);
 
+// The fused kernel with a compile-time x radius class rxc (a multiple of 4 up to
+// kFusedRcMaxClass, or 0 with symx == SYM_NONE; fused_rc.hpp).  Needs a unit x stride;
+// hipErrorNotSupported for combinations it does not cover.
+constexpr int kFusedRcMaxClass = 24;
+hipError_t launch_fused_rc(InType ti, bool is_double, int cfg, int symx, int symy, bool fast, const void *args,
+                           int64_t nx, int64_t nchunks, int64_t nouter, int rxc, int ry, hipStream_t stream);
+// LDS bytes of the radius-class fused kernel.
+size_t fused_rc_lds_bytes(int cfg, int rxc, int ry, bool is_double);
+
 // Extended lines for the boundary conditions computed from edge values (k_expand.hip):
 // `border` samples computed on each side, then `zpad` samples of 0 beyond them.
 hipError_t launch_expand(InType ti, bool is_double, const void *in, void *out, const void *geo, int d, int n,
