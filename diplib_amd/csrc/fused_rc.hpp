@@ -240,13 +240,7 @@ __device__ __forceinline__ void fused_tile_rc_body(const Tile<TB> &t, const TI *
             const int xr = (lane & 7) + 8 * (wave & 1);
             const int xl = (lane >> 3) + 8 * (wave >> 1);
             const int j = z0 - By + m * S + xr;
-            int mul = 1;
-            bool isc = false;
-            // rows outside the plane only in the first and last steps (a uniform test)
             const int jstep = z0 - By + m * S;
-            if (jstep < 0 || jstep + S > t.ny) {
-                if (j < 0 || j >= t.ny) isc = bc_fold(j, t.ny, t.bcy, &mul) < 0;
-            }
             TB acc[P];
             TB w[WN];
             const LV *wv = reinterpret_cast<const LV *>(stg + xr * PS + xl * P);
@@ -275,14 +269,23 @@ __device__ __forceinline__ void fused_tile_rc_body(const Tile<TB> &t, const TI *
                     }
                 }
             }
-            if (isc) {
+            // rows outside the plane only in the first and last steps (a uniform test):
+            // the boundary condition's sign or constant along y
+            if (jstep < 0 || jstep + S > t.ny) {
+                int mul = 1;
+                bool isc = false;
+                if (j < 0 || j >= t.ny) isc = bc_fold(j, t.ny, t.bcy, &mul) < 0;
+                if (isc) {
 #pragma unroll
-                for (int p = 0; p < P; ++p) acc[p] = t.cvaly;
-            } else if (mul < 0) {
+                    for (int p = 0; p < P; ++p) acc[p] = t.cvaly;
+                } else if (mul < 0) {
 #pragma unroll
-                for (int p = 0; p < P; ++p) acc[p] = -acc[p];
+                    for (int p = 0; p < P; ++p) acc[p] = -acc[p];
+                }
             }
-            LV *rrow = reinterpret_cast<LV *>(ring + ((m * S + xr) % RC) * PR + xl * P);
+            // ring slot of row m*S + xr: RC is a whole number of steps
+            const int slot = (m % (A + 1)) * S + xr;
+            LV *rrow = reinterpret_cast<LV *>(ring + slot * PR + xl * P);
 #pragma unroll
             for (int i = 0; i < P / LN; ++i) {
                 LV v;
