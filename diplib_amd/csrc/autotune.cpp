@@ -9,6 +9,8 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -27,6 +29,12 @@ struct TuneRow {
 std::once_flag g_tune_once;
 std::vector<TuneRow> g_tune;
 int g_forced = -2;
+
+// The persistent key -> value cache of measured choices.
+std::once_flag g_cache_once;
+std::mutex g_cache_mu;
+std::map<std::string, int> g_cache;
+std::string g_cache_path;
 
 std::string library_dir() {
     Dl_info info;
@@ -53,6 +61,27 @@ void load_table() {
         if (line[0] == '#') continue;
         TuneRow r;
         if (std::sscanf(line, "%d %d %d %d", &r.is_double, &r.rlo, &r.rhi, &r.cfg) == 4) g_tune.push_back(r);
+    }
+    std::fclose(fp);
+}
+
+std::string cache_path() {
+    if (const char *f = std::getenv("DIPAMD_TUNE_FILE")) return f;
+    return library_dir() + "/tune_cache.txt";
+}
+
+// Lines "<key>\t<value>"; a later line overrides an earlier one for the same key.
+void load_cache() {
+    g_cache_path = cache_path();
+    FILE *fp = std::fopen(g_cache_path.c_str(), "r");
+    if (!fp) return;
+    char line[512];
+    while (std::fgets(line, sizeof(line), fp)) {
+        if (line[0] == '#') continue;
+        char *tab = std::strchr(line, '\t');
+        if (!tab) continue;
+        *tab = 0;
+        g_cache[line] = std::atoi(tab + 1);
     }
     std::fclose(fp);
 }
@@ -148,6 +177,43 @@ size_t lds_limit_bytes() {
         return static_cast<size_t>(v < 65536 ? v : 65536);
     }();
     return limit;
+}
+
+std::string tune_device_tag() {
+    static std::string tag = [] {
+        int dev = 0, ncu = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        hipDeviceProp_t prop;
+        std::string arch = "gfx";
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) arch = prop.gcnArchName;
+        const size_t colon = arch.find(':');
+        if (colon != std::string::npos) arch = arch.substr(0, colon);
+        return arch + "-" + std::to_string(ncu) + "cu";
+    }();
+    return tag;
+}
+
+bool tune_get(const std::string &key, int *value) {
+    std::call_once(g_cache_once, load_cache);
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    auto it = g_cache.find(tune_device_tag() + " " + key);
+    if (it == g_cache.end()) return false;
+    *value = it->second;
+    return true;
+}
+
+void tune_put(const std::string &key, int value) {
+    std::call_once(g_cache_once, load_cache);
+    const std::string full = tune_device_tag() + " " + key;
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    g_cache[full] = value;
+    if (const char *v = std::getenv("DIPAMD_TUNE_WRITE"))
+        if (v[0] == '0') return;
+    FILE *fp = std::fopen(g_cache_path.c_str(), "a");
+    if (!fp) return;  // read-only location: the choice stays in memory for this process
+    std::fprintf(fp, "%s\t%d\n", full.c_str(), value);
+    std::fclose(fp);
 }
 
 int choose_fused_config(bool is_double, int rx, int ry, int64_t nx, int64_t ny, uint32_t flags) {

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -42,9 +43,22 @@ std::pair<int, int> batch_chunk_rule() {
 }
 }  // namespace
 
+// Images are grouped by radius class: the x radius rounded up to a multiple of 4 (the
+// radius-class tile, up to 24) and the ring depth of the y radius (A = ceil(2 ry / 16)).
+// Each group is one persistent launch whose LDS and register use fit its own radii, with
+// its own tile config.  Radii beyond the classes run the runtime-radius tile (rxc 0).
+struct GroupKey {
+    int rxc;  // 4..24, or 0: the runtime-radius tile
+    int ya;   // ring depth class of the y radius
+    int wc;   // floor(log2) of the group's mean image width (a tuning key, not a grouping key)
+    bool operator<(const GroupKey &o) const { return rxc != o.rxc ? rxc < o.rxc : ya < o.ya; }
+    bool operator==(const GroupKey &o) const { return rxc == o.rxc && ya == o.ya; }
+};
+
 // One persistent launch of a plan: a tile config, its dynamic LDS bytes, a slice of the
 // work-item table and the grid.
 struct BatchGroup {
+    GroupKey key;
     int cfg;
     size_t lds;
     int64_t item0, nitems;
@@ -71,19 +85,34 @@ struct dipamd_batch_plan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t cap_stream = nullptr;
+    // concurrent groups: side streams (default priority) and the fork / join events
+    std::vector<hipStream_t> side;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
 };
 
 namespace {
-
-std::mutex g_tune_mutex;
-std::map<std::tuple<int, int, int, int>, int> g_batch_tuned;  // (in dtype, fast, rmax, bc) -> cfg
 
 // Counters of the groups sit 64 bytes apart (one per cache line).
 constexpr int kCounterStride = 16;
 constexpr int kMaxGroups = 16;
 
-// The host side of a plan: per-image descriptors, work items (longest-processing-time
-// order), taps, and the launch groups.  No device memory is touched.
+int ring_class(int ry) { return (2 * ry + kFusedStep - 1) / kFusedStep; }
+// Largest y radius of a ring class.
+int ring_class_max_ry(int ya) { return (ya * kFusedStep) / 2; }
+
+bool use_rc_tile(int rx, int ry) { return fused_rc_enabled() && rc_class_of(rx) <= kFusedRcMaxClass && ry <= 24; }
+
+// Cache key of a group's tile config.
+std::string group_tune_key(int in_dtype, bool fast, const int bc[2], const GroupKey &k) {
+    return "batch in" + std::to_string(in_dtype) + (fast ? " fast" : " exact") + " bc" + std::to_string(bc[0]) + "," +
+           std::to_string(bc[1]) + " rxc" + std::to_string(k.rxc) + " ya" + std::to_string(k.ya) + " w" +
+           std::to_string(k.wc);
+}
+
+// The host side of a plan: per-image descriptors, taps, and per group its work items
+// (longest-processing-time order) and launch shape.  No device memory is touched.
+// `cfg_of` picks each group's tile config (falling back to one whose LDS fits).
 template <typename TB>
 struct BatchTables {
     std::vector<BatchImage<TB>> ims;
@@ -92,15 +121,18 @@ struct BatchTables {
     std::vector<BatchGroup> groups;
 };
 
-template <typename TB>
-int plan_tables(const dipamd_batch_image *images, int n, int in_dtype, double truncation, bool fast, int cfg,
+template <typename TB, typename CfgOf>
+int plan_tables(const dipamd_batch_image *images, int n, int in_dtype, double truncation, bool fast, CfgOf cfg_of,
                 BatchTables<TB> &T) {
     const bool compute_double = in_dtype == DIPAMD_DT_DFLOAT;
+    const bool is_double = sizeof(TB) == 8;
     T.ims.assign(static_cast<size_t>(n), BatchImage<TB>{});
     T.items.clear();
     T.taps.clear();
     T.groups.clear();
-    int rmax_x = 0, rmax_y = 0;
+    const int64_t vb = in_dtype == DIPAMD_DT_UINT8 ? 4 : 16;  // bytes per vector load
+    const int64_t es = static_cast<int64_t>(dt_size(in_dtype));
+    std::map<GroupKey, std::vector<int>> members;
     for (int i = 0; i < n; ++i) {
         const dipamd_batch_image &b = images[i];
         if (!b.in || !b.out) return set_error(DIPAMD_E_NOT_FORGED, "Image is not forged");
@@ -133,58 +165,86 @@ int plan_tables(const dipamd_batch_image *images, int n, int in_dtype, double tr
         im.rx = rx;
         im.ry = ry;
         im.tap_offset = static_cast<int32_t>(T.taps.size());
-        im.pad_ = 0;
+        im.vec_ok = (reinterpret_cast<uintptr_t>(b.in) % static_cast<uintptr_t>(vb) == 0 &&
+                     (b.in_stride_y * es) % vb == 0)
+                        ? 1
+                        : 0;
         for (double v : hx) T.taps.push_back(static_cast<TB>(v));
         for (double v : hy) T.taps.push_back(static_cast<TB>(v));
-        rmax_x = std::max(rmax_x, rx);
-        rmax_y = std::max(rmax_y, ry);
+        const GroupKey key = use_rc_tile(rx, ry) ? GroupKey{rc_class_of(rx), ring_class(ry), 0} : GroupKey{0, 0, 0};
+        members[key].push_back(i);
     }
-    // a tile config whose LDS (staging + ring for the largest radii) fits; the
-    // requested one when it does
-    const size_t limit = lds_limit_bytes();
-    if (fused_lds_bytes(cfg, rmax_x, rmax_y, sizeof(TB) == 8) > limit) {
-        int fit = -1;
-        for (int c = kNumFusedConfigs - 1; c >= 0; --c)
-            if (!(sizeof(TB) == 8 && c == 2) && fused_lds_bytes(c, rmax_x, rmax_y, sizeof(TB) == 8) <= limit) {
-                fit = c;
-                break;
-            }
-        if (fit < 0) return set_error(DIPAMD_E_UNSUPPORTED, "batched form: tile does not fit in LDS");
-        cfg = fit;
-    }
-    // work items, costliest images first (longest-processing-time order)
-    const int txc = kFusedConfigs[cfg].txc;
-    std::vector<int> order(static_cast<size_t>(n));
-    for (int i = 0; i < n; ++i) order[static_cast<size_t>(i)] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-        const auto &A = T.ims[static_cast<size_t>(a)];
-        const auto &B = T.ims[static_cast<size_t>(b)];
-        return (A.rx + A.ry + 2) > (B.rx + B.ry + 2);
-    });
-    for (int i : order) {
-        const auto &im = T.ims[static_cast<size_t>(i)];
-        const int ntiles = static_cast<int>((im.nx + txc - 1) / txc);
-        // row chunks: long enough that the y halo stays small, short enough to balance
-        int ch = std::max(batch_chunk_rule().first, ((batch_chunk_rule().second * im.ry + 15) / 16) * 16);
-        ch = std::min<int>(ch, static_cast<int>(((im.ny + 15) / 16) * 16));
-        for (int64_t z = 0; z < im.ny; z += ch)
-            for (int t = 0; t < ntiles; ++t)
-                T.items.push_back({i, t, static_cast<int32_t>(z), static_cast<int32_t>(std::min<int64_t>(z + ch, im.ny))});
-    }
-    if (T.items.size() > static_cast<size_t>(INT32_MAX))
-        return set_error(DIPAMD_E_UNSUPPORTED, "batched form: too many work items");
-    BatchGroup g;
-    g.cfg = cfg;
-    g.lds = fused_lds_bytes(cfg, rmax_x, rmax_y, sizeof(TB) == 8);
-    g.item0 = 0;
-    g.nitems = static_cast<int64_t>(T.items.size());
+    if (members.size() > static_cast<size_t>(kMaxGroups))
+        return set_error(DIPAMD_E_UNSUPPORTED, "batched form: too many launch groups");
     int dev = 0, ncu = 256;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    const size_t limit = lds_limit_bytes();
     const InType ti = in_dtype == DIPAMD_DT_UINT8 ? IN_U8 : IN_BUF;
-    const int occ = batched_occupancy(ti, sizeof(TB) == 8, g.cfg, fast, g.lds);
-    g.grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(static_cast<int64_t>(ncu) * occ, g.nitems)));
-    T.groups.push_back(g);
+    // groups with the largest radii first: their items are the longest
+    for (auto it = members.rbegin(); it != members.rend(); ++it) {
+        GroupKey key = it->first;
+        std::vector<int> &mem = it->second;
+        int rmax_x = 0, rmax_y = 0;
+        double wsum = 0;
+        for (int i : mem) {
+            rmax_x = std::max(rmax_x, T.ims[static_cast<size_t>(i)].rx);
+            rmax_y = std::max(rmax_y, T.ims[static_cast<size_t>(i)].ry);
+            wsum += static_cast<double>(T.ims[static_cast<size_t>(i)].nx);
+        }
+        key.wc = std::max(0, static_cast<int>(std::floor(std::log2(std::max(1.0, wsum / mem.size())))));
+        // the LDS of a group is sized for its ring class, so one config serves the class
+        const int ry_lds = key.rxc > 0 ? ring_class_max_ry(key.ya) : rmax_y;
+        auto lds_of = [&](int c) {
+            return key.rxc > 0 ? fused_rc_lds_bytes(c, key.rxc, ry_lds, is_double)
+                               : fused_lds_bytes(c, rmax_x, rmax_y, is_double);
+        };
+        auto usable = [&](int c) { return c >= 0 && c < kNumFusedConfigs && !(is_double && c == 2) && lds_of(c) <= limit; };
+        int cfg = cfg_of(key);
+        if (!usable(cfg)) {
+            cfg = -1;
+            for (int c = kNumFusedConfigs - 1; c >= 0; --c)
+                if (usable(c)) {
+                    cfg = c;
+                    break;
+                }
+            if (cfg < 0) return set_error(DIPAMD_E_UNSUPPORTED, "batched form: tile does not fit in LDS");
+        }
+        const int txc = kFusedConfigs[cfg].txc;
+        std::stable_sort(mem.begin(), mem.end(), [&](int a, int b) {
+            const auto &A = T.ims[static_cast<size_t>(a)];
+            const auto &B = T.ims[static_cast<size_t>(b)];
+            return (A.rx + A.ry + 2) > (B.rx + B.ry + 2);
+        });
+        BatchGroup g;
+        g.key = key;
+        g.cfg = cfg;
+        g.lds = lds_of(cfg);
+        g.item0 = static_cast<int64_t>(T.items.size());
+        for (int i : mem) {
+            const auto &im = T.ims[static_cast<size_t>(i)];
+            const int ntiles = static_cast<int>((im.nx + txc - 1) / txc);
+            // row chunks: long enough that the y halo stays small, short enough to balance
+            int ch = std::max(batch_chunk_rule().first, ((batch_chunk_rule().second * im.ry + 15) / 16) * 16);
+            ch = std::min<int>(ch, static_cast<int>(((im.ny + 15) / 16) * 16));
+            for (int64_t z = 0; z < im.ny; z += ch)
+                for (int t = 0; t < ntiles; ++t)
+                    T.items.push_back(
+                        {i, t, static_cast<int32_t>(z), static_cast<int32_t>(std::min<int64_t>(z + ch, im.ny))});
+        }
+        g.nitems = static_cast<int64_t>(T.items.size()) - g.item0;
+        if (T.items.size() > static_cast<size_t>(INT32_MAX))
+            return set_error(DIPAMD_E_UNSUPPORTED, "batched form: too many work items");
+        int occ = 1;
+        if (key.rxc > 0) {
+            if (launch_batched_rc(ti, is_double, cfg, fast, key.rxc, nullptr, 0, g.lds, nullptr, &occ) != hipSuccess)
+                occ = 1;
+        } else {
+            occ = batched_occupancy(ti, is_double, cfg, fast, g.lds);
+        }
+        g.grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(static_cast<int64_t>(ncu) * occ, g.nitems)));
+        T.groups.push_back(g);
+    }
     return DIPAMD_OK;
 }
 
@@ -214,13 +274,11 @@ std::vector<uint8_t> group_args(const BatchTables<TB> &T, size_t gi, const void 
     return bytes;
 }
 
-template <typename TB>
-int build_plan(dipamd_batch_plan *P, const dipamd_batch_image *images, int n) {
+template <typename TB, typename CfgOf>
+int build_plan(dipamd_batch_plan *P, const dipamd_batch_image *images, int n, CfgOf cfg_of) {
     BatchTables<TB> T;
-    int st = plan_tables<TB>(images, n, P->in_dtype, P->truncation, P->fast, P->cfg, T);
+    int st = plan_tables<TB>(images, n, P->in_dtype, P->truncation, P->fast, cfg_of, T);
     if (st != DIPAMD_OK) return st;
-    if (T.groups.size() > static_cast<size_t>(kMaxGroups))
-        return set_error(DIPAMD_E_UNSUPPORTED, "batched form: too many launch groups");
     P->cfg = T.groups[0].cfg;
     P->nitems = static_cast<int64_t>(T.items.size());
     P->groups = T.groups;
@@ -251,6 +309,12 @@ void free_plan_buffers(dipamd_batch_plan *P) {
     P->exec = nullptr;
     P->graph = nullptr;
     P->cap_stream = nullptr;
+    for (hipStream_t sk : P->side) (void)hipStreamDestroy(sk);
+    for (hipEvent_t ek : P->join) (void)hipEventDestroy(ek);
+    if (P->fork) (void)hipEventDestroy(P->fork);
+    P->side.clear();
+    P->join.clear();
+    P->fork = nullptr;
     if (P->d_images) (void)hipFree(P->d_images);
     if (P->d_items) (void)hipFree(P->d_items);
     if (P->d_taps) (void)hipFree(P->d_taps);
@@ -259,10 +323,18 @@ void free_plan_buffers(dipamd_batch_plan *P) {
     P->d_counter = nullptr;
 }
 
-int make_plan(dipamd_batch_plan *P, int cfg) {
-    P->cfg = cfg;
-    return P->is_double ? build_plan<double>(P, P->host_images.data(), P->n)
-                        : build_plan<float>(P, P->host_images.data(), P->n);
+template <typename CfgOf>
+int make_plan(dipamd_batch_plan *P, CfgOf cfg_of) {
+    return P->is_double ? build_plan<double>(P, P->host_images.data(), P->n, cfg_of)
+                        : build_plan<float>(P, P->host_images.data(), P->n, cfg_of);
+}
+
+// One group's launch (its counter cleared first).
+hipError_t launch_group(const BatchGroup &g, const std::vector<uint8_t> &args, int32_t *counter, int in_dtype,
+                        bool is_double, bool fast, hipStream_t stream) {
+    const InType ti = in_dtype == DIPAMD_DT_UINT8 ? IN_U8 : IN_BUF;
+    if (g.key.rxc > 0) return launch_batched_rc(ti, is_double, g.cfg, fast, g.key.rxc, args.data(), g.grid, g.lds, stream, nullptr);
+    return launch_batched(ti, is_double, g.cfg, fast, args.data(), g.grid, g.lds, stream);
 }
 
 // The groups' launches on `stream`: counters cleared by a kernel (no memset node
@@ -270,16 +342,63 @@ int make_plan(dipamd_batch_plan *P, int cfg) {
 int launch_groups(const std::vector<BatchGroup> &groups, const std::vector<std::vector<uint8_t>> &args,
                   int32_t *d_counter, int in_dtype, bool is_double, bool fast, hipStream_t stream) {
     hipError_t e = launch_reset_counter(d_counter, static_cast<int>(groups.size()) * kCounterStride, stream);
-    const InType ti = in_dtype == DIPAMD_DT_UINT8 ? IN_U8 : IN_BUF;
     for (size_t gi = 0; e == hipSuccess && gi < groups.size(); ++gi)
-        e = launch_batched(ti, is_double, groups[gi].cfg, fast, args[gi].data(), groups[gi].grid, groups[gi].lds,
-                           stream);
+        e = launch_group(groups[gi], args[gi], d_counter, in_dtype, is_double, fast, stream);
     if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("batched launch: ") + hipGetErrorString(e));
     return DIPAMD_OK;
 }
 
+// Side streams for concurrent groups: DIPAMD_BATCH_STREAMS (default 3, besides the
+// caller's; 0 runs the groups one after another).
+int batch_side_streams() {
+    static int n = [] {
+        const char *v = std::getenv("DIPAMD_BATCH_STREAMS");
+        const int k = v ? std::atoi(v) : 3;
+        return std::max(0, std::min(k, 8));
+    }();
+    return n;
+}
+
+// The plan's groups on `stream`.  With several groups and side streams, the groups run
+// concurrently: the counters are cleared on `stream`, every side stream waits for that,
+// group g goes to stream (g mod (1 + sides)) -- 0 being the caller's --, and `stream`
+// waits for every side stream's last group.  A persistent group that runs out of items
+// frees its CUs for the others, so one group's tail overlaps the next group's work.
 int run_plan(dipamd_batch_plan *P, hipStream_t stream) {
-    return launch_groups(P->groups, P->args, P->d_counter, P->in_dtype, P->is_double, P->fast, stream);
+    const int ns = batch_side_streams();
+    if (P->groups.size() < 2 || ns == 0)
+        return launch_groups(P->groups, P->args, P->d_counter, P->in_dtype, P->is_double, P->fast, stream);
+    hipError_t e = hipSuccess;
+    const int nside = std::min<int>(ns, static_cast<int>(P->groups.size()) - 1);
+    if (P->side.empty()) {
+        for (int k = 0; k < nside && e == hipSuccess; ++k) {
+            hipStream_t sk = nullptr;
+            e = hipStreamCreateWithFlags(&sk, hipStreamNonBlocking);
+            if (e == hipSuccess) P->side.push_back(sk);
+        }
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&P->fork, hipEventDisableTiming);
+        for (int k = 0; k < nside && e == hipSuccess; ++k) {
+            hipEvent_t ek = nullptr;
+            e = hipEventCreateWithFlags(&ek, hipEventDisableTiming);
+            if (e == hipSuccess) P->join.push_back(ek);
+        }
+        if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("batch streams: ") + hipGetErrorString(e));
+    }
+    e = launch_reset_counter(P->d_counter, static_cast<int>(P->groups.size()) * kCounterStride, stream);
+    if (e == hipSuccess) e = hipEventRecord(P->fork, stream);
+    for (size_t k = 0; e == hipSuccess && k < P->side.size(); ++k) e = hipStreamWaitEvent(P->side[k], P->fork, 0);
+    const size_t nq = P->side.size() + 1;
+    for (size_t gi = 0; e == hipSuccess && gi < P->groups.size(); ++gi) {
+        const size_t q = gi % nq;
+        e = launch_group(P->groups[gi], P->args[gi], P->d_counter, P->in_dtype, P->is_double, P->fast,
+                         q == 0 ? stream : P->side[q - 1]);
+    }
+    for (size_t k = 0; e == hipSuccess && k < P->side.size(); ++k) {
+        e = hipEventRecord(P->join[k], P->side[k]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(stream, P->join[k], 0);
+    }
+    if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("batched launch: ") + hipGetErrorString(e));
+    return DIPAMD_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -314,11 +433,11 @@ int scratch_address(void **ptr) {
 
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
-template <typename TB>
+template <typename TB, typename CfgOf>
 int run_noalloc(const dipamd_batch_image *images, int n, int in_dtype, const int bc[2], double truncation, bool fast,
-                int cfg, hipStream_t stream) {
+                CfgOf cfg_of, hipStream_t stream) {
     BatchTables<TB> T;
-    int st = plan_tables<TB>(images, n, in_dtype, truncation, fast, cfg, T);
+    int st = plan_tables<TB>(images, n, in_dtype, truncation, fast, cfg_of, T);
     if (st != DIPAMD_OK) return st;
     if (T.groups.size() > static_cast<size_t>(kMaxGroups))
         return set_error(DIPAMD_E_UNSUPPORTED, "batched form: too many launch groups");
@@ -414,58 +533,71 @@ int dipamd_batch_create(dipamd_batch_plan **plan, const dipamd_batch_image *imag
     P->truncation = truncation;
     P->host_images.assign(images, images + n);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    int cfg = tile_config;
-    if (P->is_double && cfg == 2) cfg = 1;
-    if (cfg < 0) {
-        // autotune over the tile configs on this batch (timed on the caller's stream),
-        // memoised per (input type, fast, largest radius, boundary condition)
-        int rmax = 0;
-        for (int i = 0; i < n; ++i) {
-            const double s2 = std::max(images[i].sigma_x, images[i].sigma_y);
-            rmax = std::max(rmax, gauss_radius(s2, 0, truncation, in_dtype == DIPAMD_DT_DFLOAT));
+    if (tile_config >= 0) {
+        const int c = (P->is_double && tile_config == 2) ? 1 : tile_config;
+        st = make_plan(P, [c](const GroupKey &) { return c; });
+    } else {
+        // each group's tile config: the persisted measurement where there is one, else
+        // measured now on this batch (every config, each group timed alone on the
+        // caller's stream) and persisted
+        std::map<GroupKey, int> chosen;
+        std::vector<GroupKey> keys;
+        if ((st = make_plan(P, [](const GroupKey &) { return 1; })) == DIPAMD_OK)
+            for (const BatchGroup &g : P->groups) keys.push_back(g.key);
+        free_plan_buffers(P);
+        std::vector<GroupKey> todo;
+        for (const GroupKey &k : keys) {
+            int v = -1;
+            if (tune_get(group_tune_key(in_dtype, P->fast, bcs, k), &v) && v >= 0 && v < kNumFusedConfigs)
+                chosen[k] = v;
+            else
+                todo.push_back(k);
         }
-        const auto key = std::make_tuple(in_dtype, P->fast ? 1 : 0, rmax, bcs[0] * 16 + bcs[1]);
-        {
-            std::lock_guard<std::mutex> lk(g_tune_mutex);
-            auto it = g_batch_tuned.find(key);
-            if (it != g_batch_tuned.end()) cfg = it->second;
-        }
-        if (cfg < 0) {
-            float best = 1e30f;
-            int best_cfg = 0;
+        if (st == DIPAMD_OK && !todo.empty()) {
+            std::map<GroupKey, float> best;
             hipEvent_t e0, e1;
             (void)hipEventCreate(&e0);
             (void)hipEventCreate(&e1);
             for (int c = 0; c < kNumFusedConfigs; ++c) {
                 if (P->is_double && c == 2) continue;
-                if ((st = make_plan(P, c)) != DIPAMD_OK) {
+                if (make_plan(P, [c](const GroupKey &) { return c; }) != DIPAMD_OK) {
                     free_plan_buffers(P);
                     continue;
                 }
-                float ms = 0, tot = 0;
-                bool ok = run_plan(P, s) == DIPAMD_OK;  // warm-up
-                for (int rep = 0; ok && rep < 2; ++rep) {
-                    (void)hipEventRecord(e0, s);
-                    ok = run_plan(P, s) == DIPAMD_OK;
-                    (void)hipEventRecord(e1, s);
-                    (void)hipEventSynchronize(e1);
-                    (void)hipEventElapsedTime(&ms, e0, e1);
-                    tot += ms;
+                for (size_t gi = 0; gi < P->groups.size(); ++gi) {
+                    const BatchGroup &g = P->groups[gi];
+                    if (g.cfg != c || std::find(todo.begin(), todo.end(), g.key) == todo.end()) continue;
+                    int32_t *ctr = P->d_counter + gi * kCounterStride;
+                    float tot = 0, ms = 0;
+                    bool ok = true;
+                    for (int rep = 0; ok && rep < 3; ++rep) {  // the first run warms up
+                        (void)hipEventRecord(e0, s);
+                        ok = launch_reset_counter(ctr, 1, s) == hipSuccess &&
+                             launch_group(g, P->args[gi], ctr, P->in_dtype, P->is_double, P->fast, s) == hipSuccess;
+                        (void)hipEventRecord(e1, s);
+                        (void)hipEventSynchronize(e1);
+                        (void)hipEventElapsedTime(&ms, e0, e1);
+                        if (rep > 0) tot += ms;
+                    }
+                    auto it = best.find(g.key);
+                    if (ok && (it == best.end() || tot < it->second)) {
+                        best[g.key] = tot;
+                        chosen[g.key] = c;
+                    }
                 }
                 free_plan_buffers(P);
-                if (ok && tot < best) {
-                    best = tot;
-                    best_cfg = c;
-                }
             }
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
-            cfg = best_cfg;
-            std::lock_guard<std::mutex> lk(g_tune_mutex);
-            g_batch_tuned[key] = cfg;
+            for (const GroupKey &k : todo)
+                if (chosen.count(k)) tune_put(group_tune_key(in_dtype, P->fast, bcs, k), chosen[k]);
         }
+        st = make_plan(P, [&chosen](const GroupKey &k) {
+            auto it = chosen.find(k);
+            return it != chosen.end() ? it->second : 1;
+        });
     }
-    if ((st = make_plan(P, cfg)) != DIPAMD_OK) {
+    if (st != DIPAMD_OK) {
         free_plan_buffers(P);
         delete P;
         return st;
@@ -534,22 +666,16 @@ int dipamd_batch_gauss_noalloc(const dipamd_batch_image *images, int32_t n, int3
     if (tile_config >= kNumFusedConfigs) return set_error(DIPAMD_E_PARAMETER, "tile config out of range");
     const bool is_double = out_dtype == DIPAMD_DT_DFLOAT;
     const bool fast = (flags & DIPAMD_FAST) != 0;
-    int cfg = tile_config;
-    if (cfg < 0) {
-        cfg = 1;
-        int rmax = 0;
-        for (int i = 0; i < n; ++i)
-            rmax = std::max(rmax, gauss_radius(std::max(images[i].sigma_x, images[i].sigma_y), 0, truncation,
-                                               in_dtype == DIPAMD_DT_DFLOAT));
-        const auto key = std::make_tuple(static_cast<int>(in_dtype), fast ? 1 : 0, rmax, bcs[0] * 16 + bcs[1]);
-        std::lock_guard<std::mutex> lk(g_tune_mutex);
-        auto it = g_batch_tuned.find(key);
-        if (it != g_batch_tuned.end()) cfg = it->second;
-    }
-    if (is_double && cfg == 2) cfg = 1;
+    // each group's tile config: the caller's, or the persisted measurement, or the default
+    auto cfg_of = [&](const GroupKey &k) {
+        if (tile_config >= 0) return (is_double && tile_config == 2) ? 1 : static_cast<int>(tile_config);
+        int v = -1;
+        if (tune_get(group_tune_key(in_dtype, fast, bcs, k), &v) && v >= 0 && v < kNumFusedConfigs) return v;
+        return 1;
+    };
     hipStream_t s = static_cast<hipStream_t>(stream);
-    return is_double ? run_noalloc<double>(images, n, in_dtype, bcs, truncation, fast, cfg, s)
-                     : run_noalloc<float>(images, n, in_dtype, bcs, truncation, fast, cfg, s);
+    return is_double ? run_noalloc<double>(images, n, in_dtype, bcs, truncation, fast, cfg_of, s)
+                     : run_noalloc<float>(images, n, in_dtype, bcs, truncation, fast, cfg_of, s);
 }
 
 int dipamd_batch_destroy(dipamd_batch_plan *plan) {

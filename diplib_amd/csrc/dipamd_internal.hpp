@@ -71,6 +71,14 @@ int broadcast_bc(const int32_t *bc, int nbc, int ndim, int *out);
 hipError_t scratch_alloc(void **p, size_t bytes, hipStream_t stream);
 void scratch_free(void *p, hipStream_t stream);
 
+// Persistent tuning cache (autotune.cpp): measured choices keyed by a string naming the
+// decision and its parameters, per device ("gfx950-256cu").  The file is DIPAMD_TUNE_FILE
+// or tune_cache.txt beside the library, one "<device> <key>\t<value>" per line, a later
+// line overriding an earlier one; tune_put appends (DIPAMD_TUNE_WRITE=0: memory only).
+std::string tune_device_tag();
+bool tune_get(const std::string &key, int *value);
+void tune_put(const std::string &key, int value);
+
 // Dynamic LDS a workgroup may use on this device (64 KiB unless the device reports
 // less); kernels are sized against it.
 size_t lds_limit_bytes();

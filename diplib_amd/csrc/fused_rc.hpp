@@ -320,7 +320,7 @@ __device__ __forceinline__ void fused_tile_rc_body(const Tile<TB> &t, const TI *
 template <typename TI, typename TB, int TXC, int P, int SX, int SY, bool FAST, int RXC>
 __device__ __forceinline__ void fused_tile_rc(const Tile<TB> &t, const TI *__restrict__ src, TB *__restrict__ dst,
                                               TB *lds, const TB *tx, const TB *ty) {
-    if (t.c0 - RXC >= 0 && t.c0 + TXC + RXC <= t.nx)
+    if (t.vec_ok && t.c0 - RXC >= 0 && t.c0 + TXC + RXC <= t.nx)
         fused_tile_rc_body<TI, TB, TXC, P, SX, SY, FAST, RXC, true>(t, src, dst, lds, tx, ty);
     else
         fused_tile_rc_body<TI, TB, TXC, P, SX, SY, FAST, RXC, false>(t, src, dst, lds, tx, ty);
@@ -355,7 +355,57 @@ __global__ __launch_bounds__(kBlock) void k_fused2d_rc(const FusedArgs<TB> a, in
     t.bcy = a.bcy;
     t.cvalx = a.cvalx;
     t.cvaly = a.cvaly;
+    t.vec_ok = 1;  // checked by the host
     fused_tile_rc<TI, TB, TXC, P, SX, SY, FAST, RXC>(t, static_cast<const TI *>(t.in), t.out, lds, a.tx, a.ty);
+}
+
+}  // namespace dipamd
+
+namespace dipamd {
+
+// K4R: the batched persistent form on the radius-class tile.  One launch covers the
+// work items of one radius class (x radius up to RXC): each workgroup takes the next
+// item from the group's counter, copies that image's taps to LDS and runs the tile.
+// The loop ends when the counter passes the item count, so every wave exits.
+template <typename TI, typename TB, int TXC, int P, bool FAST, int RXC>
+__global__ __launch_bounds__(kBlock) void k_batched_rc(const BatchArgs<TB> a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ int s_item;
+    __shared__ TB s_taps[2 * kFusedCap];
+    TB *lds = reinterpret_cast<TB *>(smem_raw);
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(a.counter, 1);
+        lds_barrier();
+        const int item = s_item;
+        if (item >= a.nitems) break;  // uniform: every wave of the workgroup leaves
+        const BatchItem it = a.items[item];
+        const BatchImage<TB> im = a.images[it.image];
+        const int ntx = im.rx + 1, nty = im.ry + 1;
+        for (int k = threadIdx.x; k < ntx + nty; k += blockDim.x) s_taps[k] = a.taps[im.tap_offset + k];
+        lds_barrier();  // taps visible; s_item read by all before it is rewritten
+        Tile<TB> t;
+        t.in = im.in;
+        t.out = im.out;
+        t.nx = static_cast<int>(im.nx);
+        t.ny = static_cast<int>(im.ny);
+        t.sxin = 1;
+        t.syin = im.syin;
+        t.sxout = 1;
+        t.syout = im.syout;
+        t.c0 = it.tile * TXC;
+        t.z0 = it.z0;
+        t.zend = it.z1;
+        t.rx = im.rx;
+        t.ry = im.ry;
+        t.bcx = a.bcx;
+        t.bcy = a.bcy;
+        t.cvalx = a.cvalx;
+        t.cvaly = a.cvaly;
+        t.vec_ok = im.vec_ok;
+        fused_tile_rc<TI, TB, TXC, P, SYM_EVEN, SYM_EVEN, FAST, RXC>(t, static_cast<const TI *>(t.in), t.out, lds,
+                                                                     s_taps, s_taps + ntx);
+        lds_barrier();  // LDS (ring, taps) free before the next item
+    }
 }
 
 }  // namespace dipamd

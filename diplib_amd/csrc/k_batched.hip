@@ -68,6 +68,20 @@ int batched_occupancy(InType ti, bool is_double, int cfg, bool fast, size_t lds)
     return fast ? occupancy_cfg<float, float, true>(cfg, lds) : occupancy_cfg<float, float, false>(cfg, lds);
 }
 
+hipError_t batched_rc_entry_u8(int cfg, bool fast, int rxc, const void *args, int grid, size_t lds, hipStream_t st,
+                               int *occ);
+hipError_t batched_rc_entry_f32(int cfg, bool fast, int rxc, const void *args, int grid, size_t lds, hipStream_t st,
+                                int *occ);
+hipError_t batched_rc_entry_f64(int cfg, bool fast, int rxc, const void *args, int grid, size_t lds, hipStream_t st,
+                                int *occ);
+
+hipError_t launch_batched_rc(InType ti, bool is_double, int cfg, bool fast, int rxc, const void *args, int grid,
+                             size_t lds, hipStream_t stream, int *occ) {
+    if (is_double) return batched_rc_entry_f64(cfg, fast, rxc, args, grid, lds, stream, occ);
+    if (ti == IN_U8) return batched_rc_entry_u8(cfg, fast, rxc, args, grid, lds, stream, occ);
+    return batched_rc_entry_f32(cfg, fast, rxc, args, grid, lds, stream, occ);
+}
+
 hipError_t launch_reset_counter(int32_t *counter, int n, hipStream_t stream) {
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(256), 0, stream, counter, n);
     return hipGetLastError();

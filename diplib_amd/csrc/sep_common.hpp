@@ -206,7 +206,7 @@ struct BatchImage {
     int64_t nx, ny, syin, syout;
     int32_t rx, ry;
     int32_t tap_offset;  // into the taps table: rx+1 x taps then ry+1 y taps
-    int32_t pad_;
+    int32_t vec_ok;      // input rows allow the radius-class kernel's vector loads
 };
 
 struct BatchItem {

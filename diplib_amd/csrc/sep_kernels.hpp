@@ -46,6 +46,9 @@ hipError_t launch_fused(InType ti, bool is_double, int cfg, int symx, int symy, 
 // kFusedRcMaxClass, or 0 with symx == SYM_NONE; fused_rc.hpp).  Needs a unit x stride;
 // hipErrorNotSupported for combinations it does not cover.
 constexpr int kFusedRcMaxClass = 24;
+// Radius class of an x radius for the radius-class kernels: the next multiple of 4, at
+// least 4 (a radius of 0 runs the centre tap only).
+inline int rc_class_of(int r) { return r <= 4 ? 4 : ((r + 3) / 4) * 4; }
 hipError_t launch_fused_rc(InType ti, bool is_double, int cfg, int symx, int symy, bool fast, const void *args,
                            int64_t nx, int64_t nchunks, int64_t nouter, int rxc, int ry, hipStream_t stream);
 // LDS bytes of the radius-class fused kernel.

@@ -496,6 +496,7 @@ struct Tile {
     int c0, z0, zend;  // first column, first and end output rows
     int rx, ry, bcx, bcy;
     TB cvalx, cvaly;
+    int vec_ok;  // rows allow vector loads (aligned origin and row stride; fused_rc.hpp)
 };
 
 // LDS row pitch (words) of a staged or ring row: padded, and congruent to P/2
@@ -669,6 +670,7 @@ __global__ __launch_bounds__(kBlock) void k_fused2d(const FusedArgs<TB> a, int64
     t.bcy = a.bcy;
     t.cvalx = a.cvalx;
     t.cvaly = a.cvaly;
+    t.vec_ok = 0;
     fused_tile<TI, TB, TXC, P, SX, SY, FAST>(t, static_cast<const TI *>(t.in), t.out, lds, a.tx, a.ty);
 }
 
@@ -713,6 +715,7 @@ __global__ __launch_bounds__(kBlock) void k_batched(const BatchArgs<TB> a) {
         t.bcy = a.bcy;
         t.cvalx = a.cvalx;
         t.cvaly = a.cvaly;
+        t.vec_ok = 0;
         fused_tile<TI, TB, TXC, P, SYM_EVEN, SYM_EVEN, FAST>(t, static_cast<const TI *>(t.in), t.out, lds, s_taps,
                                                              s_taps + ntx);
         lds_barrier();  // LDS (ring, taps) free before the next item
