@@ -10,7 +10,7 @@ from .batch import GaussBatch, gauss_batch_noalloc  # noqa: F401
 from .derivative import (Derivative, Dgg, Gradient, GradientMagnitude, Hessian, Laplace,  # noqa: F401
                          LaplaceMinusDgg, LaplacePlusDgg, StructureTensor)
 from .gauss import (Gauss, GaussFIR, GaussFT, GaussIIR, MakeHalfGaussian, OneDimensionalFilter,  # noqa: F401
-                    SeparableConvolution, best_gauss_method, gauss_radius, image_struct)
+                    SeparableConvolution, autotune, best_gauss_method, gauss_radius, image_struct)
 
 __all__ = ["Gauss", "GaussFIR", "GaussFT", "GaussIIR", "SeparableConvolution", "OneDimensionalFilter", "MakeHalfGaussian", "DipError",
            "gauss_radius", "best_gauss_method", "lib", "Derivative", "Gradient", "GradientMagnitude", "Hessian",

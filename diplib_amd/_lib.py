@@ -93,6 +93,8 @@ def lib():
     L.dipamd_gauss_ft.argtypes = [IMG, IMG, P, I32, P, I32, P, I32, D, U32, P]
     L.dipamd_derivative_outputs.argtypes = [I32, I32]
     L.dipamd_structure_tensor.argtypes = [IMG, IMG, I32, P, I32, P, I32, P, I32, D, U32, P]
+    if hasattr(L, "dipamd_autotune"):
+        L.dipamd_autotune.argtypes = [U32, P]
     if hasattr(L, "dipamd_gauss_fir_backward"):
         L.dipamd_gauss_fir_backward.argtypes = [IMG, IMG, IMG, P, P, I32, P, I32, P, I32, D, U32, P]
     _lib = L
@@ -112,7 +114,9 @@ EXPORTED_SYMBOLS = [
     "dipamd_batch_info", "dipamd_gauss_fir_backward", "dipamd_gauss_radius", "dipamd_last_error", "dipamd_version",
     "dipamd_trace_enable", "dipamd_trace_read", "dipamd_derivative", "dipamd_derivative_outputs",
     "dipamd_structure_tensor", "dipamd_gauss_iir", "dipamd_gauss_ft", "dipamd_batch_gauss_noalloc",
+    "dipamd_autotune",
 ]
+TUNE_QUICK = 256
 
 FN_GRADIENT, FN_GRADIENT_MAGNITUDE, FN_HESSIAN, FN_LAPLACE = 0, 1, 2, 3
 IIR_FORWARD_BACKWARD, IIR_DISCRETE_TIME_FIT = 1, 2

@@ -368,8 +368,7 @@ static int run_single(const View &src, const View &dst, int ndim, int d, const I
         // 0 column tiles, 1 LDS ring, 2 global sweep; by default the column tiles for
         // long filters (VALU-bound: the LDS tile keeps the window loads cheap) and the
         // global sweep for short ones (HBM-bound: no staging phase, full occupancy)
-        int kind = ysweep_kind();
-        if (kind < 0) kind = F.border >= kColsweepMinBorder ? 0 : 2;
+        int kind = plan_ysweep(is_double, fast, F.border);
         const bool lane_ok = src.strides[lane] == 1 && src.sizes[d] < (int64_t(1) << 31);
         const int rows = colsweep_rows(F.border, is_double);
         if (kind == 0 && lane_ok && rows > 0) {
@@ -698,7 +697,10 @@ int separable_device(const View &in, const View &out, int ndim, const InternFilt
     struct PassPlan {
         bool fused;
         int d0, d1;
+        int plan;  // fused passes: plan_fusion's choice
     };
+    // the first pass reads uint8 directly; other types are converted to the buffer type
+    const int first_src_dt = (in.dt == DIPAMD_DT_UINT8 && !is_double) ? DIPAMD_DT_UINT8 : buf_dt;
     std::vector<PassPlan> passes;
     size_t i0 = 0;
     if (!(flags & DIPAMD_NO_FUSION) && order.size() >= 2) {
@@ -711,16 +713,19 @@ int separable_device(const View &in, const View &out, int ndim, const InternFilt
                           in.sizes[order[0]] * std::llabs(in.strides[order[0]]) < (int64_t(1) << 31) &&
                           in.sizes[order[1]] < (int64_t(1) << 31);
         const bool index_bcs = !bc_value_based(bc[order[0]]) && !bc_value_based(bc[order[1]]);
-        if (symok && fits && index_bcs) {
-            const int cfg = choose_fused_config(is_double, Fx.border, Fy.border, in.sizes[order[0]], in.sizes[order[1]],
-                                                flags);
-            if (cfg >= 0) {
-                passes.push_back({true, order[0], order[1]});
+        // taps beyond the border would be dropped: a symmetric filter's ntaps is border + 1
+        const bool whole = static_cast<int>(Fx.data.size()) == Fx.border + 1 &&
+                           static_cast<int>(Fy.data.size()) == Fy.border + 1;
+        if (symok && fits && index_bcs && whole) {
+            const int plan = plan_fusion(first_src_dt, is_double, fast, Fx.border, Fy.border, in.sizes[order[0]],
+                                         in.sizes[order[1]], flags);
+            if (plan >= 0) {
+                passes.push_back({true, order[0], order[1], plan});
                 i0 = 2;
             }
         }
     }
-    for (size_t i = i0; i < order.size(); ++i) passes.push_back({false, order[i], -1});
+    for (size_t i = i0; i < order.size(); ++i) passes.push_back({false, order[i], -1, -1});
 
     // Source of the first pass: the input, converted first if its type has no kernel.
     View src = in;
@@ -768,12 +773,15 @@ int separable_device(const View &in, const View &out, int ndim, const InternFilt
         const PassPlan &pp = passes[k];
         if (pp.fused) {
             st = -100;
-            if (!is_double && !(flags & DIPAMD_FORCE_FUSION))
+            if (pp.plan == 3 && !is_double)
                 st = run_stream2d(src, dst, ndim, pp.d0, pp.d1, *filters[pp.d0], *filters[pp.d1], bc[pp.d0],
                                   bc[pp.d1], fast, stream);
             if (st == -100) {
-                const int cfg = choose_fused_config(is_double, filters[pp.d0]->border, filters[pp.d1]->border,
-                                                    src.sizes[pp.d0], src.sizes[pp.d1], flags);
+                const int cfg = pp.plan != 3 ? pp.plan
+                                             : plan_fused_config(src.dt == DIPAMD_DT_UINT8 ? DIPAMD_DT_UINT8 : buf_dt,
+                                                                 is_double, fast, filters[pp.d0]->border,
+                                                                 filters[pp.d1]->border, src.sizes[pp.d0],
+                                                                 src.sizes[pp.d1]);
                 st = is_double ? run_fused<double>(src, dst, ndim, pp.d0, pp.d1, *filters[pp.d0], *filters[pp.d1],
                                                    bc[pp.d0], bc[pp.d1], fast, cfg, stream)
                                : run_fused<float>(src, dst, ndim, pp.d0, pp.d1, *filters[pp.d0], *filters[pp.d1],

@@ -91,9 +91,15 @@ int cast_from_buffer(const View &src, const View &dst, int ndim, hipStream_t str
 int separable_device(const View &in, const View &out, int ndim, const InternFilter *const *filters, const int *bc,
                      const bool *process, uint32_t flags, hipStream_t stream);
 
-// Tile-shape choice for the fused 2-D kernel (autotune.cpp): a config index, or -1
-// to run one pass per axis instead.
-int choose_fused_config(bool is_double, int rx, int ry, int64_t nx, int64_t ny, uint32_t flags);
+// Schedule choices (autotune.cpp), measured by dipamd_autotune where available.
+// plan_fusion: whether and how the first two passes fuse (-1 one pass per axis, 0..2
+// the fused kernel's tile config, 3 the uint8 register-streaming kernel first);
+// src_dt is the first pass's source type (uint8 or the buffer type).
+int plan_fusion(int src_dt, bool is_double, bool fast, int rx, int ry, int64_t nx, int64_t ny, uint32_t flags);
+// The fused tile config where the streaming kernel does not run (-1: none fits).
+int plan_fused_config(int src_dt, bool is_double, bool fast, int rx, int ry, int64_t nx, int64_t ny);
+// Kernel of a single pass along a strided axis: 0 column tiles, 1 LDS ring, 2 sweep.
+int plan_ysweep(bool is_double, bool fast, int r);
 // Whether the register-streaming fused kernel may be used (DIPAMD_STREAM2D=0 disables).
 bool stream2d_enabled();
 // DIPAMD_ZSWEEP=0 turns the register-window sweep off (experiments).
@@ -104,9 +110,6 @@ bool fused_rc_enabled();
 // `dflt`, or DIPAMD_SWEEP_BLOCKS when set (experiments).
 int sweep_target_blocks(int dflt);
 int stream2d_target_waves();  // waves per launch of the register-streaming kernel
-// Kernel for a pass along a strided axis: 0 column tiles, 1 LDS ring, 2 global sweep
-// (DIPAMD_YSWEEP=col|ring|global; column tiles by default).
-int ysweep_kind();
 // Tile config of the LDS-ring sweep along a strided axis (y-only), or -1.
 int choose_ysweep_config(bool is_double, int ry);
 

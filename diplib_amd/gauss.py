@@ -283,3 +283,13 @@ def MakeHalfGaussian(sigma, derivativeOrder=0, truncation=3.0, dataType="SFLOAT"
 
 def gauss_radius(sigma, order=0, truncation=3.0, is_double=False):
     return lib().dipamd_gauss_radius(float(sigma), int(order), float(truncation), int(is_double))
+
+
+def autotune(fast=False, quick=False):
+    """Measures the plain path's schedule choices on this GPU (dipamd_autotune) and
+    stores them in the persistent tuning cache the planner reads on every call.
+    Results never depend on the choices; only the speed does."""
+    dev = torch.device("cuda", torch.cuda.current_device())
+    flags = (_lib.FAST if fast else _lib.EXACT) | (_lib.TUNE_QUICK if quick else 0)
+    with torch.cuda.device(dev):
+        check(lib().dipamd_autotune(flags, _stream_handle(dev)))

@@ -271,6 +271,18 @@ int dipamd_gauss_fir_backward(const dipamd_image *x, const dipamd_image *grad_ou
                               int32_t norders, const int32_t *bc, int32_t nbc, double truncation, uint32_t flags,
                               void *stream);
 
+/* ---- schedule tuning (no reference counterpart) ----
+ * Measures, on synthetic planes of this device, every candidate schedule of the plain
+ * path per (source type uint8 / sfloat / dfloat, radius class, plane size class): one
+ * pass per axis, the fused kernel with each tile config, the uint8 streaming kernel,
+ * and the kernel of a single strided pass; the fastest of each goes to the persistent
+ * tuning cache (DIPAMD_TUNE_FILE, else tune_cache.txt beside the library), which the
+ * planner consults on every call.  flags: DIPAMD_FAST tunes the fast arithmetic;
+ * DIPAMD_TUNE_QUICK uses smaller planes and fewer repetitions.  Allocates its own
+ * planes and synchronises `stream`; results do not depend on the choices. */
+#define DIPAMD_TUNE_QUICK 256u
+int dipamd_autotune(uint32_t flags, void *stream);
+
 /* Radius of the tap array used for (sigma, order) as GaussFIR computes it (gauss.cpp:38-51). */
 int dipamd_gauss_radius(double sigma, int32_t order, double truncation, int32_t is_double);
 
