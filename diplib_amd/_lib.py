@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(PKG, "libdipamd.so")
 MAX_DIMS = 8
 
 DT_UINT8, DT_UINT16, DT_UINT32, DT_SINT8, DT_SINT16, DT_SINT32, DT_SFLOAT, DT_DFLOAT, DT_BIN = range(9)
+DT_SCOMPLEX, DT_DCOMPLEX = 9, 10
 
 BC_CODES = {
     "": 0, "default": 0, "mirror": 0, "asym mirror": 1, "antisym reflect": 2, "periodic": 3,
@@ -32,7 +33,7 @@ class DipamdImage(ctypes.Structure):
 
 class DipamdFilter(ctypes.Structure):
     _fields_ = [("taps", ctypes.POINTER(ctypes.c_double)), ("ntaps", ctypes.c_int32), ("origin", ctypes.c_int32),
-                ("symmetry", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("symmetry", ctypes.c_int32), ("is_complex", ctypes.c_int32)]
 
 
 class DipamdBatchImage(ctypes.Structure):

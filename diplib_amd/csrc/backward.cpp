@@ -253,6 +253,7 @@ extern "C" int dipamd_gauss_fir_backward(const dipamd_image *x, const dipamd_ima
     hipStream_t stream = static_cast<hipStream_t>(stream_);
     int st;
     if ((st = check_image(x, "x")) != DIPAMD_OK) return st;
+    if ((st = require_real(x)) != DIPAMD_OK || (st = require_real(grad_out)) != DIPAMD_OK) return st;
     if ((st = check_image(grad_out, "grad_out")) != DIPAMD_OK) return st;
     const int nd = x->ndim;
     const bool want_dx = grad_in && grad_in->origin;

@@ -97,6 +97,8 @@ size_t dt_size(int dt) {
         case DIPAMD_DT_SINT32:
         case DIPAMD_DT_SFLOAT: return 4;
         case DIPAMD_DT_DFLOAT: return 8;
+        case DIPAMD_DT_SCOMPLEX: return 8;
+        case DIPAMD_DT_DCOMPLEX: return 16;
         default: return 0;
     }
 }
@@ -835,6 +837,13 @@ int check_image(const dipamd_image *im, const char *what) {
     return DIPAMD_OK;
 }
 
+int require_real(const dipamd_image *im) {
+    if (im && (im->dtype == DIPAMD_DT_SCOMPLEX || im->dtype == DIPAMD_DT_DCOMPLEX))
+        return set_error(DIPAMD_E_UNSUPPORTED, "complex images are taken by the separable convolution and the FIR "
+                                               "Gaussian only");
+    return DIPAMD_OK;
+}
+
 View to_view(const dipamd_image *im) {
     View v;
     v.p = im->origin;
@@ -920,6 +929,14 @@ int dipamd_separable_convolution(const dipamd_image *in, const dipamd_image *out
         return set_error(DIPAMD_E_ARRAY_LENGTH, "Array parameter has the wrong number of elements");
     int bcs[DIPAMD_MAX_DIMS];
     if ((st = broadcast_bc(bc, nbc, nd, bcs)) != DIPAMD_OK) return st;
+    // a complex filter or a complex image: the complex buffer path (convolution.cpp:331-346)
+    bool any_complex = false;
+    for (int i = 0; i < nfilters; ++i) any_complex = any_complex || filters[i].is_complex != 0;
+    if (any_complex || in->dtype == DIPAMD_DT_SCOMPLEX || in->dtype == DIPAMD_DT_DCOMPLEX)
+        return separable_complex(to_view(in), to_view(out), nd, filters, nfilters, bcs, process, any_complex,
+                                 static_cast<hipStream_t>(stream));
+    if (out->dtype == DIPAMD_DT_SCOMPLEX || out->dtype == DIPAMD_DT_DCOMPLEX)
+        return set_error(DIPAMD_E_UNSUPPORTED, "a real result into a complex output image");
     const bool use_double = suggest_flex(in->dtype) == DIPAMD_DT_DFLOAT;
     std::vector<InternFilter> F(static_cast<size_t>(nfilters));
     for (int i = 0; i < nfilters; ++i) {
@@ -955,7 +972,8 @@ int dipamd_gauss_fir(const dipamd_image *in, const dipamd_image *out, const doub
         return set_error(DIPAMD_E_ARRAY_LENGTH, "Array parameter has the wrong number of elements");
     if (norders != 0 && norders != 1 && norders != nd)
         return set_error(DIPAMD_E_ARRAY_LENGTH, "Array parameter has the wrong number of elements");
-    const bool compute_double = in->dtype == DIPAMD_DT_DFLOAT;
+    // the taps' precision (gauss.cpp:254): double for dfloat and dcomplex images
+    const bool compute_double = in->dtype == DIPAMD_DT_DFLOAT || in->dtype == DIPAMD_DT_DCOMPLEX;
     std::vector<dipamd_filter> fl(static_cast<size_t>(nd));
     std::vector<std::vector<double>> taps(static_cast<size_t>(nd));
     uint8_t proc[DIPAMD_MAX_DIMS];

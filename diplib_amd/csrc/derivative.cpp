@@ -208,6 +208,7 @@ int dipamd_derivative(int32_t function, const dipamd_image *in, const dipamd_ima
                       int32_t nprocess, double truncation, uint32_t flags, void *stream) {
     int st;
     if ((st = check_image(in, "in")) != DIPAMD_OK) return st;
+    if ((st = require_real(in)) != DIPAMD_OK) return st;
     if (function == DIPAMD_FN_STRUCTURE_TENSOR)
         return set_error(DIPAMD_E_PARAMETER, "the structure tensor takes dipamd_structure_tensor");
     const int nd = in->ndim;
@@ -262,6 +263,7 @@ int dipamd_structure_tensor(const dipamd_image *in, const dipamd_image *outs, in
                             const int32_t *bc, int32_t nbc, double truncation, uint32_t flags, void *stream) {
     int st;
     if ((st = check_image(in, "in")) != DIPAMD_OK) return st;
+    if ((st = require_real(in)) != DIPAMD_OK) return st;
     const int nd = in->ndim;
     double gsig[DIPAMD_MAX_DIMS], tsig[DIPAMD_MAX_DIMS];
     if ((st = array_param(gradient_sigmas, ngs, nd, 1.0, gsig)) != DIPAMD_OK) return st;

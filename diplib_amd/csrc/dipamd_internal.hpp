@@ -90,6 +90,12 @@ int cast_from_buffer(const View &src, const View &dst, int ndim, hipStream_t str
 
 int separable_device(const View &in, const View &out, int ndim, const InternFilter *const *filters, const int *bc,
                      const bool *process, uint32_t flags, hipStream_t stream);
+// The separable convolution over a complex buffer type (complex.cpp): a complex filter
+// in the array (any_complex_filter) or a complex input.
+int separable_complex(const View &in, const View &out, int ndim, const dipamd_filter *filters, int nfilters,
+                      const int *bc, const uint8_t *process, bool any_complex_filter, hipStream_t stream);
+// A real-valued entry point rejects complex images with DIPAMD_E_UNSUPPORTED.
+int require_real(const dipamd_image *im);
 
 // Schedule choices (autotune.cpp), measured by dipamd_autotune where available.
 // plan_fusion: whether and how the first two passes fuse (-1 one pass per axis, 0..2

@@ -341,6 +341,7 @@ extern "C" int dipamd_gauss_ft(const dipamd_image *in, const dipamd_image *out, 
     (void)flags;
     int st;
     if ((st = check_image(in, "in")) != DIPAMD_OK) return st;
+    if ((st = require_real(in)) != DIPAMD_OK) return st;
     if ((st = check_image(out, "out")) != DIPAMD_OK) return st;
     const int nd = in->ndim;
     if (out->ndim != nd) return set_error(DIPAMD_E_SIZES, "Sizes don't match");

@@ -25,6 +25,7 @@ from ._lib import DipamdFilter, DipamdImage, check, lib
 _TORCH_DT = {
     torch.uint8: _lib.DT_UINT8, torch.int8: _lib.DT_SINT8, torch.int16: _lib.DT_SINT16, torch.int32: _lib.DT_SINT32,
     torch.float32: _lib.DT_SFLOAT, torch.float64: _lib.DT_DFLOAT, torch.bool: _lib.DT_BIN,
+    torch.complex64: _lib.DT_SCOMPLEX, torch.complex128: _lib.DT_DCOMPLEX,
 }
 for _name, _code in (("uint16", _lib.DT_UINT16), ("uint32", _lib.DT_UINT32)):
     if hasattr(torch, _name):
@@ -32,9 +33,19 @@ for _name, _code in (("uint16", _lib.DT_UINT16), ("uint32", _lib.DT_UINT32)):
 
 
 def _flex_torch(dt):
+    """SuggestFlex (datatype.cpp:109-124)."""
+    if dt in (torch.complex64, torch.complex128):
+        return dt
     if dt in (torch.float64, torch.int32) or (hasattr(torch, "uint32") and dt == torch.uint32):
         return torch.float64
     return torch.float32
+
+
+def _complex_torch(dt):
+    """SuggestComplex (datatype.cpp:95-106)."""
+    if dt in (torch.float64, torch.int32, torch.complex128) or (hasattr(torch, "uint32") and dt == torch.uint32):
+        return torch.complex128
+    return torch.complex64
 
 
 def image_struct(t):
@@ -224,12 +235,25 @@ def Gauss(in_, sigmas=(1.0,), derivativeOrder=(0,), method="best", boundaryCondi
 
 
 class OneDimensionalFilter:
-    """dip::OneDimensionalFilter (linear.h:84-90), real filters."""
+    """dip::OneDimensionalFilter (linear.h:84-90).  A complex filter is given as
+    Python complex values, or with isComplex=True as (re, im) pairs of floats as the
+    reference stores it."""
 
-    def __init__(self, filter, origin=-1, symmetry="general"):
-        self.filter = [float(v) for v in filter]
+    def __init__(self, filter, origin=-1, symmetry="general", isComplex=False):
+        vals = list(filter)
+        if any(isinstance(v, complex) for v in vals) and not isComplex:
+            isComplex = True
+            flat = []
+            for v in vals:
+                c = complex(v)
+                flat += [c.real, c.imag]
+            vals = flat
+        self.filter = [float(v) for v in vals]
+        if isComplex and len(self.filter) % 2:
+            raise _lib.DipError(_lib.E_PARAMETER, "Complex filter must have an even number of values")
         self.origin = int(origin)
         self.symmetry = symmetry
+        self.isComplex = bool(isComplex)
 
 
 def SeparableConvolution(in_, filter, boundaryCondition=(), process=(), out=None, fast=False, fusion=None):
@@ -248,17 +272,20 @@ def SeparableConvolution(in_, filter, boundaryCondition=(), process=(), out=None
             raise _lib.DipError(_lib.E_PARAMETER, "Symmetry string not recognized: " + f.symmetry)
         taps = (ctypes.c_double * max(1, len(f.filter)))(*f.filter)
         keep.append(taps)
+        cplx = getattr(f, "isComplex", False)
         farr[i].taps = ctypes.cast(taps, ctypes.POINTER(ctypes.c_double))
-        farr[i].ntaps = len(f.filter)
+        farr[i].ntaps = len(f.filter) // 2 if cplx else len(f.filter)
         farr[i].origin = f.origin
         farr[i].symmetry = _lib.SYM_CODES[f.symmetry]
+        farr[i].is_complex = 1 if cplx else 0
     proc = list(process) if process is not None else []
     if len(proc) not in (0, nd):
         raise _lib.DipError(_lib.E_ARRAY_LENGTH, "Array parameter has the wrong number of elements")
     p_arr = (ctypes.c_uint8 * nd)(*[1 if p else 0 for p in proc]) if proc else None
     bcs = _bc_codes(boundaryCondition)
     b_arr = (ctypes.c_int32 * max(1, len(bcs)))(*bcs)
-    y = _prepare_out(x, out, _flex_torch(x.dtype))
+    any_complex = any(getattr(f, "isComplex", False) for f in filters)
+    y = _prepare_out(x, out, _complex_torch(x.dtype) if any_complex else _flex_torch(x.dtype))
     a_in, a_out = image_struct(x), image_struct(y)
     flags = _lib.FAST if fast else _lib.EXACT
     if fusion is False:

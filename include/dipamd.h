@@ -24,7 +24,11 @@ extern "C" {
  * default truncation of 3. */
 #define DIPAMD_MAX_RADIUS 255
 
-/* Sample types (dip::DataType, datatype.h), real types only. */
+/* Sample types (dip::DataType, datatype.h).  The complex types hold (re, im) pairs of
+ * float / double; strides of a complex image count complex samples.  Complex images
+ * and complex filters are taken by dipamd_separable_convolution and dipamd_gauss_fir
+ * (a Gaussian filters the real and imaginary parts, as the reference's does); the
+ * other entry points take real images. */
 enum {
     DIPAMD_DT_UINT8 = 0,
     DIPAMD_DT_UINT16 = 1,
@@ -34,7 +38,9 @@ enum {
     DIPAMD_DT_SINT32 = 5,
     DIPAMD_DT_SFLOAT = 6,
     DIPAMD_DT_DFLOAT = 7,
-    DIPAMD_DT_BIN = 8
+    DIPAMD_DT_BIN = 8,
+    DIPAMD_DT_SCOMPLEX = 9,
+    DIPAMD_DT_DCOMPLEX = 10
 };
 
 /* Boundary conditions (dip::BoundaryCondition, boundary.h:65-80, same order). */
@@ -59,15 +65,15 @@ enum {
    them (one extra pass over the image per axis that uses one); the batched and backward
    forms return DIPAMD_E_UNSUPPORTED for them. */
 
-/* Filter symmetry (convolution.cpp:40-48 FilterSymmetry, real filters). */
+/* Filter symmetry (convolution.cpp:40-48 FilterSymmetry). */
 enum {
     DIPAMD_SYM_GENERAL = 0,
     DIPAMD_SYM_EVEN = 1,
     DIPAMD_SYM_ODD = 2,
-    DIPAMD_SYM_CONJ = 3,   /* real filter: same as EVEN */
+    DIPAMD_SYM_CONJ = 3,   /* without a complex filter in the array: same as EVEN */
     DIPAMD_SYM_D_EVEN = 4,
     DIPAMD_SYM_D_ODD = 5,
-    DIPAMD_SYM_D_CONJ = 6  /* real filter: same as D_EVEN */
+    DIPAMD_SYM_D_CONJ = 6  /* without a complex filter in the array: same as D_EVEN */
 };
 
 /* Flags. */
@@ -112,14 +118,16 @@ typedef struct dipamd_image {
     int64_t strides[DIPAMD_MAX_DIMS];   /* in samples, may be negative */
 } dipamd_image;
 
-/* A real 1-D filter (dip::OneDimensionalFilter, linear.h:84-90). `taps` is HOST
- * memory, in the user's (convolution) order; origin -1 = centre. */
+/* A 1-D filter (dip::OneDimensionalFilter, linear.h:84-90). `taps` is HOST memory,
+ * in the user's (convolution) order; origin -1 = centre.  With is_complex != 0 the
+ * filter is complex: `taps` holds ntaps (re, im) pairs (2 * ntaps doubles), as
+ * OneDimensionalFilter::filter holds them with isComplex. */
 typedef struct dipamd_filter {
     const double *taps;
     int32_t ntaps;
     int32_t origin;
     int32_t symmetry;                   /* DIPAMD_SYM_* */
-    int32_t reserved;
+    int32_t is_complex;
 } dipamd_filter;
 
 /* Replaces dip::GaussFIR (include/diplib/linear.h:300-318; src/linear/gauss.cpp:238-286)
@@ -196,8 +204,11 @@ int dipamd_structure_tensor(const dipamd_image *in, const dipamd_image *outs, in
                             const int32_t *bc, int32_t nbc, double truncation, uint32_t flags, void *stream);
 
 /* Replaces dip::SeparableConvolution (include/diplib/linear.h:114-130;
- * src/linear/convolution.cpp:319-421) for real filters.  nfilters: 1 or ndim.
- * process: NULL (all) or ndim bytes. */
+ * src/linear/convolution.cpp:319-421).  nfilters: 1 or ndim.  process: NULL (all) or
+ * ndim bytes.  With a complex filter in the array the computation runs in
+ * SuggestComplex(in) (scomplex, or dcomplex for 32-bit integer and double input), with
+ * a complex input in SuggestFlex(in) (its own complex type); the output must then be
+ * complex (a real output would receive the modulus in the reference; not taken here). */
 int dipamd_separable_convolution(const dipamd_image *in, const dipamd_image *out, const dipamd_filter *filters,
                                  int32_t nfilters, const int32_t *bc, int32_t nbc, const uint8_t *process,
                                  uint32_t flags, void *stream);

@@ -76,13 +76,13 @@ class RunTimeError : public Error {
     using Error::Error;
 };
 
-/* The real sample types of datatype.h. */
+/* The sample types of datatype.h (without the 64-bit integers). */
 class DataType {
   public:
     enum class DT : int32_t {
         UINT8 = DIPAMD_DT_UINT8, UINT16 = DIPAMD_DT_UINT16, UINT32 = DIPAMD_DT_UINT32, SINT8 = DIPAMD_DT_SINT8,
         SINT16 = DIPAMD_DT_SINT16, SINT32 = DIPAMD_DT_SINT32, SFLOAT = DIPAMD_DT_SFLOAT, DFLOAT = DIPAMD_DT_DFLOAT,
-        BIN = DIPAMD_DT_BIN
+        BIN = DIPAMD_DT_BIN, SCOMPLEX = DIPAMD_DT_SCOMPLEX, DCOMPLEX = DIPAMD_DT_DCOMPLEX
     };
     constexpr DataType() : dt_(DT::SFLOAT) {}
     constexpr DataType(DT dt) : dt_(dt) {}  // NOLINT
@@ -93,14 +93,23 @@ class DataType {
         switch (dt_) {
             case DT::UINT8: case DT::SINT8: case DT::BIN: return 1;
             case DT::UINT16: case DT::SINT16: return 2;
-            case DT::DFLOAT: return 8;
+            case DT::DFLOAT: case DT::SCOMPLEX: return 8;
+            case DT::DCOMPLEX: return 16;
             default: return 4;
         }
     }
+    bool IsComplex() const { return dt_ == DT::SCOMPLEX || dt_ == DT::DCOMPLEX; }
     /* datatype.cpp:109-124 */
     static DataType SuggestFlex(DataType t) {
+        if (t.IsComplex()) return t;
         return (t.dt_ == DT::UINT32 || t.dt_ == DT::SINT32 || t.dt_ == DT::DFLOAT) ? DataType(DT::DFLOAT)
                                                                                   : DataType(DT::SFLOAT);
+    }
+    /* datatype.cpp:95-106 */
+    static DataType SuggestComplex(DataType t) {
+        return (t.dt_ == DT::UINT32 || t.dt_ == DT::SINT32 || t.dt_ == DT::DFLOAT || t.dt_ == DT::DCOMPLEX)
+                   ? DataType(DT::DCOMPLEX)
+                   : DataType(DT::SCOMPLEX);
     }
 
   private:
@@ -115,6 +124,8 @@ constexpr DataType DT_SINT32{DataType::DT::SINT32};
 constexpr DataType DT_SFLOAT{DataType::DT::SFLOAT};
 constexpr DataType DT_DFLOAT{DataType::DT::DFLOAT};
 constexpr DataType DT_BIN{DataType::DT::BIN};
+constexpr DataType DT_SCOMPLEX{DataType::DT::SCOMPLEX};
+constexpr DataType DT_DCOMPLEX{DataType::DT::DCOMPLEX};
 
 using DataSegment = std::shared_ptr<void>;
 /* image.h:80-87: a shared pointer that does not own the data. */
@@ -286,34 +297,40 @@ inline int32_t SymmetryCode(String const &s) {
     if (s == "d-conj") return DIPAMD_SYM_D_CONJ;
     throw ParameterError("Symmetry string not recognized: " + s);
 }
-/* ReForge `out` for the result of `in` (framework_separable.cpp:72-76, 126). */
-inline void PrepareOutput(Image const &in, Image &out) {
+/* ReForge `out` for a result of type `dt` from `in` (framework_separable.cpp:72-76, 126). */
+inline void PrepareOutput(Image const &in, Image &out, DataType dt) {
     if (!in.IsForged()) throw ParameterError("Image is not forged");
     const bool same = out.IsForged() && out.Origin() == in.Origin() && out.Sizes() == in.Sizes() &&
                       out.Strides() == in.Strides() && out.DataType() == in.DataType();
     if (same) {
-        if (out.DataType() != DataType::SuggestFlex(in.DataType()) && !out.IsProtected()) {
+        if (out.DataType() != dt && !out.IsProtected()) {
             // in-place but the output type changes: a new data segment for out
-            Image fresh(in.Sizes(), 1, DataType::SuggestFlex(in.DataType()));
+            Image fresh(in.Sizes(), 1, dt);
             out = fresh;
         }
         return;
     }
-    out.ReForge(in.Sizes(), 1, DataType::SuggestFlex(in.DataType()), Option::AcceptDataTypeChange::DO_ALLOW);
+    out.ReForge(in.Sizes(), 1, dt, Option::AcceptDataTypeChange::DO_ALLOW);
 }
+inline void PrepareOutput(Image const &in, Image &out) { PrepareOutput(in, out, DataType::SuggestFlex(in.DataType())); }
 }  // namespace detail
 
-/* dip::SeparableConvolution (linear.h:114-130), real filters. */
+/* dip::SeparableConvolution (linear.h:114-130); a complex filter (filter holds re, im
+ * pairs) makes the result SuggestComplex(in), as convolution.cpp:331-346. */
 inline void SeparableConvolution(Image const &in, Image &out, OneDimensionalFilterArray const &filterArray,
                                  StringArray const &boundaryCondition = {}, BooleanArray process = {}) {
     if (!in.IsForged()) throw ParameterError("Image is not forged");
     Image src = in;  // keeps the input's data alive if `out` is `in`
-    detail::PrepareOutput(in, out);
+    bool complexFilter = false;
+    for (auto const &o : filterArray) complexFilter = complexFilter || o.isComplex;
+    detail::PrepareOutput(in, out, complexFilter ? DataType::SuggestComplex(in.DataType())
+                                                 : DataType::SuggestFlex(in.DataType()));
     std::vector<dipamd_filter> f;
     for (auto const &o : filterArray) {
-        if (o.isComplex) throw ParameterError("Complex filters are not supported by this library");
-        f.push_back(dipamd_filter{o.filter.data(), static_cast<int32_t>(o.filter.size()),
-                                  static_cast<int32_t>(o.origin), detail::SymmetryCode(o.symmetry), 0});
+        if (o.isComplex && (o.filter.size() & 1)) throw ParameterError("Complex filter must have an even number of values");
+        const size_t n = o.isComplex ? o.filter.size() / 2 : o.filter.size();
+        f.push_back(dipamd_filter{o.filter.data(), static_cast<int32_t>(n), static_cast<int32_t>(o.origin),
+                                  detail::SymmetryCode(o.symmetry), o.isComplex ? 1 : 0});
     }
     std::vector<int32_t> bc = detail::BoundaryCodes(boundaryCondition);
     std::vector<uint8_t> proc(process.begin(), process.end());

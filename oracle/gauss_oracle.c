@@ -386,6 +386,14 @@ DEFINE_LINE_FILTER(line_filter_d, double, d)
 DEFINE_EXPAND(expand_f, float, FLT_MAX)
 DEFINE_EXPAND(expand_d, double, DBL_MAX)
 
+/* The boundary extension of one real line in place: `buf` points at sample 0 of n
+ * samples with `border` writable samples before and after (float or double).  Exposed
+ * for the complex restatement, which extends the real and imaginary parts of a line
+ * separately as copy_buffer.cpp:802-808 does. */
+int orc_expand_line(void *buf, int is_double, long n, long border, int bc) {
+    return is_double ? expand_d((double *)buf, n, border, border, bc) : expand_f((float *)buf, n, border, border, bc);
+}
+
 /* ---------------------------------------------------------------------------
  * Sample conversion: load as buffer type (a static_cast, exact for these
  * types), store with clamp_cast semantics (clamp_cast.h:210-235: clamp to the

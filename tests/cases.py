@@ -13,6 +13,9 @@ def make_input(shape, dtype, seed, impulses=0.0):
     (fraction `impulses` of the pixels, seeded positions)."""
     rng = np.random.default_rng(seed)
     dt = np.dtype(dtype)
+    if dt.kind == "c":
+        ft = np.float64 if dt == np.complex128 else np.float32
+        return (rng.random(shape).astype(ft) + 1j * rng.random(shape).astype(ft)).astype(dt)
     if dt.kind == "f":
         x = rng.random(shape, dtype=np.float64).astype(dt) if dt == np.float64 else rng.random(shape, dtype=np.float32)
     elif dt == np.bool_:
@@ -344,4 +347,69 @@ SEP_CASES += [
     ("sep_even_f32_out_u8_64x48", (48, 64), "float32", [([0.25, 0.5], "even", -1)], ["mirror"], None, "uint8"),
     ("sep_odd_u8_out_s16_64x48", (48, 64), "uint8", [([-1.0, 0.0], "odd", -1)], ["mirror"], None, "int16"),
     ("sep_even_u8_r4_256x64", (64, 256), "uint8", [(sep_taps(5, 911), "even", -1)], ["periodic"], None, None),
+]
+
+
+# Complex separable convolution (dip::SeparableConvolution with a complex buffer type):
+# (name, numpy shape, dtype, [(taps, symmetry, origin, is_complex), ...], boundary list,
+# process or None, out dtype or None).  Complex taps are Python complex values.
+def csep_taps(n, seed):
+    rng = np.random.default_rng(seed)
+    return list(rng.standard_normal(n) + 1j * rng.standard_normal(n))
+
+
+CSEP_CASES = []
+# complex filters of every symmetry (conj / d-conj act as such only here), real input
+for i, sym in enumerate(SEP_SYMS):
+    t = sym.replace("-", "")
+    for n in (3, 4):
+        CSEP_CASES.append((f"csep_{t}_n{n}_f32_37x29", (29, 37), "float32",
+                           [(csep_taps(n, 1100 + 10 * i + n), sym, -1, True)], ["mirror"], None, None))
+# complex filters on complex input, off-centre origins (d-conj origin 0: one sample
+# past the border), per-dim filters
+for i, sym in enumerate(SEP_SYMS):
+    t = sym.replace("-", "")
+    CSEP_CASES.append((f"csep_{t}_c64_origin0_31x23", (23, 31), "complex64",
+                       [(csep_taps(4, 1200 + i), sym, 0, True), (csep_taps(3, 1220 + i), "general", 2, True)],
+                       ["mirror", "periodic"], None, None))
+# real filters of a complex filter array (stored as (f, 0)) next to a complex one;
+# real filters with conj symmetry act as even when no filter of the array is complex
+CSEP_CASES += [
+    ("csep_mixed_real_complex_f32_40x30", (30, 40), "float32",
+     [(sep_taps(5, 1300), "even", -1, False), (csep_taps(3, 1301), "conj", -1, True)], ["mirror"], None, None),
+    ("csep_mixed_dconj_real_u8_40x30", (30, 40), "uint8",
+     [(sep_taps(3, 1302, neg=True), "d-conj", -1, False), (csep_taps(2, 1303), "odd", -1, True)],
+     ["add zeros"], None, None),
+    ("csep_real_filters_c64_45x33", (33, 45), "complex64",
+     [(sep_taps(5, 1304), "conj", -1, False), (sep_taps(4, 1305), "d-odd", 1, False)], ["mirror"], None, None),
+    ("csep_real_general_c128_45x33", (33, 45), "complex128",
+     [(sep_taps(6, 1306), "general", 2, False)], ["periodic"], None, None),
+]
+# every boundary condition, including those computed from the edge values (applied to
+# the real and imaginary parts separately), with short lines
+for j, bc in enumerate(("mirror", "asym mirror", "periodic", "asym periodic", "add zeros", "add max", "add min",
+                        "zero order", "antisym reflect", "first order", "second order", "third order")):
+    t = bc.replace(" ", "_")
+    CSEP_CASES.append((f"csep_general_{t}_c64_33x25", (25, 33), "complex64",
+                       [(csep_taps(5, 1400 + j), "general", 1, True), (csep_taps(4, 1420 + j), "d-conj", -1, True)],
+                       [bc], None, None))
+    CSEP_CASES.append((f"csep_even_{t}_short_5x3", (3, 5), "float32",
+                       [(csep_taps(6, 1440 + j), "even", -1, True)], [bc], None, None))
+# double precision buffers, 3-D, 1-D, process masks, long filters, narrower output
+CSEP_CASES += [
+    ("csep_conj_f64_45x33", (33, 45), "float64", [(csep_taps(4, 1500), "conj", -1, True)], ["mirror"], None, None),
+    ("csep_dconj_s32_45x33", (33, 45), "int32", [(csep_taps(3, 1501), "d-conj", 0, True)], ["periodic"], None,
+     None),
+    ("csep_mixed_c64_17x13x9", (9, 13, 17), "complex64",
+     [(csep_taps(3, 1502), "conj", -1, True), (sep_taps(4, 1503), "general", 1, False),
+      (csep_taps(2, 1504), "d-odd", 3, True)], ["mirror", "periodic", "add zeros"], None, None),
+    ("csep_general_1d_c128_150", (150,), "complex128", [(csep_taps(9, 1505), "general", 6, True)], ["mirror"], None,
+     None),
+    ("csep_process_y_u16_40x30", (30, 40), "uint16", [(csep_taps(3, 1506), "odd", -1, True)], ["mirror"], [0, 1],
+     None),
+    ("csep_identity_c64_40x30", (30, 40), "complex64", [([1.0], "general", -1, False)], ["mirror"], None, None),
+    ("csep_long_conj_f32_120x70", (70, 120), "float32", [(csep_taps(25, 1507), "conj", -1, True)], ["mirror"], None,
+     None),
+    ("csep_c128_out_c64_40x30", (30, 40), "complex128", [(csep_taps(4, 1508), "even", -1, True)], ["mirror"], None,
+     "complex64"),
 ]

@@ -112,11 +112,26 @@ def write_sep_cases(fname="sep_cases.npz", seed0=11000):
     np.savez_compressed(os.path.join(HERE, fname), **arrs)
 
 
+def write_csep_cases(fname="csep_cases.npz", seed0=12000):
+    """Complex separable convolution fixtures (complex filters and/or complex images),
+    written by the reference."""
+    arrs = {}
+    for i, (name, shape, dt, filters, bc, proc, odt) in enumerate(cases.CSEP_CASES):
+        x = cases.make_input(shape, dt, seed=seed0 + i)
+        arrs[name + "__in"] = x
+        arrs[name + "__out"] = oc.separable(x, [f[0] for f in filters], [f[1] for f in filters], bc, proc, odt,
+                                            origins=[f[2] for f in filters], use_ref=True,
+                                            is_complex=[f[3] for f in filters])
+    np.savez_compressed(os.path.join(HERE, fname), **arrs)
+
+
 def main():
     assert oc.ref_available(), "build oracle/_ref/libdipref.so first (make -f oracle/ref.mk)"
     only = sys.argv[sys.argv.index("--only") + 1] if "--only" in sys.argv else None
     if only in (None, "sep"):
         write_sep_cases()
+    if only in (None, "csep"):
+        write_csep_cases()
     if only in (None, "extrap"):
         write_cases(cases.EXTRAP_CASES, "extrap_cases.npz", 3000)
     if only in (None, "ft"):

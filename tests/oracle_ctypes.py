@@ -39,6 +39,7 @@ def orc():
         lib.orc_bc_map.argtypes = [ctypes.c_long, ctypes.c_long, I, ctypes.POINTER(I)]
         lib.orc_bc_map.restype = ctypes.c_long
         lib.orc_gauss_iir.argtypes = [P, I, P, I, I, P, P, I, P, I, P, I, P, I, I, D, I]
+        lib.orc_expand_line.argtypes = [P, I, ctypes.c_long, ctypes.c_long, I]
         _orc = lib
     return _orc
 
@@ -79,8 +80,8 @@ def _sizes(arr):
 
 def _flex(dt):
     dt = np.dtype(dt)
-    return np.dtype(np.float64) if dt in (np.dtype(np.uint32), np.dtype(np.int32), np.dtype(np.float64)) \
-        else np.dtype(np.float32)
+    return np.dtype(np.float64) if dt in (np.dtype(np.uint32), np.dtype(np.int32), np.dtype(np.float64),
+                                          np.dtype(np.complex128)) else np.dtype(np.float32)
 
 
 def _as_list(v, n):
@@ -302,6 +303,16 @@ def _ft_oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import gauss_ft_oracle
     return gauss_ft_oracle
+
+
+def separable_complex(x, filters, symmetries, bc=("mirror",), process=None, out_dtype=None, origins=None,
+                      is_complex=None):
+    """dip::SeparableConvolution with a complex buffer type: the numpy restatement
+    (oracle/separable_complex_oracle.py), boundary extension by the C oracle."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import separable_complex_oracle
+    return separable_complex_oracle.separable(x, filters, symmetries, bc, process, origins, is_complex, orc(),
+                                              out_dtype)
 
 
 def derivative(which, x, sigmas, bc=("mirror",), process=None, truncation=3.0, iir=False, tensor_iir=False,

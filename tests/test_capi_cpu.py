@@ -97,6 +97,19 @@ def test_argument_errors_without_device():
     assert L.dipamd_separable_convolution(ctypes.byref(good), ctypes.byref(good), ctypes.byref(f), 1, None, 0,
                                           None, 0, None) == _lib.E_PARAMETER
     assert b"Origin" in L.dipamd_last_error()
+    # complex: a complex filter needs a complex output image; an odd-length complex
+    # filter is refused when the Python layer builds it
+    f.origin = -1
+    f.is_complex = 1
+    f.ntaps = 1
+    assert L.dipamd_separable_convolution(ctypes.byref(good), ctypes.byref(good), ctypes.byref(f), 1, None, 0,
+                                          None, 0, None) == _lib.E_UNSUPPORTED
+    assert b"complex output" in L.dipamd_last_error()
+    from diplib_amd import OneDimensionalFilter
+    with pytest.raises(_lib.DipError):
+        OneDimensionalFilter([1.0, 2.0, 3.0], isComplex=True)
+    cf = OneDimensionalFilter([1 + 2j, -0.5j])
+    assert cf.isComplex and cf.filter == [1.0, 2.0, -0.0, -0.5]
 
 
 def test_best_method_resolution():

@@ -26,6 +26,7 @@ EXTRAP = np.load(os.path.join(HERE, "golden", "extrap_cases.npz"))
 DERIV = np.load(os.path.join(HERE, "golden", "deriv_cases.npz"))
 DERIV2 = np.load(os.path.join(HERE, "golden", "deriv2_cases.npz"))
 SEP = np.load(os.path.join(HERE, "golden", "sep_cases.npz"))
+CSEP = np.load(os.path.join(HERE, "golden", "csep_cases.npz"))
 
 
 @pytest.mark.parametrize("sigma", cases.TAP_SIGMAS)
@@ -214,6 +215,73 @@ def test_separable_dense_convolution(sym, n, origin, bc):
             if idx >= 0:
                 exp[i] += h[j] * sgn * x[0, idx]
     got = oc.separable(x, [taps, [1.0]], [sym, "general"], [bc], process=[1, 0], origins=[origin, -1])
+    assert np.allclose(got[0], exp, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("case", cases.CSEP_CASES, ids=[c[0] for c in cases.CSEP_CASES])
+def test_oracle_complex_separable_matches_reference(case):
+    """dip::SeparableConvolution with a complex buffer type (complex filters, complex
+    images, real filters in a complex array, value-based boundary conditions): the numpy
+    restatement against the reference's outputs, bit for bit."""
+    name, shape, dt, filters, bc, proc, odt = case
+    x = CSEP[name + "__in"]
+    exp = CSEP[name + "__out"]
+    got = oc.separable_complex(x, [f[0] for f in filters], [f[1] for f in filters], bc, proc, odt,
+                               origins=[f[2] for f in filters], is_complex=[f[3] for f in filters])
+    assert got.dtype == exp.dtype
+    assert np.array_equal(got.view(np.uint8), exp.view(np.uint8))
+
+
+def _full_complex_filter(taps, sym):
+    """The full kernel of a complex filter description (linear.h:70-80): conj
+    [a,b,c,b*,a*], d-conj [a,b,c,c*,b*,a*]; the others as for real taps."""
+    t = list(taps)
+    if sym == "conj":
+        return t + [np.conj(v) for v in t[-2::-1]]
+    if sym == "d-conj":
+        return t + [np.conj(v) for v in t[::-1]]
+    return _full_filter(t, sym)
+
+
+@pytest.mark.parametrize("sym", ["general", "even", "odd", "conj", "d-even", "d-odd", "d-conj"])
+@pytest.mark.parametrize("n,origin", [(3, -1), (4, -1), (4, 0), (4, 5)])
+@pytest.mark.parametrize("bc", ["mirror", "periodic", "first order"])
+def test_complex_separable_dense_convolution(sym, n, origin, bc):
+    """The complex restatement against the textbook complex convolution out[i] =
+    sum_j h[j] x[i + o - j] with the full complex kernel, on a complex line; boundary
+    samples from the real extension of each part (the C restatement's, checked against
+    np.pad above).  A conjugate on the wrong side, a swapped real/imaginary product or a
+    wrong d-* pairing fails this."""
+    rng = np.random.default_rng(n * 11 + max(origin, 0))
+    taps = list(rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    h = _full_complex_filter(taps, sym)
+    size = len(h)
+    if origin >= size:
+        pytest.skip("origin outside the filter")
+    o = size // 2 if origin < 0 else origin
+    oi = size - o - 1
+    border = max(oi, size - oi - 1)
+    if sym.startswith("d-"):
+        o -= 1
+    W = 13
+    x = rng.standard_normal((1, W)) + 1j * rng.standard_normal((1, W))
+    ext = np.zeros(W + 2 * border, np.complex128)
+    for part, f in ((0, np.real), (1, np.imag)):
+        buf = np.zeros(W + 2 * border)
+        buf[border:border + W] = f(x[0])
+        assert oc.orc().orc_expand_line(buf[border:].ctypes.data_as(oc.ctypes.c_void_p), 1, W, border,
+                                        oc.BC[bc]) == 0
+        ext += buf if part == 0 else 1j * buf
+    exp = np.zeros(W, np.complex128)
+    for i in range(W):
+        for j in range(size):
+            q = i + o - j
+            if q < -border:
+                assert sym.startswith("d-") and q == -border - 1 and i == 0
+                continue
+            exp[i] += h[j] * ext[q + border]
+    got = oc.separable_complex(x, [taps, [1.0]], [sym, "general"], [bc], process=[1, 0], origins=[origin, -1],
+                               is_complex=[True, False])
     assert np.allclose(got[0], exp, rtol=1e-12, atol=1e-12)
 
 
