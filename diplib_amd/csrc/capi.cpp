@@ -531,46 +531,31 @@ static int run_fused(const View &src, const View &dst, int ndim, int dx, int dy,
     return DIPAMD_OK;
 }
 
-// The register-streaming fused kernel for uint8 planes and radii <= 4 (stream2d.hpp).
-// Returns -100 when its conditions do not hold (the caller then uses the LDS kernel).
-static int run_stream2d(const View &src, const View &dst, int ndim, int dx, int dy, const InternFilter &Fx,
-                        const InternFilter &Fy, int bcx, int bcy, bool fast, hipStream_t stream) {
-    if (!stream2d_enabled()) return -100;
-    auto sym_ok = [](const InternFilter &F) {
-        return (F.sym == DIPAMD_SYM_EVEN || F.sym == DIPAMD_SYM_ODD) && F.shift == 0 && F.border >= 1 &&
-               F.border <= kStreamRMaxHost && static_cast<int>(F.data.size()) == F.border + 1;
-    };
-    if (src.dt != DIPAMD_DT_UINT8 || !sym_ok(Fx) || !sym_ok(Fy)) return -100;
+// Geometry, boundary folds and the wave split of the register-streaming kernels
+// (stream2d.hpp, k_stream2d_deriv.hip) for a uint8 source plane stack; -100 when their
+// conditions on the source do not hold.  Taps, radii and outputs are left to the caller.
+static int stream_setup(StreamArgs &a, const View &src, const View &dst, int ndim, int dx, int dy, int bcx, int bcy) {
+    if (src.dt != DIPAMD_DT_UINT8) return -100;
     if (src.strides[dx] != 1 || dst.strides[dx] != 1) return -100;
     if (src.sizes[dx] >= (int64_t(1) << 30) || src.sizes[dy] >= (int64_t(1) << 30)) return -100;
     if (src.sizes[dx] < 16 || src.sizes[dy] < 16) return -100;  // its boundary folds reflect once
     if (src.sizes[dx] & 3) return -100;                          // whole 4-sample words per row
+    if (!bc_supported(bcx) || !bc_supported(bcy)) return -100;
     // word loads need 4-byte aligned rows in every plane
     if ((reinterpret_cast<uintptr_t>(src.p) & 3) || (src.strides[dy] & 3)) return -100;
     for (int d = 0; d < ndim; ++d)
         if (d != dx && d != dy && src.sizes[d] > 1 && (src.strides[d] & 3)) return -100;
-    StreamArgs a;
     std::memset(&a, 0, sizeof(a));
     a.in = static_cast<const uint8_t *>(src.p);
-    a.out = static_cast<float *>(dst.p);
     a.nx = static_cast<int>(src.sizes[dx]);
     a.ny = static_cast<int>(src.sizes[dy]);
     a.syin = src.strides[dy];
     a.syout = dst.strides[dy];
-    a.rx = Fx.border;
-    a.ry = Fy.border;
     a.bcx = bcx;
     a.bcy = bcy;
     a.cvalx = const_value<float>(bcx);
     a.cvaly = const_value<float>(bcy);
-    for (size_t k = 0; k < Fx.data.size(); ++k) a.tx[k] = static_cast<float>(Fx.data[k]);
-    for (size_t k = 0; k < Fy.data.size(); ++k) a.ty[k] = static_cast<float>(Fy.data[k]);
     fill_outer(a.outer, src, dst, ndim, dx, dy);
-    bool aligned = (reinterpret_cast<uintptr_t>(dst.p) & 15) == 0 && (dst.strides[dy] & 3) == 0;
-    for (int d = 0; d < ndim; ++d)
-        if (d != dx && d != dy && dst.sizes[d] > 1 && (dst.strides[d] & 3)) aligned = false;
-    if (!aligned) return -100;  // 16-byte stores of 4 outputs
-    a.aligned_out = 1;
     // one-reflection row fold coefficients (fold_once in stream2d.hpp, as arithmetic)
     const int ny = static_cast<int>(src.sizes[dy]);
     switch (bcy) {
@@ -602,9 +587,85 @@ static int run_stream2d(const View &src, const View &dst, int ndim, int dx, int 
     a.nchunks = static_cast<int>((a.ny + ch - 1) / ch);
     a.nwaves = base * a.nchunks;
     a.nplanes = nouter;
+    return DIPAMD_OK;
+}
+
+// Output rows of a streaming kernel: 16-byte stores of 4 samples.
+static bool stream_out_aligned(const View &dst, int ndim, int dx, int dy) {
+    if (dst.strides[dx] != 1) return false;
+    bool aligned = (reinterpret_cast<uintptr_t>(dst.p) & 15) == 0 && (dst.strides[dy] & 3) == 0;
+    for (int d = 0; d < ndim; ++d)
+        if (d != dx && d != dy && dst.sizes[d] > 1 && (dst.strides[d] & 3)) aligned = false;
+    return aligned;
+}
+
+// The register-streaming fused kernel for uint8 planes and radii <= 4 (stream2d.hpp).
+// Returns -100 when its conditions do not hold (the caller then uses the LDS kernel).
+static int run_stream2d(const View &src, const View &dst, int ndim, int dx, int dy, const InternFilter &Fx,
+                        const InternFilter &Fy, int bcx, int bcy, bool fast, hipStream_t stream) {
+    if (!stream2d_enabled()) return -100;
+    auto sym_ok = [](const InternFilter &F) {
+        return (F.sym == DIPAMD_SYM_EVEN || F.sym == DIPAMD_SYM_ODD) && F.shift == 0 && F.border >= 1 &&
+               F.border <= kStreamRMaxHost && static_cast<int>(F.data.size()) == F.border + 1;
+    };
+    if (!sym_ok(Fx) || !sym_ok(Fy)) return -100;
+    if (!stream_out_aligned(dst, ndim, dx, dy)) return -100;  // 16-byte stores of 4 outputs
+    StreamArgs a;
+    if (stream_setup(a, src, dst, ndim, dx, dy, bcx, bcy) != DIPAMD_OK) return -100;
+    a.out = static_cast<float *>(dst.p);
+    a.aligned_out = 1;
+    a.rx = Fx.border;
+    a.ry = Fy.border;
+    for (size_t k = 0; k < Fx.data.size(); ++k) a.tx[k] = static_cast<float>(Fx.data[k]);
+    for (size_t k = 0; k < Fy.data.size(); ++k) a.ty[k] = static_cast<float>(Fy.data[k]);
     TraceScope ts("stream2d", npx_of(src, ndim) * static_cast<int64_t>(1 + sizeof(float)), stream,
                   npx_of(src, ndim) * (filter_ops(Fx, fast) + filter_ops(Fy, fast)));
     hipError_t err = launch_stream2d(Fx.sym, Fy.sym, fast, &a, stream);
+    if (err != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(err));
+    return DIPAMD_OK;
+}
+
+int run_stream2d_deriv(int mode, const View &src, const View *outs, int nouts, const InternFilter *const *fx,
+                       const InternFilter *const *fy, int bcx, int bcy, bool fast, hipStream_t stream) {
+    if (!stream2d_enabled() || nouts < 1 || nouts > 3) return -100;
+    const int ndim = 2, dx = 0, dy = 1;
+    for (int o = 0; o < 3; ++o) {
+        const InternFilter *f[2] = {fx[o], fy[o]};
+        for (const InternFilter *F : f)
+            if (F && !((F->sym == (o == 1 ? DIPAMD_SYM_ODD : DIPAMD_SYM_EVEN)) && F->shift == 0 && F->border >= 1 &&
+                       F->border <= kStreamRMaxHost && static_cast<int>(F->data.size()) == F->border + 1))
+                return -100;
+    }
+    for (int k = 0; k < nouts; ++k) {
+        if (outs[k].dt != DIPAMD_DT_SFLOAT || !stream_out_aligned(outs[k], ndim, dx, dy)) return -100;
+        if (views_overlap(outs[k], src, ndim)) return -100;
+        for (int j = 0; j < k; ++j)
+            if (views_overlap(outs[k], outs[j], ndim)) return -100;
+    }
+    DerivStreamArgs a;
+    std::memset(&a, 0, sizeof(a));
+    if (stream_setup(a.s, src, outs[0], ndim, dx, dy, bcx, bcy) != DIPAMD_OK) return -100;
+    for (int k = 0; k < nouts; ++k) {
+        a.out[k] = static_cast<float *>(outs[k].p);
+        a.syout[k] = outs[k].strides[dy];
+    }
+    int64_t ops = 0;
+    for (int o = 0; o < 3; ++o) {
+        // an order the function does not use: radius 1, zero taps (never read)
+        a.rx[o] = fx[o] ? fx[o]->border : 1;
+        a.ry[o] = fy[o] ? fy[o]->border : 1;
+        if (fx[o]) {
+            for (size_t k = 0; k < fx[o]->data.size(); ++k) a.tx[o][k] = static_cast<float>(fx[o]->data[k]);
+            ops += filter_ops(*fx[o], fast);
+        }
+        if (fy[o]) {
+            for (size_t k = 0; k < fy[o]->data.size(); ++k) a.ty[o][k] = static_cast<float>(fy[o]->data[k]);
+            ops += filter_ops(*fy[o], fast);
+        }
+    }
+    TraceScope ts("stream2d_deriv", npx_of(src, ndim) * static_cast<int64_t>(1 + nouts * sizeof(float)), stream,
+                  npx_of(src, ndim) * ops);
+    const hipError_t err = launch_stream2d_deriv(mode, fast, &a, stream);
     if (err != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(err));
     return DIPAMD_OK;
 }

@@ -1,17 +1,21 @@
 // The derivative family on the Gaussian FIR path (derivative.cpp:120-521 of the
 // reference) and the structure tensor (structure_tensor.cpp:32-57): dip::Gradient,
 // GradientMagnitude, Hessian, Laplace, Dgg, LaplacePlusDgg, LaplaceMinusDgg and
-// StructureTensor with the FIR method.  Each Gaussian derivative is the plain drop-in
-// call (dipamd_gauss_fir, or dipamd_gauss_iir with DIPAMD_GAUSS_IIR); the combined forms
-// add one element-wise pass
-// (k_elementwise.hip) with the reference's operation order and type.
+// StructureTensor with the FIR method.  A 2-D uint8 image whose Gaussian derivatives
+// all have radius <= 4 takes the one-launch form (k_stream2d_deriv.hip: one read of the
+// input, every output written once, bit-identical to the separate calls).  Otherwise
+// each Gaussian derivative is the plain drop-in call (dipamd_gauss_fir, or
+// dipamd_gauss_iir with DIPAMD_GAUSS_IIR) and the combined forms add one element-wise
+// pass (k_elementwise.hip) with the reference's operation order and type.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <string>
 #include <vector>
 
 #include "dipamd.h"
 #include "dipamd_internal.hpp"
+#include "sep_common.hpp"
 #include "sep_kernels.hpp"
 
 namespace dipamd {
@@ -109,6 +113,57 @@ int gauss(int method, const dipamd_image *in, const dipamd_image *out, const dou
     return dipamd_gauss_fir(in, out, sig, in->ndim, orders, norders, bc, nbc, truncation,
                             flags & static_cast<uint32_t>(DIPAMD_FAST | DIPAMD_NO_FUSION | DIPAMD_FORCE_FUSION),
                             stream);
+}
+
+// DIPAMD_DERIV_FUSION=0 turns the one-launch form off (experiments).
+bool deriv_fusion_enabled() {
+    static const bool on = [] {
+        const char *e = std::getenv("DIPAMD_DERIV_FUSION");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
+// The one-launch form (run_stream2d_deriv) for a 2-D uint8 image, or -100.
+int try_stream_derivative(int32_t function, const dipamd_image *in, const dipamd_image *outs, int32_t nouts,
+                          const double *sig, const int32_t *bc, int32_t nbc, double truncation, uint32_t flags,
+                          void *stream) {
+    if (!deriv_fusion_enabled() || (flags & DIPAMD_NO_FUSION) || (flags & (DIPAMD_GAUSS_IIR | DIPAMD_GAUSS_FT)))
+        return -100;
+    if (in->ndim != 2 || in->dtype != DIPAMD_DT_UINT8) return -100;
+    int mode, maxorder;
+    switch (function) {
+        case DIPAMD_FN_GRADIENT: mode = kDerivGradient, maxorder = 1; break;
+        case DIPAMD_FN_GRADIENT_MAGNITUDE: mode = kDerivGradientMagnitude, maxorder = 1; break;
+        case DIPAMD_FN_LAPLACE: mode = kDerivLaplace, maxorder = 2; break;
+        case DIPAMD_FN_HESSIAN: mode = kDerivHessian, maxorder = 2; break;
+        case DIPAMD_FN_DGG: mode = kDerivDgg, maxorder = 2; break;
+        case DIPAMD_FN_LAPLACE_PLUS_DGG: mode = kDerivLaplacePlusDgg, maxorder = 2; break;
+        case DIPAMD_FN_LAPLACE_MINUS_DGG: mode = kDerivLaplaceMinusDgg, maxorder = 2; break;
+        default: return -100;
+    }
+    int bcs[2];
+    if (nbc != 0 && nbc != 1 && nbc != 2) return -100;
+    for (int d = 0; d < 2; ++d) bcs[d] = nbc == 0 ? DIPAMD_BC_SYMMETRIC_MIRROR : (nbc == 1 ? bc[0] : bc[d]);
+    // the orders each function combines (Laplace: 0 and 2 only)
+    InternFilter F[2][3];
+    const InternFilter *fx[3] = {nullptr, nullptr, nullptr}, *fy[3] = {nullptr, nullptr, nullptr};
+    for (int o = 0; o <= maxorder; ++o) {
+        if (function == DIPAMD_FN_LAPLACE && o == 1) continue;
+        for (int d = 0; d < 2; ++d) {
+            if (gauss_radius(sig[d], o, truncation, false) > kStreamRMaxHost) return -100;
+            const std::vector<double> t = half_gaussian(sig[d], o, truncation, false);
+            if (intern_filter(t.data(), static_cast<int>(t.size()), (o & 1) ? DIPAMD_SYM_ODD : DIPAMD_SYM_EVEN, -1,
+                              F[d][o]) != DIPAMD_OK)
+                return -100;
+        }
+        fx[o] = &F[0][o];
+        fy[o] = &F[1][o];
+    }
+    View v[3];
+    for (int k = 0; k < nouts && k < 3; ++k) v[k] = to_view(&outs[k]);
+    return run_stream2d_deriv(mode, to_view(in), v, nouts, fx, fy, bcs[0], bcs[1], (flags & DIPAMD_FAST) != 0,
+                              static_cast<hipStream_t>(stream));
 }
 
 int derivative_method(uint32_t flags) {
@@ -225,6 +280,9 @@ int dipamd_derivative(int32_t function, const dipamd_image *in, const dipamd_ima
     const int want = dipamd_derivative_outputs(function, n);
     if (want < 0) return want;
     if ((st = check_outputs(function, in, outs, nouts, want)) != DIPAMD_OK) return st;
+    if (n == 2 && (st = try_stream_derivative(function, in, outs, nouts, sig, bc, nbc, truncation, flags, stream)) !=
+                      -100)
+        return st;
     if (function == DIPAMD_FN_GRADIENT)  // derivative.cpp:160-169
         return derivatives(in, gradient_orders(nd, dims), outs, sig, bc, nbc, truncation, flags, stream);
     if (function == DIPAMD_FN_HESSIAN)  // derivative.cpp:348-369

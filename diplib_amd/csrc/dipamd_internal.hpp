@@ -94,6 +94,12 @@ int separable_device(const View &in, const View &out, int ndim, const InternFilt
 // in the array (any_complex_filter) or a complex input.
 int separable_complex(const View &in, const View &out, int ndim, const dipamd_filter *filters, int nfilters,
                       const int *bc, const uint8_t *process, bool any_complex_filter, hipStream_t stream);
+// The derivative family's one-launch form on a 2-D uint8 image (k_stream2d_deriv.hip):
+// mode kDeriv*, fx[o] / fy[o] the interned x / y Gaussian derivative of order o (null
+// where the function does not use it), outs the sfloat outputs.  -100 when its
+// conditions do not hold (the caller then makes the separate calls).
+int run_stream2d_deriv(int mode, const View &src, const View *outs, int nouts, const InternFilter *const *fx,
+                       const InternFilter *const *fy, int bcx, int bcy, bool fast, hipStream_t stream);
 // A real-valued entry point rejects complex images with DIPAMD_E_UNSUPPORTED.
 int require_real(const dipamd_image *im);
 

@@ -2,8 +2,9 @@
 // 404-416, 424-484) and of the structure tensor (structure_tensor.cpp:55;
 // arithmetic.cpp:116-170), over contiguous buffer-type images, in the reference's order
 // and type: each product and sum rounded in the buffer type (no contraction), the
-// square root correctly rounded (computed in double, which rounds once more exactly
-// for a square root of a float), dip::Abs for a single dim.
+// square root correctly rounded (hipcc's default for sqrtf; on gfx950 it equals the
+// double square root rounded to float for every one of the 2^32 float inputs, NaNs
+// included), dip::Abs for a single dim.
 #include "sep_kernels_impl.hpp"
 
 namespace dipamd {
@@ -25,7 +26,7 @@ template <typename T>
 __device__ __forceinline__ T ew_sqrt(T v);
 template <>
 __device__ __forceinline__ float ew_sqrt<float>(float v) {
-    return static_cast<float>(__builtin_sqrt(static_cast<double>(v)));
+    return __builtin_sqrtf(v);
 }
 template <>
 __device__ __forceinline__ double ew_sqrt<double>(double v) {

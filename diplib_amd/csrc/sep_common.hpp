@@ -198,6 +198,28 @@ struct StreamArgs {
     Outer outer;          // stack of planes
 };
 
+// The derivative family's fused streaming form (stream2d_deriv.hpp): every Gaussian
+// derivative of orders 0..2 per axis that the function needs, from one read of the
+// input, combined in registers.  `s` carries the geometry, strips and folds (its out,
+// rx, ry, tx, ty are unused); filters are indexed by derivative order.
+enum : int32_t {
+    kDerivGradient = 0,
+    kDerivGradientMagnitude = 1,
+    kDerivLaplace = 2,
+    kDerivHessian = 3,
+    kDerivDgg = 4,
+    kDerivLaplacePlusDgg = 5,
+    kDerivLaplaceMinusDgg = 6,
+};
+struct DerivStreamArgs {
+    StreamArgs s;
+    float *out[3];
+    int64_t syout[3];
+    int32_t rx[3], ry[3];
+    float tx[3][kStreamRMax + 1];
+    float ty[3][kStreamRMax + 1];
+};
+
 // Batched form: per-image parameters, work items and kernel arguments.
 template <typename TB>
 struct BatchImage {

@@ -35,6 +35,8 @@ hipError_t launch_zsweep(InType ti, bool is_double, int r, int sym, bool fast, c
                          int64_t nchunks, int64_t nouter, hipStream_t stream);
 // Register-streaming fused kernel for uint8 planes (k_stream2d.hip).
 hipError_t launch_stream2d(int symx, int symy, bool fast, const void *args, hipStream_t stream);
+// The derivative family's streaming form (k_stream2d_deriv.hip); args: DerivStreamArgs.
+hipError_t launch_stream2d_deriv(int mode, bool fast, const void *args, hipStream_t stream);
 // Column-tile sweep: rows per block for a border (0 when the block does not fit).
 int colsweep_rows(int border, bool is_double);
 hipError_t launch_colsweep(InType ti, bool is_double, int sym, bool fast, const void *args, int64_t e,

@@ -128,6 +128,18 @@ for fn in ("gradient", "gradmag", "hessian", "laplace"):
         (f"{fn}_f32_50x40_s0x3", fn, (40, 50), "float32", [0.0, 3.0], ["asym mirror"]),
     ]
 
+# 2-D uint8 images whose Gaussian derivatives all have radius <= 4 (sigma <= 1): the
+# one-launch form of the family (several strips, a partial last strip, every index-map
+# boundary condition).
+for fn in ("gradient", "gradmag", "hessian", "laplace"):
+    DERIV_CASES += [
+        (f"{fn}_u8_96x64_s1", fn, (64, 96), "uint8", [1.0], ["mirror"]),
+        (f"{fn}_u8_128x40_s08_periodic", fn, (40, 128), "uint8", [0.8], ["periodic"]),
+        (f"{fn}_u8_64x48_s1x07_mixedbc", fn, (48, 64), "uint8", [1.0, 0.7], ["asym mirror", "add zeros"]),
+        (f"{fn}_u8_300x20_s1_zero_order", fn, (20, 300), "uint8", [1.0], ["zero order"]),
+        (f"{fn}_u8_64x32_s1_addmax", fn, (32, 64), "uint8", [1.0], ["add max", "asym periodic"]),
+    ]
+
 # The Dgg family and the structure tensor ("structure:<tensor sigma>"), same layout.
 DERIV2_CASES = []
 for fn in ("dgg", "laplace+dgg", "laplace-dgg", "structure:3"):
@@ -141,6 +153,14 @@ for fn in ("dgg", "laplace+dgg", "laplace-dgg", "structure:3"):
     ]
 # a flat image: the gradient is zero everywhere and SafeDivide gives 0
 DERIV2_CASES.append(("dgg_flat_f32_30x20", "dgg", (20, 30), "flat", [2.0], ["mirror"]))
+# the one-launch form (2-D uint8, sigma <= 1)
+for fn in ("dgg", "laplace+dgg", "laplace-dgg"):
+    t = fn.replace("+", "p").replace("-", "m")
+    DERIV2_CASES += [
+        (f"{t}_u8_96x64_s1", fn, (64, 96), "uint8", [1.0], ["mirror"]),
+        (f"{t}_u8_300x40_s08_periodic", fn, (40, 300), "uint8", [0.8], ["periodic"]),
+        (f"{t}_u8_64x48_s1x07_mixedbc", fn, (48, 64), "uint8", [1.0, 0.7], ["asym mirror", "add zeros"]),
+    ]
 
 TAP_SIGMAS = [0.5, 0.8, 1.0, 1.5, 2.0, 3.0, 4.0, 5.0, 8.0, 16.0, 31.7]
 

@@ -59,6 +59,18 @@ def test_derivative_family_larger_vs_oracle(cuda, fn):
     assert np.array_equal(_elements(fn, y), oc.derivative(fn, v, [1.0, 2.0, 1.5], ["periodic"]))
 
 
+@pytest.mark.parametrize("fn", sorted(FN))
+def test_derivative_one_launch_vs_oracle(cuda, fn):
+    """2-D uint8 images with every Gaussian derivative of radius <= 4 take the one-launch
+    form: bit-identical to the oracle's separate derivatives and combination, over three
+    strips (the last one partial), isotropic and per-dim sigmas and boundary conditions."""
+    x = cases.make_input((520, 700), "uint8", seed=55)
+    for sig, bc in (([1.0], ["mirror"]), ([0.9, 1.0], ["periodic", "zero order"])):
+        y = FN[fn](torch.from_numpy(x).to(cuda), sig, "FIR", bc)
+        exp = oc.derivative(fn, x, sig, bc)
+        assert np.array_equal(_elements(fn, y).view(np.uint8), exp.view(np.uint8)), (fn, sig, bc)
+
+
 def test_derivative_family_process_and_errors(cuda):
     x = torch.rand((30, 40), device=cuda)
     g = dip.Gradient(x, [2.0], "FIR", ["mirror"], process=[True, False])
