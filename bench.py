@@ -196,6 +196,9 @@ def main():
     value = px_per_step / (ms_per_step * 1e-3) / 1e9
     log(f"headline: {ms_per_step:.3f} ms/step, {value:.2f} GPix/s")
 
+    copy_roof = measure_copy_roof(L, torch, dev)
+    log(f"copy roof: {copy_roof}")
+
     # dominant kernel over the timed region
     agg = {}
     for name, ms, by, ops in trace:
@@ -211,6 +214,7 @@ def main():
         kernels[name] = {"launches": cnt, "total_ms": round(tot, 4), "avg_ms": round(avg_ms, 5),
                          "bytes_per_launch": by // cnt, "achieved_GBs": round(gbs, 1),
                          "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                         "copy_roof_frac": round(gbs / roof_for(copy_roof, by / cnt / (SIZE * SIZE)), 4),
                          "ops_per_launch": ops // cnt,
                          "valu_frac": round((ops / cnt) / (avg_ms * 1e-3) / VALU_PEAK_OPS, 4)}
     dom = max(agg.items(), key=lambda kv: kv[1][0])[0] if agg else None
@@ -226,7 +230,9 @@ def main():
     if dom:
         k = kernels[dom]
         hbm = {"achieved": k["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": k["hbm_frac"],
-               "bytes_per_launch": k["bytes_per_launch"]}
+               "bytes_per_launch": k["bytes_per_launch"],
+               "measured_copy_GBs": roof_for(copy_roof, k["bytes_per_launch"] / (SIZE * SIZE)),
+               "frac_of_measured_copy": k["copy_roof_frac"]}
         # exact-mode arithmetic is one FP32 add or multiply per lane-op (no FMA, no packed
         # math), so its ceiling is the unpacked VALU rate; the published FP32 peak counts
         # an FMA as two FLOPs and packed pairs twice (4x this rate)
@@ -241,25 +247,36 @@ def main():
                     "bytes_per_launch": k["bytes_per_launch"], "avg_launch_ms": k["avg_ms"],
                     "hbm": hbm, "valu": valu}
 
-    # per-case rates (event-timed) and roofline fractions
+    # per-case rates (event-timed) and roofline fractions; each case also once more with
+    # the kernel trace on, whose per-call kernel time (kernel_ms) separates the kernels
+    # from the launch gaps between calls
     per_case = []
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     for fast in (False, True):
         for i, (s, o) in enumerate(combos):
-            for _ in range(2):
+            for _ in range(3):
                 dip.GaussFIR(x, [s], list(o), ["mirror"], 3.0, out=outs[i], fast=fast)
-            reps = 5
+            reps = 20
+            torch.cuda.synchronize()
             ev0.record()
             for _ in range(reps):
                 dip.GaussFIR(x, [s], list(o), ["mirror"], 3.0, out=outs[i], fast=fast)
             ev1.record()
             torch.cuda.synchronize()
             ms = ev0.elapsed_time(ev1) / reps
+            L.dipamd_trace_enable(1)
+            for _ in range(reps):
+                dip.GaussFIR(x, [s], list(o), ["mirror"], 3.0, out=outs[i], fast=fast)
+            torch.cuda.synchronize()
+            kms = sum(t[1] for t in read_trace(L)) / reps
+            L.dipamd_trace_enable(0)
             gbs = SIZE * SIZE * 5 / (ms * 1e-3) / 1e9
             ops = SIZE * SIZE * valu_ops_per_px([s, s], list(o), fast)
             per_case.append({"sigma": s, "orders": list(o), "mode": "fast" if fast else "exact",
-                             "ms": round(ms, 4), "GPix_s": round(SIZE * SIZE / (ms * 1e-3) / 1e9, 2),
+                             "ms": round(ms, 4), "kernel_ms": round(kms, 4),
+                             "GPix_s": round(SIZE * SIZE / (ms * 1e-3) / 1e9, 2),
                              "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                             "copy_roof_frac": round(gbs / copy_roof["u8_to_f32_GBs"], 4),
                              "valu_frac": round(ops / (ms * 1e-3) / VALU_PEAK_OPS, 4)})
     fast_ms = sum(c["ms"] for c in per_case if c["mode"] == "fast")
     fast_value = px_per_step / (fast_ms * 1e-3) / 1e9
@@ -297,6 +314,7 @@ def main():
                                "{(0,0),(1,0),(0,2)}, mirror, truncation 3 (9 calls per step)",
                    "image": [SIZE, SIZE], "in_dtype": "uint8", "out_dtype": "sfloat", "mode": "exact"},
         "roofline": roofline,
+        "copy_roof": copy_roof,
         "cpu_baseline": cpu,
         "bitexact_vs_reference": bitexact,
         "fast_mode_GPix_s": round(fast_value, 3),
@@ -305,6 +323,43 @@ def main():
     }
     line.update(extra)
     print(json.dumps(line), flush=True)
+
+
+def measure_copy_roof(L, torch, dev, reps=20):
+    """What the memory system sustains for the FIR path's two traffic shapes, measured
+    with the library's streaming copy kernels (k_roof.hip, interleaved layout: every
+    load and store instruction of a wave covers one contiguous span) on 8192^2 samples:
+    uint8 -> sfloat (5 bytes per sample) and sfloat -> sfloat (8)."""
+    n = SIZE * SIZE
+    out = {"samples": n, "kernel": "k_copy_u8_f32_il / k_copy_f32_f32_il"}
+    st = torch.cuda.current_stream(dev).cuda_stream
+    dst = torch.empty(n, dtype=torch.float32, device=dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    from diplib_amd import _lib
+    for key, src, dt in (("u8_to_f32", torch.randint(0, 256, (n,), dtype=torch.uint8, device=dev), 0),
+                         ("f32_to_f32", torch.rand(n, device=dev), 1)):
+        code = _lib.DT_UINT8 if dt == 0 else _lib.DT_SFLOAT
+        for _ in range(3):
+            if L.dipamd_measure_copy(src.data_ptr(), code, dst.data_ptr(), n, 2, st) != 0:
+                raise RuntimeError("copy kernel failed")
+        e0.record()
+        for _ in range(reps):
+            L.dipamd_measure_copy(src.data_ptr(), code, dst.data_ptr(), n, 2, st)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[key + "_us"] = round(ms * 1e3, 2)
+        out[key + "_GBs"] = round(n * (src.element_size() + 4) / (ms * 1e-3) / 1e9, 1)
+        ok = torch.equal(dst, src.float()) if dt == 0 else torch.equal(dst, src)
+        if not ok:
+            raise RuntimeError("copy kernel result wrong")
+    return out
+
+
+def roof_for(copy_roof, bytes_per_px):
+    """The measured copy roof of the traffic shape nearest to bytes_per_px (5: uint8 in,
+    sfloat out; 8: sfloat in and out)."""
+    return copy_roof["u8_to_f32_GBs"] if bytes_per_px < 6.5 else copy_roof["f32_to_f32_GBs"]
 
 
 def bench_volume(dip, torch, cases, dev):

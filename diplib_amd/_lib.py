@@ -96,6 +96,8 @@ def lib():
     L.dipamd_structure_tensor.argtypes = [IMG, IMG, I32, P, I32, P, I32, P, I32, D, U32, P]
     if hasattr(L, "dipamd_autotune"):
         L.dipamd_autotune.argtypes = [U32, P]
+    if hasattr(L, "dipamd_measure_copy"):  # measurement helper, not part of the drop-in interface
+        L.dipamd_measure_copy.argtypes = [P, I32, P, ctypes.c_int64, U32, P]
     if hasattr(L, "dipamd_gauss_fir_backward"):
         L.dipamd_gauss_fir_backward.argtypes = [IMG, IMG, IMG, P, P, I32, P, I32, P, I32, D, U32, P]
     _lib = L
