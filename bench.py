@@ -103,7 +103,8 @@ def main():
                          "512x512x256 sfloat volume, sigma (2,2,1); or 'iir 00' for dip::GaussIIR sigma 16, "
                          "'ft 00' for dip::GaussFT sigma 0.5, on the 8192^2 input; or 'bwd mirror' for "
                          "forward + backward through the autograd function, 32 sfloat 1024^2 planes, sigma 3 "
-                         "on x and y, order 0) --steps times, then a 256 MiB device copy as the calibration "
+                         "on x and y, order 0; or 'deriv laplace' etc. for the derivative family at sigma 1 "
+                         "and 4) --steps times, then a 256 MiB device copy as the calibration "
                          "kernel for the traffic counters")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -138,6 +139,18 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log(f"case backward {a.case[1]}: {a.steps} forward + backward calls, then {a.steps} copies of 256 MiB")
+        return
+    if a.case and a.case[0] == "deriv":
+        # the derivative family on the 8192^2 uint8 input, sigma 1 then 4 (FIR)
+        fn = {"gradient": dip.Gradient, "gradmag": dip.GradientMagnitude, "hessian": dip.Hessian,
+              "laplace": dip.Laplace, "dgg": dip.Dgg}[a.case[1]]
+        xd = torch.from_numpy(cases.make_input((SIZE, SIZE), "uint8", seed=2)).to(dev)
+        for s in (1.0, 4.0):
+            for _ in range(a.steps):
+                y = fn(xd, [s], "FIR", ["mirror"])
+            torch.cuda.synchronize()
+            del y
+        log(f"case derivative {a.case[1]}: {a.steps} calls at sigma 1, then {a.steps} at sigma 4")
         return
     if a.case and a.case[0] == "vol":
         xv = torch.from_numpy(cases.make_input((256, 512, 512), "float32", 3, 0.01)).to(dev)
