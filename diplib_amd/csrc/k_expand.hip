@@ -1,14 +1,26 @@
-// Boundary conditions whose extension is computed from the edge values (see
-// sep_extend.hpp).  The filter kernels read boundary extensions through an
-// index map, which these conditions do not have, so a pass with one of them first
-// writes the extended lines into a buffer that is 2*border longer along the pass
-// axis (this kernel), filters that buffer, and keeps the interior.
+// Extended lines: every line of an image along one dim written with its boundary
+// extension into a buffer 2*border longer along that dim.  The filter kernels read
+// index-map conditions through the map, but the conditions computed from the edge
+// values (sep_extend.hpp) have none, so a pass with one of them filters this buffer;
+// the line buffers of the plug-in framework (lines.cpp) use it for every condition
+// (index-map ones fold the sample, negate it or give the constant).
 #include "sep_extend.hpp"
 #include "sep_kernels_impl.hpp"
 
 namespace dipamd {
 
 namespace {
+
+template <typename TB>
+__device__ __forceinline__ TB type_max();
+template <>
+__device__ __forceinline__ float type_max<float>() {
+    return __FLT_MAX__;
+}
+template <>
+__device__ __forceinline__ double type_max<double>() {
+    return __DBL_MAX__;
+}
 
 // One thread per sample of the extended buffer (contiguous, dim `d` of size
 // n + 2 (border + zpad)); the zpad outermost samples of each side are 0.
@@ -37,6 +49,16 @@ __global__ __launch_bounds__(kBlock) void k_expand(const TI *in, TB *out, const 
             v = to_buf<TB>(line[static_cast<int64_t>(i) * st]);
         } else if (i < -border || i > n - 1 + border) {
             v = TB(0);
+        } else if (bc != DIPAMD_BC_ANTISYMMETRIC_REFLECT && bc != DIPAMD_BC_FIRST_ORDER_EXTRAPOLATE &&
+                   bc != DIPAMD_BC_SECOND_ORDER_EXTRAPOLATE && bc != DIPAMD_BC_THIRD_ORDER_EXTRAPOLATE) {
+            // an index-map condition: the folded sample, its negation, or the constant
+            const int f = bc_fold_packed(i, n, bc);
+            if (f < 0) {
+                v = bc == DIPAMD_BC_ADD_MAX_VALUE ? type_max<TB>() : (bc == DIPAMD_BC_ADD_MIN_VALUE ? -type_max<TB>() : TB(0));
+            } else {
+                const TB s = to_buf<TB>(line[static_cast<int64_t>(f >> 1) * st]);
+                v = (f & 1) ? -s : s;
+            }
         } else if (i < 0) {
             v = extension_value<TI, TB>(line, st, n, 0, -i, border, bc);
         } else {

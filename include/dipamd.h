@@ -213,6 +213,30 @@ int dipamd_separable_convolution(const dipamd_image *in, const dipamd_image *out
                                  int32_t nfilters, const int32_t *bc, int32_t nbc, const uint8_t *process,
                                  uint32_t flags, void *stream);
 
+/* The line buffers of the separable framework for user line filters -- the plug-in
+ * operator API of dip::Framework::Separable (include/diplib/framework.h:589-669,
+ * 760-770; framework_separable.cpp:146-331), whose device form is the template
+ * dip::Framework::Separable in include/diplib_amd_framework.hpp.
+ *
+ * A line buffer holds every line of an image along `dim` interleaved: sample i of
+ * line l at element i * nlines + l (lines numbered over the other dims, dim 0
+ * fastest; nlines from dipamd_separable_lines_count), so one thread per line reads
+ * and writes whole cache lines across a wave.  buffer_dt is DIPAMD_DT_SFLOAT or
+ * DIPAMD_DT_DFLOAT.
+ *
+ * dipamd_separable_lines_gather replaces the framework's input copy (CopyBuffer with the
+ * cast to the buffer type, copy_buffer.cpp:284-350) and boundary extension
+ * (ExpandBuffer, copy_buffer.cpp:751-813, every condition): `buffer` receives
+ * nlines * (sizes[dim] + 2 border) samples, sample i of a line being its extended
+ * sample i - border.  dipamd_separable_lines_scatter replaces the output copy
+ * (CopyBuffer with clamp_cast to the image's type): `buffer` holds nlines * sizes[dim]
+ * filtered samples, written to `out` along `dim`. */
+int64_t dipamd_separable_lines_count(const dipamd_image *im, int32_t dim);
+int dipamd_separable_lines_gather(const dipamd_image *in, int32_t dim, int32_t border, int32_t bc, int32_t buffer_dt,
+                                  void *buffer, void *stream);
+int dipamd_separable_lines_scatter(const void *buffer, int32_t buffer_dt, const dipamd_image *out, int32_t dim,
+                                   void *stream);
+
 /* Replaces dip::MakeHalfGaussian (include/diplib/generation.h:638-653;
  * src/linear/gauss.cpp:151-165): writes r+1 host doubles, x = 0 LAST, returns the
  * count (or a negative error code). */

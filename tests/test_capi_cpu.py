@@ -112,6 +112,39 @@ def test_argument_errors_without_device():
     assert cf.isComplex and cf.filter == [1.0, 2.0, -0.0, -0.5]
 
 
+def test_line_buffer_argument_errors_without_device():
+    """The plug-in framework's line-buffer entry points validate before any HIP call."""
+    L = _lib.lib()
+    L.dipamd_separable_lines_count.restype = ctypes.c_int64
+    good = _img(ctypes.c_void_p(0x1000), _lib.DT_SFLOAT, [8, 6, 5])
+    assert L.dipamd_separable_lines_count(ctypes.byref(good), 1) == 40
+    assert L.dipamd_separable_lines_count(ctypes.byref(good), 3) == _lib.E_PARAMETER
+    buf = ctypes.c_void_p(0x2000)
+    assert L.dipamd_separable_lines_gather(ctypes.byref(good), 0, 2, 0, _lib.DT_UINT8, buf, None) == _lib.E_UNSUPPORTED
+    assert L.dipamd_separable_lines_gather(ctypes.byref(good), 0, -1, 0, _lib.DT_SFLOAT, buf, None) == _lib.E_PARAMETER
+    assert L.dipamd_separable_lines_gather(ctypes.byref(good), 0, 2, 99, _lib.DT_SFLOAT, buf, None) == _lib.E_PARAMETER
+    assert L.dipamd_separable_lines_scatter(None, _lib.DT_SFLOAT, ctypes.byref(good), 0, None) == _lib.E_PARAMETER
+    assert L.dipamd_separable_lines_scatter(buf, _lib.DT_SFLOAT, ctypes.byref(good), 5, None) == _lib.E_PARAMETER
+
+
+def test_framework_header_compiles(tmp_path):
+    """include/diplib_amd_framework.hpp compiles for gfx950 with a user line filter (no GPU needed)."""
+    import subprocess
+    src = tmp_path / "t.hip"
+    src.write_text('#include "diplib_amd_framework.hpp"\n'
+                   'struct Copy { __device__ void Filter(dip::Framework::SeparableLineFilterParameters const &p) '
+                   'const { float const *i = static_cast<float const *>(p.inBuffer.buffer); '
+                   'float *o = static_cast<float *>(p.outBuffer.buffer); '
+                   'for (dip::uint k = 0; k < p.inBuffer.length; ++k) o[k * p.outBuffer.stride] = '
+                   'i[k * p.inBuffer.stride]; } };\n'
+                   'void f(dip::Image const &a, dip::Image &b) { dip::Framework::Separable(a, b, dip::DT_SFLOAT, '
+                   'dip::DT_SFLOAT, {}, {1}, {"mirror"}, Copy{}); }\n')
+    p = subprocess.run(["/opt/rocm/bin/hipcc", "-std=c++17", "--offload-arch=gfx950", "-c", "-I",
+                        os.path.join(ROOT, "include"), str(src), "-o", str(tmp_path / "t.o")],
+                       capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr
+
+
 def test_best_method_resolution():
     from diplib_amd import best_gauss_method
     assert best_gauss_method([2.0], [0]) == "fir"
