@@ -7,11 +7,16 @@
  *   dip::SeparableConvolution  (include/diplib/linear.h:114-130)
  *   dip::MakeHalfGaussian      (include/diplib/generation.h:638-653)
  * over the subset of dip::Image the path uses (include/diplib/library/image.h):
- * sizes, strides in samples (dim 0 fastest), data type, a shared data segment,
- * Forge / ReForge / Strip / Protect, and the external-data constructor
- * (image.h:458-491).  Pixel data live in DEVICE memory; Forge allocates with
- * hipMalloc.  Calls run asynchronously on the stream set with dipamd::SetStream
- * (the null stream by default), through the C ABI of include/dipamd.h.
+ * sizes, strides in samples (dim 0 fastest), tensor images (elements stored as planes;
+ * filters treat each element as a scalar image, as the reference's AsScalarImage),
+ * data type, a shared data segment, Forge / ReForge / Strip / Protect, and the
+ * external-data constructor (image.h:458-491).  Pixel data live in DEVICE memory by
+ * default (Forge allocates with hipMalloc); an image in HOST memory (MemoryLocation::
+ * HOST: pinned by Forge, or the caller's own buffer) is copied to the device, filtered
+ * there and its result copied back, so host callers need no other change.  Device
+ * calls run asynchronously on the stream set with dipamd::SetStream (the null stream
+ * by default), through the C ABI of include/dipamd.h; calls on host images return when
+ * the result is in place.
  *
  * Output rules follow Framework::Separable (framework_separable.cpp:126): `out` is
  * reforged to the input's sizes and SuggestFlex(in) type unless it is protected,
@@ -138,22 +143,36 @@ namespace Option {
 enum class AcceptDataTypeChange { DONT_ALLOW, DO_ALLOW };
 }
 
-/* The part of dip::Image the Gaussian path uses, with device-resident pixels. */
+/* Where an image's pixels live: device memory (the default, what the kernels read) or
+ * host memory (the reference's ordinary images; every function copies such an image
+ * to the device, runs there and copies the result back). */
+enum class MemoryLocation { DEVICE, HOST };
+
+/* The part of dip::Image the Gaussian path uses: sizes, strides in samples (dim 0
+ * fastest), a tensor of one or more elements per pixel, data type, data segment and
+ * its memory location.  Forge lays a tensor image out as planes (tensor stride = the
+ * number of pixels), so every element is a contiguous image. */
 class Image {
   public:
     Image() = default;
-    /* Forged image of the given sizes and type (normal strides), on the device. */
-    explicit Image(UnsignedArray sizes, uint tensorElems = 1, dip::DataType dt = DT_SFLOAT)
-        : dataType_(dt), sizes_(std::move(sizes)) {
-        if (tensorElems != 1) throw ParameterError("Tensor images are not supported by this library");
+    /* Forged image of the given sizes, tensor elements and type (normal strides). */
+    explicit Image(UnsignedArray sizes, uint tensorElems = 1, dip::DataType dt = DT_SFLOAT,
+                   MemoryLocation where = MemoryLocation::DEVICE)
+        : dataType_(dt), sizes_(std::move(sizes)), tensorElements_(tensorElems), onHost_(where == MemoryLocation::HOST) {
+        if (tensorElems < 1) throw ParameterError("Tensor must have at least one element");
         Forge();
     }
-    /* Image around existing device data (image.h:458-491); no ownership if `data` says so. */
-    Image(DataSegment const &data, void *origin, dip::DataType dt, UnsignedArray sizes, IntegerArray strides = {})
-        : dataType_(dt), sizes_(std::move(sizes)), strides_(std::move(strides)), data_(data), origin_(origin) {
+    /* Image around existing data (image.h:458-491); no ownership if `data` says so.  A
+     * tensor image gives its element count and tensor stride (0: one element). */
+    Image(DataSegment const &data, void *origin, dip::DataType dt, UnsignedArray sizes, IntegerArray strides = {},
+          uint tensorElems = 1, sint tensorStride = 0, MemoryLocation where = MemoryLocation::DEVICE)
+        : dataType_(dt), sizes_(std::move(sizes)), strides_(std::move(strides)), tensorElements_(tensorElems),
+          tensorStride_(tensorStride), onHost_(where == MemoryLocation::HOST), data_(data), origin_(origin) {
         if (!data || !origin) throw ParameterError("Bad data pointer");
+        if (tensorElems < 1) throw ParameterError("Tensor must have at least one element");
         if (strides_.empty()) SetNormalStrides();
         if (strides_.size() != sizes_.size()) throw ParameterError("Strides array size does not match image dimensionality");
+        if (tensorElems == 1) tensorStride_ = 1;
     }
 
     bool IsForged() const { return origin_ != nullptr; }
@@ -162,13 +181,19 @@ class Image {
     uint Size(uint d) const { return sizes_.at(d); }
     IntegerArray const &Strides() const { return strides_; }
     sint Stride(uint d) const { return strides_.at(d); }
+    uint TensorElements() const { return tensorElements_; }
+    sint TensorStride() const { return tensorStride_; }
+    bool IsScalar() const { return tensorElements_ == 1; }
+    bool IsOnHost() const { return onHost_; }
     dip::DataType DataType() const { return dataType_; }
     void *Origin() const { return origin_; }
+    DataSegment const &Data() const { return data_; }
     uint NumberOfPixels() const {
         uint n = 1;
         for (uint s : sizes_) n *= s;
         return n;
     }
+    uint NumberOfSamples() const { return NumberOfPixels() * tensorElements_; }
     bool IsProtected() const { return protect_; }
     bool Protect(bool set = true) {
         bool old = protect_;
@@ -183,29 +208,56 @@ class Image {
         if (IsForged()) throw ParameterError("Image is forged");
         dataType_ = dt;
     }
-    /* Allocates device memory for the current sizes and type (normal strides). */
+    void SetTensorElements(uint n) {
+        if (IsForged()) throw ParameterError("Image is forged");
+        if (n < 1) throw ParameterError("Tensor must have at least one element");
+        tensorElements_ = n;
+    }
+    /* Memory of the next Forge (an unforged image only). */
+    void SetMemoryLocation(MemoryLocation where) {
+        if (IsForged()) throw ParameterError("Image is forged");
+        onHost_ = where == MemoryLocation::HOST;
+    }
+    /* Tensor element k as a scalar image sharing the data. */
+    Image TensorElement(uint k) const {
+        if (!IsForged()) throw ParameterError("Image is not forged");
+        if (k >= tensorElements_) throw ParameterError("Index out of range");
+        void *o = static_cast<char *>(origin_) + static_cast<sint>(k) * tensorStride_ * static_cast<sint>(dataType_.SizeOf());
+        Image e(data_, o, dataType_, sizes_, strides_, 1, 0, onHost_ ? MemoryLocation::HOST : MemoryLocation::DEVICE);
+        e.protect_ = protect_;
+        return e;
+    }
+    /* Allocates memory for the current sizes, tensor and type (normal strides; a
+     * tensor as planes): device memory (hipMalloc) or pinned host memory. */
     void Forge() {
         if (IsForged()) return;
         SetNormalStrides();
-        std::size_t bytes = NumberOfPixels() * dataType_.SizeOf();
+        tensorStride_ = tensorElements_ > 1 ? static_cast<sint>(NumberOfPixels()) : 1;
+        std::size_t bytes = NumberOfSamples() * dataType_.SizeOf();
         void *p = nullptr;
-        if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) throw RunTimeError("hipMalloc failed");
-        data_ = DataSegment(p, [](void *q) { (void)hipFree(q); });
+        if (onHost_) {
+            if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess)
+                throw RunTimeError("hipHostMalloc failed");
+            data_ = DataSegment(p, [](void *q) { (void)hipHostFree(q); });
+        } else {
+            if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) throw RunTimeError("hipMalloc failed");
+            data_ = DataSegment(p, [](void *q) { (void)hipFree(q); });
+        }
         origin_ = p;
     }
-    /* image.h ReForge: keep the data when sizes and type fit, else reallocate (or
-     * throw for a protected image; a protected image keeps its type). */
+    /* image.h ReForge: keep the data when sizes, tensor and type fit, else reallocate
+     * (or throw for a protected image; a protected image keeps its type). */
     void ReForge(UnsignedArray const &sizes, uint tensorElems, dip::DataType dt,
                  Option::AcceptDataTypeChange acceptDataTypeChange = Option::AcceptDataTypeChange::DONT_ALLOW) {
-        if (tensorElems != 1) throw ParameterError("Tensor images are not supported by this library");
         if (IsForged()) {
-            if (sizes_ == sizes && (dataType_ == dt ||
-                                    (protect_ && acceptDataTypeChange == Option::AcceptDataTypeChange::DO_ALLOW)))
+            if (sizes_ == sizes && tensorElements_ == tensorElems &&
+                (dataType_ == dt || (protect_ && acceptDataTypeChange == Option::AcceptDataTypeChange::DO_ALLOW)))
                 return;
             if (protect_) throw ParameterError("Image is protected");
             Strip();
         }
         sizes_ = sizes;
+        tensorElements_ = tensorElems;
         dataType_ = dt;
         Forge();
     }
@@ -215,18 +267,31 @@ class Image {
         origin_ = nullptr;
         strides_.clear();
     }
-    /* Host <-> device copies (normal strides only). */
+    /* Normal strides and, for a tensor, planes: the samples form one contiguous block. */
+    bool IsContiguousBlock() const {
+        sint s = 1;
+        for (uint d = 0; d < sizes_.size(); ++d) {
+            if (sizes_[d] > 1 && strides_[d] != s) return false;
+            s *= static_cast<sint>(sizes_[d]);
+        }
+        return tensorElements_ == 1 || tensorStride_ == s;
+    }
+    /* Copies between this image and a host buffer of NumberOfSamples() samples (the
+     * planes of a tensor one after the other); the image's samples must form one block. */
     void CopyFromHost(void const *host) {
         if (!IsForged()) throw ParameterError("Image is not forged");
-        if (hipMemcpy(origin_, host, NumberOfPixels() * dataType_.SizeOf(), hipMemcpyHostToDevice) != hipSuccess)
+        if (!IsContiguousBlock()) throw ParameterError("Copy needs normal strides");
+        if (hipMemcpy(origin_, host, NumberOfSamples() * dataType_.SizeOf(), hipMemcpyDefault) != hipSuccess)
             throw RunTimeError("hipMemcpy failed");
     }
     void CopyToHost(void *host) const {
         if (!IsForged()) throw ParameterError("Image is not forged");
-        if (hipMemcpy(host, origin_, NumberOfPixels() * dataType_.SizeOf(), hipMemcpyDeviceToHost) != hipSuccess)
+        if (!IsContiguousBlock()) throw ParameterError("Copy needs normal strides");
+        if (hipMemcpy(host, origin_, NumberOfSamples() * dataType_.SizeOf(), hipMemcpyDefault) != hipSuccess)
             throw RunTimeError("hipMemcpy failed");
     }
-    /* The C ABI view of this image. */
+    /* The C ABI view of this image; a tensor image as a scalar image with the tensor as
+     * one more dim (the reference's AsScalarImage, framework_separable.cpp:105-116). */
     dipamd_image View() const {
         dipamd_image v{};
         v.origin = origin_;
@@ -235,6 +300,12 @@ class Image {
         for (uint d = 0; d < sizes_.size() && d < DIPAMD_MAX_DIMS; ++d) {
             v.sizes[d] = static_cast<int64_t>(sizes_[d]);
             v.strides[d] = static_cast<int64_t>(strides_[d]);
+        }
+        if (tensorElements_ > 1) {
+            if (sizes_.size() >= DIPAMD_MAX_DIMS) throw ParameterError("Dimensionality not supported");
+            v.sizes[v.ndim] = static_cast<int64_t>(tensorElements_);
+            v.strides[v.ndim] = static_cast<int64_t>(tensorStride_);
+            ++v.ndim;
         }
         return v;
     }
@@ -252,6 +323,9 @@ class Image {
     dip::DataType dataType_ = DT_SFLOAT;
     UnsignedArray sizes_;
     IntegerArray strides_;
+    uint tensorElements_ = 1;
+    sint tensorStride_ = 1;
+    bool onHost_ = false;
     DataSegment data_;
     void *origin_ = nullptr;
     bool protect_ = false;
@@ -297,29 +371,104 @@ inline int32_t SymmetryCode(String const &s) {
     if (s == "d-conj") return DIPAMD_SYM_D_CONJ;
     throw ParameterError("Symmetry string not recognized: " + s);
 }
-/* ReForge `out` for a result of type `dt` from `in` (framework_separable.cpp:72-76, 126). */
+/* ReForge `out` for a result of type `dt` from `in` (framework_separable.cpp:72-76, 126),
+ * with the input's tensor. */
 inline void PrepareOutput(Image const &in, Image &out, DataType dt) {
     if (!in.IsForged()) throw ParameterError("Image is not forged");
     const bool same = out.IsForged() && out.Origin() == in.Origin() && out.Sizes() == in.Sizes() &&
-                      out.Strides() == in.Strides() && out.DataType() == in.DataType();
+                      out.Strides() == in.Strides() && out.DataType() == in.DataType() &&
+                      out.TensorElements() == in.TensorElements() && out.TensorStride() == in.TensorStride();
     if (same) {
         if (out.DataType() != dt && !out.IsProtected()) {
             // in-place but the output type changes: a new data segment for out
-            Image fresh(in.Sizes(), 1, dt);
+            Image fresh(in.Sizes(), in.TensorElements(), dt);
             out = fresh;
         }
         return;
     }
-    out.ReForge(in.Sizes(), 1, dt, Option::AcceptDataTypeChange::DO_ALLOW);
+    out.ReForge(in.Sizes(), in.TensorElements(), dt, Option::AcceptDataTypeChange::DO_ALLOW);
+}
+
+/* Host images: the call `run(deviceIn, deviceOut)` works on a device copy of a host
+ * input (normal strides needed) and, when `out` is in host memory (or unforged with a
+ * host input), into a device image whose result is copied back into `out`.  Returns
+ * false (and runs nothing) when every image is on the device. */
+template <typename F>
+bool Staged(Image const &in, Image &out, F &&run) {
+    const bool outHost = out.IsForged() ? out.IsOnHost() : in.IsOnHost();
+    if (!in.IsOnHost() && !outHost) return false;
+    if (!in.IsForged()) throw ParameterError("Image is not forged");
+    Image din = in;
+    if (in.IsOnHost()) {
+        if (!in.IsContiguousBlock()) throw ParameterError("Host images need normal strides");
+        din = Image(in.Sizes(), in.TensorElements(), in.DataType());
+        if (hipMemcpyAsync(din.Origin(), in.Origin(), in.NumberOfSamples() * in.DataType().SizeOf(), hipMemcpyDefault,
+                           dipamd::CurrentStream()) != hipSuccess)
+            throw RunTimeError("hipMemcpy failed");
+    }
+    if (!outHost) {
+        run(din, out);
+        if (hipStreamSynchronize(dipamd::CurrentStream()) != hipSuccess) throw RunTimeError("stream failed");
+        return true;
+    }
+    Image dout;
+    if (out.IsForged()) {
+        dout = Image(out.Sizes(), out.TensorElements(), out.DataType());
+        dout.Protect(out.IsProtected());
+    }
+    run(din, dout);
+    if (hipStreamSynchronize(dipamd::CurrentStream()) != hipSuccess) throw RunTimeError("stream failed");
+    if (!out.IsForged()) out.SetMemoryLocation(MemoryLocation::HOST);
+    out.ReForge(dout.Sizes(), dout.TensorElements(), dout.DataType(), Option::AcceptDataTypeChange::DO_ALLOW);
+    if (!out.IsContiguousBlock()) throw ParameterError("Host images need normal strides");
+    if (hipMemcpy(out.Origin(), dout.Origin(), dout.NumberOfSamples() * dout.DataType().SizeOf(), hipMemcpyDefault) !=
+        hipSuccess)
+        throw RunTimeError("hipMemcpy failed");
+    return true;
+}
+
+/* A per-dim parameter for the C ABI view of a tensor image, whose last dim is the
+ * tensor: broadcast to the image's dims, then `fill` for the tensor dim (which no
+ * filter processes).  Scalar images keep the parameter as given. */
+template <typename T>
+std::vector<T> TensorDim(std::vector<T> v, Image const &in, T dflt, T fill) {
+    if (in.IsScalar()) return v;
+    const uint nd = in.Dimensionality();
+    if (v.empty()) {
+        v.assign(nd, dflt);
+    } else if (v.size() == 1) {
+        v.assign(nd, v[0]);
+    } else if (v.size() != nd) {
+        throw ParameterError("Array parameter has the wrong number of elements");
+    }
+    v.push_back(fill);
+    return v;
+}
+inline StringArray TensorDimBC(StringArray bc, Image const &in) {
+    if (in.IsScalar() || bc.size() <= 1) return bc;  // broadcast by the C ABI
+    return TensorDim<String>(std::move(bc), in, "", "mirror");
 }
 inline void PrepareOutput(Image const &in, Image &out) { PrepareOutput(in, out, DataType::SuggestFlex(in.DataType())); }
 }  // namespace detail
 
 /* dip::SeparableConvolution (linear.h:114-130); a complex filter (filter holds re, im
  * pairs) makes the result SuggestComplex(in), as convolution.cpp:331-346. */
-inline void SeparableConvolution(Image const &in, Image &out, OneDimensionalFilterArray const &filterArray,
-                                 StringArray const &boundaryCondition = {}, BooleanArray process = {}) {
+inline void SeparableConvolution(Image const &in, Image &out, OneDimensionalFilterArray const &c_filterArray,
+                                 StringArray const &c_boundaryCondition = {}, BooleanArray process = {}) {
+    if (detail::Staged(in, out, [&](Image const &i, Image &o) {
+            SeparableConvolution(i, o, c_filterArray, c_boundaryCondition, process);
+        }))
+        return;
     if (!in.IsForged()) throw ParameterError("Image is not forged");
+    // a tensor image: its tensor dim gets the identity filter and is not processed
+    OneDimensionalFilterArray filterArray =
+        detail::TensorDim<OneDimensionalFilter>(c_filterArray, in, OneDimensionalFilter{}, OneDimensionalFilter{{1.0}});
+    const StringArray boundaryCondition = detail::TensorDimBC(c_boundaryCondition, in);
+    {
+        std::vector<char> p(process.begin(), process.end());
+        p = detail::TensorDim<char>(p, in, 1, 0);
+        process.assign(p.begin(), p.end());
+    }
     Image src = in;  // keeps the input's data alive if `out` is `in`
     bool complexFilter = false;
     for (auto const &o : filterArray) complexFilter = complexFilter || o.isComplex;
@@ -349,11 +498,17 @@ inline Image SeparableConvolution(Image const &in, OneDimensionalFilterArray con
 /* dip::GaussFIR (linear.h:300-318). */
 inline void GaussFIR(Image const &in, Image &out, FloatArray sigmas = {1.0}, UnsignedArray derivativeOrder = {0},
                      StringArray const &boundaryCondition = {}, dfloat truncation = 3) {
+    if (detail::Staged(in, out, [&](Image const &i, Image &o) {
+            GaussFIR(i, o, sigmas, derivativeOrder, boundaryCondition, truncation);
+        }))
+        return;
     if (!in.IsForged()) throw ParameterError("Image is not forged");
+    sigmas = detail::TensorDim<dfloat>(std::move(sigmas), in, 1.0, 0.0);
+    derivativeOrder = detail::TensorDim<uint>(std::move(derivativeOrder), in, 0, 0);
     Image src = in;
     detail::PrepareOutput(in, out);
     std::vector<int32_t> ord(derivativeOrder.begin(), derivativeOrder.end());
-    std::vector<int32_t> bc = detail::BoundaryCodes(boundaryCondition);
+    std::vector<int32_t> bc = detail::BoundaryCodes(detail::TensorDimBC(boundaryCondition, in));
     dipamd_image a = src.View(), b = out.View();
     detail::Check(dipamd_gauss_fir(&a, &b, sigmas.data(), static_cast<int32_t>(sigmas.size()), ord.data(),
                                    static_cast<int32_t>(ord.size()), bc.data(), static_cast<int32_t>(bc.size()),
@@ -370,7 +525,14 @@ inline Image GaussFIR(Image const &in, FloatArray sigmas = {1.0}, UnsignedArray 
 inline void GaussIIR(Image const &in, Image &out, FloatArray sigmas = {1.0}, UnsignedArray derivativeOrder = {0},
                      StringArray const &boundaryCondition = {}, UnsignedArray filterOrder = {},
                      String const &designMethod = "discrete time fit", dfloat truncation = 3) {
+    if (detail::Staged(in, out, [&](Image const &i, Image &o) {
+            GaussIIR(i, o, sigmas, derivativeOrder, boundaryCondition, filterOrder, designMethod, truncation);
+        }))
+        return;
     if (!in.IsForged()) throw ParameterError("Image is not forged");
+    sigmas = detail::TensorDim<dfloat>(std::move(sigmas), in, 1.0, 0.0);
+    derivativeOrder = detail::TensorDim<uint>(std::move(derivativeOrder), in, 0, 0);
+    if (!filterOrder.empty()) filterOrder = detail::TensorDim<uint>(std::move(filterOrder), in, 3, 3);
     int32_t method;
     if (designMethod == "discrete time fit") {
         method = DIPAMD_IIR_DISCRETE_TIME_FIT;
@@ -383,7 +545,7 @@ inline void GaussIIR(Image const &in, Image &out, FloatArray sigmas = {1.0}, Uns
     detail::PrepareOutput(in, out);
     std::vector<int32_t> ord(derivativeOrder.begin(), derivativeOrder.end());
     std::vector<int32_t> fo(filterOrder.begin(), filterOrder.end());
-    std::vector<int32_t> bc = detail::BoundaryCodes(boundaryCondition);
+    std::vector<int32_t> bc = detail::BoundaryCodes(detail::TensorDimBC(boundaryCondition, in));
     dipamd_image a = src.View(), b = out.View();
     detail::Check(dipamd_gauss_iir(&a, &b, sigmas.data(), static_cast<int32_t>(sigmas.size()), ord.data(),
                                    static_cast<int32_t>(ord.size()), bc.data(), static_cast<int32_t>(bc.size()),
@@ -403,13 +565,19 @@ inline Image GaussIIR(Image const &in, FloatArray sigmas = {1.0}, UnsignedArray 
 inline void GaussFT(Image const &in, Image &out, FloatArray sigmas = {1.0}, UnsignedArray derivativeOrder = {0},
                     dfloat truncation = 3, String const &inRepresentation = "spatial",
                     String const &outRepresentation = "spatial", StringArray const &boundaryCondition = {}) {
+    if (detail::Staged(in, out, [&](Image const &i, Image &o) {
+            GaussFT(i, o, sigmas, derivativeOrder, truncation, inRepresentation, outRepresentation, boundaryCondition);
+        }))
+        return;
     if (!in.IsForged()) throw ParameterError("Image is not forged");
+    sigmas = detail::TensorDim<dfloat>(std::move(sigmas), in, 1.0, 0.0);
+    derivativeOrder = detail::TensorDim<uint>(std::move(derivativeOrder), in, 0, 0);
     if (inRepresentation != "spatial" || outRepresentation != "spatial")
         throw ParameterError("GaussFT in or out of the frequency domain is not provided by this library");
     Image src = in;
     detail::PrepareOutput(in, out);
     std::vector<int32_t> ord(derivativeOrder.begin(), derivativeOrder.end());
-    std::vector<int32_t> bc = detail::BoundaryCodes(boundaryCondition);
+    std::vector<int32_t> bc = detail::BoundaryCodes(detail::TensorDimBC(boundaryCondition, in));
     dipamd_image a = src.View(), b = out.View();
     detail::Check(dipamd_gauss_ft(&a, &b, sigmas.data(), static_cast<int32_t>(sigmas.size()), ord.data(),
                                   static_cast<int32_t>(ord.size()), bc.data(), static_cast<int32_t>(bc.size()),
@@ -464,10 +632,11 @@ inline Image Gauss(Image const &in, FloatArray sigmas = {1.0}, UnsignedArray der
 }
 
 /* The derivative family (derivative.cpp:120-521; linear.h:666, 697, 862, 904) with the
- * Gaussian FIR, IIR or FT method ("best" resolves to one of them).  This shim has no
- * tensor images: Gradient and
- * Hessian return their tensor elements as an array of images, in dip's element order
- * (Hessian: the diagonal first, then the upper triangle column by column). */
+ * Gaussian FIR, IIR or FT method ("best" resolves to one of them), on scalar images.
+ * Gradient, Hessian and StructureTensor give tensor images in dip's element order
+ * (Hessian and the structure tensor: the diagonal first, then the upper triangle
+ * column by column); overloads taking an ImageArray give the elements as separate
+ * images. */
 using ImageArray = std::vector<Image>;
 namespace detail {
 /* The Gaussian method the derivatives run (dip::Derivative passes `method` on to
@@ -511,10 +680,40 @@ inline int DifferentiatedDims(Image const &in, FloatArray const &sigmas, Boolean
     if (n < 1) throw ParameterError("Dimensionality not supported");
     return n;
 }
+inline void RequireScalar(Image const &in) {
+    if (!in.IsForged()) throw ParameterError("Image is not forged");
+    if (!in.IsScalar()) throw ParameterError("Image is not scalar");
+}
+/* The tensor result as one tensor image (each element a plane of `out`). */
+inline void TensorResult(int32_t fn, Image const &in, Image &out, FloatArray const &sigmas, String const &method,
+                         StringArray const &bc, BooleanArray const &process, dfloat truncation) {
+    if (Staged(in, out, [&](Image const &i, Image &o) {
+            TensorResult(fn, i, o, sigmas, method, bc, process, truncation);
+        }))
+        return;
+    RequireScalar(in);
+    const uint32_t mf = GaussMethodFlag(method, sigmas);
+    const int n = dipamd_derivative_outputs(fn, DifferentiatedDims(in, sigmas, process));
+    if (n < 0) Check(n);
+    Image src = in;
+    if (out.IsForged() && out.Origin() == in.Origin()) out = Image();  // the result never overwrites the input
+    out.ReForge(in.Sizes(), static_cast<uint>(n), DataType::SuggestFlex(in.DataType()),
+                Option::AcceptDataTypeChange::DO_ALLOW);
+    std::vector<dipamd_image> views;
+    for (int k = 0; k < n; ++k) views.push_back(out.TensorElement(static_cast<uint>(k)).View());
+    RunDerivative(fn, src, views, sigmas, bc, process, truncation, mf);
+}
 inline void TensorResult(int32_t fn, Image const &in, ImageArray &out, FloatArray const &sigmas,
                          String const &method, StringArray const &bc, BooleanArray const &process,
                          dfloat truncation) {
-    if (!in.IsForged()) throw ParameterError("Image is not forged");
+    RequireScalar(in);
+    if (in.IsOnHost()) {
+        Image t;
+        TensorResult(fn, in, t, sigmas, method, bc, process, truncation);
+        out.clear();
+        for (uint k = 0; k < t.TensorElements(); ++k) out.push_back(t.TensorElement(k));
+        return;
+    }
     const uint32_t mf = GaussMethodFlag(method, sigmas);
     const int n = dipamd_derivative_outputs(fn, DifferentiatedDims(in, sigmas, process));
     if (n < 0) Check(n);
@@ -528,7 +727,11 @@ inline void TensorResult(int32_t fn, Image const &in, ImageArray &out, FloatArra
 }
 inline void ScalarResult(int32_t fn, Image const &in, Image &out, FloatArray const &sigmas, String const &method,
                          StringArray const &bc, BooleanArray const &process, dfloat truncation) {
-    if (!in.IsForged()) throw ParameterError("Image is not forged");
+    if (Staged(in, out, [&](Image const &i, Image &o) {
+            ScalarResult(fn, i, o, sigmas, method, bc, process, truncation);
+        }))
+        return;
+    RequireScalar(in);
     const uint32_t mf = GaussMethodFlag(method, sigmas);
     DifferentiatedDims(in, sigmas, process);
     Image src = in;
@@ -538,9 +741,29 @@ inline void ScalarResult(int32_t fn, Image const &in, Image &out, FloatArray con
 }
 }  // namespace detail
 
+inline void Gradient(Image const &in, Image &out, FloatArray sigmas = {1.0}, String const &method = "best",
+                     StringArray const &boundaryCondition = {}, BooleanArray process = {}, dfloat truncation = 3) {
+    detail::TensorResult(DIPAMD_FN_GRADIENT, in, out, sigmas, method, boundaryCondition, process, truncation);
+}
+inline Image Gradient(Image const &in, FloatArray sigmas = {1.0}, String const &method = "best",
+                      StringArray const &boundaryCondition = {}, BooleanArray process = {}, dfloat truncation = 3) {
+    Image out;
+    Gradient(in, out, std::move(sigmas), method, boundaryCondition, std::move(process), truncation);
+    return out;
+}
 inline void Gradient(Image const &in, ImageArray &out, FloatArray sigmas = {1.0}, String const &method = "best",
                      StringArray const &boundaryCondition = {}, BooleanArray process = {}, dfloat truncation = 3) {
     detail::TensorResult(DIPAMD_FN_GRADIENT, in, out, sigmas, method, boundaryCondition, process, truncation);
+}
+inline void Hessian(Image const &in, Image &out, FloatArray sigmas = {1.0}, String const &method = "best",
+                    StringArray const &boundaryCondition = {}, BooleanArray process = {}, dfloat truncation = 3) {
+    detail::TensorResult(DIPAMD_FN_HESSIAN, in, out, sigmas, method, boundaryCondition, process, truncation);
+}
+inline Image Hessian(Image const &in, FloatArray sigmas = {1.0}, String const &method = "best",
+                     StringArray const &boundaryCondition = {}, BooleanArray process = {}, dfloat truncation = 3) {
+    Image out;
+    Hessian(in, out, std::move(sigmas), method, boundaryCondition, std::move(process), truncation);
+    return out;
 }
 inline void Hessian(Image const &in, ImageArray &out, FloatArray sigmas = {1.0}, String const &method = "best",
                     StringArray const &boundaryCondition = {}, BooleanArray process = {}, dfloat truncation = 3) {
@@ -585,32 +808,46 @@ inline void LaplaceMinusDgg(Image const &in, Image &out, FloatArray sigmas = {1.
     detail::ScalarResult(DIPAMD_FN_LAPLACE_MINUS_DGG, in, out, sigmas, method, boundaryCondition, process,
                          truncation);
 }
-/* dip::StructureTensor (analysis.h:454; structure_tensor.cpp:32-57) without a mask: the
- * symmetric-matrix elements as an array of images. */
-inline void StructureTensor(Image const &in, Image const &mask, ImageArray &out, FloatArray gradientSigmas = {1.0},
+/* dip::StructureTensor (analysis.h:454; structure_tensor.cpp:32-57) without a mask:
+ * the symmetric-matrix elements as a tensor image (or, with an ImageArray, as
+ * separate images). */
+inline void StructureTensor(Image const &in, Image const &mask, Image &out, FloatArray gradientSigmas = {1.0},
                             FloatArray tensorSigmas = {5.0}, String const &method = "best",
                             StringArray const &boundaryCondition = {}, dfloat truncation = 3) {
-    if (!in.IsForged()) throw ParameterError("Image is not forged");
     if (mask.IsForged()) throw ParameterError("StructureTensor with a mask is not provided by this library");
+    if (detail::Staged(in, out, [&](Image const &i, Image &o) {
+            StructureTensor(i, mask, o, gradientSigmas, tensorSigmas, method, boundaryCondition, truncation);
+        }))
+        return;
+    detail::RequireScalar(in);
     const uint32_t tf = detail::GaussMethodFlag(method, tensorSigmas);
     const uint32_t flags = detail::GaussMethodFlag(method, gradientSigmas) |
                            (tf == DIPAMD_GAUSS_IIR ? DIPAMD_TENSOR_IIR : (tf == DIPAMD_GAUSS_FT ? DIPAMD_TENSOR_FT : 0u));
     const int n = dipamd_derivative_outputs(DIPAMD_FN_STRUCTURE_TENSOR,
                                             detail::DifferentiatedDims(in, gradientSigmas, {}));
     if (n < 0) detail::Check(n);
-    out.clear();
+    Image src = in;
+    if (out.IsForged() && out.Origin() == in.Origin()) out = Image();
+    out.ReForge(in.Sizes(), static_cast<uint>(n), DataType::SuggestFlex(in.DataType()),
+                Option::AcceptDataTypeChange::DO_ALLOW);
     std::vector<dipamd_image> views;
-    for (int k = 0; k < n; ++k) {
-        out.emplace_back(in.Sizes(), 1, DataType::SuggestFlex(in.DataType()));
-        views.push_back(out.back().View());
-    }
+    for (int k = 0; k < n; ++k) views.push_back(out.TensorElement(static_cast<uint>(k)).View());
     std::vector<int32_t> bc = detail::BoundaryCodes(boundaryCondition);
-    dipamd_image a = in.View();
+    dipamd_image a = src.View();
     detail::Check(dipamd_structure_tensor(&a, views.data(), n, gradientSigmas.data(),
                                           static_cast<int32_t>(gradientSigmas.size()), tensorSigmas.data(),
                                           static_cast<int32_t>(tensorSigmas.size()), bc.data(),
                                           static_cast<int32_t>(bc.size()), truncation, DIPAMD_EXACT | flags,
                                           dipamd::CurrentStream()));
+}
+inline void StructureTensor(Image const &in, Image const &mask, ImageArray &out, FloatArray gradientSigmas = {1.0},
+                            FloatArray tensorSigmas = {5.0}, String const &method = "best",
+                            StringArray const &boundaryCondition = {}, dfloat truncation = 3) {
+    Image t;
+    StructureTensor(in, mask, t, std::move(gradientSigmas), std::move(tensorSigmas), method, boundaryCondition,
+                    truncation);
+    out.clear();
+    for (uint k = 0; k < t.TensorElements(); ++k) out.push_back(t.TensorElement(k));
 }
 
 /* dip::Derivative (derivative.cpp:88-115), Gaussian methods: dip::Gauss. */

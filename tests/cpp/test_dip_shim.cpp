@@ -129,6 +129,69 @@ int main(int argc, char **argv) {
             dip::Gradient(img, gi, {12.0}, "best", {"mirror"});
             CHECK(gi.size() == 2 && host(gi[0]) == iir, "dip::Gradient best -> the IIR derivatives");
         }
+        // tensor images: every element filtered as a scalar image (AsScalarImage); the
+        // Gradient and Hessian as tensor images; a tensor element is a view
+        {
+            auto host = [&](dip::Image const &im) {
+                std::vector<float> v(im.NumberOfSamples());
+                (void)hipDeviceSynchronize();
+                im.CopyToHost(v.data());
+                return v;
+            };
+            dip::Image t({W, H}, 2, dip::DT_SFLOAT);
+            std::vector<float> planes(2 * W * H);
+            for (size_t i = 0; i < W * H; ++i) {
+                planes[i] = in[i];
+                planes[W * H + i] = in[W * H - 1 - i];
+            }
+            t.CopyFromHost(planes.data());
+            dip::Image tg = dip::Gauss(t, {sigma}, {0}, "FIR", {"mirror"}, 3);
+            std::vector<float> rev(in.rbegin(), in.rend());
+            dip::Image r({W, H}, 1, dip::DT_SFLOAT);
+            r.CopyFromHost(rev.data());
+            std::vector<float> e1 = host(dip::Gauss(r, {sigma}, {0}, "FIR", {"mirror"}, 3));
+            std::vector<float> tgh = host(tg);
+            CHECK(tg.TensorElements() == 2 && std::memcmp(tgh.data(), exp.data(), W * H * 4) == 0 &&
+                      std::memcmp(tgh.data() + W * H, e1.data(), W * H * 4) == 0,
+                  "tensor image: each element filtered as a scalar image");
+            dip::Image tsep = dip::SeparableConvolution(t, fa, {"mirror"});
+            CHECK(host(tsep) == tgh, "tensor image through SeparableConvolution");
+            dip::Image gt = dip::Gradient(img, {sigma}, "FIR", {"mirror"});
+            dip::ImageArray ga;
+            dip::Gradient(img, ga, {sigma}, "FIR", {"mirror"});
+            CHECK(gt.TensorElements() == 2 && host(gt.TensorElement(0)) == host(ga[0]) &&
+                      host(gt.TensorElement(1)) == host(ga[1]),
+                  "dip::Gradient as a tensor image");
+            dip::Image ht;
+            dip::Hessian(img, ht, {sigma}, "FIR", {"mirror"});
+            CHECK(ht.TensorElements() == 3, "dip::Hessian as a tensor image of 3 elements");
+            bool threw_t = false;
+            try {
+                dip::Laplace(t, {sigma});
+            } catch (dip::ParameterError const &) {
+                threw_t = true;
+            }
+            CHECK(threw_t, "the derivative family takes scalar images");
+        }
+        // host images: staged through the device, the result in host memory
+        {
+            dip::Image hin({W, H}, 1, dip::DT_SFLOAT, dip::MemoryLocation::HOST);
+            std::memcpy(hin.Origin(), in.data(), W * H * sizeof(float));
+            dip::Image hout = dip::Gauss(hin, {sigma}, {0}, "FIR", {"mirror"}, 3);
+            CHECK(hout.IsOnHost() && std::memcmp(hout.Origin(), exp.data(), W * H * 4) == 0,
+                  "host image in, host image out, bit-identical");
+            std::vector<float> user(in);  // the caller's own host buffer around an image
+            dip::Image wrap(dip::NonOwnedRefToDataSegment(user.data()), user.data(), dip::DT_SFLOAT, {W, H}, {}, 1, 0,
+                            dip::MemoryLocation::HOST);
+            dip::Gauss(wrap, wrap, {sigma}, {0}, "FIR", {"mirror"}, 3);
+            CHECK(std::memcmp(user.data(), exp.data(), W * H * 4) == 0, "in place on a host buffer");
+            dip::Image dev_from_host({W, H}, 1, dip::DT_SFLOAT);
+            dip::GaussFIR(hin, dev_from_host, {sigma}, {0}, {"mirror"}, 3);
+            std::vector<float> dv(W * H);
+            (void)hipDeviceSynchronize();
+            dev_from_host.CopyToHost(dv.data());
+            CHECK(!dev_from_host.IsOnHost() && dv == exp, "host input into a device output");
+        }
         // protected output of another type keeps its type (clamp_cast)
         dip::Image prot({W, H}, 1, dip::DT_DFLOAT);
         prot.Protect();
