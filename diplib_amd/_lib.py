@@ -120,6 +120,7 @@ EXPORTED_SYMBOLS = [
     "dipamd_autotune",
 ]
 TUNE_QUICK = 256
+FT_IN_FREQUENCY, FT_OUT_FREQUENCY = 256, 512
 
 FN_GRADIENT, FN_GRADIENT_MAGNITUDE, FN_HESSIAN, FN_LAPLACE = 0, 1, 2, 3
 IIR_FORWARD_BACKWARD, IIR_DISCRETE_TIME_FIT = 1, 2

@@ -33,6 +33,22 @@ struct FtGeo {
     int64_t lut_off[kFtMaxT];            // offset of each dim's LUT in the LUT array
 };
 
+// The centred-spectrum path (GaussFT with a frequency-domain input or output): a
+// contiguous complex array of the image's sizes, dim 0 fastest.
+struct FtCGeo {
+    int32_t ndim;
+    int32_t pad_;
+    int64_t size[DIPAMD_MAX_DIMS];
+    int64_t shift[DIPAMD_MAX_DIMS];     // cyclic shift of each dim (k_ft_cshift)
+    int64_t lut_off[DIPAMD_MAX_DIMS];   // LUT of each dim in the LUT array, -1: none
+    int64_t total;
+};
+// out[(i_d + shift_d) mod size_d] = in[i] for every dim (complex, contiguous)
+hipError_t launch_ft_cshift(bool is_double, const void *in, void *out, const FtCGeo &g, hipStream_t st);
+// data *= the product of each dim's LUT entry (in dim order), then times `scale`
+hipError_t launch_ft_cmultiply(bool is_double, void *data, const void *luts, double scale, const FtCGeo &g,
+                               hipStream_t st);
+
 hipError_t launch_ft_expand(int in_dt, bool is_double, const void *in, void *pad, const FtGeo &g, hipStream_t st);
 hipError_t launch_ft_multiply(bool is_double, void *spec, const void *luts, const FtGeo &g, hipStream_t st);
 hipError_t launch_ft_crop(bool is_double, const void *pad, void *out, int out_dt, double scale, const FtGeo &g,

@@ -1,4 +1,4 @@
-// dip::GaussFT on the device (src/linear/gauss.cpp:397-487) for real images in the
+// dip::GaussFT on the device (src/linear/gauss.cpp:397-487).  Real images in the
 // spatial domain: the image is extended by the boundary condition to a 5-smooth size
 // that holds its Gaussian border (ExpandInput, gauss.cpp:380-393; without a condition
 // it is transformed as it is), transformed with rocFFT (real to half-complex), the
@@ -11,6 +11,9 @@
 // double for dfloat, as DIPlib's DFT.  Where that pass's padded length is odd the
 // reference's output is not shift invariant (a defect of its odd-length real DFT path);
 // this implementation keeps the exact convolution there.
+// A frequency-domain input or output (gauss_ft_centred) takes the complex path: the
+// centred DFT of every dim (rocFFT complex-to-complex between cyclic shifts), the
+// product with the centred responses, and the centred inverse where asked.
 #include <rocfft/rocfft.h>
 
 #include <algorithm>
@@ -25,6 +28,7 @@
 
 #include "dipamd_internal.hpp"
 #include "ft_common.hpp"
+#include "complex_kernels.hpp"
 
 namespace dipamd {
 
@@ -45,9 +49,9 @@ int64_t next_smooth5(int64_t n) {
 }
 
 // Frequency response of one dim (GaussFTLineFilter's constructor, gauss.cpp:294-345) in
-// the centred order of DIPlib's DFT, then rotated to FFT order (frequency k at k).
+// the centred order of DIPlib's DFT (frequency 0 at size / 2).
 template <typename TF>
-std::vector<std::complex<TF>> ft_response(int64_t size, double sigma, int order, double truncation) {
+std::vector<std::complex<TF>> ft_response_centred(int64_t size, double sigma, int order, double truncation) {
     using C = std::complex<TF>;
     std::vector<C> lut(static_cast<size_t>(size), C(0));
     const int64_t origin = size / 2;
@@ -74,6 +78,15 @@ std::vector<std::complex<TF>> ft_response(int64_t size, double sigma, int order,
     } else {
         std::fill(lut.begin(), lut.end(), C(1));
     }
+    return lut;
+}
+
+// The same rotated to FFT order (frequency k at k).
+template <typename TF>
+std::vector<std::complex<TF>> ft_response(int64_t size, double sigma, int order, double truncation) {
+    using C = std::complex<TF>;
+    const std::vector<C> lut = ft_response_centred<TF>(size, sigma, order, truncation);
+    const int64_t origin = size / 2;
     std::vector<C> out(lut.size());
     for (int64_t k = 0; k < size; ++k) out[static_cast<size_t>(k)] = lut[static_cast<size_t>((k + origin) % size)];
     return out;
@@ -127,6 +140,31 @@ int run_fft(rocfft_plan plan, void *in, void *out, hipStream_t stream, std::vect
     const rocfft_status s = rocfft_execute(plan, ib, ob, info);
     rocfft_execution_info_destroy(info);
     if (s != rocfft_status_success) return set_error(DIPAMD_E_HIP, "rocFFT execution failed");
+    return DIPAMD_OK;
+}
+
+// Complex-to-complex plans of the centred path (keyed by lengths, precision, direction).
+std::map<std::tuple<int64_t, int64_t, int64_t, int, bool, bool>, rocfft_plan> g_cplans;
+
+int get_cplan(const int64_t *len, int rank, bool is_double, bool inverse, rocfft_plan *plan) {
+    std::call_once(g_fft_once, []() { rocfft_setup(); });
+    const auto key = std::make_tuple(len[0], rank > 1 ? len[1] : 1, rank > 2 ? len[2] : 1, rank, is_double, inverse);
+    std::lock_guard<std::mutex> lk(g_fft_mu);
+    auto it = g_cplans.find(key);
+    if (it != g_cplans.end()) {
+        *plan = it->second;
+        return DIPAMD_OK;
+    }
+    size_t lengths[3] = {1, 1, 1};
+    for (int k = 0; k < rank; ++k) lengths[k] = static_cast<size_t>(len[k]);
+    rocfft_plan p = nullptr;
+    const rocfft_status s = rocfft_plan_create(
+        &p, rocfft_placement_inplace,
+        inverse ? rocfft_transform_type_complex_inverse : rocfft_transform_type_complex_forward,
+        is_double ? rocfft_precision_double : rocfft_precision_single, static_cast<size_t>(rank), lengths, 1, nullptr);
+    if (s != rocfft_status_success) return set_error(DIPAMD_E_HIP, "rocFFT plan creation failed");
+    g_cplans[key] = p;
+    *plan = p;
     return DIPAMD_OK;
 }
 
@@ -331,6 +369,119 @@ int gauss_ft_device(const View &in, const View &out, int ndim, const double *sig
     return finish(DIPAMD_OK);
 }
 
+// GaussFT with a frequency-domain input or output (gauss.cpp:397-487 without
+// ExpandInput): the centred DFT of the input (fourier.cpp:116-150: ifftshift, DFT,
+// fftshift; unscaled) unless it is a spectrum, times each filtered dim's centred
+// response, and the centred inverse (scaled by 1/N) unless a spectrum is asked for.
+int gauss_ft_centred(const View &in, const View &out, int ndim, const double *sig, const int *ord, bool in_freq,
+                     bool out_freq, double truncation, hipStream_t stream) {
+    const bool is_double = in.dt == DIPAMD_DT_DFLOAT || in.dt == DIPAMD_DT_DCOMPLEX || in.dt == DIPAMD_DT_SINT32 ||
+                           in.dt == DIPAMD_DT_UINT32;
+    if (out.dt != DIPAMD_DT_SCOMPLEX && out.dt != DIPAMD_DT_DCOMPLEX)
+        return set_error(DIPAMD_E_UNSUPPORTED, "a frequency-domain GaussFT gives a complex image");
+    if (in_freq && in.dt != DIPAMD_DT_SCOMPLEX && in.dt != DIPAMD_DT_DCOMPLEX)
+        return set_error(DIPAMD_E_PARAMETER, "Data type not supported");
+    if (ndim > 3) return set_error(DIPAMD_E_UNSUPPORTED, "the FT method transforms at most 3 dimensions at once");
+    const int64_t n = npx_of(in, ndim);
+    const size_t ce = is_double ? 16 : 8;
+    std::vector<void *> scratch;
+    auto finish = [&](int st) {
+        for (void *p : scratch) scratch_free(p, stream);
+        return st;
+    };
+    auto alloc = [&](size_t bytes, void **p) -> int {
+        if (scratch_alloc(p, bytes ? bytes : 1, stream) != hipSuccess)
+            return set_error(DIPAMD_E_HIP, "scratch allocation failed");
+        scratch.push_back(*p);
+        return DIPAMD_OK;
+    };
+    void *a = nullptr, *b = nullptr, *luts = nullptr;
+    int st;
+    if ((st = alloc(static_cast<size_t>(n) * ce, &a)) != DIPAMD_OK) return finish(st);
+    if ((st = alloc(static_cast<size_t>(n) * ce, &b)) != DIPAMD_OK) return finish(st);
+    CastGeo cg;
+    std::memset(&cg, 0, sizeof(cg));
+    cg.ndim = ndim;
+    cg.total = n;
+    int64_t s = 1;
+    for (int d = 0; d < ndim; ++d) {
+        cg.size[d] = in.sizes[d];
+        cg.sin[d] = in.strides[d];
+        cg.sout[d] = s;
+        s *= in.sizes[d];
+    }
+    hipError_t e = (in.dt == DIPAMD_DT_SCOMPLEX || in.dt == DIPAMD_DT_DCOMPLEX)
+                       ? launch_complex_copy(in.dt == DIPAMD_DT_DCOMPLEX, is_double, in.p, a, &cg, stream)
+                       : launch_to_complex(in.dt, is_double, in.p, a, &cg, stream);
+    if (e != hipSuccess) return finish(set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e)));
+    FtCGeo g;
+    std::memset(&g, 0, sizeof(g));
+    g.ndim = ndim;
+    g.total = n;
+    int64_t len[3] = {1, 1, 1};
+    for (int d = 0; d < ndim; ++d) {
+        g.size[d] = in.sizes[d];
+        g.lut_off[d] = -1;
+        len[d] = in.sizes[d];
+    }
+    auto shift = [&](const void *src, void *dst, bool to_corner) -> int {
+        for (int d = 0; d < ndim; ++d) g.shift[d] = to_corner ? (g.size[d] - g.size[d] / 2) % g.size[d] : g.size[d] / 2;
+        const hipError_t err = launch_ft_cshift(is_double, src, dst, g, stream);
+        return err == hipSuccess ? DIPAMD_OK : set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(err));
+    };
+    auto transform = [&](bool inverse) -> int {
+        // a -> b (origin to the corner), DFT in place, b -> a (origin to the middle)
+        int r;
+        if ((r = shift(a, b, true)) != DIPAMD_OK) return r;
+        rocfft_plan plan;
+        if ((r = get_cplan(len, ndim, is_double, inverse, &plan)) != DIPAMD_OK) return r;
+        if ((r = run_fft(plan, b, b, stream, scratch)) != DIPAMD_OK) return r;
+        return shift(b, a, false);
+    };
+    if (!in_freq && (st = transform(false)) != DIPAMD_OK) return finish(st);
+    // the responses of the filtered dims, centred, in the transform's precision
+    std::vector<double> lutd;
+    std::vector<float> lutf;
+    int64_t lut_len = 0;
+    for (int d = 0; d < ndim; ++d) {
+        if (!(sig[d] > 0.0 || ord[d] > 0)) continue;
+        g.lut_off[d] = lut_len;
+        if (is_double) {
+            for (const auto &c : ft_response_centred<double>(in.sizes[d], sig[d], ord[d], truncation)) {
+                lutd.push_back(c.real());
+                lutd.push_back(c.imag());
+            }
+        } else {
+            for (const auto &c : ft_response_centred<float>(in.sizes[d], sig[d], ord[d], truncation)) {
+                lutf.push_back(c.real());
+                lutf.push_back(c.imag());
+            }
+        }
+        lut_len += in.sizes[d];
+    }
+    if (lut_len > 0) {
+        if ((st = alloc(static_cast<size_t>(lut_len) * ce, &luts)) != DIPAMD_OK) return finish(st);
+        if (hipMemcpyAsync(luts, is_double ? static_cast<const void *>(lutd.data()) : static_cast<const void *>(lutf.data()),
+                           static_cast<size_t>(lut_len) * ce, hipMemcpyHostToDevice, stream) != hipSuccess)
+            return finish(set_error(DIPAMD_E_HIP, "copy of the frequency response failed"));
+        e = launch_ft_cmultiply(is_double, a, luts, 1.0, g, stream);
+        if (e != hipSuccess) return finish(set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e)));
+    }
+    if (!out_freq) {
+        if ((st = transform(true)) != DIPAMD_OK) return finish(st);
+        for (int d = 0; d < ndim; ++d) g.lut_off[d] = -1;
+        e = launch_ft_cmultiply(is_double, a, nullptr, 1.0 / static_cast<double>(n), g, stream);
+        if (e != hipSuccess) return finish(set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e)));
+    }
+    for (int d = 0; d < ndim; ++d) {
+        cg.sin[d] = cg.sout[d];
+        cg.sout[d] = out.strides[d];
+    }
+    e = launch_complex_copy(is_double, out.dt == DIPAMD_DT_DCOMPLEX, a, out.p, &cg, stream);
+    if (e != hipSuccess) return finish(set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e)));
+    return finish(DIPAMD_OK);
+}
+
 }  // namespace dipamd
 
 using namespace dipamd;
@@ -338,10 +489,10 @@ using namespace dipamd;
 extern "C" int dipamd_gauss_ft(const dipamd_image *in, const dipamd_image *out, const double *sigmas, int32_t nsigmas,
                                const int32_t *orders, int32_t norders, const int32_t *bc, int32_t nbc,
                                double truncation, uint32_t flags, void *stream) {
-    (void)flags;
     int st;
+    const bool in_freq = (flags & DIPAMD_FT_IN_FREQUENCY) != 0, out_freq = (flags & DIPAMD_FT_OUT_FREQUENCY) != 0;
     if ((st = check_image(in, "in")) != DIPAMD_OK) return st;
-    if ((st = require_real(in)) != DIPAMD_OK) return st;
+    if (!in_freq && !out_freq && (st = require_real(in)) != DIPAMD_OK) return st;
     if ((st = check_image(out, "out")) != DIPAMD_OK) return st;
     const int nd = in->ndim;
     if (out->ndim != nd) return set_error(DIPAMD_E_SIZES, "Sizes don't match");
@@ -362,6 +513,13 @@ extern "C" int dipamd_gauss_ft(const dipamd_image *in, const dipamd_image *out, 
         } else if (s[d] < 0) {
             s[d] = 0;
         }
+    }
+    if (in_freq || out_freq) {
+        if (nbc > 0)
+            return set_error(DIPAMD_E_UNSUPPORTED,
+                             "a frequency-domain GaussFT takes no boundary condition (the image is not extended)");
+        return gauss_ft_centred(to_view(in), to_view(out), nd, s, o, in_freq, out_freq, truncation,
+                                static_cast<hipStream_t>(stream));
     }
     int bcs[DIPAMD_MAX_DIMS] = {};
     const bool expand = nbc > 0;

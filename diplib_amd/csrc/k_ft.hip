@@ -181,4 +181,75 @@ hipError_t launch_ft_crop(bool is_double, const void *pad, void *out, int out_dt
     return hipGetLastError();
 }
 
+namespace {
+
+template <typename TB>
+__global__ __launch_bounds__(kFtBlock) void k_ft_cshift(const Cplx<TB> *in, Cplx<TB> *out, const FtCGeo g) {
+    for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < g.total;
+         t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        int64_t r = t, o = 0, st = 1;
+        for (int d = 0; d < g.ndim; ++d) {
+            const int64_t n = g.size[d];
+            const int64_t i = r % n;
+            r /= n;
+            int64_t j = i + g.shift[d];
+            j = j >= n ? j - n : j;
+            o += j * st;
+            st *= n;
+        }
+        out[o] = in[t];
+    }
+}
+
+// the centred response products in dim order, then the scale (the reference multiplies
+// the spectrum by each dim's response; the inverse scales by 1/N)
+template <typename TB>
+__global__ __launch_bounds__(kFtBlock) void k_ft_cmultiply(Cplx<TB> *data, const Cplx<TB> *luts, TB scale,
+                                                           const FtCGeo g) {
+    for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < g.total;
+         t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        int64_t r = t;
+        Cplx<TB> v = data[t];
+        for (int d = 0; d < g.ndim; ++d) {
+            const int64_t i = r % g.size[d];
+            r /= g.size[d];
+            if (g.lut_off[d] >= 0) {
+                const Cplx<TB> h = luts[g.lut_off[d] + i];
+                v = Cplx<TB>{v.x * h.x - v.y * h.y, v.x * h.y + v.y * h.x};
+            }
+        }
+        if (scale != TB(1)) v = Cplx<TB>{v.x * scale, v.y * scale};
+        data[t] = v;
+    }
+}
+
+unsigned cgrid(int64_t total) {
+    int64_t n = (total + kFtBlock - 1) / kFtBlock;
+    return static_cast<unsigned>(n > 65536 ? 65536 : (n < 1 ? 1 : n));
+}
+
+}  // namespace
+
+hipError_t launch_ft_cshift(bool is_double, const void *in, void *out, const FtCGeo &g, hipStream_t st) {
+    if (is_double)
+        hipLaunchKernelGGL(k_ft_cshift<double>, dim3(cgrid(g.total)), dim3(kFtBlock), 0, st,
+                           static_cast<const Cplx<double> *>(in), static_cast<Cplx<double> *>(out), g);
+    else
+        hipLaunchKernelGGL(k_ft_cshift<float>, dim3(cgrid(g.total)), dim3(kFtBlock), 0, st,
+                           static_cast<const Cplx<float> *>(in), static_cast<Cplx<float> *>(out), g);
+    return hipGetLastError();
+}
+
+hipError_t launch_ft_cmultiply(bool is_double, void *data, const void *luts, double scale, const FtCGeo &g,
+                               hipStream_t st) {
+    if (is_double)
+        hipLaunchKernelGGL(k_ft_cmultiply<double>, dim3(cgrid(g.total)), dim3(kFtBlock), 0, st,
+                           static_cast<Cplx<double> *>(data), static_cast<const Cplx<double> *>(luts), scale, g);
+    else
+        hipLaunchKernelGGL(k_ft_cmultiply<float>, dim3(cgrid(g.total)), dim3(kFtBlock), 0, st,
+                           static_cast<Cplx<float> *>(data), static_cast<const Cplx<float> *>(luts),
+                           static_cast<float>(scale), g);
+    return hipGetLastError();
+}
+
 }  // namespace dipamd

@@ -187,12 +187,16 @@ def GaussIIR(in_, sigmas=(1.0,), derivativeOrder=(0,), boundaryCondition=(), fil
 
 def GaussFT(in_, sigmas=(1.0,), derivativeOrder=(0,), truncation=3.0, inRepresentation="spatial",
             outRepresentation="spatial", boundaryCondition=(), out=None):
-    """dip::GaussFT (linear.h:355-364; gauss.cpp:397-487) for real images in the
-    spatial domain: the Gaussian and its derivatives of any order through the
-    frequency domain (rocFFT), equal to the reference to FFT rounding.  An empty
-    `boundaryCondition` transforms the image as it is (periodic)."""
-    if inRepresentation != "spatial" or outRepresentation != "spatial":
-        raise NotImplementedError("GaussFT in or out of the frequency domain is outside this library")
+    """dip::GaussFT (linear.h:355-364; gauss.cpp:397-487): the Gaussian and its
+    derivatives of any order through the frequency domain (rocFFT), equal to the
+    reference to FFT rounding.  An empty `boundaryCondition` transforms the image as it
+    is (periodic).  inRepresentation / outRepresentation "frequency": the input is / the
+    result is the centred spectrum (dip::FourierTransform's), complex; those forms take
+    no boundary condition."""
+    for r in (inRepresentation, outRepresentation):
+        if r not in ("spatial", "frequency"):
+            raise _lib.DipError(_lib.E_PARAMETER, "Invalid flag: " + r)
+    in_freq, out_freq = inRepresentation == "frequency", outRepresentation == "frequency"
     x, was_np = _to_device(in_)
     sig = [float(s) for s in _as_list(sigmas)]
     ords = [int(o) for o in _as_list(derivativeOrder)]
@@ -200,15 +204,17 @@ def GaussFT(in_, sigmas=(1.0,), derivativeOrder=(0,), truncation=3.0, inRepresen
     if len(sig) not in (0, 1, nd) or len(ords) not in (0, 1, nd):
         raise _lib.DipError(_lib.E_ARRAY_LENGTH, "Array parameter has the wrong number of elements")
     bcs = _bc_codes(boundaryCondition)
-    y = _prepare_out(x, out, _flex_torch(x.dtype))
+    freq = in_freq or out_freq
+    y = _prepare_out(x, out, _complex_torch(x.dtype) if freq else _flex_torch(x.dtype))
     L = lib()
     a_in, a_out = image_struct(x), image_struct(y)
     s_arr = (ctypes.c_double * max(1, len(sig)))(*sig)
     o_arr = (ctypes.c_int32 * max(1, len(ords)))(*ords)
     b_arr = (ctypes.c_int32 * max(1, len(bcs)))(*bcs)
+    flags = (_lib.FT_IN_FREQUENCY if in_freq else 0) | (_lib.FT_OUT_FREQUENCY if out_freq else 0)
     with torch.cuda.device(x.device):
         check(L.dipamd_gauss_ft(ctypes.byref(a_in), ctypes.byref(a_out), s_arr, len(sig), o_arr, len(ords), b_arr,
-                                len(bcs), float(truncation), 0, _stream_handle(x.device)))
+                                len(bcs), float(truncation), flags, _stream_handle(x.device)))
     if was_np and out is None:
         return y.cpu().numpy()
     return y

@@ -94,7 +94,11 @@ enum {
     DIPAMD_TENSOR_FT = 64,
     /* dipamd_gauss_fir_backward: grad_sigma is ONE device double, the sum over the dims
      * of dL/dsigma_d -- the derivative for one sigma shared by every filtered dim. */
-    DIPAMD_GRAD_SIGMA_TOTAL = 128
+    DIPAMD_GRAD_SIGMA_TOTAL = 128,
+    /* dipamd_gauss_ft: the input is a spectrum ("frequency" inRepresentation) / the
+     * result is one ("frequency" outRepresentation); see dipamd_gauss_ft. */
+    DIPAMD_FT_IN_FREQUENCY = 256,
+    DIPAMD_FT_OUT_FREQUENCY = 512
 };
 
 /* Error codes. */
@@ -156,14 +160,27 @@ int dipamd_gauss_iir(const dipamd_image *in, const dipamd_image *out, const doub
                      const int32_t *filter_orders, int32_t nfilter_orders, int32_t design_method,
                      double truncation, uint32_t flags, void *stream);
 
-/* dip::GaussFT (src/linear/gauss.cpp:397-487; include/diplib/linear.h:355-364) for real
- * images, spatial domain in and out: the Gaussian and its derivatives of any order as a
- * product in the frequency domain (rocFFT; single precision for 8-bit and sfloat images,
- * double for dfloat, as the reference's DFT).  nbc == 0 transforms the image as it is
- * (periodic); otherwise the image is extended by the boundary condition (the ones that
- * copy samples) to a 2-3-5-smooth size holding its Gaussian border, and cropped back.
- * dip::Gauss with method "FT" (and "best" for sigma < 0.8 or order > 3) passes "mirror".
- * At most three dims may be filtered.  Results match the reference to FFT rounding. */
+/* dip::GaussFT (src/linear/gauss.cpp:397-487; include/diplib/linear.h:355-364): the
+ * Gaussian and its derivatives of any order as a product in the frequency domain
+ * (rocFFT; single precision for 8-bit and sfloat images, double for dfloat, as the
+ * reference's DFT).  Spatial domain in and out (real images): nbc == 0 transforms the
+ * image as it is (periodic); otherwise the image is extended by the boundary condition
+ * (the ones that copy samples) to a 2-3-5-smooth size holding its Gaussian border, and
+ * cropped back.  dip::Gauss with method "FT" (and "best" for sigma < 0.8 or order > 3)
+ * passes "mirror".  At most three dims may be filtered.
+ *
+ * With DIPAMD_FT_OUT_FREQUENCY the result is the spectrum: the centred DFT of the input
+ * (origin in the middle, unscaled: dip::FourierTransform's default) times the
+ * separable response; with DIPAMD_FT_IN_FREQUENCY the input is such a (complex)
+ * spectrum, multiplied by the response and, unless the output is a spectrum too,
+ * transformed back (centred inverse DFT, scaled by 1/N) into a complex image.  These
+ * forms take no boundary condition (nbc == 0: the reference extends only spatial
+ * inputs, to a larger size), any real or complex input, at most three dims, and a
+ * complex `out` (DIPAMD_DT_SCOMPLEX / DCOMPLEX, SuggestComplex(in) normally).  Where
+ * the reference transforms a real image whose real-to-complex dim has an odd length,
+ * its spectrum deviates from the DFT (that path of its DFT is not exact); this
+ * implementation gives the DFT there, as the reference does for complex input.
+ * Results match the reference to FFT rounding. */
 int dipamd_gauss_ft(const dipamd_image *in, const dipamd_image *out, const double *sigmas, int32_t nsigmas,
                     const int32_t *orders, int32_t norders, const int32_t *bc, int32_t nbc, double truncation,
                     uint32_t flags, void *stream);

@@ -561,7 +561,8 @@ inline Image GaussIIR(Image const &in, FloatArray sigmas = {1.0}, UnsignedArray 
     return out;
 }
 
-/* dip::GaussFT (linear.h:355-364) for real images, spatial domain in and out. */
+/* dip::GaussFT (linear.h:355-364); "frequency" representations give / take the centred
+ * spectrum (complex), without a boundary condition. */
 inline void GaussFT(Image const &in, Image &out, FloatArray sigmas = {1.0}, UnsignedArray derivativeOrder = {0},
                     dfloat truncation = 3, String const &inRepresentation = "spatial",
                     String const &outRepresentation = "spatial", StringArray const &boundaryCondition = {}) {
@@ -572,16 +573,19 @@ inline void GaussFT(Image const &in, Image &out, FloatArray sigmas = {1.0}, Unsi
     if (!in.IsForged()) throw ParameterError("Image is not forged");
     sigmas = detail::TensorDim<dfloat>(std::move(sigmas), in, 1.0, 0.0);
     derivativeOrder = detail::TensorDim<uint>(std::move(derivativeOrder), in, 0, 0);
-    if (inRepresentation != "spatial" || outRepresentation != "spatial")
-        throw ParameterError("GaussFT in or out of the frequency domain is not provided by this library");
+    for (auto const *r : {&inRepresentation, &outRepresentation})
+        if (*r != "spatial" && *r != "frequency") throw ParameterError("Invalid flag: " + *r);
+    const uint32_t reprFlags = (inRepresentation == "frequency" ? DIPAMD_FT_IN_FREQUENCY : 0u) |
+                               (outRepresentation == "frequency" ? DIPAMD_FT_OUT_FREQUENCY : 0u);
     Image src = in;
-    detail::PrepareOutput(in, out);
+    detail::PrepareOutput(in, out, reprFlags ? DataType::SuggestComplex(in.DataType())
+                                             : DataType::SuggestFlex(in.DataType()));
     std::vector<int32_t> ord(derivativeOrder.begin(), derivativeOrder.end());
     std::vector<int32_t> bc = detail::BoundaryCodes(detail::TensorDimBC(boundaryCondition, in));
     dipamd_image a = src.View(), b = out.View();
     detail::Check(dipamd_gauss_ft(&a, &b, sigmas.data(), static_cast<int32_t>(sigmas.size()), ord.data(),
                                   static_cast<int32_t>(ord.size()), bc.data(), static_cast<int32_t>(bc.size()),
-                                  truncation, 0, dipamd::CurrentStream()));
+                                  truncation, reprFlags, dipamd::CurrentStream()));
 }
 inline Image GaussFT(Image const &in, FloatArray sigmas = {1.0}, UnsignedArray derivativeOrder = {0},
                      dfloat truncation = 3, String const &inRepresentation = "spatial",

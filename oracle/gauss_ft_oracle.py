@@ -181,3 +181,42 @@ def gauss_ft(x, sigmas, orders=(0,), bc=(), truncation=3.0):
     for d in tdims:
         sl[nd - 1 - d] = slice(offs[d], offs[d] + sizes[d])
     return y[tuple(sl)].astype(np.float32 if single else np.float64)
+
+
+def centred_fft(a, inverse=False):
+    """DIPlib's FourierTransform of every dim with the origin in the middle
+    (fourier.cpp:116-150 CopyForDFT with shift, ShiftCornerToCenter): fftshift of the
+    DFT of the ifftshift of the input; the forward transform unscaled, the inverse scaled
+    by 1/N (fourier.cpp:555-565)."""
+    a = np.fft.ifftshift(np.asarray(a, np.complex128))
+    b = np.fft.ifftn(a) if inverse else np.fft.fftn(a)
+    return np.fft.fftshift(b)
+
+
+def gauss_ft_frequency(x, sigmas, orders=(0,), truncation=3.0, in_frequency=False, out_frequency=True):
+    """dip::GaussFT(x, sigmas, orders, truncation, inRepr, outRepr) without a boundary
+    condition (gauss.cpp:397-487) where a domain is "frequency": the centred spectrum of
+    the input (or the input itself when it is a spectrum) times the separable response
+    of each dim (GaussFTLineFilter in its centred order), returned as a spectrum or
+    transformed back (complex: the inverse of a spectrum input is not taken as real).
+    Complex result of the input's complex type (SuggestComplex)."""
+    x = np.asarray(x)
+    nd = x.ndim
+    single = x.dtype not in (np.float64, np.complex128, np.int32, np.uint32)
+    sig = list(sigmas) if len(sigmas) == nd else list(sigmas) * nd
+    ords = list(orders) if len(orders) == nd else (list(orders) * nd if orders else [0] * nd)
+    sizes = list(reversed(x.shape))
+    for d in range(nd):
+        if sizes[d] == 1:
+            sig[d], ords[d] = 0.0, 0
+        elif sig[d] < 0:
+            sig[d] = 0.0
+    spec = np.asarray(x, np.complex128) if in_frequency else centred_fft(x)
+    for d in range(nd):
+        if sig[d] > 0 or ords[d] > 0:
+            lut = _response(sizes[d], sig[d], ords[d], truncation, single).astype(np.complex128)
+            shape = [1] * nd
+            shape[nd - 1 - d] = lut.size
+            spec = spec * lut.reshape(shape)
+    y = spec if out_frequency else centred_fft(spec, inverse=True)
+    return y.astype(np.complex64 if single else np.complex128)

@@ -130,6 +130,28 @@ int dipref_gauss_ft( void const* in, int in_dtype, void* out, int out_dtype, int
    }
 }
 
+// dip::GaussFT( in, out, sigmas, orders, truncation, inRepr, outRepr, {} ) with the
+// domains named ("spatial" / "frequency"); the output is written into `out` (protected,
+// so of out_dtype and the input's sizes: no boundary extension).
+int dipref_gauss_ft_repr( void const* in, int in_dtype, void* out, int out_dtype, int ndim, int64_t const* sizes,
+                          double const* sigmas, int nsigmas, int const* orders, int norders, double truncation,
+                          char const* in_repr, char const* out_repr ) {
+   try {
+      dip::Image imgIn = Wrap( in, in_dtype, ndim, sizes );
+      dip::Image imgOut = Wrap( out, out_dtype, ndim, sizes );
+      imgOut.Protect();
+      dip::FloatArray s( static_cast< dip::uint >( nsigmas ));
+      for( int ii = 0; ii < nsigmas; ++ii ) { s[ static_cast< dip::uint >( ii ) ] = sigmas[ ii ]; }
+      dip::UnsignedArray o( static_cast< dip::uint >( norders ));
+      for( int ii = 0; ii < norders; ++ii ) { o[ static_cast< dip::uint >( ii ) ] = static_cast< dip::uint >( orders[ ii ] ); }
+      dip::GaussFT( imgIn, imgOut, s, o, truncation, in_repr, out_repr, {} );
+      return 0;
+   } catch( std::exception const& e ) {
+      g_err = e.what();
+      return 1;
+   }
+}
+
 // In-place form: `buf` is both the input and the output (same dtype, which must be
 // sfloat or dfloat).
 int dipref_gauss_inplace( void* buf, int dtype, int ndim, int64_t const* sizes, double const* sigmas, int nsigmas,

@@ -264,6 +264,24 @@ FT_ODD_CASES = [
 # The derivative family and the structure tensor through the FT method ("best" with a sigma
 # below 0.8, or "gaussft"); compared with a tolerance (FFT rounding).  Dgg is left out: its
 # division by the squared gradient magnitude amplifies rounding where the gradient vanishes.
+# dip::GaussFT with a frequency-domain input or output (no boundary condition):
+# (name, numpy shape, dtype, sigmas (dip order), orders, inRepresentation, outRepresentation).
+# Real inputs are 1-D or 2-D with an even x length: the reference's real-to-complex path
+# gives no exact DFT for an odd length of that dim, nor at the Nyquist plane of a third
+# dim (see the oracle tests); complex inputs take its exact complex path.
+FT_REPR_CASES = [
+    ("ftr_f32_64x48_s2_spec", (48, 64), "float32", [2.0], [0], "spatial", "frequency"),
+    ("ftr_u8_62x40_s15_o10_spec", (40, 62), "uint8", [1.5], [1, 0], "spatial", "frequency"),
+    ("ftr_f64_40x31_s3_o02_spec", (31, 40), "float64", [3.0], [0, 2], "spatial", "frequency"),
+    ("ftr_c64_20x16x10_s1_spec", (10, 16, 20), "complex64", [1.0, 1.0, 0.8], [0], "spatial", "frequency"),
+    ("ftr_f32_1d_50_s05_o3_spec", (50,), "float32", [0.5], [3], "spatial", "frequency"),
+    ("ftr_c64_45x33_s2_spec", (33, 45), "complex64", [2.0], [0, 1], "spatial", "frequency"),
+    ("ftr_c64_64x48_s2_inverse", (48, 64), "complex64", [2.0], [0], "frequency", "spatial"),
+    ("ftr_c128_33x20_o11_inverse", (20, 33), "complex128", [1.0], [1, 1], "frequency", "spatial"),
+    ("ftr_c64_40x30_s1_spec_to_spec", (30, 40), "complex64", [1.0, 2.0], [0], "frequency", "frequency"),
+    ("ftr_c64_37x29x5_inverse", (5, 29, 37), "complex64", [1.5, 0.0, 1.0], [0], "frequency", "spatial"),
+]
+
 DERIV_FT_CASES = []
 for fn in ("gradient", "gradmag", "hessian", "laplace", "structure:0.6", "structure:3"):
     t = fn.replace(":", "").replace(".", "")

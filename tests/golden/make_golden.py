@@ -92,6 +92,15 @@ def write_ft_cases(fname="ft_cases.npz", seed0=9500):
     np.savez_compressed(os.path.join(HERE, fname), **arrs)
 
 
+def write_ft_repr_cases(fname="ft_repr_cases.npz", seed0=9800):
+    arrs = {}
+    for i, (name, shape, dt, sig, ords, ir, orr) in enumerate(cases.FT_REPR_CASES):
+        x = cases.make_input(shape, dt, seed=seed0 + i)
+        arrs[name + "__in"] = x
+        arrs[name + "__out"] = oc.ref_gauss_ft_repr(x, sig, ords, 3.0, ir, orr)
+    np.savez_compressed(os.path.join(HERE, fname), **arrs)
+
+
 def write_derivative_ft_cases(fname="deriv_ft_cases.npz", seed0=9900):
     arrs = {}
     for i, (name, fn, shape, dt, sig, bc, method) in enumerate(cases.DERIV_FT_CASES):
@@ -136,6 +145,7 @@ def main():
         write_cases(cases.EXTRAP_CASES, "extrap_cases.npz", 3000)
     if only in (None, "ft"):
         write_ft_cases()
+        write_ft_repr_cases()
         write_derivative_ft_cases()
     if only in (None, "iir"):
         write_iir_cases()

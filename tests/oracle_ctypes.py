@@ -60,6 +60,7 @@ def ref():
         lib.dipref_last_error.restype = ctypes.c_char_p
         if hasattr(lib, "dipref_gauss_ft"):
             lib.dipref_gauss_ft.argtypes = [P, I, P, I, I, P, P, I, P, I, P, I, D]
+            lib.dipref_gauss_ft_repr.argtypes = [P, I, P, I, I, P, P, I, P, I, D, ctypes.c_char_p, ctypes.c_char_p]
         if hasattr(lib, "dipref_gauss_iir"):
             lib.dipref_gauss_iir.argtypes = [P, I, P, I, I, P, P, I, P, I, P, I, P, I, ctypes.c_char_p, D, I]
         if hasattr(lib, "dipref_derivative"):
@@ -303,6 +304,26 @@ def _ft_oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import gauss_ft_oracle
     return gauss_ft_oracle
+
+
+def ref_gauss_ft_repr(x, sigmas, orders=(0,), truncation=3.0, in_repr="spatial", out_repr="frequency",
+                      out_dtype=None):
+    """The reference's dip::GaussFT(x, sigmas, orders, truncation, in_repr, out_repr, {})
+    (no boundary extension; the result has the input's sizes)."""
+    x = np.ascontiguousarray(x)
+    if out_dtype is None:
+        out_dtype = np.complex128 if x.dtype in (np.float64, np.complex128, np.int32, np.uint32) else np.complex64
+        if out_repr == "spatial" and in_repr == "spatial":
+            out_dtype = _flex(x.dtype)
+    out = np.zeros(x.shape, np.dtype(out_dtype))
+    sizes = _sizes(x)
+    sig = np.ascontiguousarray(list(sigmas), np.float64)
+    ords = np.ascontiguousarray(list(orders), np.int32)
+    st = ref().dipref_gauss_ft_repr(_ptr(x), DT[x.dtype], _ptr(out), DT[out.dtype], x.ndim, _ptr(sizes), _ptr(sig),
+                                    len(sig), _ptr(ords), len(ords), truncation, in_repr.encode(), out_repr.encode())
+    if st != 0:
+        raise RuntimeError(ref().dipref_last_error().decode())
+    return out
 
 
 def separable_complex(x, filters, symmetries, bc=("mirror",), process=None, out_dtype=None, origins=None,

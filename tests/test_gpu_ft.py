@@ -83,7 +83,51 @@ def test_ft_shift_invariance_and_routing(cuda):
     assert torch.equal(dip.Gauss(x, [0.5], [0, 0], "best"), ref)
     y4 = dip.GaussFT(x, [2.0], [4, 0], 3.0, "spatial", "spatial", ["mirror"])
     assert torch.equal(dip.Gauss(x, [2.0], [4, 0], "best"), y4)  # order > 3
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(dip.DipError):  # a real image is no spectrum (gauss.cpp:412-414)
         dip.GaussFT(x, [1.0], [0], 3.0, "frequency", "spatial", [])
     with pytest.raises(dip.DipError):
         dip.GaussFT(x, [1.0], [0], 3.0, "spatial", "spatial", ["third order"])
+
+
+FT_REPR = np.load(os.path.join(HERE, "golden", "ft_repr_cases.npz"))
+
+
+def _close_c(got, exp):
+    tol = 1e-5 if exp.dtype == np.complex64 else 1e-12
+    err = np.max(np.abs(got.astype(np.complex128) - exp)) / max(np.max(np.abs(exp)), 1e-300)
+    return err <= tol, err
+
+
+@pytest.mark.parametrize("case", cases.FT_REPR_CASES, ids=[c[0] for c in cases.FT_REPR_CASES])
+def test_ft_frequency_vs_reference(cuda, case):
+    """dip::GaussFT with a frequency-domain input and / or output (the centred
+    spectrum) against the reference's own results, to FFT rounding."""
+    name, shape, dt, sig, ords, ir, orr = case
+    x = FT_REPR[name + "__in"]
+    exp = FT_REPR[name + "__out"]
+    y = dip.GaussFT(torch.from_numpy(x).to(cuda), sig, ords, 3.0, ir, orr)
+    got = y.cpu().numpy()
+    assert got.dtype == exp.dtype and got.shape == exp.shape
+    ok, err = _close_c(got, exp)
+    assert ok, f"{name}: {err:.3e}"
+
+
+@pytest.mark.parametrize("shape,dt", [((44, 31), "float32"), ((10, 16, 20), "float32"), ((9,), "float64")])
+def test_ft_frequency_exact_dft_vs_oracle(cuda, shape, dt):
+    """Where the reference's real-input spectrum is not the DFT (odd real-to-complex
+    length, a third dim), the result is the DFT: against the oracle."""
+    x = cases.make_input(shape, dt, seed=31)
+    exp = fto.gauss_ft_frequency(x, [1.0], [1], 3.0)
+    got = dip.GaussFT(torch.from_numpy(x).to(cuda), [1.0], [1], 3.0, "spatial", "frequency").cpu().numpy()
+    ok, err = _close_c(got, exp)
+    assert ok, f"{shape}: {err:.3e}"
+
+
+def test_ft_frequency_errors(cuda):
+    x = torch.rand((20, 30), device=cuda)
+    with pytest.raises(dip.DipError):
+        dip.GaussFT(x, [1.0], [0], 3.0, "spatial", "frequency", ["mirror"])  # no boundary condition here
+    with pytest.raises(dip.DipError):
+        dip.GaussFT(x, [1.0], [0], 3.0, "frequency", "spatial")  # a spectrum is complex
+    with pytest.raises(dip.DipError):
+        dip.GaussFT(x, [1.0], [0], 3.0, "spatial", "fourier")

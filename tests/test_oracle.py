@@ -632,6 +632,60 @@ def test_ft_reference_odd_length_defect():
     assert np.allclose(rb, fto.gauss_ft(b, [0.5], [0], []), atol=1e-6)  # the centre one is right
 
 
+FT_REPR = np.load(os.path.join(HERE, "golden", "ft_repr_cases.npz"))
+
+
+@pytest.mark.parametrize("case", cases.FT_REPR_CASES, ids=[c[0] for c in cases.FT_REPR_CASES])
+def test_ft_frequency_oracle_matches_reference(case):
+    """The frequency-domain forms of dip::GaussFT (centred spectrum out, spectrum in,
+    both): the oracle against the reference's own outputs, to FFT rounding."""
+    name, shape, dt, sig, ords, ir, orr = case
+    x = FT_REPR[name + "__in"]
+    exp = FT_REPR[name + "__out"]
+    got = fto.gauss_ft_frequency(x, sig, ords, 3.0, ir == "frequency", orr == "frequency")
+    assert got.dtype == exp.dtype and got.shape == exp.shape
+    tol = 1e-5 if exp.dtype == np.complex64 else 1e-12
+    assert np.max(np.abs(got.astype(np.complex128) - exp)) <= tol * np.max(np.abs(exp))
+
+
+def test_ft_frequency_oracle_properties():
+    """What the mathematics fixes for the centred spectrum: an impulse at the origin
+    (the centre pixel) has the flat spectrum 1; the spectrum of a real image is
+    conjugate-symmetric about the centre; sigma 0, order 0 returns the spectrum itself;
+    spectrum out followed by spectrum in and back gives the spatial-domain result (an
+    even order: an odd order's response at the unpaired Nyquist bin of an even size
+    makes the inverse complex);
+    the Gaussian's spectrum at the centre (DC) is 1."""
+    x = np.zeros((8, 10))
+    x[4, 5] = 1.0
+    assert np.allclose(fto.gauss_ft_frequency(x, [0.0], [0]), 1.0, atol=1e-12)
+    r = np.random.default_rng(4).random((8, 10))
+    X = fto.gauss_ft_frequency(r, [0.0], [0])
+    assert np.allclose(X[1:, 1:], np.conj(X[1:, 1:][::-1, ::-1]), atol=1e-9)
+    assert np.allclose(X, np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(r))), atol=1e-12)
+    Y = fto.gauss_ft_frequency(r, [1.2], [2, 0])
+    back = fto.gauss_ft_frequency(Y.astype(np.complex128), [0.0], [0], in_frequency=True, out_frequency=False)
+    direct = fto.gauss_ft(r, [1.2], [2, 0], [])
+    assert np.allclose(back.real, direct, atol=1e-10) and np.max(np.abs(back.imag)) < 1e-10
+    g = fto.gauss_ft_frequency(x, [2.0], [0])
+    assert abs(g[4, 5] - 1.0) < 1e-12
+
+
+@pytest.mark.skipif(not oc.ref_available(), reason="reference library not built")
+def test_ft_frequency_reference_odd_length_defect():
+    """Recorded behaviour of the reference: the spectrum of a REAL image is not the DFT
+    where its real-to-complex dim (x) has an odd length, nor at the Nyquist plane of a
+    third dim (the complex input of the same values gives the DFT), so the real-input
+    fixtures are 1-D or 2-D with x even."""
+    for shape in ((44, 31), (10, 16, 20)):  # odd x; a third dim (its Nyquist plane)
+        x = cases.make_input(shape, "float32", seed=3)
+        r_real = oc.ref_gauss_ft_repr(x, [1.0], [0], 3.0, "spatial", "frequency")
+        r_cplx = oc.ref_gauss_ft_repr(x.astype(np.complex64), [1.0], [0], 3.0, "spatial", "frequency")
+        exact = fto.gauss_ft_frequency(x, [1.0], [0])
+        assert np.max(np.abs(r_cplx - exact)) <= 1e-5 * np.max(np.abs(exact))
+        assert np.max(np.abs(r_real - exact)) > 5e-5 * np.max(np.abs(exact))
+
+
 DERIV_FT = np.load(os.path.join(HERE, "golden", "deriv_ft_cases.npz"))
 
 
