@@ -94,6 +94,9 @@ def lib():
     L.dipamd_gauss_ft.argtypes = [IMG, IMG, P, I32, P, I32, P, I32, D, U32, P]
     L.dipamd_derivative_outputs.argtypes = [I32, I32]
     L.dipamd_structure_tensor.argtypes = [IMG, IMG, I32, P, I32, P, I32, P, I32, D, U32, P]
+    L.dipamd_normalized_convolution.argtypes = [IMG, IMG, IMG, P, I32, P, I32, D, U32, P]
+    L.dipamd_normalized_differential_convolution.argtypes = [IMG, IMG, IMG, I32, P, I32, P, I32, D, U32, P]
+    L.dipamd_structure_tensor_masked.argtypes = [IMG, IMG, IMG, I32, P, I32, P, I32, P, I32, D, U32, P]
     if hasattr(L, "dipamd_autotune"):
         L.dipamd_autotune.argtypes = [U32, P]
     if hasattr(L, "dipamd_measure_copy"):  # measurement helper, not part of the drop-in interface
@@ -117,7 +120,9 @@ EXPORTED_SYMBOLS = [
     "dipamd_batch_info", "dipamd_gauss_fir_backward", "dipamd_gauss_radius", "dipamd_last_error", "dipamd_version",
     "dipamd_trace_enable", "dipamd_trace_read", "dipamd_derivative", "dipamd_derivative_outputs",
     "dipamd_structure_tensor", "dipamd_gauss_iir", "dipamd_gauss_ft", "dipamd_batch_gauss_noalloc",
-    "dipamd_autotune",
+    "dipamd_autotune", "dipamd_normalized_convolution", "dipamd_normalized_differential_convolution",
+    "dipamd_structure_tensor_masked", "dipamd_separable_lines_count", "dipamd_separable_lines_gather",
+    "dipamd_separable_lines_scatter",
 ]
 TUNE_QUICK = 256
 FT_IN_FREQUENCY, FT_OUT_FREQUENCY = 256, 512

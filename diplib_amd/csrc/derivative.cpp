@@ -21,6 +21,8 @@
 namespace dipamd {
 hipError_t launch_combine(bool is_double, int op, const void *const *d, int nd, void *const *out, int64_t n,
                           hipStream_t stream);
+hipError_t launch_masked(bool is_double, int op, const void *x, const void *m, void *a, const void *den, const void *dw,
+                         const void *dm, void *out, int64_t n, hipStream_t stream);
 }  // namespace dipamd
 
 using namespace dipamd;
@@ -239,9 +241,171 @@ int combine(int op, bool is_double, const std::vector<dipamd_image> &terms, int 
     return DIPAMD_OK;
 }
 
+// The masked forms (derivative.cpp:522-600): the mask's Gaussian (the denominator) and
+// the weighted input, in buffer-type scratch images.
+struct MaskedTerms {
+    dipamd_image den, w, m;
+};
+
+int64_t npx_of_image(const dipamd_image *im) {
+    int64_t n = 1;
+    for (int d = 0; d < im->ndim; ++d) n *= im->sizes[d];
+    return n;
+}
+
+int masked_terms(const dipamd_image *in, const dipamd_image *mask, int buf_dt, const double *sig, const int32_t *bc,
+                 int32_t nbc, double truncation, uint32_t flags, Scratch &scr, MaskedTerms *t) {
+    int st;
+    if ((st = check_image(mask, "mask")) != DIPAMD_OK) return st;
+    if ((st = require_real(mask)) != DIPAMD_OK) return st;
+    if (mask->ndim != in->ndim) return set_error(DIPAMD_E_SIZES, "Sizes don't match");
+    for (int d = 0; d < in->ndim; ++d)
+        if (mask->sizes[d] != in->sizes[d]) return set_error(DIPAMD_E_SIZES, "Sizes don't match");
+    const int method = derivative_method(flags);
+    const bool dbl = buf_dt == DIPAMD_DT_DFLOAT;
+    dipamd_image xc;
+    if ((st = scr.make(in, buf_dt, &t->den)) != DIPAMD_OK || (st = scr.make(in, buf_dt, &t->w)) != DIPAMD_OK ||
+        (st = scr.make(in, buf_dt, &t->m)) != DIPAMD_OK || (st = scr.make(in, buf_dt, &xc)) != DIPAMD_OK)
+        return st;
+    // denominator = Gauss( mask ) (in the buffer type: SafeDivide's computation type)
+    if ((st = gauss(method, mask, &t->den, sig, nullptr, 0, bc, nbc, truncation, flags, scr.stream)) != DIPAMD_OK)
+        return st;
+    // weighted = the input masked (binary mask) or times the mask, in the buffer type
+    if ((st = cast_to_buffer(to_view(in), to_view(&xc), in->ndim, dbl, scr.stream)) != DIPAMD_OK) return st;
+    if ((st = cast_to_buffer(to_view(mask), to_view(&t->m), in->ndim, dbl, scr.stream)) != DIPAMD_OK) return st;
+    const hipError_t e = launch_masked(dbl, mask->dtype == DIPAMD_DT_BIN ? 0 : 1, xc.origin, t->m.origin, t->w.origin,
+                                       nullptr, nullptr, nullptr, nullptr, npx_of_image(in), scr.stream);
+    if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return DIPAMD_OK;
+}
+
+// NormalizedDifferentialConvolution along `dim` into the contiguous buffer-type `out`.
+int ndc_into(const dipamd_image *mask, const MaskedTerms &t, int dim, int buf_dt, const double *sig, const int32_t *bc,
+             int32_t nbc, double truncation, uint32_t flags, Scratch &scr, const dipamd_image *out) {
+    int st;
+    const int method = derivative_method(flags);
+    const bool dbl = buf_dt == DIPAMD_DT_DFLOAT;
+    const int nd = t.w.ndim;
+    dipamd_image nc, dm, dw;
+    if ((st = scr.make(&t.w, buf_dt, &nc)) != DIPAMD_OK || (st = scr.make(&t.w, buf_dt, &dm)) != DIPAMD_OK ||
+        (st = scr.make(&t.w, buf_dt, &dw)) != DIPAMD_OK)
+        return st;
+    std::vector<int32_t> ord(static_cast<size_t>(nd), 0);
+    ord[static_cast<size_t>(dim)] = 1;
+    const int64_t n = npx_of_image(&t.w);
+    // NC = SafeDivide( Gauss( weighted ), denominator )
+    if ((st = gauss(method, &t.w, &nc, sig, nullptr, 0, bc, nbc, truncation, flags, scr.stream)) != DIPAMD_OK)
+        return st;
+    hipError_t e = launch_masked(dbl, 2, nullptr, nullptr, nc.origin, t.den.origin, nullptr, nullptr, nullptr, n,
+                                 scr.stream);
+    if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    // the derivatives of the mask and of the weighted input along `dim`
+    if ((st = gauss(method, mask, &dm, sig, ord.data(), nd, bc, nbc, truncation, flags, scr.stream)) != DIPAMD_OK)
+        return st;
+    if ((st = gauss(method, &t.w, &dw, sig, ord.data(), nd, bc, nbc, truncation, flags, scr.stream)) != DIPAMD_OK)
+        return st;
+    // out = SafeDivide( dw - NC * dm, denominator )
+    e = launch_masked(dbl, 3, nullptr, nullptr, nc.origin, t.den.origin, dw.origin, dm.origin, out->origin, n,
+                      scr.stream);
+    if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return DIPAMD_OK;
+}
+
+// Writes the contiguous buffer-type `res` into `out` (a copy when out is not contiguous).
+int deliver(const dipamd_image *res, const dipamd_image *out, hipStream_t stream) {
+    if (res->origin == out->origin) return DIPAMD_OK;
+    return cast_from_buffer(to_view(res), to_view(out), out->ndim, stream);
+}
+
 }  // namespace
 
 extern "C" {
+
+int dipamd_normalized_convolution(const dipamd_image *in, const dipamd_image *mask, const dipamd_image *out,
+                                  const double *sigmas, int32_t nsigmas, const int32_t *bc, int32_t nbc,
+                                  double truncation, uint32_t flags, void *stream) {
+    int st;
+    if ((st = check_image(in, "in")) != DIPAMD_OK) return st;
+    if ((st = require_real(in)) != DIPAMD_OK) return st;
+    if ((st = check_outputs(DIPAMD_FN_LAPLACE, in, out, 1, 1)) != DIPAMD_OK) return st;
+    const int buf_dt = suggest_flex(in->dtype);
+    if (out->dtype != buf_dt)
+        return set_error(DIPAMD_E_UNSUPPORTED, "the output must have the floating-point type of the input");
+    double sig[DIPAMD_MAX_DIMS];
+    if ((st = array_param(sigmas, nsigmas, in->ndim, 1.0, sig)) != DIPAMD_OK) return st;
+    Scratch scr{static_cast<hipStream_t>(stream), {}};
+    MaskedTerms t;
+    if ((st = masked_terms(in, mask, buf_dt, sig, bc, nbc, truncation, flags, scr, &t)) != DIPAMD_OK) return st;
+    dipamd_image g;
+    if ((st = scr.make(in, buf_dt, &g)) != DIPAMD_OK) return st;
+    // out = SafeDivide( Gauss( weighted ), Gauss( mask ) ) (derivative.cpp:522-542)
+    if ((st = gauss(derivative_method(flags), &t.w, &g, sig, nullptr, 0, bc, nbc, truncation, flags, stream)) != DIPAMD_OK)
+        return st;
+    const hipError_t e = launch_masked(buf_dt == DIPAMD_DT_DFLOAT, 2, nullptr, nullptr, g.origin, t.den.origin, nullptr,
+                                       nullptr, nullptr, npx_of_image(in), scr.stream);
+    if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return deliver(&g, out, scr.stream);
+}
+
+int dipamd_normalized_differential_convolution(const dipamd_image *in, const dipamd_image *mask,
+                                               const dipamd_image *out, int32_t dimension, const double *sigmas,
+                                               int32_t nsigmas, const int32_t *bc, int32_t nbc, double truncation,
+                                               uint32_t flags, void *stream) {
+    int st;
+    if ((st = check_image(in, "in")) != DIPAMD_OK) return st;
+    if ((st = require_real(in)) != DIPAMD_OK) return st;
+    if ((st = check_outputs(DIPAMD_FN_LAPLACE, in, out, 1, 1)) != DIPAMD_OK) return st;
+    if (dimension < 0 || dimension >= in->ndim) return set_error(DIPAMD_E_PARAMETER, "Parameter has invalid value");
+    const int buf_dt = suggest_flex(in->dtype);
+    if (out->dtype != buf_dt)
+        return set_error(DIPAMD_E_UNSUPPORTED, "the output must have the floating-point type of the input");
+    double sig[DIPAMD_MAX_DIMS];
+    if ((st = array_param(sigmas, nsigmas, in->ndim, 1.0, sig)) != DIPAMD_OK) return st;
+    Scratch scr{static_cast<hipStream_t>(stream), {}};
+    MaskedTerms t;
+    if ((st = masked_terms(in, mask, buf_dt, sig, bc, nbc, truncation, flags, scr, &t)) != DIPAMD_OK) return st;
+    dipamd_image r;
+    if ((st = scr.make(in, buf_dt, &r)) != DIPAMD_OK) return st;
+    if ((st = ndc_into(mask, t, dimension, buf_dt, sig, bc, nbc, truncation, flags, scr, &r)) != DIPAMD_OK) return st;
+    return deliver(&r, out, scr.stream);
+}
+
+int dipamd_structure_tensor_masked(const dipamd_image *in, const dipamd_image *mask, const dipamd_image *outs,
+                                   int32_t nouts, const double *gradient_sigmas, int32_t ngs,
+                                   const double *tensor_sigmas, int32_t nts, const int32_t *bc, int32_t nbc,
+                                   double truncation, uint32_t flags, void *stream) {
+    int st;
+    if ((st = check_image(in, "in")) != DIPAMD_OK) return st;
+    if ((st = require_real(in)) != DIPAMD_OK) return st;
+    const int nd = in->ndim;
+    // every dim gets its normalized derivative (structure_tensor.cpp:46-52)
+    const int want = nd * (nd + 1) / 2;
+    if ((st = check_outputs(DIPAMD_FN_STRUCTURE_TENSOR, in, outs, nouts, want)) != DIPAMD_OK) return st;
+    const int buf_dt = suggest_flex(in->dtype);
+    for (int k = 0; k < nouts; ++k)
+        if (outs[k].dtype != buf_dt)
+            return set_error(DIPAMD_E_UNSUPPORTED, "the output must have the floating-point type of the input");
+    double gsig[DIPAMD_MAX_DIMS], tsig[DIPAMD_MAX_DIMS];
+    if ((st = array_param(gradient_sigmas, ngs, nd, 1.0, gsig)) != DIPAMD_OK) return st;
+    if ((st = array_param(tensor_sigmas, nts, nd, 5.0, tsig)) != DIPAMD_OK) return st;
+    Scratch scr{static_cast<hipStream_t>(stream), {}};
+    MaskedTerms t;
+    if ((st = masked_terms(in, mask, buf_dt, gsig, bc, nbc, truncation, flags, scr, &t)) != DIPAMD_OK) return st;
+    std::vector<dipamd_image> g(static_cast<size_t>(nd));
+    for (int d = 0; d < nd; ++d) {
+        if ((st = scr.make(in, buf_dt, &g[static_cast<size_t>(d)])) != DIPAMD_OK) return st;
+        if ((st = ndc_into(mask, t, d, buf_dt, gsig, bc, nbc, truncation, flags, scr, &g[static_cast<size_t>(d)])) !=
+            DIPAMD_OK)
+            return st;
+    }
+    // g g^T, then each element smoothed with the tensor sigmas (structure_tensor.cpp:55-56)
+    if ((st = combine(5, buf_dt == DIPAMD_DT_DFLOAT, g, nd, outs, nouts, scr, in)) != DIPAMD_OK) return st;
+    for (int k = 0; k < nouts; ++k) {
+        st = gauss(tensor_method(flags), &outs[k], &outs[k], tsig, nullptr, 0, bc, nbc, truncation, flags, stream);
+        if (st != DIPAMD_OK) return st;
+    }
+    return DIPAMD_OK;
+}
 
 int dipamd_derivative_outputs(int32_t function, int32_t ndiff) {
     if (ndiff < 1 || ndiff > DIPAMD_MAX_DIMS) return set_error(DIPAMD_E_PARAMETER, "Dimensionality not supported");

@@ -130,4 +130,54 @@ hipError_t launch_combine(bool is_double, int op, const void *const *d, int nd, 
     return hipGetLastError();
 }
 
+namespace {
+
+// The masked forms (derivative.cpp:522-600), over contiguous buffer-type arrays, each
+// image operation rounded in the buffer type.  op 0: w = m != 0 ? x : 0 (a binary mask,
+// Image::Mask); op 1: w = x * m (MultiplySampleWise); op 2: a = SafeDivide(a, den);
+// op 3: out = SafeDivide(dw - a * dm, den) with a = NC (the NDC's last lines).
+template <typename T>
+__device__ __forceinline__ T safe_div(T a, T b) {
+    return b == T(0) ? T(0) : a / b;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_masked(int op, const T *x, const T *m, T *a, const T *den, const T *dw,
+                                                   const T *dm, T *out, int64_t n) {
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        if (op == 0) {
+            a[i] = m[i] != T(0) ? x[i] : T(0);
+        } else if (op == 1) {
+            a[i] = x[i] * m[i];
+        } else if (op == 2) {
+            a[i] = safe_div(a[i], den[i]);
+        } else {
+            const T p = a[i] * dm[i];
+            const T d = dw[i] - p;
+            out[i] = safe_div(d, den[i]);
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t launch_masked(bool is_double, int op, const void *x, const void *m, void *a, const void *den, const void *dw,
+                         const void *dm, void *out, int64_t n, hipStream_t stream) {
+    int64_t nblk = (n + kBlock - 1) / kBlock;
+    if (nblk > 65536) nblk = 65536;
+    if (nblk < 1) nblk = 1;
+    if (is_double)
+        hipLaunchKernelGGL(k_masked<double>, dim3(static_cast<unsigned>(nblk)), dim3(kBlock), 0, stream, op,
+                           static_cast<const double *>(x), static_cast<const double *>(m), static_cast<double *>(a),
+                           static_cast<const double *>(den), static_cast<const double *>(dw),
+                           static_cast<const double *>(dm), static_cast<double *>(out), n);
+    else
+        hipLaunchKernelGGL(k_masked<float>, dim3(static_cast<unsigned>(nblk)), dim3(kBlock), 0, stream, op,
+                           static_cast<const float *>(x), static_cast<const float *>(m), static_cast<float *>(a),
+                           static_cast<const float *>(den), static_cast<const float *>(dw),
+                           static_cast<const float *>(dm), static_cast<float *>(out), n);
+    return hipGetLastError();
+}
+
 }  // namespace dipamd

@@ -137,16 +137,62 @@ def LaplaceMinusDgg(in_, sigmas=(1.0,), method="best", boundaryCondition=(), pro
                           out, fast)
 
 
+def _masked_args(x, mask, sigmas, boundaryCondition):
+    m, _ = _to_device(mask)
+    if m.device != x.device:
+        m = m.to(x.device)
+    nd = x.dim()
+    sig = [float(s) for s in _as_list(sigmas)] or [1.0]
+    sig = sig * nd if len(sig) == 1 else sig
+    if len(sig) != nd:
+        raise _lib.DipError(_lib.E_ARRAY_LENGTH, "Array parameter has the wrong number of elements")
+    bcs = _bc_codes(boundaryCondition)
+    return m, sig, (ctypes.c_double * nd)(*sig), (ctypes.c_int32 * max(1, len(bcs)))(*bcs), len(bcs)
+
+
+def NormalizedConvolution(in_, mask, sigmas=(1.0,), method="best", boundaryCondition=(), truncation=3.0, out=None):
+    """dip::NormalizedConvolution (derivative.cpp:522-542): Gauss(in x mask) / Gauss(mask),
+    0 where the denominator is 0; a boolean mask zeroes the pixels outside it."""
+    x, was_np = _to_device(in_)
+    m, sig, s_arr, b_arr, nbc = _masked_args(x, mask, sigmas, boundaryCondition)
+    y = torch.empty(x.shape, dtype=_flex_torch(x.dtype), device=x.device) if out is None else out
+    flags = _gauss_method(method, sig, [0] * x.dim())
+    a_in, a_m, a_out = image_struct(x), image_struct(m), image_struct(y)
+    with torch.cuda.device(x.device):
+        check(lib().dipamd_normalized_convolution(ctypes.byref(a_in), ctypes.byref(a_m), ctypes.byref(a_out), s_arr,
+                                                  len(sig), b_arr, nbc, float(truncation), flags,
+                                                  _stream_handle(x.device)))
+    return y.cpu().numpy() if (was_np and out is None) else y
+
+
+def NormalizedDifferentialConvolution(in_, mask, dimension=0, sigmas=(1.0,), method="best", boundaryCondition=(),
+                                      truncation=3.0, out=None):
+    """dip::NormalizedDifferentialConvolution (derivative.cpp:544-600): the first
+    derivative along `dimension` (dip order) of the masked input, normalized by the
+    mask's Gaussian."""
+    x, was_np = _to_device(in_)
+    m, sig, s_arr, b_arr, nbc = _masked_args(x, mask, sigmas, boundaryCondition)
+    y = torch.empty(x.shape, dtype=_flex_torch(x.dtype), device=x.device) if out is None else out
+    flags = _gauss_method(method, sig, [1] * x.dim())
+    a_in, a_m, a_out = image_struct(x), image_struct(m), image_struct(y)
+    with torch.cuda.device(x.device):
+        check(lib().dipamd_normalized_differential_convolution(
+            ctypes.byref(a_in), ctypes.byref(a_m), ctypes.byref(a_out), int(dimension), s_arr, len(sig), b_arr, nbc,
+            float(truncation), flags, _stream_handle(x.device)))
+    return y.cpu().numpy() if (was_np and out is None) else y
+
+
 def StructureTensor(in_, mask=None, gradientSigmas=(1.0,), tensorSigmas=(5.0,), method="best",
                     boundaryCondition=(), truncation=3.0, fast=False):
-    """dip::StructureTensor (structure_tensor.cpp:32-57) without a mask: the outer
-    product of the gradient, each element smoothed with `tensorSigmas`.  Elements on the
-    last axis, as dip stores a symmetric matrix (diagonal first, then the upper triangle
-    column by column)."""
-    if mask is not None:
-        raise NotImplementedError("StructureTensor with a mask (normalized differential convolution) is outside "
-                                  "this library")
+    """dip::StructureTensor (structure_tensor.cpp:32-57): the outer product of the
+    gradient, each element smoothed with `tensorSigmas`; with a mask the gradient is the
+    normalized differential convolution of every dim.  Elements on the last axis, as dip
+    stores a symmetric matrix (diagonal first, then the upper triangle column by
+    column)."""
     x, was_np = _to_device(in_)
+    if mask is not None:
+        return _structure_tensor_masked(x, was_np, mask, gradientSigmas, tensorSigmas, method, boundaryCondition,
+                                        truncation, fast)
     nd = x.dim()
     gs = [float(s) for s in _as_list(gradientSigmas)] or [1.0]
     ts = [float(s) for s in _as_list(tensorSigmas)] or [5.0]
@@ -171,5 +217,28 @@ def StructureTensor(in_, mask=None, gradientSigmas=(1.0,), tensorSigmas=(5.0,), 
     with torch.cuda.device(x.device):
         check(lib().dipamd_structure_tensor(ctypes.byref(a_in), imgs, n, g_arr, nd, t_arr, nd, b_arr, len(bcs),
                                             float(truncation), flags, _stream_handle(x.device)))
+    y = planes.movedim(0, -1)
+    return y.cpu().numpy() if was_np else y
+
+
+def _structure_tensor_masked(x, was_np, mask, gradientSigmas, tensorSigmas, method, boundaryCondition, truncation,
+                             fast):
+    nd = x.dim()
+    m, gs, g_arr, b_arr, nbc = _masked_args(x, mask, gradientSigmas, boundaryCondition)
+    ts = [float(s) for s in _as_list(tensorSigmas)] or [5.0]
+    ts = ts * nd if len(ts) == 1 else ts
+    if len(ts) != nd:
+        raise _lib.DipError(_lib.E_ARRAY_LENGTH, "Array parameter has the wrong number of elements")
+    flags = (_lib.FAST if fast else _lib.EXACT) | _gauss_method(method, gs, [1] * nd) | \
+        _gauss_method(method, ts, [0] * nd, tensor=True)
+    n = nd * (nd + 1) // 2
+    planes = torch.empty((n,) + tuple(x.shape), dtype=_flex_torch(x.dtype), device=x.device)
+    imgs = (_lib.DipamdImage * n)(*[image_struct(planes[i]) for i in range(n)])
+    t_arr = (ctypes.c_double * nd)(*ts)
+    a_in, a_m = image_struct(x), image_struct(m)
+    with torch.cuda.device(x.device):
+        check(lib().dipamd_structure_tensor_masked(ctypes.byref(a_in), ctypes.byref(a_m), imgs, n, g_arr, nd, t_arr,
+                                                   nd, b_arr, nbc, float(truncation), flags,
+                                                   _stream_handle(x.device)))
     y = planes.movedim(0, -1)
     return y.cpu().numpy() if was_np else y

@@ -220,6 +220,29 @@ int dipamd_structure_tensor(const dipamd_image *in, const dipamd_image *outs, in
                             const double *gradient_sigmas, int32_t ngs, const double *tensor_sigmas, int32_t nts,
                             const int32_t *bc, int32_t nbc, double truncation, uint32_t flags, void *stream);
 
+/* The masked forms.  Replaces dip::NormalizedConvolution (src/linear/derivative.cpp:522-542;
+ * include/diplib/linear.h) -- SafeDivide( Gauss( in x mask ), Gauss( mask ) ) -- and
+ * dip::NormalizedDifferentialConvolution (derivative.cpp:544-600) along `dimension`:
+ * SafeDivide( Derivative( in x mask ) - NC x Derivative( mask ), Gauss( mask ) ) with
+ * NC the normalized convolution; a binary mask (DIPAMD_DT_BIN) zeroes the pixels outside
+ * it, a real one multiplies; each image operation rounded in SuggestFlex(in), the
+ * result of that type, bit-identical to the reference.  flags select the Gaussian method
+ * (DIPAMD_GAUSS_IIR / DIPAMD_GAUSS_FT; FIR otherwise).  `out` may be `in`. */
+int dipamd_normalized_convolution(const dipamd_image *in, const dipamd_image *mask, const dipamd_image *out,
+                                  const double *sigmas, int32_t nsigmas, const int32_t *bc, int32_t nbc,
+                                  double truncation, uint32_t flags, void *stream);
+int dipamd_normalized_differential_convolution(const dipamd_image *in, const dipamd_image *mask,
+                                               const dipamd_image *out, int32_t dimension, const double *sigmas,
+                                               int32_t nsigmas, const int32_t *bc, int32_t nbc, double truncation,
+                                               uint32_t flags, void *stream);
+/* Replaces dip::StructureTensor with a mask (structure_tensor.cpp:46-56): the normalized
+ * derivative of every dim as the gradient, then as dipamd_structure_tensor; outs:
+ * ndim(ndim+1)/2 images of type SuggestFlex(in). */
+int dipamd_structure_tensor_masked(const dipamd_image *in, const dipamd_image *mask, const dipamd_image *outs,
+                                   int32_t nouts, const double *gradient_sigmas, int32_t ngs,
+                                   const double *tensor_sigmas, int32_t nts, const int32_t *bc, int32_t nbc,
+                                   double truncation, uint32_t flags, void *stream);
+
 /* Replaces dip::SeparableConvolution (include/diplib/linear.h:114-130;
  * src/linear/convolution.cpp:319-421).  nfilters: 1 or ndim.  process: NULL (all) or
  * ndim bytes.  With a complex filter in the array the computation runs in

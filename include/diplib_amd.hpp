@@ -812,15 +812,19 @@ inline void LaplaceMinusDgg(Image const &in, Image &out, FloatArray sigmas = {1.
     detail::ScalarResult(DIPAMD_FN_LAPLACE_MINUS_DGG, in, out, sigmas, method, boundaryCondition, process,
                          truncation);
 }
-/* dip::StructureTensor (analysis.h:454; structure_tensor.cpp:32-57) without a mask:
- * the symmetric-matrix elements as a tensor image (or, with an ImageArray, as
+/* dip::StructureTensor (analysis.h:454; structure_tensor.cpp:32-57), with or without a
+ * mask: the symmetric-matrix elements as a tensor image (or, with an ImageArray, as
  * separate images). */
 inline void StructureTensor(Image const &in, Image const &mask, Image &out, FloatArray gradientSigmas = {1.0},
                             FloatArray tensorSigmas = {5.0}, String const &method = "best",
                             StringArray const &boundaryCondition = {}, dfloat truncation = 3) {
-    if (mask.IsForged()) throw ParameterError("StructureTensor with a mask is not provided by this library");
     if (detail::Staged(in, out, [&](Image const &i, Image &o) {
-            StructureTensor(i, mask, o, gradientSigmas, tensorSigmas, method, boundaryCondition, truncation);
+            Image m = mask;
+            if (mask.IsForged() && mask.IsOnHost()) {  // a device copy of a host mask
+                m = Image(mask.Sizes(), 1, mask.DataType());
+                m.CopyFromHost(mask.Origin());
+            }
+            StructureTensor(i, m, o, gradientSigmas, tensorSigmas, method, boundaryCondition, truncation);
         }))
         return;
     detail::RequireScalar(in);
@@ -835,9 +839,23 @@ inline void StructureTensor(Image const &in, Image const &mask, Image &out, Floa
     out.ReForge(in.Sizes(), static_cast<uint>(n), DataType::SuggestFlex(in.DataType()),
                 Option::AcceptDataTypeChange::DO_ALLOW);
     std::vector<dipamd_image> views;
-    for (int k = 0; k < n; ++k) views.push_back(out.TensorElement(static_cast<uint>(k)).View());
     std::vector<int32_t> bc = detail::BoundaryCodes(boundaryCondition);
     dipamd_image a = src.View();
+    if (mask.IsForged()) {
+        // structure_tensor.cpp:46-52: the normalized derivative of every dim
+        const uint nd = in.Dimensionality();
+        out.ReForge(in.Sizes(), nd * (nd + 1) / 2, DataType::SuggestFlex(in.DataType()),
+                    Option::AcceptDataTypeChange::DO_ALLOW);
+        for (uint k = 0; k < out.TensorElements(); ++k) views.push_back(out.TensorElement(k).View());
+        dipamd_image m = mask.View();
+        detail::Check(dipamd_structure_tensor_masked(&a, &m, views.data(), static_cast<int32_t>(views.size()),
+                                                     gradientSigmas.data(), static_cast<int32_t>(gradientSigmas.size()),
+                                                     tensorSigmas.data(), static_cast<int32_t>(tensorSigmas.size()),
+                                                     bc.data(), static_cast<int32_t>(bc.size()), truncation,
+                                                     DIPAMD_EXACT | flags, dipamd::CurrentStream()));
+        return;
+    }
+    for (int k = 0; k < n; ++k) views.push_back(out.TensorElement(static_cast<uint>(k)).View());
     detail::Check(dipamd_structure_tensor(&a, views.data(), n, gradientSigmas.data(),
                                           static_cast<int32_t>(gradientSigmas.size()), tensorSigmas.data(),
                                           static_cast<int32_t>(tensorSigmas.size()), bc.data(),
@@ -852,6 +870,37 @@ inline void StructureTensor(Image const &in, Image const &mask, ImageArray &out,
                     truncation);
     out.clear();
     for (uint k = 0; k < t.TensorElements(); ++k) out.push_back(t.TensorElement(k));
+}
+
+/* dip::NormalizedConvolution and dip::NormalizedDifferentialConvolution
+ * (derivative.cpp:522-600) with the Gaussian methods. */
+inline void NormalizedConvolution(Image const &in, Image const &mask, Image &out, FloatArray sigmas = {1.0},
+                                  String const &method = "best", StringArray const &boundaryCondition = {},
+                                  dfloat truncation = 3) {
+    detail::RequireScalar(in);
+    if (in.IsOnHost() || mask.IsOnHost() || (out.IsForged() && out.IsOnHost()))
+        throw ParameterError("NormalizedConvolution takes device images");
+    Image src = in;
+    detail::PrepareOutput(in, out);
+    std::vector<int32_t> bc = detail::BoundaryCodes(boundaryCondition);
+    dipamd_image a = src.View(), m = mask.View(), b = out.View();
+    detail::Check(dipamd_normalized_convolution(&a, &m, &b, sigmas.data(), static_cast<int32_t>(sigmas.size()),
+                                                bc.data(), static_cast<int32_t>(bc.size()), truncation,
+                                                detail::GaussMethodFlag(method, sigmas), dipamd::CurrentStream()));
+}
+inline void NormalizedDifferentialConvolution(Image const &in, Image const &mask, Image &out, uint dimension = 0,
+                                              FloatArray sigmas = {1.0}, String const &method = "best",
+                                              StringArray const &boundaryCondition = {}, dfloat truncation = 3) {
+    detail::RequireScalar(in);
+    if (in.IsOnHost() || mask.IsOnHost() || (out.IsForged() && out.IsOnHost()))
+        throw ParameterError("NormalizedDifferentialConvolution takes device images");
+    Image src = in;
+    detail::PrepareOutput(in, out);
+    std::vector<int32_t> bc = detail::BoundaryCodes(boundaryCondition);
+    dipamd_image a = src.View(), m = mask.View(), b = out.View();
+    detail::Check(dipamd_normalized_differential_convolution(
+        &a, &m, &b, static_cast<int32_t>(dimension), sigmas.data(), static_cast<int32_t>(sigmas.size()), bc.data(),
+        static_cast<int32_t>(bc.size()), truncation, detail::GaussMethodFlag(method, sigmas), dipamd::CurrentStream()));
 }
 
 /* dip::Derivative (derivative.cpp:88-115), Gaussian methods: dip::Gauss. */

@@ -270,6 +270,49 @@ int dipref_derivative( char const* which, void const* in, int in_dtype, void* ou
    }
 }
 
+// The masked forms: "nc" dip::NormalizedConvolution, "ndc:<dim>"
+// dip::NormalizedDifferentialConvolution along <dim>, "structure:<tensor sigma>"
+// dip::StructureTensor with the mask; as dipref_derivative otherwise.
+int dipref_derivative_masked( char const* which, void const* in, int in_dtype, void const* mask, int mask_dtype,
+                              void* out, int out_dtype, int cap, int* nelem, int ndim, int64_t const* sizes,
+                              double const* sigmas, int nsigmas, char const* const* bcs, int nbc, char const* method,
+                              double truncation ) {
+   try {
+      dip::Image imgIn = Wrap( in, in_dtype, ndim, sizes );
+      dip::Image imgMask = Wrap( mask, mask_dtype, ndim, sizes );
+      dip::FloatArray s( static_cast< dip::uint >( nsigmas ));
+      for( int ii = 0; ii < nsigmas; ++ii ) { s[ static_cast< dip::uint >( ii ) ] = sigmas[ ii ]; }
+      dip::StringArray bc;
+      for( int ii = 0; ii < nbc; ++ii ) { bc.emplace_back( bcs[ ii ] ); }
+      std::string w( which );
+      dip::Image res;
+      if( w == "nc" ) { dip::NormalizedConvolution( imgIn, imgMask, res, s, method, bc, truncation ); }
+      else if( w.substr( 0, 4 ) == "ndc:" ) {
+         dip::NormalizedDifferentialConvolution( imgIn, imgMask, res, std::stoul( w.substr( 4 )), s, method, bc, truncation );
+      }
+      else if( w.substr( 0, 10 ) == "structure:" ) {
+         dip::StructureTensor( imgIn, imgMask, res, s, { std::stod( w.substr( 10 )) }, method, bc, truncation );
+      }
+      else { throw std::runtime_error( "unknown function" ); }
+      int n = static_cast< int >( res.TensorElements() );
+      *nelem = n;
+      if( n > cap ) { g_err = "capacity"; return 1; }
+      dip::uint npx = 1;
+      for( int ii = 0; ii < ndim; ++ii ) { npx *= static_cast< dip::uint >( sizes[ ii ] ); }
+      std::size_t es = out_dtype == 7 ? 8 : 4;
+      for( int ii = 0; ii < n; ++ii ) {
+         dip::Image dst = Wrap( static_cast< char* >( out ) + static_cast< std::size_t >( ii ) * npx * es, out_dtype,
+                                ndim, sizes );
+         dst.Protect();
+         dst.Copy( res[ static_cast< dip::uint >( ii ) ] );
+      }
+      return 0;
+   } catch( std::exception const& e ) {
+      g_err = e.what();
+      return 1;
+   }
+}
+
 int dipref_max_threads() { return static_cast< int >( dip::GetNumberOfThreads() ); }
 
 } // extern "C"

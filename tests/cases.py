@@ -451,3 +451,28 @@ CSEP_CASES += [
     ("csep_c128_out_c64_40x30", (30, 40), "complex128", [(csep_taps(4, 1508), "even", -1, True)], ["mirror"], None,
      "complex64"),
 ]
+
+
+# The masked forms: dip::NormalizedConvolution ("nc"), NormalizedDifferentialConvolution
+# ("ndc:<dim>") and the masked StructureTensor ("structure:<tensor sigma>").
+# (name, function, numpy shape, dtype, mask ("bin": seeded 70 % true, "f32": seeded
+# uniform [0, 1) with zeros), sigmas (dip order), boundary list)
+MASKED_CASES = []
+for fn in ("nc", "ndc:0", "ndc:1", "structure:3"):
+    t = fn.replace(":", "")
+    MASKED_CASES += [
+        (f"{t}_f32_52x40_bin", fn, (40, 52), "float32", "bin", [2.0], ["mirror"]),
+        (f"{t}_u8_45x33_f32mask", fn, (33, 45), "uint8", "f32", [1.5], ["periodic"]),
+        (f"{t}_f64_30x24_bin", fn, (24, 30), "float64", "bin", [1.0, 2.0], ["mirror"]),
+    ]
+MASKED_CASES.append(("ndc2_f32_20x16x9_bin", "ndc:2", (9, 16, 20), "float32", "bin", [1.0], ["mirror"]))
+MASKED_CASES.append(("structure2_f32_20x16x9_bin", "structure:2", (9, 16, 20), "float32", "bin", [1.0], ["mirror"]))
+
+
+def make_mask(shape, kind, seed):
+    rng = np.random.default_rng(seed)
+    if kind == "bin":
+        return rng.random(shape) < 0.7
+    m = rng.random(shape).astype(np.float32)
+    m[rng.random(shape) < 0.2] = 0.0  # zeros: SafeDivide's 0 where the mask's Gaussian vanishes locally
+    return m

@@ -101,6 +101,17 @@ def write_ft_repr_cases(fname="ft_repr_cases.npz", seed0=9800):
     np.savez_compressed(os.path.join(HERE, fname), **arrs)
 
 
+def write_masked_cases(fname="masked_cases.npz", seed0=13000):
+    arrs = {}
+    for i, (name, fn, shape, dt, mk, sig, bc) in enumerate(cases.MASKED_CASES):
+        x = cases.make_input(shape, dt, seed=seed0 + i)
+        m = cases.make_mask(shape, mk, seed=seed0 + 500 + i)
+        arrs[name + "__in"] = x
+        arrs[name + "__mask"] = m
+        arrs[name + "__out"] = oc.ref_derivative_masked(fn, x, m, sig, bc)
+    np.savez_compressed(os.path.join(HERE, fname), **arrs)
+
+
 def write_derivative_ft_cases(fname="deriv_ft_cases.npz", seed0=9900):
     arrs = {}
     for i, (name, fn, shape, dt, sig, bc, method) in enumerate(cases.DERIV_FT_CASES):
@@ -153,6 +164,7 @@ def main():
     if only in (None, "deriv"):
         write_derivative_cases()
         write_derivative_cases(cases.DERIV2_CASES, "deriv2_cases.npz", 5000)
+        write_masked_cases()
     if only is not None:
         return
     # taps

@@ -63,6 +63,9 @@ def ref():
             lib.dipref_gauss_ft_repr.argtypes = [P, I, P, I, I, P, P, I, P, I, D, ctypes.c_char_p, ctypes.c_char_p]
         if hasattr(lib, "dipref_gauss_iir"):
             lib.dipref_gauss_iir.argtypes = [P, I, P, I, I, P, P, I, P, I, P, I, P, I, ctypes.c_char_p, D, I]
+        if hasattr(lib, "dipref_derivative_masked"):
+            lib.dipref_derivative_masked.argtypes = [ctypes.c_char_p, P, I, P, I, P, I, I, ctypes.POINTER(I), I, P, P,
+                                                     I, P, I, ctypes.c_char_p, D]
         if hasattr(lib, "dipref_derivative"):
             lib.dipref_derivative.argtypes = [ctypes.c_char_p, P, I, P, I, I, ctypes.POINTER(I), I, P, P, I, P, I,
                                               ctypes.c_char_p, D]
@@ -298,6 +301,77 @@ def ref_derivative(which, x, sigmas, bc=("mirror",), method="gaussfir", truncati
     if st != 0:
         raise RuntimeError(ref().dipref_last_error().decode())
     return out[:n.value].copy()
+
+
+def ref_derivative_masked(which, x, mask, sigmas, bc=("mirror",), method="gaussfir", truncation=3.0):
+    """The reference's dip::NormalizedConvolution ("nc"), NormalizedDifferentialConvolution
+    ("ndc:<dim>") and masked StructureTensor ("structure:<tensor sigma>"), elements on a
+    new first axis."""
+    x = np.ascontiguousarray(x)
+    mask = np.ascontiguousarray(mask)
+    odt = _flex(x.dtype)
+    cap = 64
+    out = np.zeros((cap,) + x.shape, odt)
+    sizes = _sizes(x)
+    sig = np.ascontiguousarray(list(sigmas), np.float64)
+    bcl = list(bc) if not isinstance(bc, str) else [bc]
+    arr = (ctypes.c_char_p * max(1, len(bcl)))(*[b.encode() for b in bcl])
+    n = ctypes.c_int(0)
+    st = ref().dipref_derivative_masked(which.encode(), _ptr(x), DT[x.dtype], _ptr(mask), DT[mask.dtype], _ptr(out),
+                                        DT[odt], cap, ctypes.byref(n), x.ndim, _ptr(sizes), _ptr(sig), len(sig), arr,
+                                        len(bcl), method.encode(), truncation)
+    if st != 0:
+        raise RuntimeError(ref().dipref_last_error().decode())
+    return out[:n.value].copy()
+
+
+def _safe_divide(a, b):
+    """dip::SafeDivide: a / b, 0 where b is 0 (arithmetic.h)."""
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return np.where(b == 0, a.dtype.type(0), a / np.where(b == 0, b.dtype.type(1), b))
+
+
+def derivative_masked(which, x, mask, sigmas, bc=("mirror",), truncation=3.0):
+    """The oracle's restatement of the masked forms (derivative.cpp:522-600,
+    structure_tensor.cpp:46-56) on the FIR Gaussian, each image operation rounded in the
+    buffer type: the mask's Gaussian as the denominator; the input weighted by the mask
+    (a binary mask zeroes the pixels outside it, a real one multiplies); for "nc" the
+    weighted Gaussian over the denominator; for "ndc:<dim>" the derivative of the
+    weighted input minus (weighted Gaussian / denominator) x (the mask's derivative), over
+    the denominator; "structure:<t>" the tensor of the per-dim NDCs (every dim), each
+    product then Gaussian-smoothed with sigma t."""
+    x = np.asarray(x)
+    nd = x.ndim
+    dt = _flex(x.dtype)
+    sig = list(sigmas) if len(sigmas) == nd else list(sigmas) * nd
+    den = gauss_fir(mask, sig, [0], bc, truncation).astype(dt)
+    if mask.dtype == np.bool_:
+        w = np.where(mask, x.astype(dt), dt.type(0))
+    else:
+        w = (x.astype(dt) * mask.astype(dt)).astype(dt)
+    if which == "nc":
+        return _safe_divide(gauss_fir(w, sig, [0], bc, truncation).astype(dt), den)[None]
+
+    def ndc(dim):
+        nc = _safe_divide(gauss_fir(w, sig, [0], bc, truncation).astype(dt), den)
+        o = [0] * nd
+        o[dim] = 1
+        dm = gauss_fir(mask, sig, o, bc, truncation).astype(dt)
+        dw = gauss_fir(w, sig, o, bc, truncation).astype(dt)
+        return _safe_divide((dw - (nc * dm).astype(dt)).astype(dt), den)
+
+    if which.startswith("ndc:"):
+        return ndc(int(which.split(":")[1]))[None]
+    if which.startswith("structure:"):
+        tsig = float(which.split(":")[1])
+        g = [ndc(d) for d in range(nd)]
+        zero = dt.type(0)
+        el = [zero + g[c] * g[c] for c in range(nd)]
+        for c in range(1, nd):
+            for r in range(c):
+                el.append(zero + g[r] * g[c])
+        return np.stack([gauss_fir(e, [tsig], [0], bc, truncation) for e in el])
+    raise ValueError(which)
 
 
 def _ft_oracle():

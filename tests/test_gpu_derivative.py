@@ -144,3 +144,43 @@ def test_derivative_ft_vs_reference(cuda, case):
     tol = 1e-5 if exp.dtype != np.float64 else 1e-12
     err = np.max(np.abs(got.astype(np.float64) - exp)) / np.max(np.abs(exp))
     assert err <= tol, f"{name}: {err:.3e}"
+
+
+MASKED = np.load(os.path.join(HERE, "golden", "masked_cases.npz"))
+
+
+def _masked_call(fn, x, m, sig, bc):
+    if fn == "nc":
+        return dip.NormalizedConvolution(x, m, sig, "FIR", bc)[None]
+    if fn.startswith("ndc:"):
+        return dip.NormalizedDifferentialConvolution(x, m, int(fn[4:]), sig, "FIR", bc)[None]
+    y = dip.StructureTensor(x, m, sig, [float(fn.split(":")[1])], "FIR", bc)
+    return y.movedim(-1, 0)
+
+
+@pytest.mark.parametrize("case", cases.MASKED_CASES, ids=[c[0] for c in cases.MASKED_CASES])
+def test_masked_bitexact(cuda, case):
+    """dip::NormalizedConvolution, NormalizedDifferentialConvolution and the masked
+    StructureTensor against the reference's own outputs, bit for bit."""
+    name, fn, shape, dt, mk, sig, bc = case
+    x = torch.from_numpy(MASKED[name + "__in"]).to(cuda)
+    m = torch.from_numpy(MASKED[name + "__mask"]).to(cuda)
+    exp = MASKED[name + "__out"]
+    got = _masked_call(fn, x, m, sig, bc).cpu().numpy()
+    assert got.dtype == exp.dtype and got.shape == exp.shape
+    assert np.array_equal(got.view(np.uint8), exp.view(np.uint8)), \
+        f"{name}: max abs diff {np.nanmax(np.abs(got.astype(np.float64) - exp.astype(np.float64)))}"
+
+
+def test_masked_larger_vs_oracle_and_errors(cuda):
+    x = cases.make_input((300, 257), "uint8", seed=61)
+    m = cases.make_mask((300, 257), "bin", seed=62)
+    exp = oc.derivative_masked("structure:4", x, m, [2.0], ["mirror"])
+    got = dip.StructureTensor(torch.from_numpy(x).to(cuda), torch.from_numpy(m).to(cuda), [2.0], [4.0], "FIR",
+                              ["mirror"]).movedim(-1, 0).cpu().numpy()
+    assert np.array_equal(got, exp)
+    xt = torch.from_numpy(x).to(cuda)
+    with pytest.raises(dip.DipError):  # mask of other sizes
+        dip.NormalizedConvolution(xt, torch.ones((10, 10), device=cuda), [2.0])
+    with pytest.raises(dip.DipError):  # dimension out of range
+        dip.NormalizedDifferentialConvolution(xt, torch.from_numpy(m).to(cuda), 2, [2.0])

@@ -700,3 +700,34 @@ def test_oracle_derivative_ft_matches_reference(case):
     got = oc.derivative(fn, x, sig, bc, None, 3.0, m == "iir", tm == "iir", m == "ft", tm == "ft")
     assert got.shape == exp.shape and got.dtype == exp.dtype
     assert np.max(np.abs(got.astype(np.float64) - exp)) <= ft_tolerance(exp.dtype) * np.max(np.abs(exp))
+
+
+MASKED = np.load(os.path.join(HERE, "golden", "masked_cases.npz"))
+
+
+@pytest.mark.parametrize("case", cases.MASKED_CASES, ids=[c[0] for c in cases.MASKED_CASES])
+def test_oracle_masked_matches_reference(case):
+    """dip::NormalizedConvolution, NormalizedDifferentialConvolution and the masked
+    StructureTensor: the oracle's restatement (FIR Gaussians of the C oracle, image
+    operations in numpy in the buffer type) against the reference's outputs, bit for bit."""
+    name, fn, shape, dt, mk, sig, bc = case
+    x, m = MASKED[name + "__in"], MASKED[name + "__mask"]
+    exp = MASKED[name + "__out"]
+    got = oc.derivative_masked(fn, x, m, sig, bc)
+    assert got.dtype == exp.dtype and got.shape == exp.shape
+    assert np.array_equal(got.view(np.uint8), exp.view(np.uint8))
+
+
+def test_oracle_masked_properties():
+    """What the definitions fix: with a mask of ones the normalized convolution is the
+    Gaussian divided by the Gaussian of ones (1 away from the border effects of a
+    mirror condition: exactly the Gaussian there), the NDC is the derivative minus
+    0 x (the mask's derivative, 0) over 1; a mask that is zero everywhere gives 0."""
+    x = cases.make_input((30, 40), "float64", seed=7)
+    ones = np.ones(x.shape, np.float64)
+    g = oc.gauss_fir(x, [2.0], [0], ["mirror"])
+    assert np.allclose(oc.derivative_masked("nc", x, ones, [2.0], ["mirror"])[0], g, rtol=1e-12, atol=1e-12)
+    d = oc.gauss_fir(x, [2.0], [1, 0], ["mirror"])
+    assert np.allclose(oc.derivative_masked("ndc:0", x, ones, [2.0], ["mirror"])[0], d, rtol=1e-9, atol=1e-12)
+    zero = np.zeros(x.shape, bool)
+    assert not oc.derivative_masked("ndc:1", x, zero, [2.0], ["mirror"]).any()
