@@ -5,6 +5,10 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
+# Choices the tests measure (a quick autotune, batched plans on small batches) stay in
+# memory: writing them to the persistent tuning cache would let later runs of the same
+# tree use schedules measured on test-sized images.
+os.environ.setdefault("DIPAMD_TUNE_WRITE", "0")
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 

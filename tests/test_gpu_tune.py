@@ -1,6 +1,6 @@
-"""The schedule tuner (dipamd_autotune): it measures every candidate schedule and
-stores the winners in the persistent cache; whatever it chooses, results stay bit for
-bit the reference's (the schedules differ in speed only)."""
+"""The schedule tuner (dipamd_autotune): it measures every candidate schedule and keeps
+the winners (in memory here: the tests do not write the persistent cache); whatever it
+chooses, results stay bit for bit the reference's (the schedules differ in speed only)."""
 import os
 
 import numpy as np
@@ -16,12 +16,8 @@ pytestmark = pytest.mark.gpu
 
 def test_autotune_then_results_unchanged(cuda):
     print("[tune] quick autotune", flush=True)
+    assert os.environ.get("DIPAMD_TUNE_WRITE") == "0"  # the choices stay in this process
     dip.autotune(quick=True)
-    path = os.environ.get("DIPAMD_TUNE_FILE") or os.path.join(os.path.dirname(dip.__file__), "tune_cache.txt")
-    if os.access(os.path.dirname(os.path.abspath(path)), os.W_OK):
-        with open(path) as f:
-            keys = [line.split("\t")[0] for line in f if "\t" in line]
-        assert any(" fuse " in k for k in keys) and any(" ysweep " in k for k in keys)
     for shape, dt, sig, ords in (((300, 1024), "uint8", 1.0, (0, 1)), ((256, 520), "float32", 4.0, (1, 0)),
                                  ((128, 96), "float64", 2.0, (0, 2)), ((64, 130, 70), "float32", 3.0, (0, 0, 1))):
         x = cases.make_input(shape, dt, seed=101)
