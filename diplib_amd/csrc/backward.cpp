@@ -39,6 +39,14 @@ hipError_t launch_sum_doubles(const double *in, int n, double *out, hipStream_t 
 size_t bwd2d_lds_bytes(int rc);
 int bwd2d_class(int rx, int ry);
 int bwd2d_tile_columns(int rc);
+hipError_t launch_bwd2d_sum(const double *partials, int64_t nblocks, double *gs, int gs_mode, int gs_n,
+                            hipStream_t stream);
+bool bwd2s_launch_y3(int cx, int symx, int symy, const Bwd2sArgs &a, int64_t nblocks, hipStream_t stream);
+bool bwd2s_launch_y5(int cx, int symx, int symy, const Bwd2sArgs &a, int64_t nblocks, hipStream_t stream);
+bool bwd2s_launch_y7(int cx, int symx, int symy, const Bwd2sArgs &a, int64_t nblocks, hipStream_t stream);
+bool bwd2s_launch_y9(int cx, int symx, int symy, const Bwd2sArgs &a, int64_t nblocks, hipStream_t stream);
+bool bwd2s_launch_y11(int cx, int symx, int symy, const Bwd2sArgs &a, int64_t nblocks, hipStream_t stream);
+bool bwd2s_launch_y13(int cx, int symx, int symy, const Bwd2sArgs &a, int64_t nblocks, hipStream_t stream);
 
 namespace {
 
@@ -127,6 +135,135 @@ int bwd_mode(int bc) {
     }
 }
 
+// Odd radius class of the streaming backward (3..13), 0 beyond.
+int bwd2s_class(int r) {
+    if (r > 13) return 0;
+    return r <= 3 ? 3 : (r | 1);
+}
+
+int bwd2s_forced() {
+    static int on = [] {
+        const char *v = std::getenv("DIPAMD_BWD_STREAM");
+        return v ? std::atoi(v) : -1;
+    }();
+    return on;
+}
+
+// The streaming 2-D backward (k_bwd2s.hpp), under the same conditions as the fused one
+// below plus radius classes it has instances for; -100 when it does not apply.
+int streaming_backward_2d(const View &x, const View &g, const View &dx, int nd, const InternFilter &F0,
+                          const InternFilter &F1, const InternFilter &dF0, const InternFilter &dF1, int bc0, int bc1,
+                          double *gs, int gs_mode, int gs_n, hipStream_t stream) {
+    const int forced = bwd2s_forced();
+    if (forced == 0) return -100;
+    if (x.dt != DIPAMD_DT_SFLOAT || g.dt != DIPAMD_DT_SFLOAT) return -100;
+    const bool want_dx = dx.p != nullptr;
+    if (x.strides[0] != 1 || g.strides[0] != 1 || (want_dx && dx.strides[0] != 1)) return -100;
+    const InternFilter *Fs[2] = {&F0, &F1};
+    const int bcs[2] = {bc0, bc1};
+    for (int d = 0; d < 2; ++d) {
+        const InternFilter &f = *Fs[d];
+        if ((f.sym != DIPAMD_SYM_EVEN && f.sym != DIPAMD_SYM_ODD) || f.shift != 0) return -100;
+        if (bwd_mode(bcs[d]) < 0) return -100;
+        if (x.sizes[d] < 2 * f.border + 1 || x.sizes[d] >= (int64_t(1) << 30)) return -100;
+    }
+    const int cx = bwd2s_class(F0.border), cy = bwd2s_class(F1.border);
+    if (cx == 0 || cx != cy) return -100;  // equal classes (others measured faster tiled)
+    if (static_cast<int>(F0.data.size()) > kB2sCap || static_cast<int>(F1.data.size()) > kB2sCap) return -100;
+    if (nd - 2 > kMaxOuter) return -100;
+    // narrow planes keep the tiled kernel unless asked otherwise (a strip is 64 columns)
+    if (forced < 0 && x.sizes[0] < 256) return -100;
+    auto plane_bytes = [&](const View &v) -> int64_t {
+        if (v.strides[1] <= 0) return -1;
+        return ((x.sizes[1] - 1) * v.strides[1] + x.sizes[0]) * static_cast<int64_t>(sizeof(float));
+    };
+    const int64_t kMaxPlane = (int64_t(1) << 31) - 256;
+    const int64_t xb = plane_bytes(x), gb = plane_bytes(g), db = want_dx ? plane_bytes(dx) : 0;
+    if (xb < 0 || gb < 0 || db < 0 || xb > kMaxPlane || gb > kMaxPlane || db > kMaxPlane) return -100;
+    Bwd2sArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.x = static_cast<const float *>(x.p);
+    a.g = static_cast<const float *>(g.p);
+    a.dx = want_dx ? static_cast<float *>(dx.p) : nullptr;
+    a.want_dx = want_dx ? 1 : 0;
+    a.xbytes = static_cast<int>(xb);
+    a.gbytes = static_cast<int>(gb);
+    a.dxbytes = static_cast<int>(db);
+    a.nx = static_cast<int>(x.sizes[0]);
+    a.ny = static_cast<int>(x.sizes[1]);
+    a.syx = x.strides[1];
+    a.syg = g.strides[1];
+    a.sydx = want_dx ? dx.strides[1] : 0;
+    a.rx = F0.border;
+    a.ry = F1.border;
+    a.bcx = bc0;
+    a.bcy = bc1;
+    a.modex = bwd_mode(bc0);
+    a.modey = bwd_mode(bc1);
+    int64_t nplanes = 1;
+    a.outer.n = 0;
+    for (int d = 2; d < nd; ++d) {
+        if (x.sizes[d] <= 1) continue;
+        const int k = a.outer.n++;
+        a.outer.size[k] = x.sizes[d];
+        a.outer.sin[k] = x.strides[d];
+        a.outer.sout[k] = want_dx ? dx.strides[d] : 0;
+        a.gplane[k] = g.strides[d];
+        nplanes *= x.sizes[d];
+    }
+    // adjoint taps (odd filters negated); entries beyond the radius stay zero
+    auto put = [](float *dst, const InternFilter &f) {
+        for (size_t k = 0; k < f.data.size(); ++k)
+            dst[k] = static_cast<float>((f.sym == DIPAMD_SYM_ODD && k > 0) ? -f.data[k] : f.data[k]);
+    };
+    put(a.txa, F0);
+    put(a.tdxa, dF0);
+    put(a.tya, F1);
+    put(a.tdya, dF1);
+    a.nstrips = (a.nx + 63) / 64;
+    // chunks: about 4096 waves (one per slot of the card at three waves per SIMD, and a
+    // third more: longer chunks stream fewer warm-up rows per output row, measured best
+    // on 32 x 1024^2), at least 64 rows (and more than ry, so the rows folded onto stay
+    // in the chunk that folds them); the last chunk takes the remainder
+    const int64_t cols = static_cast<int64_t>(a.nstrips) * nplanes;
+    int64_t chunk = (static_cast<int64_t>(a.ny) * cols + 4095) / 4096;
+    static const int force_chunk = [] {
+        const char *v = std::getenv("DIPAMD_BWD_STREAM_CHUNK");
+        return v ? std::atoi(v) : 0;
+    }();
+    if (force_chunk > 0) chunk = force_chunk;
+    chunk = std::max<int64_t>(chunk, std::max(64, a.ry + 1));
+    chunk = std::min<int64_t>(chunk, a.ny);
+    a.chunk = static_cast<int>(chunk);
+    a.nchunks = static_cast<int>(std::max<int64_t>(1, a.ny / chunk));
+    a.nwaves = cols * a.nchunks;
+    const int64_t nblocks = (a.nwaves + 3) / 4;
+    if (nblocks >= (int64_t(1) << 31)) return -100;
+    double *partials = nullptr;
+    hipError_t e = scratch_alloc(reinterpret_cast<void **>(&partials), static_cast<size_t>(2 * nblocks) * sizeof(double),
+                                 stream);
+    if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("scratch allocation: ") + hipGetErrorString(e));
+    a.partials = partials;
+    bool ok = false;
+    {
+        TraceScope ts("bwd2s", static_cast<int64_t>(a.nx) * a.ny * nplanes * (want_dx ? 12 : 8), stream);
+        switch (cy) {
+            case 3: ok = bwd2s_launch_y3(cx, F0.sym, F1.sym, a, nblocks, stream); break;
+            case 5: ok = bwd2s_launch_y5(cx, F0.sym, F1.sym, a, nblocks, stream); break;
+            case 7: ok = bwd2s_launch_y7(cx, F0.sym, F1.sym, a, nblocks, stream); break;
+            case 9: ok = bwd2s_launch_y9(cx, F0.sym, F1.sym, a, nblocks, stream); break;
+            case 11: ok = bwd2s_launch_y11(cx, F0.sym, F1.sym, a, nblocks, stream); break;
+            case 13: ok = bwd2s_launch_y13(cx, F0.sym, F1.sym, a, nblocks, stream); break;
+        }
+        e = ok ? hipGetLastError() : hipSuccess;
+    }
+    if (ok && e == hipSuccess && gs) e = launch_bwd2d_sum(partials, nblocks, gs, gs_mode, gs_n, stream);
+    scratch_free(partials, stream);
+    if (!ok) return -100;
+    if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("backward launch: ") + hipGetErrorString(e));
+    return DIPAMD_OK;
+}
+
 // The fused 2-D backward (k_bwd2d.hip) for the passes along dims 0 and 1, on views:
 // x (the forward input), g (dL/d(output of those two passes)), dx (dL/dx, or p == null
 // for none), in float with unit x strides, symmetric filters of radius <= 16, no line
@@ -138,6 +275,10 @@ int bwd_mode(int bc) {
 int fused_backward_2d(const View &x, const View &g, const View &dx, int nd, const InternFilter &F0,
                       const InternFilter &F1, const InternFilter &dF0, const InternFilter &dF1, int bc0, int bc1,
                       double *gs, int gs_mode, int gs_n, hipStream_t stream) {
+    {
+        const int st = streaming_backward_2d(x, g, dx, nd, F0, F1, dF0, dF1, bc0, bc1, gs, gs_mode, gs_n, stream);
+        if (st != -100) return st;
+    }
     if (!bwd_fused_enabled()) return -100;
     if (x.dt != DIPAMD_DT_SFLOAT || g.dt != DIPAMD_DT_SFLOAT) return -100;
     const bool want_dx = dx.p != nullptr;

@@ -41,6 +41,13 @@ __global__ __launch_bounds__(kBlock) void k_bwd2d_sum(const double *partials, in
         for (int i = 2 + static_cast<int>(threadIdx.x); i < nout; i += kBlock) out[i] = 0.0;
 }
 
+// The fixed-order sum of per-block partials (for the streaming backward, k_bwd2s).
+hipError_t launch_bwd2d_sum(const double *partials, int64_t nblocks, double *gs, int gs_mode, int gs_n,
+                            hipStream_t stream) {
+    hipLaunchKernelGGL(k_bwd2d_sum, dim3(1), dim3(kBlock), 0, stream, partials, nblocks, gs, gs_mode, gs_n);
+    return hipGetLastError();
+}
+
 namespace {
 
 constexpr int kBwdClasses[] = {3, 5, 7, 9, 11, 13, 16};  // 20 would exceed 64 KB of LDS (67200 B)

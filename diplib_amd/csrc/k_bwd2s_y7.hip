@@ -1,0 +1,10 @@
+// Streaming backward instances with y radius class 7.
+#include "k_bwd2s_inst.hpp"
+
+namespace dipamd {
+
+bool bwd2s_launch_y7(int cx, int symx, int symy, const Bwd2sArgs &a, int64_t nblocks, hipStream_t stream) {
+    return bwd2s_launch_y<7>(cx, symx, symy, a, nblocks, stream);
+}
+
+}  // namespace dipamd

@@ -291,4 +291,28 @@ constexpr int kBwdZero = 0, kBwdPeriodic = 1, kBwdFold = 2;
 constexpr int kBwdStep = 16;   // rows per step
 constexpr int kBwdSumAdd = 0, kBwdSumWrite = 1, kBwdSumTotal = 2;  // how dL/dsigma reaches its output
 
+// Arguments of the streaming backward kernel (k_bwd2s.hpp).
+constexpr int kB2sCap = 16;  // taps per filter (radius <= 15)
+
+struct Bwd2sArgs {
+    const float *x;
+    const float *g;
+    float *dx;
+    double *partials;  // two per workgroup: dL/dsigma_x, dL/dsigma_y
+    int nx, ny;
+    int64_t syx, syg, sydx;  // row strides (samples)
+    int rx, ry;
+    int bcx, bcy;
+    int modex, modey;  // kBwdZero, kBwdPeriodic, kBwdFold
+    int nstrips, nchunks, chunk;  // rows per chunk; the last chunk runs to ny
+    int64_t nwaves;
+    int want_dx;
+    int xbytes, gbytes, dxbytes;  // byte extent of one plane of x, g, dx (each < 2^31)
+    Outer outer;                  // sin: x, sout: dx
+    int64_t gplane[kMaxOuter];
+    float txa[kB2sCap], tdxa[kB2sCap];  // adjoint x filter and adjoint x derivative
+    float tya[kB2sCap], tdya[kB2sCap];  // adjoint y filter and adjoint y derivative
+};
+
+
 }  // namespace dipamd
