@@ -12,10 +12,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
-#include <atomic>
 #include <cstdio>
 #include <mutex>
 #include <string>
@@ -28,6 +28,7 @@
 #include "sep_config.hpp"
 #include "sep_kernels.hpp"
 #include "fused_rc.hpp"
+#include "fused3d.hpp"
 
 namespace dipamd {
 
@@ -531,6 +532,176 @@ static int run_fused(const View &src, const View &dst, int ndim, int dx, int dy,
     return DIPAMD_OK;
 }
 
+// The fused x+y+z pass (fused3d.hpp) over dims d0 (contiguous), d1, d2 of a float
+// source into dst; -100 when its conditions do not hold (the caller then plans the
+// passes separately).
+bool fused3d_launch_exact_x4(int, int, int, const Fused3Args &, int64_t, size_t, hipStream_t, int *);
+bool fused3d_launch_exact_x8(int, int, int, const Fused3Args &, int64_t, size_t, hipStream_t, int *);
+bool fused3d_launch_exact_x12(int, int, int, const Fused3Args &, int64_t, size_t, hipStream_t, int *);
+bool fused3d_launch_fast_x4(int, int, int, const Fused3Args &, int64_t, size_t, hipStream_t, int *);
+bool fused3d_launch_fast_x8(int, int, int, const Fused3Args &, int64_t, size_t, hipStream_t, int *);
+bool fused3d_launch_fast_x12(int, int, int, const Fused3Args &, int64_t, size_t, hipStream_t, int *);
+
+static bool fused3d_enabled() {
+    static bool on = [] {
+        const char *v = std::getenv("DIPAMD_FUSE3D");
+        return !(v && v[0] == '0');
+    }();
+    return on;
+}
+
+// Whether the fused 3-D pass takes dims d0, d1, d2 of `src` (float, d0 contiguous).
+static bool fused3d_applies(const View &src, int ndim, int d0, int d1, int d2, const InternFilter &Fx,
+                            const InternFilter &Fy, const InternFilter &Fz, const int *bc, bool fast) {
+    if (!fused3d_enabled() || src.dt != DIPAMD_DT_SFLOAT || src.strides[d0] != 1) return false;
+    for (const InternFilter *f : {&Fx, &Fy, &Fz})
+        if ((f->sym != DIPAMD_SYM_EVEN && f->sym != DIPAMD_SYM_ODD) || f->shift != 0 ||
+            static_cast<int>(f->data.size()) != f->border + 1)
+            return false;
+    if (Fz.sym != DIPAMD_SYM_EVEN) return false;
+    if (Fx.border < 1 || Fx.border > 12 || Fy.border < 1 || Fy.border > kF3MaxRy || Fz.border < 1 || Fz.border > 4)
+        return false;
+    // index-based conditions only: every sample outside is a sample inside, maybe negated
+    for (int d : {d0, d1, d2})
+        if (bc[d] != DIPAMD_BC_SYMMETRIC_MIRROR && bc[d] != DIPAMD_BC_ASYMMETRIC_MIRROR &&
+            bc[d] != DIPAMD_BC_PERIODIC && bc[d] != DIPAMD_BC_ASYMMETRIC_PERIODIC &&
+            bc[d] != DIPAMD_BC_ZERO_ORDER_EXTRAPOLATE)
+            return false;
+    // The fused pass moves about 8 bytes of HBM per sample against 16 for the separate
+    // passes, but is VALU-bound: past about 56 operations per sample (the x stage also
+    // filters the 2 ry halo rows of each 48-row tile) the separate passes are faster.
+    const double ops = filter_ops(Fx, fast) * (kF3TY + 2.0 * Fy.border) / kF3TY + filter_ops(Fy, fast) +
+                       filter_ops(Fz, fast);
+    if (ops > 56.0) return false;
+    // a tile of 64 x 48 mostly inside and enough planes to stream
+    if (src.sizes[d0] < 64 || src.sizes[d1] < 32 || src.sizes[d2] < 8) return false;
+    // in-plane offsets of the staged samples are kept as int, two bits of flags below them
+    if (src.strides[d1] <= 0 || src.strides[d1] * src.sizes[d1] >= (int64_t(1) << 29)) return false;
+    for (int d : {d0, d1, d2})
+        if (src.sizes[d] >= (int64_t(1) << 30)) return false;
+    return f3_lds_bytes(std::max(4, rc_class(Fx.border)), kF3MaxPlanes) <= lds_limit_bytes();
+}
+
+static int run_fused3d(const View &src, const View &dst, int ndim, int d0, int d1, int d2, const InternFilter &Fx,
+                       const InternFilter &Fy, const InternFilter &Fz, const int *bc, bool fast, hipStream_t stream) {
+    Fused3Args a;
+    std::memset(&a, 0, sizeof(a));
+    a.in = src.p;
+    a.out = static_cast<float *>(dst.p);
+    a.nx = static_cast<int>(src.sizes[d0]);
+    a.ny = static_cast<int>(src.sizes[d1]);
+    a.nz = static_cast<int>(src.sizes[d2]);
+    a.syin = src.strides[d1];
+    a.szin = src.strides[d2];
+    a.sxout = dst.strides[d0];
+    a.syout = dst.strides[d1];
+    a.szout = dst.strides[d2];
+    a.rx = Fx.border;
+    a.ry = Fy.border;
+    a.bcx = bc[d0];
+    a.bcy = bc[d1];
+    a.bcz = bc[d2];
+    for (size_t k = 0; k < Fx.data.size(); ++k) a.tx[k] = static_cast<float>(Fx.data[k]);
+    for (size_t k = 0; k < Fy.data.size(); ++k) a.ty[k] = static_cast<float>(Fy.data[k]);
+    for (size_t k = 0; k < Fz.data.size(); ++k) a.tz[k] = static_cast<float>(Fz.data[k]);
+    // dims beyond the three
+    int dims[DIPAMD_MAX_DIMS];
+    int n = 0;
+    int64_t nouter = 1;
+    for (int d = 0; d < ndim; ++d)
+        if (d != d0 && d != d1 && d != d2 && src.sizes[d] > 1) dims[n++] = d;
+    a.outer.n = n;
+    for (int i = 0; i < kMaxOuter; ++i) {
+        a.outer.size[i] = 1;
+        a.outer.sin[i] = 0;
+        a.outer.sout[i] = 0;
+    }
+    for (int i = 0; i < n; ++i) {
+        a.outer.size[i] = src.sizes[dims[i]];
+        a.outer.sin[i] = src.strides[dims[i]];
+        a.outer.sout[i] = dst.strides[dims[i]];
+        nouter *= src.sizes[dims[i]];
+    }
+    bool aligned = reinterpret_cast<uintptr_t>(src.p) % 16 == 0;
+    for (int d = 0; d < ndim; ++d)
+        if (d != d0 && src.sizes[d] > 1 && (src.strides[d] * 4) % 16 != 0) aligned = false;
+    a.vec_ok = aligned ? 1 : 0;
+    a.ntx = (a.nx + kF3TX - 1) / kF3TX;
+    a.nty = (a.ny + kF3TY - 1) / kF3TY;
+    const int64_t tiles = static_cast<int64_t>(a.ntx) * a.nty * nouter;
+    const int rxc = std::max(4, rc_class(Fx.border));
+    const int rz = Fz.border;
+    auto launch = [&](int64_t nblocks, size_t lds, int *occ) {
+        switch (rxc) {
+            case 4: return fast ? fused3d_launch_fast_x4(Fx.sym, Fy.sym, rz, a, nblocks, lds, stream, occ)
+                                : fused3d_launch_exact_x4(Fx.sym, Fy.sym, rz, a, nblocks, lds, stream, occ);
+            case 8: return fast ? fused3d_launch_fast_x8(Fx.sym, Fy.sym, rz, a, nblocks, lds, stream, occ)
+                                : fused3d_launch_exact_x8(Fx.sym, Fy.sym, rz, a, nblocks, lds, stream, occ);
+            case 12: return fast ? fused3d_launch_fast_x12(Fx.sym, Fy.sym, rz, a, nblocks, lds, stream, occ)
+                                 : fused3d_launch_exact_x12(Fx.sym, Fy.sym, rz, a, nblocks, lds, stream, occ);
+        }
+        return false;
+    };
+    // Resident workgroups on the device (per instance: x class, symmetries, z radius,
+    // arithmetic), at the largest plane map
+    static std::atomic<int> occ_cache[2][3][4][4];
+    std::atomic<int> &oc = occ_cache[fast ? 1 : 0][rxc / 4 - 1][(Fx.sym == DIPAMD_SYM_ODD) * 2 + (Fy.sym == DIPAMD_SYM_ODD)][rz - 1];
+    int per_cu = oc.load(std::memory_order_relaxed);
+    if (per_cu <= 0) {
+        if (!launch(0, f3_lds_bytes(rxc, kF3MaxPlanes), &per_cu)) return -100;
+        oc.store(per_cu, std::memory_order_relaxed);
+    }
+    int dev = 0, ncu = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    const int64_t slots = static_cast<int64_t>(per_cu) * std::max(ncu, 1);
+    // Plane chunks: a chunk streams 2 rz planes beyond its outputs (and its set-up costs
+    // about two more), and a launch of a little more than a whole number of resident
+    // rounds leaves most of the device idle in its last round.  The count of chunks
+    // minimising (planes per workgroup) x (rounds) is taken: whole rounds up to two,
+    // then fractional ones (the scheduler balances many rounds) plus half a round for
+    // the tail.
+    static const int force = [] {
+        const char *v = std::getenv("DIPAMD_FUSE3D_ZCHUNK");
+        return v ? std::atoi(v) : 0;
+    }();
+    const int zmax = kF3MaxPlanes - 2 * rz;
+    int64_t zchunk = 0;
+    if (force > 0) {
+        zchunk = force;
+    } else {
+        double best = 1e300;
+        const int64_t nmax = std::max<int64_t>(1, a.nz / 8);
+        for (int64_t nn = 1; nn <= nmax && nn <= 4096; ++nn) {
+            const int64_t ch = (a.nz + nn - 1) / nn;
+            if (ch > zmax) continue;
+            const int64_t n2 = (a.nz + ch - 1) / ch;
+            if (n2 != nn) continue;  // the same chunking as a smaller count
+            const double wg = static_cast<double>(tiles * n2);
+            const double rounds = wg <= 2.0 * slots ? std::ceil(wg / slots) : wg / slots + 0.5;
+            const double cost = static_cast<double>(ch + 2 * rz + 2) * rounds;
+            if (cost < best * 0.999) {
+                best = cost;
+                zchunk = ch;
+            }
+        }
+        if (zchunk == 0) zchunk = std::min<int64_t>(a.nz, zmax);
+    }
+    zchunk = std::min<int64_t>(std::min<int64_t>(zchunk, zmax), a.nz);
+    a.zchunk = static_cast<int>(zchunk);
+    a.nzchunks = static_cast<int>((a.nz + zchunk - 1) / zchunk);
+    const int64_t nblocks = tiles * a.nzchunks;
+    if (nblocks >= (int64_t(1) << 31)) return -100;
+    const size_t lds = f3_lds_bytes(rxc, a.zchunk + 2 * rz);
+    TraceScope ts("fused3d", npx_of(src, ndim) * 8, stream,
+                  npx_of(src, ndim) * (filter_ops(Fx, fast) + filter_ops(Fy, fast) + filter_ops(Fz, fast)));
+    const bool ok = launch(nblocks, lds, nullptr);
+    if (!ok) return -100;
+    const hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(err));
+    return DIPAMD_OK;
+}
+
 // Geometry, boundary folds and the wave split of the register-streaming kernels
 // (stream2d.hpp, k_stream2d_deriv.hip) for a uint8 source plane stack; -100 when their
 // conditions on the source do not hold.  Taps, radii and outputs are left to the caller.
@@ -760,13 +931,20 @@ int separable_device(const View &in, const View &out, int ndim, const InternFilt
     struct PassPlan {
         bool fused;
         int d0, d1;
-        int plan;  // fused passes: plan_fusion's choice
+        int plan;  // fused passes: plan_fusion's choice; -3: the fused x+y+z pass (d2)
+        int d2;
     };
     // the first pass reads uint8 directly; other types are converted to the buffer type
     const int first_src_dt = (in.dt == DIPAMD_DT_UINT8 && !is_double) ? DIPAMD_DT_UINT8 : buf_dt;
     std::vector<PassPlan> passes;
     size_t i0 = 0;
-    if (!(flags & DIPAMD_NO_FUSION) && order.size() >= 2) {
+    if (!(flags & DIPAMD_NO_FUSION) && !is_double && order.size() >= 3 && in.dt == DIPAMD_DT_SFLOAT &&
+        fused3d_applies(in, ndim, order[0], order[1], order[2], *filters[order[0]], *filters[order[1]],
+                        *filters[order[2]], bc, fast)) {
+        passes.push_back({true, order[0], order[1], -3, order[2]});
+        i0 = 3;
+    }
+    if (i0 == 0 && !(flags & DIPAMD_NO_FUSION) && order.size() >= 2) {
         const InternFilter &Fx = *filters[order[0]];
         const InternFilter &Fy = *filters[order[1]];
         const bool symok = (Fx.sym == DIPAMD_SYM_EVEN || Fx.sym == DIPAMD_SYM_ODD) &&
@@ -783,12 +961,12 @@ int separable_device(const View &in, const View &out, int ndim, const InternFilt
             const int plan = plan_fusion(first_src_dt, is_double, fast, Fx.border, Fy.border, in.sizes[order[0]],
                                          in.sizes[order[1]], flags);
             if (plan >= 0) {
-                passes.push_back({true, order[0], order[1], plan});
+                passes.push_back({true, order[0], order[1], plan, -1});
                 i0 = 2;
             }
         }
     }
-    for (size_t i = i0; i < order.size(); ++i) passes.push_back({false, order[i], -1, -1});
+    for (size_t i = i0; i < order.size(); ++i) passes.push_back({false, order[i], -1, -1, -1});
 
     // Source of the first pass: the input, converted first if its type has no kernel.
     View src = in;
@@ -834,7 +1012,11 @@ int separable_device(const View &in, const View &out, int ndim, const InternFilt
             }
         }
         const PassPlan &pp = passes[k];
-        if (pp.fused) {
+        if (pp.fused && pp.plan == -3) {
+            st = run_fused3d(src, dst, ndim, pp.d0, pp.d1, pp.d2, *filters[pp.d0], *filters[pp.d1], *filters[pp.d2], bc,
+                             fast, stream);
+            if (st == -100) st = set_error(DIPAMD_E_HIP, "fused 3-D pass: no kernel instance");
+        } else if (pp.fused) {
             st = -100;
             if (pp.plan == 3 && !is_double)
                 st = run_stream2d(src, dst, ndim, pp.d0, pp.d1, *filters[pp.d0], *filters[pp.d1], bc[pp.d0],
