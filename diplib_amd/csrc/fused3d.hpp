@@ -10,16 +10,16 @@
 //     samples already in flight (register prefetch);
 //   * x stage: every staged row is x-filtered, four outputs per item, the window read
 //     with 128-bit LDS loads at compile-time offsets; rows outside the plane take the
-//     y condition's sign or constant;
+//     y condition's sign;
 //   * y stage: a thread owns one column and six rows and y-filters them from the
-//     x-filtered rows; a plane outside the volume takes the z condition's sign or
-//     constant;
+//     x-filtered rows; a plane outside the volume takes the z condition's sign;
 //   * z stage: the six values enter a register ring of W = 2 RZ + 1 planes whose
 //     slot indices are compile-time constants (the plane loop is unrolled by W); once
 //     it is full the centre plane's outputs are z-filtered and stored.
-// Every output sample is computed in the reference's order on each axis
-// (convolution.cpp:214-241) from the same float intermediates as the pass-by-pass
-// path, so results are bit-identical to it.
+// The conditions are index-based (mirrors, periodic, zero order): a sample outside is
+// a sample inside, maybe negated.  Every output sample is computed in the reference's
+// order on each axis (convolution.cpp:214-241) from the same float intermediates as
+// the pass-by-pass path, so results are bit-identical to it.
 #pragma once
 
 #include <utility>
