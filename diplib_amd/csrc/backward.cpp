@@ -168,8 +168,10 @@ int streaming_backward_2d(const View &x, const View &g, const View &dx, int nd, 
         if (x.sizes[d] < 2 * f.border + 1 || x.sizes[d] >= (int64_t(1) << 30)) return -100;
     }
     const int cx = bwd2s_class(F0.border), cy = bwd2s_class(F1.border);
-    if (cx == 0 || cx != cy) return -100;  // equal classes (others measured faster tiled)
+    // equal or adjacent classes (a first derivative along one axis)
+    if (cx == 0 || !(cx == cy || cx == cy + 2 || cy == cx + 2)) return -100;
     if (static_cast<int>(F0.data.size()) > kB2sCap || static_cast<int>(F1.data.size()) > kB2sCap) return -100;
+    if (F1.border + cy > kB2sFold) return -100;  // rows beyond the y extension table
     if (nd - 2 > kMaxOuter) return -100;
     // narrow planes keep the tiled kernel unless asked otherwise (a strip is 64 columns)
     if (forced < 0 && x.sizes[0] < 256) return -100;
@@ -220,6 +222,28 @@ int streaming_backward_2d(const View &x, const View &g, const View &dx, int nd, 
     put(a.tdxa, dF0);
     put(a.tya, F1);
     put(a.tdya, dF1);
+    // zero-order x fold: suffix sums of the adjoint taps, H[m] = sum over k = m+1..rx
+    auto suffix = [](float *dst, const InternFilter &f) {
+        const int r = static_cast<int>(f.data.size()) - 1;
+        for (int m = 0; m < r; ++m) {
+            double t = 0.0;
+            for (int k = m + 1; k <= r; ++k) t += (f.sym == DIPAMD_SYM_ODD) ? -f.data[k] : f.data[k];
+            dst[m] = static_cast<float>(t);
+        }
+    };
+    suffix(a.hzv, F0);
+    suffix(a.hzw, dF0);
+    // the y extension of the rows just outside the plane (source row and sign, or a zero
+    // row for the constant conditions)
+    for (int i = 0; i < kB2sFold; ++i) {
+        for (int side = 0; side < 2; ++side) {
+            const int64_t q = side == 0 ? -1 - i : a.ny + i;
+            int mul = 1;
+            bool is_const = false;
+            const int64_t src = bc_map(q, a.ny, bc1, &mul, &is_const);
+            a.yfold[side * kB2sFold + i] = is_const ? -1 : static_cast<int>((src << 1) | (mul < 0 ? 1 : 0));
+        }
+    }
     a.nstrips = (a.nx + 63) / 64;
     // chunks: about 4096 waves (one per slot of the card at three waves per SIMD, and a
     // third more: longer chunks stream fewer warm-up rows per output row, measured best

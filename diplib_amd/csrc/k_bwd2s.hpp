@@ -68,33 +68,6 @@ __device__ __forceinline__ void for_each_slot(std::integer_sequence<int, U...>, 
     (f(std::integral_constant<int, U>{}), ...);
 }
 
-// bc_fold of a wave-uniform coordinate, kept in scalar registers (row offsets are
-// scalar operands of the buffer instructions).
-__device__ __forceinline__ int uniform_fold(int j, int n, int bc, int *mul) {
-    int m;
-    const int s = __builtin_amdgcn_readfirstlane(bc_fold(j, n, bc, &m));
-    *mul = __builtin_amdgcn_readfirstlane(m);
-    return s;
-}
-
-// Folded x halo terms of target column c for v (adjoint taps ta) and w (tda), from
-// the staged row (column q at srow[q - cs0]).  Called only in the strips at the
-// edges of a folding condition; kept out of line so the unrolled rows stay small.  The
-// tap pointers come from the kernel: a called function has no kernel-argument segment
-// pointer of its own.
-template <int SX, int RXC>
-__device__ __noinline__ float2 b2s_xfold(const float *srow, int cs0, int c, int rx, int bcx, int nx, ctap *ta,
-                                         ctap *tda) {
-    auto src = [&](int q) { return srow[q - cs0]; };
-    float txa[RXC + 1], tdxa[RXC + 1];
-#pragma unroll
-    for (int k = 0; k <= RXC; ++k) {
-        txa[k] = ta[k];
-        tdxa[k] = tda[k];
-    }
-    return make_float2(fold_sum<SX, RXC>(txa, rx, bcx, nx, c, src), fold_sum<SX, RXC>(tdxa, rx, bcx, nx, c, src));
-}
-
 template <int P>
 struct B2sVec;
 template <>
@@ -232,6 +205,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
 #pragma unroll
             for (int p = 0; p < P; ++p) vr[u][p] = wr[u][p] = 0.f;
 
+        // a row q outside 0..ny-1 (wave-uniform): its source row under the y extension
+        // (-1: a constant zero row) and sign, from the table the host filled for rows
+        // -kB2sFold..-1 and ny..ny+kB2sFold-1 (no call, a scalar load)
+        auto yext = [&](int q, int *m) -> int {
+            const int i = __builtin_amdgcn_readfirstlane(q < 0 ? -q - 1 : kB2sFold + (q - ny));
+            const int e = a.yfold[i];
+            *m = (e & 1) ? -1 : 1;
+            return e < 0 ? -1 : (e >> 1);
+        };
         // row q of g: source row (wave-uniform) and factor; a zero row needs no load
         auto grow = [&](int q, int *src) -> float {
             if (q >= 0 && q < ny) {
@@ -240,7 +222,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
             }
             if (a.modey == kBwdPeriodic) {
                 int m;
-                *src = uniform_fold(q, ny, a.bcy, &m);
+                *src = yext(q, &m);
                 return static_cast<float>(m);
             }
             *src = 0;
@@ -253,7 +235,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
                 return 1.f;
             }
             int m;
-            const int s = uniform_fold(o, ny, a.bcy, &m);
+            const int s = yext(o, &m);
             if (s < 0) {
                 *src = 0;
                 return 0.f;
@@ -362,10 +344,35 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
                         };
                         if (c0 <= rx) side(lbase, tleft);
                         if (c0 + TC - 1 >= nx - 1 - rx) side(rbase, tright);
-                    } else {  // zero order: every halo position folds onto the end sample
-                        const float2 f = b2s_xfold<SX, RXC>(srow, cs0, c, rx, a.bcx, nx, txa, tdxa);
-                        v[0] += f.x;
-                        w[0] += f.y;
+                    } else {
+                        // zero order: every halo position folds onto the end sample; the
+                        // sum of their correlations over the zero-padded line is
+                        // sum_m g[m] H[m] at the left end (g[n-1-m] at the right), H[m] the
+                        // sum of the adjoint taps m+1..rx (the host's hzv, hzw)
+                        ctap *hv = taps_after(offsetof(Bwd2sArgs, hzv), win[0]);
+                        ctap *hw = taps_after(offsetof(Bwd2sArgs, hzw), win[0]);
+                        if (c == 0) {
+                            float sv = 0.f, sw = 0.f;
+#pragma unroll
+                            for (int m = 0; m < RXC; ++m) {
+                                const float gm = srow[m - cs0];
+                                sv = __builtin_fmaf(hv[m], gm, sv);
+                                sw = __builtin_fmaf(hw[m], gm, sw);
+                            }
+                            v[0] += sv;
+                            w[0] += sw;
+                        }
+                        if (c == nx - 1) {
+                            float sv = 0.f, sw = 0.f;
+#pragma unroll
+                            for (int m = 0; m < RXC; ++m) {
+                                const float gm = srow[nx - 1 - m - cs0];
+                                sv = __builtin_fmaf(hv[m], gm, sv);
+                                sw = __builtin_fmaf(hw[m], gm, sw);
+                            }
+                            v[0] += SX == SYM_EVEN ? sv : -sv;
+                            w[0] += SX == SYM_EVEN ? sw : -sw;
+                        }
                     }
                 }
                 if (!RP && rm != 1.f) {  // periodic wrap with a sign
@@ -461,7 +468,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
                         }
                     } else {  // a halo row: fold onto its target
                         int m;
-                        const int t = uniform_fold(o, ny, a.bcy, &m);
+                        const int t = yext(o, &m);
                         const int so = t * sydx4;
                         const bool first = o < 0 && (a.bcy != DIPAMD_BC_ZERO_ORDER_EXTRAPOLATE || o == -ry);
 #pragma unroll

@@ -292,7 +292,8 @@ constexpr int kBwdStep = 16;   // rows per step
 constexpr int kBwdSumAdd = 0, kBwdSumWrite = 1, kBwdSumTotal = 2;  // how dL/dsigma reaches its output
 
 // Arguments of the streaming backward kernel (k_bwd2s.hpp).
-constexpr int kB2sCap = 16;  // taps per filter (radius <= 15)
+constexpr int kB2sCap = 16;
+constexpr int kB2sFold = 32;  // rows of the y extension table on each side  // taps per filter (radius <= 15)
 
 struct Bwd2sArgs {
     const float *x;
@@ -311,6 +312,8 @@ struct Bwd2sArgs {
     Outer outer;                  // sin: x, sout: dx
     int64_t gplane[kMaxOuter];
     float txa[kB2sCap], tdxa[kB2sCap];  // adjoint x filter and adjoint x derivative
+    float hzv[kB2sCap], hzw[kB2sCap];   // zero-order x fold: sums of txa / tdxa over m+1..rx
+    int yfold[2 * kB2sFold];            // y extension of rows -1-i and ny+i: source << 1 | negate, -1 zero
     float tya[kB2sCap], tdya[kB2sCap];  // adjoint y filter and adjoint y derivative
 };
 

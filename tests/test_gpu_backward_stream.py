@@ -15,13 +15,33 @@ pytestmark = pytest.mark.gpu
 
 BCS = ["mirror", "asym mirror", "periodic", "asym periodic", "zero order", "add zeros"]
 # (numpy shape: planes, y, x), sigmas (x, y), orders (x, y)
-# (radius classes equal along x and y; unequal ones take the tiled kernel)
+# (radius classes equal along x and y, or adjacent: a first derivative along one axis)
 CASES = [((1, 300, 256), [3.0, 3.0], [0, 0]),
          ((2, 180, 280), [3.0, 3.5], [1, 0]),
          ((1, 150, 257), [1.0, 1.0], [1, 1]),
          ((1, 600, 320), [3.5, 3.0], [0, 1]),
          ((3, 90, 270), [4.0, 4.0], [0, 0]),
-         ((1, 1030, 260), [2.0, 2.0], [0, 0])]
+         ((1, 1030, 260), [2.0, 2.0], [0, 0]),
+         # adjacent classes: 11 / 9, 9 / 11, 5 / 3, 11 / 13
+         ((2, 200, 300), [3.0, 3.0], [1, 0]),
+         ((1, 260, 290), [3.0, 3.0], [0, 1]),
+         ((1, 120, 256), [1.0, 1.0], [1, 0]),
+         ((1, 300, 270), [3.5, 3.5], [0, 1])]
+
+
+def _kernels_of(fn):
+    """Names of the traced kernel launches of fn()."""
+    from diplib_amd import _lib
+    L = _lib.lib()
+    L.dipamd_trace_enable(1)
+    try:
+        fn()
+        torch.cuda.synchronize()
+        recs = (_lib.DipamdTraceRecord * 256)()
+        n = L.dipamd_trace_read(recs, 256)
+        return [recs[i].name.decode() for i in range(n)]
+    finally:
+        L.dipamd_trace_enable(0)
 
 
 def _run(x, w, sig, ords, bc):
@@ -39,7 +59,10 @@ def test_streaming_backward_vs_fp64(cuda, bc, case):
     g.manual_seed(11)
     x = torch.rand(shape, generator=g, device=cuda, dtype=torch.float32)
     w = torch.randn(shape, generator=g, device=cuda, dtype=torch.float32)
-    dx32, ds32 = _run(x, w, sig, ords, bc)
+    out = {}
+    names = _kernels_of(lambda: out.setdefault("r", _run(x, w, sig, ords, bc)))
+    assert "bwd2s" in names, names  # the streaming kernel took the case
+    dx32, ds32 = out["r"]
     dx64, ds64 = _run(x.double(), w.double(), sig, ords, bc)
     scale = dx64.abs().max().item()
     err = (dx32.double() - dx64).abs().max().item() / scale
