@@ -111,7 +111,7 @@ std::string group_tune_key(int in_dtype, bool fast, const int bc[2], const Group
 }
 
 // The host side of a plan: per-image descriptors, taps, and per group its work items
-// (longest-processing-time order) and launch shape.  No device memory is touched.
+// (longest-processing-time order: the longest items first) and launch shape.  No device memory is touched.
 // `cfg_of` picks each group's tile config (falling back to one whose LDS fits).
 template <typename TB>
 struct BatchTables {
@@ -233,6 +233,16 @@ int plan_tables(const dipamd_batch_image *images, int n, int in_dtype, double tr
                         {i, t, static_cast<int32_t>(z), static_cast<int32_t>(std::min<int64_t>(z + ch, im.ny))});
         }
         g.nitems = static_cast<int64_t>(T.items.size()) - g.item0;
+        // longest items first (the persistent workgroups take them in table order, so the
+        // short ones fill the gaps at the end): cost ~ (x-filtered rows + output rows) x
+        // operations per sample
+        std::stable_sort(T.items.begin() + g.item0, T.items.end(), [&](const BatchItem &A, const BatchItem &B) {
+            auto cost = [&](const BatchItem &it) {
+                const auto &im = T.ims[static_cast<size_t>(it.image)];
+                return static_cast<int64_t>(2 * (it.z1 - it.z0) + 2 * im.ry) * (1 + 3 * std::max(im.rx, im.ry));
+            };
+            return cost(A) > cost(B);
+        });
         if (T.items.size() > static_cast<size_t>(INT32_MAX))
             return set_error(DIPAMD_E_UNSUPPORTED, "batched form: too many work items");
         int occ = 1;
