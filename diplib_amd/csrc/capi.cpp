@@ -17,6 +17,7 @@
 #include <cstdint>
 #include <cstring>
 #include <cstdio>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -189,11 +190,40 @@ int64_t filter_ops(const InternFilter &F, bool fast) {
 }
 
 // ---------------------------------------------------------------------------
-// Device scratch (stream-ordered; the default pool keeps it cached).
+// Device scratch: a bump arena per (device, stream).
+//
+// hipMallocAsync / hipFreeAsync hold the calling thread until the device has drained
+// the work queued before them (measured: a malloc behind 144 ms of queued work returned
+// after 144 ms), which would make every call that needs an intermediate wait for the
+// device.  Calls on one stream run in stream order, so the scratch of one call can be
+// handed to the next call on the same stream at once: each stream gets blocks of
+// device memory, allocations bump through the last block, and when a call has freed
+// everything the arena starts over (blocks allocated while it grew are merged into one
+// at the next allocation).  Memory stays with the arena until the process ends (at
+// most kMaxArenas arenas; beyond that an idle one is released).  Under graph capture
+// the stream-ordered calls are used instead, so a captured graph owns its memory.
 // ---------------------------------------------------------------------------
 namespace {
 std::once_flag g_pool_once[64];
+
+struct ScratchArena {
+    std::vector<std::pair<char *, size_t>> blocks;  // allocations bump through the last
+    size_t used = 0;                                // bytes used in the last block
+    int outstanding = 0;                            // allocations not yet freed
+    uint64_t last_use = 0;
+};
+std::mutex g_arena_mu;
+std::map<std::pair<int, hipStream_t>, ScratchArena> g_arenas;
+uint64_t g_arena_clock = 0;
+constexpr size_t kArenaAlign = 256;
+constexpr size_t kArenaMinBlock = size_t(1) << 20;
+constexpr size_t kMaxArenas = 16;
+
+bool capturing(hipStream_t stream) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(stream, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
 }
+}  // namespace
 
 hipError_t scratch_alloc(void **p, size_t bytes, hipStream_t stream) {
     int dev = 0;
@@ -207,11 +237,69 @@ hipError_t scratch_alloc(void **p, size_t bytes, hipStream_t stream) {
             }
         });
     }
-    return hipMallocAsync(p, bytes ? bytes : 256, stream);
+    if (capturing(stream)) return hipMallocAsync(p, bytes ? bytes : 256, stream);
+    const size_t need = ((bytes ? bytes : 1) + kArenaAlign - 1) / kArenaAlign * kArenaAlign;
+    std::lock_guard<std::mutex> lk(g_arena_mu);
+    const auto key = std::make_pair(dev, stream);
+    auto it = g_arenas.find(key);
+    if (it == g_arenas.end()) {
+        if (g_arenas.size() >= kMaxArenas) {  // release the least recently used idle arena
+            auto victim = g_arenas.end();
+            for (auto jt = g_arenas.begin(); jt != g_arenas.end(); ++jt)
+                if (jt->second.outstanding == 0 && (victim == g_arenas.end() || jt->second.last_use < victim->second.last_use))
+                    victim = jt;
+            if (victim != g_arenas.end()) {
+                // its stream may be gone: wait for the device, then nothing uses the blocks
+                (void)hipSetDevice(victim->first.first);
+                (void)hipDeviceSynchronize();
+                for (auto &b : victim->second.blocks) (void)hipFreeAsync(b.first, nullptr);
+                (void)hipSetDevice(dev);
+                g_arenas.erase(victim);
+            }
+        }
+        it = g_arenas.emplace(key, ScratchArena{}).first;
+    }
+    ScratchArena &A = it->second;
+    A.last_use = ++g_arena_clock;
+    if (A.outstanding == 0 && A.blocks.size() > 1) {  // merge the blocks of the last growth
+        size_t total = 0;
+        for (auto &b : A.blocks) {
+            total += b.second;
+            (void)hipFreeAsync(b.first, stream);
+        }
+        A.blocks.clear();
+        A.used = 0;
+        void *q = nullptr;
+        const hipError_t e = hipMallocAsync(&q, total, stream);
+        if (e != hipSuccess) return e;
+        A.blocks.emplace_back(static_cast<char *>(q), total);
+    }
+    if (A.blocks.empty() || A.used + need > A.blocks.back().second) {
+        void *q = nullptr;
+        const hipError_t e = hipMallocAsync(&q, std::max(need, kArenaMinBlock), stream);
+        if (e != hipSuccess) return e;
+        A.blocks.emplace_back(static_cast<char *>(q), std::max(need, kArenaMinBlock));
+        A.used = 0;
+    }
+    *p = A.blocks.back().first + A.used;
+    A.used += need;
+    ++A.outstanding;
+    return hipSuccess;
 }
 
 void scratch_free(void *p, hipStream_t stream) {
-    if (p) (void)hipFreeAsync(p, stream);
+    if (!p) return;
+    if (capturing(stream)) {
+        (void)hipFreeAsync(p, stream);
+        return;
+    }
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(g_arena_mu);
+    auto it = g_arenas.find(std::make_pair(dev, stream));
+    if (it == g_arenas.end()) return;
+    ScratchArena &A = it->second;
+    if (A.outstanding > 0 && --A.outstanding == 0) A.used = 0;
 }
 
 // ---------------------------------------------------------------------------
