@@ -103,14 +103,66 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_fold(const PassArgs<TB> a) {
 template __global__ void k_adjoint_fold<float>(const PassArgs<float>);
 template __global__ void k_adjoint_fold<double>(const PassArgs<double>);
 
+// The same halo terms for the mirror conditions on lines of n >= 2B + 2 samples, one
+// thread per (line, side, t): there every halo position j = -t (j = n - 1 + t) folds
+// onto its own target (t, or n - 1 - t), so the 2B read-add-writes of a line touch
+// distinct samples and run in parallel (B times the threads of the kernel above, whose
+// one thread per line leaves a short line stack with few waves).  Each sum runs over
+// the same terms in the same order as above.
+template <typename TB>
+__global__ __launch_bounds__(kBlock) void k_adjoint_fold_pt(const PassArgs<TB> a) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int B = a.border;
+    if (idx >= a.nlines * 2 * B) return;
+    const int64_t line = idx % a.nlines;  // consecutive threads: consecutive lines
+    const int r = static_cast<int>(idx / a.nlines);
+    const int side = r / B, t = r % B + 1;
+    int64_t oin, oout;
+    outer_offsets(a.outer, line, &oin, &oout);
+    const TB *g = static_cast<const TB *>(a.in) + oin;
+    TB *out = static_cast<TB *>(a.out) + oout;
+    const int n = static_cast<int>(a.n);
+    const bool odd = a.shift != 0;
+    TB h = TB(0);
+    if (side == 0) {  // k = t..B reads sample k - t
+        for (int k = t; k <= B; ++k) h += a.taps[k] * g[static_cast<int64_t>(k - t) * a.sin];
+    } else {  // q = B - t..0 reads sample n - 1 - q
+        for (int q = B - t; q >= 0; --q) {
+            const TB c = a.taps[t + q];
+            h += (odd ? -c : c) * g[static_cast<int64_t>(n - 1 - q) * a.sin];
+        }
+    }
+    int mul;
+    const int m = bc_fold(side == 0 ? -t : n - 1 + t, n, a.bc, &mul);
+    TB *o = out + static_cast<int64_t>(m) * a.sout;
+    *o = mul < 0 ? *o - h : *o + h;
+}
+template __global__ void k_adjoint_fold_pt<float>(const PassArgs<float>);
+template __global__ void k_adjoint_fold_pt<double>(const PassArgs<double>);
+
 hipError_t launch_adjoint_fold(bool is_double, const void *args, int64_t nlines, hipStream_t stream) {
+    // (the fields read here precede the first one whose type depends on TB)
+    const PassArgs<float> &af = *static_cast<const PassArgs<float> *>(args);
+    const int B = af.border;
+    const bool mirror = af.bc == DIPAMD_BC_SYMMETRIC_MIRROR || af.bc == DIPAMD_BC_ASYMMETRIC_MIRROR;
+    if (mirror && B >= 1 && af.n >= 2 * static_cast<int64_t>(B) + 2) {
+        const int64_t nblk = (nlines * 2 * B + kBlock - 1) / kBlock;
+        if (nblk < (int64_t(1) << 31)) {
+            if (is_double)
+                hipLaunchKernelGGL(k_adjoint_fold_pt<double>, dim3(static_cast<unsigned>(nblk)), dim3(kBlock), 0,
+                                   stream, *static_cast<const PassArgs<double> *>(args));
+            else
+                hipLaunchKernelGGL(k_adjoint_fold_pt<float>, dim3(static_cast<unsigned>(nblk)), dim3(kBlock), 0,
+                                   stream, af);
+            return hipGetLastError();
+        }
+    }
     const int64_t nblk = (nlines + kBlock - 1) / kBlock;
     if (is_double) {
         hipLaunchKernelGGL(k_adjoint_fold<double>, dim3(static_cast<unsigned>(nblk)), dim3(kBlock), 0, stream,
                            *static_cast<const PassArgs<double> *>(args));
     } else {
-        hipLaunchKernelGGL(k_adjoint_fold<float>, dim3(static_cast<unsigned>(nblk)), dim3(kBlock), 0, stream,
-                           *static_cast<const PassArgs<float> *>(args));
+        hipLaunchKernelGGL(k_adjoint_fold<float>, dim3(static_cast<unsigned>(nblk)), dim3(kBlock), 0, stream, af);
     }
     return hipGetLastError();
 }
