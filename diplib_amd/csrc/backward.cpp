@@ -589,7 +589,12 @@ extern "C" int dipamd_gauss_fir_backward(const dipamd_image *x, const dipamd_ima
                 v1_pending = false;
             }
             if (want_ds) {
-                // dL/dsigma_d = < u_i , K'_i v_{i-1} >
+                // dL/dsigma_d = < u_i , K'_i v_{i-1} >: in one sweep where it applies
+                st = run_pass_dot(v[static_cast<size_t>(i - 1)], u[static_cast<size_t>(i)], nd, d, dF[d], bcs[d],
+                                  is_double, fast, gsig + d, stream);
+                if (st != -100 && st != DIPAMD_OK) return finish(st);
+            }
+            if (want_ds && st == -100) {
                 if (!have_w) {
                     if ((st = alloc(&w)) != DIPAMD_OK) return finish(st);
                     have_w = true;

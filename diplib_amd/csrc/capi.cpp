@@ -395,16 +395,24 @@ bool bc_value_based(int bc) {
 // ---------------------------------------------------------------------------
 // Single-axis and fused passes
 // ---------------------------------------------------------------------------
-static int64_t pick_chunk(int64_t len, int64_t base, int granule, int border, int64_t target_blocks) {
+// Bytes of a pass (input + output) that the device's caches hold whole (the 256 MB
+// last-level cache, with room to spare).
+constexpr int64_t kCachedBytes = int64_t(64) << 20;
+
+static int64_t pick_chunk(int64_t len, int64_t base, int granule, int border, int64_t target_blocks,
+                          bool cached = false) {
     int64_t want = std::max<int64_t>(1, (target_blocks + base - 1) / std::max<int64_t>(base, 1));
     int64_t chunk = (len + want - 1) / want;
     chunk = ((chunk + granule - 1) / granule) * granule;
     // the halo (2*border rows) is re-read by each chunk: keep it under ~1/8 of the chunk,
-    // unless that leaves fewer than 4 workgroups per CU (small images: then up to 1/2)
+    // unless that leaves fewer than 4 workgroups per CU (small images: then up to 1/2;
+    // images that the caches hold whole, `cached`: no limit, the re-reads hit the caches
+    // and the waves are what the pass lacks)
     // (a filter without border, e.g. a single tap, needs no halo: minc is one granule)
     int64_t minc = std::max<int64_t>(granule, ((16 * static_cast<int64_t>(border) + granule - 1) / granule) * granule);
     if (base * ((len + minc - 1) / minc) < 1024)
-        minc = std::max<int64_t>(granule, ((4 * static_cast<int64_t>(border) + granule - 1) / granule) * granule);
+        minc = cached ? granule
+                      : std::max<int64_t>(granule, ((4 * static_cast<int64_t>(border) + granule - 1) / granule) * granule);
     chunk = std::max<int64_t>(chunk, std::max<int64_t>(granule, minc));
     int64_t full = ((len + granule - 1) / granule) * granule;
     return std::min(chunk, full);
@@ -497,7 +505,8 @@ static int run_single(const View &src, const View &dst, int ndim, int d, const I
                          static_cast<int>(F.data.size()) == F.border + 1;
         // the register-window sweep gains from long chunks (no halo re-reads, long
         // streams), so it aims for fewer workgroups
-        a.chunk = pick_chunk(a.n, ntiles * nouter, 8, F.border, sweep_target_blocks(zsw ? 1024 : 2048));
+        const bool cached = npx_of(src, ndim) * static_cast<int64_t>(dt_size(src.dt) + sizeof(TB)) <= kCachedBytes;
+        a.chunk = pick_chunk(a.n, ntiles * nouter, 8, F.border, sweep_target_blocks(zsw ? 1024 : 2048), cached);
         const int64_t nchunks = (a.n + a.chunk - 1) / a.chunk;
         TraceScope ts(zsw ? "zsweep" : "sweep",
                       npx_of(src, ndim) * static_cast<int64_t>(dt_size(src.dt) + sizeof(TB)), stream,
@@ -1144,6 +1153,66 @@ int run_pass(const View &src, const View &dst, int ndim, int d, const InternFilt
              bool fast, hipStream_t stream) {
     return is_double ? run_single<double>(src, dst, ndim, d, F, bc, fast, stream)
                      : run_single<float>(src, dst, ndim, d, F, bc, fast, stream);
+}
+
+template <typename TB>
+static int run_pass_dot_t(const View &src, const View &u, int ndim, int d, const InternFilter &F, int bc, bool fast,
+                          double *out, hipStream_t stream) {
+    // a strided axis (the register sweep's lane axis is the one of smallest stride)
+    int lane = -1;
+    for (int k = 0; k < ndim; ++k) {
+        if (k == d || src.sizes[k] <= 1) continue;
+        if (lane < 0 || std::llabs(src.strides[k]) < std::llabs(src.strides[lane])) lane = k;
+    }
+    if (lane < 0 || std::llabs(src.strides[d]) <= std::llabs(src.strides[lane])) return -100;
+    if ((F.sym != DIPAMD_SYM_EVEN && F.sym != DIPAMD_SYM_ODD) || F.shift != 0 || F.border < 1 ||
+        F.border > kZSweepMaxR || static_cast<int>(F.data.size()) != F.border + 1 || bc_value_based(bc) ||
+        src.sizes[d] >= (int64_t(1) << 31))
+        return -100;
+    PassArgs<TB> a;
+    std::memset(&a, 0, sizeof(a));
+    a.in = src.p;
+    a.out = nullptr;
+    a.n = src.sizes[d];
+    a.sin = src.strides[d];
+    a.sout = u.strides[d];
+    a.border = F.border;
+    a.ntaps = static_cast<int32_t>(F.data.size());
+    a.bc = bc;
+    a.zlo = F.zpad ? -(F.border - F.zpad) : INT32_MIN;
+    a.cval = const_value<TB>(bc);
+    for (size_t k = 0; k < F.data.size(); ++k) a.taps[k] = static_cast<TB>(F.data[k]);
+    a.e = src.sizes[lane];
+    a.ein = src.strides[lane];
+    a.eout = u.strides[lane];
+    fill_outer(a.outer, src, u, ndim, d, lane);
+    const int64_t nouter = product_except(src, ndim, d, lane);
+    const int bs = a.e >= 256 ? 256 : 64;
+    const int64_t ntiles = (a.e + bs - 1) / bs;
+    const bool cached = npx_of(src, ndim) * static_cast<int64_t>(2 * sizeof(TB)) <= kCachedBytes;
+    a.chunk = pick_chunk(a.n, ntiles * nouter, 8, F.border, sweep_target_blocks(1024), cached);
+    const int64_t nchunks = (a.n + a.chunk - 1) / a.chunk;
+    const int64_t nblk = ntiles * nchunks * nouter;
+    if (nblk >= (int64_t(1) << 31)) return -100;
+    double *partials = nullptr;
+    hipError_t e = scratch_alloc(reinterpret_cast<void **>(&partials), static_cast<size_t>(nblk) * sizeof(double), stream);
+    if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("scratch allocation: ") + hipGetErrorString(e));
+    {
+        TraceScope ts("zsweep_dot", npx_of(src, ndim) * static_cast<int64_t>(2 * sizeof(TB)), stream,
+                      npx_of(src, ndim) * (filter_ops(F, fast) + 2));
+        e = launch_zsweep_dot(sizeof(TB) == 8, F.border, F.sym, fast, &a, a.e, nchunks, nouter, u.p, partials, out,
+                              stream);
+    }
+    scratch_free(partials, stream);
+    if (e != hipSuccess) return set_error(DIPAMD_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return DIPAMD_OK;
+}
+
+int run_pass_dot(const View &src, const View &u, int ndim, int d, const InternFilter &F, int bc, bool is_double,
+                 bool fast, double *out, hipStream_t stream) {
+    if (src.dt != (is_double ? DIPAMD_DT_DFLOAT : DIPAMD_DT_SFLOAT) || u.dt != src.dt) return -100;
+    return is_double ? run_pass_dot_t<double>(src, u, ndim, d, F, bc, fast, out, stream)
+                     : run_pass_dot_t<float>(src, u, ndim, d, F, bc, fast, out, stream);
 }
 
 // Element-wise conversion passes (exposed for the backward form).

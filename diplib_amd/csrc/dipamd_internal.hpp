@@ -85,6 +85,11 @@ size_t lds_limit_bytes();
 
 int run_pass(const View &src, const View &dst, int ndim, int d, const InternFilter &F, int bc, bool is_double,
              bool fast, hipStream_t stream);
+// *out = <u, pass(src)> in FP64 (the pass along dim d, no output written), for a
+// strided d and a symmetric filter of radius 1..12 with an index-based or constant
+// condition; -100 otherwise (run_pass and a dot product then).
+int run_pass_dot(const View &src, const View &u, int ndim, int d, const InternFilter &F, int bc, bool is_double,
+                 bool fast, double *out, hipStream_t stream);
 int cast_to_buffer(const View &src, const View &dst, int ndim, bool to_double, hipStream_t stream);
 int cast_from_buffer(const View &src, const View &dst, int ndim, hipStream_t stream);
 

@@ -33,6 +33,11 @@ hipError_t launch_sweep(InType ti, bool is_double, int sym, bool fast, const voi
 // Register-window sweep for symmetric radii 1..kZSweepMaxR (k_zsweep.hip).
 hipError_t launch_zsweep(InType ti, bool is_double, int r, int sym, bool fast, const void *args, int64_t e,
                          int64_t nchunks, int64_t nouter, hipStream_t stream);
+// The same sweep of a buffer-type image `args.in` (no output written) into
+// *out = <u, sweep(in)> in FP64, deterministic: one partial per workgroup in
+// `partials` (at least as many doubles as workgroups), then their fixed-order sum.
+hipError_t launch_zsweep_dot(bool is_double, int r, int sym, bool fast, const void *args, int64_t e, int64_t nchunks,
+                             int64_t nouter, const void *u, double *partials, double *out, hipStream_t stream);
 // Register-streaming fused kernel for uint8 planes (k_stream2d.hip).
 hipError_t launch_stream2d(int symx, int symy, bool fast, const void *args, hipStream_t stream);
 // The derivative family's streaming form (k_stream2d_deriv.hip); args: DerivStreamArgs.
