@@ -255,8 +255,9 @@ int streaming_backward_2d(const View &x, const View &g, const View &dx, int nd, 
         const char *v = std::getenv("DIPAMD_BWD_STREAM_CHUNK");
         return v ? std::atoi(v) : 0;
     }();
+    chunk = std::max<int64_t>(chunk, 64);
     if (force_chunk > 0) chunk = force_chunk;
-    chunk = std::max<int64_t>(chunk, std::max(64, a.ry + 1));
+    chunk = std::max<int64_t>(chunk, a.ry + 1);
     chunk = std::min<int64_t>(chunk, a.ny);
     a.chunk = static_cast<int>(chunk);
     a.nchunks = static_cast<int>(std::max<int64_t>(1, a.ny / chunk));
