@@ -119,9 +119,9 @@ struct RingSrcRc {
 
 // The tile.  INTERIOR: every staged column lies inside the plane (vector loads);
 // otherwise per-sample folded loads.
-template <typename TI, typename TB, int TXC, int P, int SX, int SY, bool FAST, int RXC, bool INTERIOR>
+template <typename TI, typename TB, int TXC, int P, int SX, int SY, bool FAST, int RXC, bool INTERIOR, typename TapP>
 __device__ __forceinline__ void fused_tile_rc_body(const Tile<TB> &t, const TI *__restrict__ src,
-                                                   TB *__restrict__ dst, TB *lds, const TB *tx, const TB *ty) {
+                                                   TB *__restrict__ dst, TB *lds, TapP tx, TapP ty) {
     constexpr int S = 16;
     static_assert(TXC / P == 16 && S * 16 == kBlock, "tile shape: 16 rows x 16 lanes of P outputs");
     static_assert(RXC % 4 == 0, "radius class: a multiple of 4");
@@ -317,9 +317,9 @@ __device__ __forceinline__ void fused_tile_rc_body(const Tile<TB> &t, const TI *
     }
 }
 
-template <typename TI, typename TB, int TXC, int P, int SX, int SY, bool FAST, int RXC>
+template <typename TI, typename TB, int TXC, int P, int SX, int SY, bool FAST, int RXC, typename TapP>
 __device__ __forceinline__ void fused_tile_rc(const Tile<TB> &t, const TI *__restrict__ src, TB *__restrict__ dst,
-                                              TB *lds, const TB *tx, const TB *ty) {
+                                              TB *lds, TapP tx, TapP ty) {
     if (t.vec_ok && t.c0 - RXC >= 0 && t.c0 + TXC + RXC <= t.nx)
         fused_tile_rc_body<TI, TB, TXC, P, SX, SY, FAST, RXC, true>(t, src, dst, lds, tx, ty);
     else
@@ -365,14 +365,16 @@ namespace dipamd {
 
 // K4R: the batched persistent form on the radius-class tile.  One launch covers the
 // work items of one radius class (x radius up to RXC): each workgroup takes the next
-// item from the group's counter, copies that image's taps to LDS and runs the tile.
+// item from the group's counter and runs the tile with that image's taps.
 // The loop ends when the counter passes the item count, so every wave exits.
 template <typename TI, typename TB, int TXC, int P, bool FAST, int RXC>
 __global__ __launch_bounds__(kBlock) void k_batched_rc(const BatchArgs<TB> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int s_item;
-    __shared__ TB s_taps[2 * kFusedCap];
     TB *lds = reinterpret_cast<TB *>(smem_raw);
+    // the taps table is written before the launch and only read here: through the
+    // constant address space its reads are scalar loads, as the plain kernel's taps
+    using CT = const __attribute__((address_space(4))) TB;
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(a.counter, 1);
         lds_barrier();
@@ -380,9 +382,8 @@ __global__ __launch_bounds__(kBlock) void k_batched_rc(const BatchArgs<TB> a) {
         if (item >= a.nitems) break;  // uniform: every wave of the workgroup leaves
         const BatchItem it = a.items[item];
         const BatchImage<TB> im = a.images[it.image];
-        const int ntx = im.rx + 1, nty = im.ry + 1;
-        for (int k = threadIdx.x; k < ntx + nty; k += blockDim.x) s_taps[k] = a.taps[im.tap_offset + k];
-        lds_barrier();  // taps visible; s_item read by all before it is rewritten
+        const int ntx = im.rx + 1;
+        CT *taps = (CT *)(a.taps + __builtin_amdgcn_readfirstlane(im.tap_offset));
         Tile<TB> t;
         t.in = im.in;
         t.out = im.out;
@@ -403,8 +404,8 @@ __global__ __launch_bounds__(kBlock) void k_batched_rc(const BatchArgs<TB> a) {
         t.cvaly = a.cvaly;
         t.vec_ok = im.vec_ok;
         fused_tile_rc<TI, TB, TXC, P, SYM_EVEN, SYM_EVEN, FAST, RXC>(t, static_cast<const TI *>(t.in), t.out, lds,
-                                                                     s_taps, s_taps + ntx);
-        lds_barrier();  // LDS (ring, taps) free before the next item
+                                                                     taps, taps + ntx);
+        lds_barrier();  // LDS (ring) free and s_item read by all before the next item
     }
 }
 

@@ -168,8 +168,10 @@ struct GColExtSrc {  // a column in global memory near its ends: boundary-extend
 // P outputs from a window source.  `shift` moves the window: sample q is read as
 // q - shift (GENERAL: the reversed origin; the symmetric forms: origin - (ntaps-1),
 // zero for a centred filter).
-template <typename TB, int P, int SYM, bool FAST, typename Src>
-__device__ __forceinline__ void fir_window(const TB *taps, int ntaps, int shift, Src &s, TB (&acc)[P]) {
+// `taps`: any pointer to the taps (generic, or the constant address space for taps that
+// are to be read with scalar loads).
+template <typename TB, int P, int SYM, bool FAST, typename Src, typename TapP>
+__device__ __forceinline__ void fir_window(TapP taps, int ntaps, int shift, Src &s, TB (&acc)[P]) {
     if constexpr (SYM == SYM_EVEN || SYM == SYM_ODD) {
         const int r = ntaps - 1;
         TB rw[P], lw[P];
