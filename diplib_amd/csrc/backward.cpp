@@ -245,23 +245,39 @@ int streaming_backward_2d(const View &x, const View &g, const View &dx, int nd, 
         }
     }
     a.nstrips = (a.nx + 63) / 64;
-    // chunks: about 4096 waves (one per slot of the card at three waves per SIMD, and a
-    // third more: longer chunks stream fewer warm-up rows per output row, measured best
-    // on 32 x 1024^2), at least 64 rows (and more than ry, so the rows folded onto stay
-    // in the chunk that folds them); the last chunk takes the remainder
+    // chunks: one round of waves (the kernel holds three waves per SIMD: 12 per CU;
+    // 32 x 1024^2 sigma 3 periodic 0.153 -> 0.118 ms against a third more waves); under
+    // an x fold the edge strips' rows cost more (the halo correlations) and their waves
+    // would end a round late, so there a third more, shorter chunks (order 1 mirror:
+    // 0.170 ms with one round, 0.164 with this).  At least 64 rows (and more than ry, so
+    // the rows folded onto stay in the chunk that folds them); the last chunk takes the
+    // remainder.  (Shorter chunks for the edge strips alone measured slower: their
+    // extra waves start a second round.)
+    static const int ncu = [] {
+        int dev = 0, n = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
+        return n > 0 ? n : 256;
+    }();
     const int64_t cols = static_cast<int64_t>(a.nstrips) * nplanes;
-    int64_t chunk = (static_cast<int64_t>(a.ny) * cols + 4095) / 4096;
     static const int force_chunk = [] {
         const char *v = std::getenv("DIPAMD_BWD_STREAM_CHUNK");
         return v ? std::atoi(v) : 0;
     }();
-    chunk = std::max<int64_t>(chunk, 64);
-    if (force_chunk > 0) chunk = force_chunk;
-    chunk = std::max<int64_t>(chunk, a.ry + 1);
-    chunk = std::min<int64_t>(chunk, a.ny);
+    const int minrows = std::max(64, a.ry + 1);
+    const int64_t slots = (a.modex == kBwdFold ? 16 : 12) * static_cast<int64_t>(ncu);
+    int64_t nchunks = std::max<int64_t>(1, slots / cols);
+    if (force_chunk > 0) nchunks = std::max<int64_t>(1, a.ny / std::max(force_chunk, a.ry + 1));
+    else if (a.ny / nchunks < minrows) nchunks = std::max<int64_t>(1, a.ny / minrows);
+    const int64_t chunk = a.ny / nchunks;
     a.chunk = static_cast<int>(chunk);
-    a.nchunks = static_cast<int>(std::max<int64_t>(1, a.ny / chunk));
-    a.nwaves = cols * a.nchunks;
+    a.nchunks = static_cast<int>(nchunks);
+    const int64_t nchunks_e = nchunks;
+    a.nchunks_e = static_cast<int>(nchunks_e);
+    a.chunk_e = static_cast<int>(a.ny / nchunks_e);
+    a.nplanes = nplanes;
+    const int ne = a.nstrips > 1 ? 2 : 1;
+    a.nwaves = (static_cast<int64_t>(ne) * nchunks_e + static_cast<int64_t>(a.nstrips - ne) * nchunks) * nplanes;
     const int64_t nblocks = (a.nwaves + 3) / 4;
     if (nblocks >= (int64_t(1) << 31)) return -100;
     double *partials = nullptr;

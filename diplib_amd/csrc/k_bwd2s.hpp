@@ -114,10 +114,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
     double dsx = 0.0, dsy = 0.0;
     if (active) {
         const int ne = a.nstrips > 1 ? 2 : 1;  // edge strips
-        const int64_t nedge = static_cast<int64_t>(ne) * a.nchunks * (a.nwaves / (static_cast<int64_t>(a.nstrips) * a.nchunks));
+        const int64_t nedge = static_cast<int64_t>(ne) * a.nchunks_e * a.nplanes;
         int strip;
         int64_t rest;
-        if (gw < nedge) {
+        const bool edge = gw < nedge;
+        if (edge) {
             strip = (gw % ne) == 0 ? 0 : a.nstrips - 1;
             rest = gw / ne;
         } else {
@@ -126,8 +127,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
             strip = 1 + static_cast<int>(j % ni);
             rest = j / ni;
         }
-        const int chunk = static_cast<int>(rest % a.nchunks);
-        const int64_t plane = rest / a.nchunks;
+        // edge strips may take shorter chunks (their rows cost more under an x fold)
+        const int nch = edge ? a.nchunks_e : a.nchunks;
+        const int chl = edge ? a.chunk_e : a.chunk;
+        const int chunk = static_cast<int>(rest % nch);
+        const int64_t plane = rest / nch;
         int64_t ox = 0, odx = 0, og = 0;
         {
             int64_t idx = plane;
@@ -191,10 +195,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
 
         // rows: outputs ostart .. oend-1 (halo rows of a folding condition included);
         // streamed rows ostart - RYC .. oend - 1 + RYC
-        const int z0 = chunk * a.chunk;
-        const int zend = chunk == a.nchunks - 1 ? ny : z0 + a.chunk;
+        const int z0 = chunk * chl;
+        const int zend = chunk == nch - 1 ? ny : z0 + chl;
         const int ostart = (yfold && chunk == 0) ? -ry : z0;
-        const int oend = (yfold && chunk == a.nchunks - 1) ? ny + ry : zend;
+        const int oend = (yfold && chunk == nch - 1) ? ny + ry : zend;
         const int E = (oend - ostart) + 2 * RYC;
         const int q0 = ostart - RYC;  // streamed row of e = 0
 

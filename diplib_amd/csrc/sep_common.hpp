@@ -306,7 +306,8 @@ struct Bwd2sArgs {
     int bcx, bcy;
     int modex, modey;  // kBwdZero, kBwdPeriodic, kBwdFold
     int nstrips, nchunks, chunk;  // rows per chunk; the last chunk runs to ny
-    int64_t nwaves;
+    int nchunks_e, chunk_e;       // the same for the edge strips
+    int64_t nplanes, nwaves;
     int want_dx;
     int xbytes, gbytes, dxbytes;  // byte extent of one plane of x, g, dx (each < 2^31)
     Outer outer;                  // sin: x, sout: dx
