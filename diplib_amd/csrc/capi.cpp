@@ -601,7 +601,16 @@ static int run_fused(const View &src, const View &dst, int ndim, int dx, int dy,
     const int64_t nouter = product_except(src, ndim, dx, dy);
     const int txc = kFusedConfigs[cfg].txc;
     const int64_t ntiles = (a.nx + txc - 1) / txc;
-    a.chunk = pick_chunk(a.ny, ntiles * nouter, kFusedStep, Fy.border, 2048);
+    // about 1024 workgroups (four resident per CU) for float sources: longer chunks than
+    // at 2048 halve the x-filtered halo rows per output row (32 x 1024^2 sfloat sigma 3:
+    // 91.8 -> 84.9 us); uint8 sources stay at 2048 (8192^2 sigma 4: 200 us against 204);
+    // DIPAMD_FUSED_BLOCKS overrides both for experiments
+    static const int fused_target = [] {
+        const char *v = std::getenv("DIPAMD_FUSED_BLOCKS");
+        return v ? std::atoi(v) : 0;
+    }();
+    const int64_t target = fused_target > 0 ? fused_target : (src.dt == DIPAMD_DT_UINT8 ? 2048 : 1024);
+    a.chunk = pick_chunk(a.ny, ntiles * nouter, kFusedStep, Fy.border, target);
     const int64_t nchunks = (a.ny + a.chunk - 1) / a.chunk;
     const InType ti = (src.dt == DIPAMD_DT_UINT8) ? IN_U8 : IN_BUF;
     TraceScope ts(Fx.sym == SYM_NONE ? "ringsweep" : "fused2d",
