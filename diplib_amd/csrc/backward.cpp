@@ -246,33 +246,40 @@ int streaming_backward_2d(const View &x, const View &g, const View &dx, int nd, 
     }
     a.nstrips = (a.nx + 63) / 64;
     // chunks: one round of waves (the kernel holds three waves per SIMD: 12 per CU;
-    // 32 x 1024^2 sigma 3 periodic 0.153 -> 0.118 ms against a third more waves); under
-    // an x fold the edge strips' rows cost more (the halo correlations) and their waves
-    // would end a round late, so there a third more, shorter chunks (order 1 mirror:
-    // 0.170 ms with one round, 0.164 with this).  At least 64 rows (and more than ry, so
-    // the rows folded onto stay in the chunk that folds them); the last chunk takes the
-    // remainder.  (Shorter chunks for the edge strips alone measured slower: their
-    // extra waves start a second round.)
+    // 32 x 1024^2 sigma 3 periodic 0.153 -> 0.118 ms against a third more waves).  Under
+    // an x fold the edge strips' rows cost about twice as much (the correlations at the
+    // halo positions), so they take chunks half as long, dispatched first, and the
+    // interior chunks are sized so that both together still make one round (mirror
+    // order 0: 0.154 -> 0.136 ms, order 1: 0.163 -> 0.142 ms).  At least 64 rows (and
+    // more than ry, so the rows folded onto stay in the chunk that folds them); the last
+    // chunk takes the remainder.  DIPAMD_BWD_STREAM_CHUNK / _ECHUNK set the interior /
+    // edge rows per chunk for experiments.
     static const int ncu = [] {
         int dev = 0, n = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
         return n > 0 ? n : 256;
     }();
-    const int64_t cols = static_cast<int64_t>(a.nstrips) * nplanes;
     static const int force_chunk = [] {
         const char *v = std::getenv("DIPAMD_BWD_STREAM_CHUNK");
         return v ? std::atoi(v) : 0;
     }();
+    static const int force_echunk = [] {
+        const char *v = std::getenv("DIPAMD_BWD_STREAM_ECHUNK");
+        return v ? std::atoi(v) : 0;
+    }();
+    const bool split_edges = a.modex == kBwdFold && a.nstrips > 2;
     const int minrows = std::max(64, a.ry + 1);
-    const int64_t slots = (a.modex == kBwdFold ? 16 : 12) * static_cast<int64_t>(ncu);
-    int64_t nchunks = std::max<int64_t>(1, slots / cols);
+    const int64_t units = static_cast<int64_t>(split_edges ? a.nstrips + 2 : a.nstrips) * nplanes;
+    int64_t nchunks = std::max<int64_t>(1, (12 * static_cast<int64_t>(ncu)) / units);
     if (force_chunk > 0) nchunks = std::max<int64_t>(1, a.ny / std::max(force_chunk, a.ry + 1));
     else if (a.ny / nchunks < minrows) nchunks = std::max<int64_t>(1, a.ny / minrows);
     const int64_t chunk = a.ny / nchunks;
     a.chunk = static_cast<int>(chunk);
     a.nchunks = static_cast<int>(nchunks);
-    const int64_t nchunks_e = nchunks;
+    int64_t nchunks_e = split_edges ? std::min<int64_t>(2 * nchunks, a.ny / (a.ry + 1)) : nchunks;
+    if (force_echunk > 0) nchunks_e = std::max<int64_t>(1, a.ny / std::max(force_echunk, a.ry + 1));
+    nchunks_e = std::max<int64_t>(1, nchunks_e);
     a.nchunks_e = static_cast<int>(nchunks_e);
     a.chunk_e = static_cast<int>(a.ny / nchunks_e);
     a.nplanes = nplanes;
