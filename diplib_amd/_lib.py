@@ -103,6 +103,8 @@ def lib():
         L.dipamd_measure_copy.argtypes = [P, I32, P, ctypes.c_int64, U32, P]
     if hasattr(L, "dipamd_gauss_fir_backward"):
         L.dipamd_gauss_fir_backward.argtypes = [IMG, IMG, IMG, P, P, I32, P, I32, P, I32, D, U32, P]
+    L.dipamd_gauss_fir_keep.argtypes = [IMG, IMG, P, I32, P, I32, P, I32, D, U32, IMG, ctypes.POINTER(I32), P]
+    L.dipamd_gauss_fir_backward_kept.argtypes = [IMG, IMG, IMG, P, P, I32, P, I32, P, I32, D, U32, IMG, P]
     _lib = L
     return L
 
@@ -122,7 +124,7 @@ EXPORTED_SYMBOLS = [
     "dipamd_structure_tensor", "dipamd_gauss_iir", "dipamd_gauss_ft", "dipamd_batch_gauss_noalloc",
     "dipamd_autotune", "dipamd_normalized_convolution", "dipamd_normalized_differential_convolution",
     "dipamd_structure_tensor_masked", "dipamd_separable_lines_count", "dipamd_separable_lines_gather",
-    "dipamd_separable_lines_scatter",
+    "dipamd_separable_lines_scatter", "dipamd_gauss_fir_keep", "dipamd_gauss_fir_backward_kept",
 ]
 TUNE_QUICK = 256
 FT_IN_FREQUENCY, FT_OUT_FREQUENCY = 256, 512

@@ -1,6 +1,8 @@
 """Differentiable dip::Gauss (FIR) on PyTorch-ROCm: a torch.autograd.Function whose
 forward is the library's exact drop-in and whose backward is the library's adjoint
-kernel path (C ABI: dipamd_gauss_fir_backward).
+kernel path (C ABI: dipamd_gauss_fir_keep, dipamd_gauss_fir_backward_kept: for images of
+three or more dims the forward keeps its intermediate after dims 0 and 1, which the
+sigma derivative of the later dims needs).
 
     y = gauss_fir_autograd(x, sigma, orders=(0,), bc=("mirror",), truncation=3.0, sigma_dims=None)
 
@@ -40,19 +42,49 @@ def _per_dim_sigmas(sigma, nd, sigma_dims, sigma_host=None):
     return s
 
 
+def _gauss_fir_keep(x, sig, orders, bc, truncation, fast):
+    """GaussFIR that also returns the intermediate after dims 0 and 1 (or None when the
+    first pass is not the fused one over them)."""
+    y = torch.empty(x.shape, dtype=_flex_torch(x.dtype), device=x.device)
+    keep = torch.empty(x.shape, dtype=_flex_torch(x.dtype), device=x.device)
+    a_in, a_out, a_keep = image_struct(x), image_struct(y), image_struct(keep)
+    s_arr = (ctypes.c_double * len(sig))(*sig)
+    o_arr = (ctypes.c_int32 * max(1, len(orders)))(*orders)
+    bcs = _bc_codes(bc)
+    b_arr = (ctypes.c_int32 * max(1, len(bcs)))(*bcs)
+    kept = ctypes.c_int32(0)
+    with torch.cuda.device(x.device):
+        check(lib().dipamd_gauss_fir_keep(ctypes.byref(a_in), ctypes.byref(a_out), s_arr, len(sig), o_arr,
+                                          len(orders), b_arr, len(bcs), float(truncation),
+                                          _lib.FAST if fast else _lib.EXACT, ctypes.byref(a_keep),
+                                          ctypes.byref(kept), _stream_handle(x.device)))
+    return y, (keep if kept.value else None)
+
+
 class _GaussFIRFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x, sigma, orders, bc, truncation, sigma_dims, fast, sigma_host):
         nd = x.dim()
         sig = _per_dim_sigmas(sigma, nd, sigma_dims, sigma_host)
-        y = GaussFIR(x, sig, list(orders), list(bc), truncation, fast=fast)
-        ctx.save_for_backward(x, sigma)
+        kept = None
+        if nd >= 3 and ctx.needs_input_grad[1]:
+            # the forward's intermediate after dims 0 and 1 serves the sigma derivative of
+            # the later dims in the backward, which would otherwise compute it again
+            y, kept = _gauss_fir_keep(x, sig, list(orders), list(bc), truncation, fast)
+        else:
+            y = GaussFIR(x, sig, list(orders), list(bc), truncation, fast=fast)
+        if kept is not None:
+            ctx.save_for_backward(x, sigma, kept)
+        else:
+            ctx.save_for_backward(x, sigma)
         ctx.cfg = (sig, list(orders), list(bc), float(truncation), sigma_dims, fast)
         return y
 
     @staticmethod
     def backward(ctx, g):
-        x, sigma = ctx.saved_tensors
+        saved = ctx.saved_tensors
+        x, sigma = saved[0], saved[1]
+        kept = saved[2] if len(saved) > 2 else None
         sig, orders, bc, truncation, sigma_dims, fast = ctx.cfg
         nd = x.dim()
         want_dx = ctx.needs_input_grad[0]
@@ -70,11 +102,12 @@ class _GaussFIRFunction(torch.autograd.Function):
         o_arr = (ctypes.c_int32 * max(1, len(orders)))(*orders)
         bcs = _bc_codes(bc)
         b_arr = (ctypes.c_int32 * max(1, len(bcs)))(*bcs)
+        a_k = image_struct(kept) if kept is not None else None
         with torch.cuda.device(x.device):
-            check(lib().dipamd_gauss_fir_backward(
+            check(lib().dipamd_gauss_fir_backward_kept(
                 ctypes.byref(a_x), ctypes.byref(a_g), ctypes.byref(a_dx),
                 ctypes.c_void_p(ds.data_ptr()) if want_ds else None, s_arr, nd, o_arr, len(orders), b_arr, len(bcs),
-                truncation, flags, _stream_handle(x.device)))
+                truncation, flags, ctypes.byref(a_k) if a_k is not None else None, _stream_handle(x.device)))
         # on sigma's own device: a CPU sigma (e.g. a CPU nn.Parameter) with a GPU image
         dsigma = ds.to(device=sigma.device, dtype=sigma.dtype).reshape(sigma.shape) if want_ds else None
         if want_dx:

@@ -435,10 +435,10 @@ int fused_backward_2d(const View &x, const View &g, const View &dx, int nd, cons
 
 using namespace dipamd;
 
-extern "C" int dipamd_gauss_fir_backward(const dipamd_image *x, const dipamd_image *grad_out,
-                                         const dipamd_image *grad_in, double *grad_sigma, const double *sigmas,
-                                         int32_t nsigmas, const int32_t *orders, int32_t norders, const int32_t *bc,
-                                         int32_t nbc, double truncation, uint32_t flags, void *stream_) {
+static int gauss_fir_backward_impl(const dipamd_image *x, const dipamd_image *grad_out, const dipamd_image *grad_in,
+                                   double *grad_sigma, const double *sigmas, int32_t nsigmas, const int32_t *orders,
+                                   int32_t norders, const int32_t *bc, int32_t nbc, double truncation, uint32_t flags,
+                                   const dipamd_image *kept, void *stream_) {
     hipStream_t stream = static_cast<hipStream_t>(stream_);
     int st;
     if ((st = check_image(x, "x")) != DIPAMD_OK) return st;
@@ -566,15 +566,20 @@ extern "C" int dipamd_gauss_fir_backward(const dipamd_image *x, const dipamd_ima
                             fast, stream);
         };
         if (want_ds && fuse01 && m > 2) {
-            if ((st = alloc(&v[2])) != DIPAMD_OK) return finish(st);
-            const InternFilter *fp[DIPAMD_MAX_DIMS];
-            bool proc[DIPAMD_MAX_DIMS];
-            for (int d = 0; d < nd; ++d) {
-                fp[d] = &F[d];
-                proc[d] = d == 0 || d == 1;
+            if (kept) {
+                v[2] = to_view(kept);  // the forward's own intermediate after dims 0 and 1
+            } else {
+                if ((st = alloc(&v[2])) != DIPAMD_OK) return finish(st);
+                const InternFilter *fp[DIPAMD_MAX_DIMS];
+                bool proc[DIPAMD_MAX_DIMS];
+                for (int d = 0; d < nd; ++d) {
+                    fp[d] = &F[d];
+                    proc[d] = d == 0 || d == 1;
+                }
+                if ((st = separable_device(x0, v[2], nd, fp, bcs, proc, fast ? DIPAMD_FAST : 0u, stream)) !=
+                    DIPAMD_OK)
+                    return finish(st);
             }
-            if ((st = separable_device(x0, v[2], nd, fp, bcs, proc, fast ? DIPAMD_FAST : 0u, stream)) != DIPAMD_OK)
-                return finish(st);
             v1_pending = true;
             for (int i = 3; i < m; ++i)
                 if ((st = make_v(i)) != DIPAMD_OK) return finish(st);
@@ -667,4 +672,31 @@ extern "C" int dipamd_gauss_fir_backward(const dipamd_image *x, const dipamd_ima
     }
     scratch_free(gsd, stream);
     return st;
+}
+
+extern "C" int dipamd_gauss_fir_backward(const dipamd_image *x, const dipamd_image *grad_out,
+                                         const dipamd_image *grad_in, double *grad_sigma, const double *sigmas,
+                                         int32_t nsigmas, const int32_t *orders, int32_t norders, const int32_t *bc,
+                                         int32_t nbc, double truncation, uint32_t flags, void *stream) {
+    return gauss_fir_backward_impl(x, grad_out, grad_in, grad_sigma, sigmas, nsigmas, orders, norders, bc, nbc,
+                                   truncation, flags, nullptr, stream);
+}
+
+extern "C" int dipamd_gauss_fir_backward_kept(const dipamd_image *x, const dipamd_image *grad_out,
+                                              const dipamd_image *grad_in, double *grad_sigma, const double *sigmas,
+                                              int32_t nsigmas, const int32_t *orders, int32_t norders,
+                                              const int32_t *bc, int32_t nbc, double truncation, uint32_t flags,
+                                              const dipamd_image *kept, void *stream) {
+    int st;
+    if (kept) {
+        if ((st = check_image(kept, "kept")) != DIPAMD_OK) return st;
+        if ((st = check_image(x, "x")) != DIPAMD_OK) return st;
+        if (kept->ndim != x->ndim) return set_error(DIPAMD_E_SIZES, "Sizes don't match");
+        for (int d = 0; d < x->ndim; ++d)
+            if (kept->sizes[d] != x->sizes[d]) return set_error(DIPAMD_E_SIZES, "Sizes don't match");
+        if (kept->dtype != suggest_flex(x->dtype))
+            return set_error(DIPAMD_E_PARAMETER, "kept must have the buffer type of x");
+    }
+    return gauss_fir_backward_impl(x, grad_out, grad_in, grad_sigma, sigmas, nsigmas, orders, norders, bc, nbc,
+                                   truncation, flags, kept, stream);
 }

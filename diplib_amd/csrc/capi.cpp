@@ -985,7 +985,9 @@ static int run_cast_out(const View &src, const View &dst, int ndim, hipStream_t 
 // dim d; `process[d]` the final process mask (already cleared for size-1 dims
 // and meaningless filters).
 int separable_device(const View &in, const View &out, int ndim, const InternFilter *const *filters,
-                     const int *bc, const bool *process, uint32_t flags, hipStream_t stream) {
+                     const int *bc, const bool *process, uint32_t flags, hipStream_t stream, const View *keep,
+                     bool *kept) {
+    if (kept) *kept = false;
     const int buf_dt = suggest_flex(in.dt);
     const bool is_double = buf_dt == DIPAMD_DT_DFLOAT;
     const bool fast = (flags & DIPAMD_FAST) != 0;
@@ -1118,6 +1120,11 @@ int separable_device(const View &in, const View &out, int ndim, const InternFilt
             }
         }
         const PassPlan &pp = passes[k];
+        if (k == 0 && keep && kept && pp.fused && pp.plan != -3 && pp.d0 == 0 && pp.d1 == 1 && m >= 2 &&
+            keep->dt == buf_dt && !views_overlap(*keep, src, ndim)) {
+            dst = *keep;  // the intermediate after dims 0 and 1, for the caller
+            *kept = true;
+        }
         if (pp.fused && pp.plan == -3) {
             st = run_fused3d(src, dst, ndim, pp.d0, pp.d1, pp.d2, *filters[pp.d0], *filters[pp.d1], *filters[pp.d2], bc,
                              fast, stream);
@@ -1324,9 +1331,9 @@ int dipamd_make_half_gaussian(double sigma, int32_t order, double truncation, in
     return static_cast<int>(f.size());
 }
 
-int dipamd_separable_convolution(const dipamd_image *in, const dipamd_image *out, const dipamd_filter *filters,
-                                 int32_t nfilters, const int32_t *bc, int32_t nbc, const uint8_t *process,
-                                 uint32_t flags, void *stream) {
+static int separable_convolution_impl(const dipamd_image *in, const dipamd_image *out, const dipamd_filter *filters,
+                                      int32_t nfilters, const int32_t *bc, int32_t nbc, const uint8_t *process,
+                                      uint32_t flags, void *stream, const View *keep, bool *kept) {
     int st;
     if ((st = check_image(in, "in")) != DIPAMD_OK) return st;
     if ((st = check_image(out, "out")) != DIPAMD_OK) return st;
@@ -1368,12 +1375,18 @@ int dipamd_separable_convolution(const dipamd_image *in, const dipamd_image *out
         if (in->sizes[d] == 1) proc[d] = false;
     }
     return separable_device(to_view(in), to_view(out), nd, per_dim, bcs, proc, flags,
-                            static_cast<hipStream_t>(stream));
+                            static_cast<hipStream_t>(stream), keep, kept);
 }
 
-int dipamd_gauss_fir(const dipamd_image *in, const dipamd_image *out, const double *sigmas, int32_t nsigmas,
-                     const int32_t *orders, int32_t norders, const int32_t *bc, int32_t nbc, double truncation,
-                     uint32_t flags, void *stream) {
+int dipamd_separable_convolution(const dipamd_image *in, const dipamd_image *out, const dipamd_filter *filters,
+                                 int32_t nfilters, const int32_t *bc, int32_t nbc, const uint8_t *process,
+                                 uint32_t flags, void *stream) {
+    return separable_convolution_impl(in, out, filters, nfilters, bc, nbc, process, flags, stream, nullptr, nullptr);
+}
+
+static int gauss_fir_impl(const dipamd_image *in, const dipamd_image *out, const double *sigmas, int32_t nsigmas,
+                          const int32_t *orders, int32_t norders, const int32_t *bc, int32_t nbc, double truncation,
+                          uint32_t flags, void *stream, const View *keep, bool *kept) {
     int st;
     if ((st = check_image(in, "in")) != DIPAMD_OK) return st;
     const int nd = in->ndim;
@@ -1402,7 +1415,35 @@ int dipamd_gauss_fir(const dipamd_image *in, const dipamd_image *out, const doub
             proc[d] = 1;
         }
     }
-    return dipamd_separable_convolution(in, out, fl.data(), nd, bc, nbc, proc, flags, stream);
+    return separable_convolution_impl(in, out, fl.data(), nd, bc, nbc, proc, flags, stream, keep, kept);
+}
+
+int dipamd_gauss_fir(const dipamd_image *in, const dipamd_image *out, const double *sigmas, int32_t nsigmas,
+                     const int32_t *orders, int32_t norders, const int32_t *bc, int32_t nbc, double truncation,
+                     uint32_t flags, void *stream) {
+    return gauss_fir_impl(in, out, sigmas, nsigmas, orders, norders, bc, nbc, truncation, flags, stream, nullptr,
+                          nullptr);
+}
+
+int dipamd_gauss_fir_keep(const dipamd_image *in, const dipamd_image *out, const double *sigmas, int32_t nsigmas,
+                          const int32_t *orders, int32_t norders, const int32_t *bc, int32_t nbc, double truncation,
+                          uint32_t flags, const dipamd_image *keep, int32_t *kept, void *stream) {
+    int st;
+    if (kept) *kept = 0;
+    if ((st = check_image(keep, "keep")) != DIPAMD_OK) return st;
+    if ((st = check_image(in, "in")) != DIPAMD_OK) return st;
+    if (keep->ndim != in->ndim) return set_error(DIPAMD_E_SIZES, "Sizes don't match");
+    for (int d = 0; d < in->ndim; ++d)
+        if (keep->sizes[d] != in->sizes[d]) return set_error(DIPAMD_E_SIZES, "Sizes don't match");
+    if (keep->dtype != suggest_flex(in->dtype))
+        return set_error(DIPAMD_E_PARAMETER, "keep must have the buffer type of the input");
+    const View kv = to_view(keep);
+    if (views_overlap(kv, to_view(in), in->ndim) || (out && out->origin && views_overlap(kv, to_view(out), in->ndim)))
+        return set_error(DIPAMD_E_PARAMETER, "keep must not overlap in or out");
+    bool k = false;
+    st = gauss_fir_impl(in, out, sigmas, nsigmas, orders, norders, bc, nbc, truncation, flags, stream, &kv, &k);
+    if (kept) *kept = k ? 1 : 0;
+    return st;
 }
 
 }  // extern "C"

@@ -93,8 +93,12 @@ int run_pass_dot(const View &src, const View &u, int ndim, int d, const InternFi
 int cast_to_buffer(const View &src, const View &dst, int ndim, bool to_double, hipStream_t stream);
 int cast_from_buffer(const View &src, const View &dst, int ndim, hipStream_t stream);
 
+// `keep` (optional, buffer type, the input's sizes): receives the intermediate image after
+// dims 0 and 1 when the first pass is the fused 2-D pass over them and a further pass
+// follows; *kept tells whether it was written.
 int separable_device(const View &in, const View &out, int ndim, const InternFilter *const *filters, const int *bc,
-                     const bool *process, uint32_t flags, hipStream_t stream);
+                     const bool *process, uint32_t flags, hipStream_t stream, const View *keep = nullptr,
+                     bool *kept = nullptr);
 // The separable convolution over a complex buffer type (complex.cpp): a complex filter
 // in the array (any_complex_filter) or a complex input.
 int separable_complex(const View &in, const View &out, int ndim, const dipamd_filter *filters, int nfilters,

@@ -346,6 +346,23 @@ int dipamd_gauss_fir_backward(const dipamd_image *x, const dipamd_image *grad_ou
                               int32_t norders, const int32_t *bc, int32_t nbc, double truncation, uint32_t flags,
                               void *stream);
 
+/* The forward with its intermediate kept for the backward (no reference counterpart; for
+ * autograd callers of images with three or more filtered dims): dipamd_gauss_fir, and when
+ * its first pass is the fused one over dims 0 and 1 and a further dim follows, that pass
+ * writes its result (x filtered along dims 0 and 1, the buffer type SuggestFlex(in), the
+ * sizes of in) into `keep` and *kept = 1; else *kept = 0 and keep is untouched.  keep
+ * must not overlap in or out. */
+int dipamd_gauss_fir_keep(const dipamd_image *in, const dipamd_image *out, const double *sigmas, int32_t nsigmas,
+                          const int32_t *orders, int32_t norders, const int32_t *bc, int32_t nbc, double truncation,
+                          uint32_t flags, const dipamd_image *keep, int32_t *kept, void *stream);
+
+/* dipamd_gauss_fir_backward taking that intermediate (NULL: computed again), which the
+ * sigma derivative of the dims after 0 and 1 needs (no reference counterpart). */
+int dipamd_gauss_fir_backward_kept(const dipamd_image *x, const dipamd_image *grad_out, const dipamd_image *grad_in,
+                                   double *grad_sigma, const double *sigmas, int32_t nsigmas,
+                                   const int32_t *orders, int32_t norders, const int32_t *bc, int32_t nbc,
+                                   double truncation, uint32_t flags, const dipamd_image *kept, void *stream);
+
 /* ---- schedule tuning (no reference counterpart) ----
  * Measures, on synthetic planes of this device, every candidate schedule of the plain
  * path per (source type uint8 / sfloat / dfloat, radius class, plane size class): one
