@@ -30,8 +30,11 @@
 // them), and their dot products take the extended x row (E_y x)[o].
 //
 // RXC and RYC are compile-time radius classes (taps beyond rx, ry are zero), so every
-// window and ring index is an immediate.  Arithmetic is
-// fused multiply-add; the gradients are checked against FP64 references.
+// window and ring index is an immediate.  The row loop makes no calls: rows outside the
+// plane take their source row and sign from a table the host fills (scalar loads), and
+// the zero-order x fold is a dot of the end samples with suffix sums of the adjoint taps
+// (also from the host).  Arithmetic is fused multiply-add; the gradients are checked
+// against FP64 references.
 #pragma once
 
 #include <utility>
