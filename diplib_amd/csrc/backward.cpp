@@ -173,8 +173,8 @@ int streaming_backward_2d(const View &x, const View &g, const View &dx, int nd, 
     if (static_cast<int>(F0.data.size()) > kB2sCap || static_cast<int>(F1.data.size()) > kB2sCap) return -100;
     if (F1.border + cy > kB2sFold) return -100;  // rows beyond the y extension table
     if (nd - 2 > kMaxOuter) return -100;
-    // narrow planes keep the tiled kernel unless asked otherwise (a strip is 64 columns)
-    if (forced < 0 && x.sizes[0] < 256) return -100;
+    // planes narrower than two strips keep the tiled kernel unless asked otherwise
+    if (forced < 0 && x.sizes[0] < 128) return -100;
     auto plane_bytes = [&](const View &v) -> int64_t {
         if (v.strides[1] <= 0) return -1;
         return ((x.sizes[1] - 1) * v.strides[1] + x.sizes[0]) * static_cast<int64_t>(sizeof(float));
@@ -269,7 +269,11 @@ int streaming_backward_2d(const View &x, const View &g, const View &dx, int nd, 
         return v ? std::atoi(v) : 0;
     }();
     const bool split_edges = a.modex == kBwdFold && a.nstrips > 2;
-    const int minrows = std::max(64, a.ry + 1);
+    // (a stack the caches hold whole may take short chunks: its halo re-reads hit the
+    // caches and the waves are what it lacks; 128^3 sigma 3: 48 -> 43 us against the
+    // tiled kernel)
+    const bool cached = static_cast<int64_t>(a.nx) * a.ny * nplanes * 12 <= (int64_t(64) << 20);
+    const int minrows = std::max(cached ? 16 : 64, a.ry + 1);
     const int64_t units = static_cast<int64_t>(split_edges ? a.nstrips + 2 : a.nstrips) * nplanes;
     int64_t nchunks = std::max<int64_t>(1, (12 * static_cast<int64_t>(ncu)) / units);
     if (force_chunk > 0) nchunks = std::max<int64_t>(1, a.ny / std::max(force_chunk, a.ry + 1));

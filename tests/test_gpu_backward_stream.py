@@ -1,4 +1,4 @@
-"""The streaming 2-D backward kernel (float32 stacks of planes at least 256 columns
+"""The streaming 2-D backward kernel (float32 stacks of planes at least 128 columns
 wide, k_bwd2s.hpp) against the FP64 backward (the per-axis adjoint passes, pinned by
 the reference Jacobian and by central differences in test_gpu_backward.py): every
 boundary condition it takes, both symmetries along each axis, radius classes 3..13,
@@ -26,7 +26,10 @@ CASES = [((1, 300, 256), [3.0, 3.0], [0, 0]),
          ((2, 200, 300), [3.0, 3.0], [1, 0]),
          ((1, 260, 290), [3.0, 3.0], [0, 1]),
          ((1, 120, 256), [1.0, 1.0], [1, 0]),
-         ((1, 300, 270), [3.5, 3.5], [0, 1])]
+         ((1, 300, 270), [3.5, 3.5], [0, 1]),
+         # two strips (128..255 columns), short chunks of a cached stack
+         ((4, 96, 128), [3.0, 3.0], [0, 0]),
+         ((3, 80, 200), [2.0, 3.0], [1, 0])]
 
 
 def _kernels_of(fn):
