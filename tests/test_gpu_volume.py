@@ -38,6 +38,8 @@ VOL = [
     # too many operations per sample for the fused kernel (y radius 8, z radius 4)
     ("f32_200x96x40_s25", (40, 96, 200), "float32", [2.5, 2.5, 1.3], [0], ["mirror"]),
     ("f32_260x70x90_zchunks", (90, 70, 260), "float32", [2.0, 1.0, 0.5], [0], ["mirror"]),
+    # a long z axis (more planes than one chunk's plane map holds): many plane chunks
+    ("f32_64x40x1100_longz", (1100, 40, 64), "float32", [1.0, 1.0, 1.0], [0], ["mirror"]),
     ("f32_130x64x9_asym_periodic", (9, 64, 130), "float32", [1.5, 1.5, 1.3], [0], ["asym periodic"]),
     ("f32_4d_128x64x20x2", (2, 20, 64, 128), "float32", [2.0, 2.0, 1.0, 0.0], [0], ["mirror"]),
     ("f64_64x32x16_o011", (16, 32, 64), "float64", [1.0, 2.0, 1.0], [0, 1, 1], ["periodic"]),
@@ -47,7 +49,7 @@ VOL = [
 # the cases the fused x+y+z kernel takes
 FUSED3 = {"f32_300x100x40_s221", "f32_256x96x50_o010_periodic", "f32_192x64x36_o100_bcs",
           "f32_260x70x90_zchunks", "f32_130x64x9_asym_periodic", "f32_4d_128x64x20x2",
-          "f32_130x70x40_s215", "f32_128x64x33_periodic", "f32_96x50x20_o102"}
+          "f32_130x70x40_s215", "f32_128x64x33_periodic", "f32_96x50x20_o102", "f32_64x40x1100_longz"}
 
 
 def _kernels_of(fn):
