@@ -78,14 +78,6 @@ __device__ __forceinline__ void f3_for_each(std::integer_sequence<int, U...>, F 
     (f(std::integral_constant<int, U>{}), ...);
 }
 
-// Wave-uniform boundary map (scalar registers).
-__device__ __forceinline__ int f3_fold(int j, int n, int bc, int *mul) {
-    int m;
-    const int s = __builtin_amdgcn_readfirstlane(bc_fold(j, n, bc, &m));
-    *mul = __builtin_amdgcn_readfirstlane(m);
-    return s;
-}
-
 }  // namespace
 
 template <typename TI, int SX, int SY, int SZ, int RXC, int RZ, bool FAST, bool INTERIOR>
